@@ -1,0 +1,104 @@
+/*
+ * pynear_amd C-ABI -- the drop-in boundary of the MI355X search path.
+ *
+ * The reference's boundary is the pybind11 module `_pynear` (pynear/src/PythonBindings.cpp:331-481).
+ * This header exports the same operations as plain C entry points (plain pointers and sizes, no
+ * Python or torch types) so that any binding -- ctypes, pybind11, cgo, JNI -- can sit on top.
+ * Each entry point names the reference interface it replaces.
+ *
+ * Conventions
+ *  - Every entry point returns a pn_status; on failure pn_last_error() (thread-local) holds the
+ *    message. PN_EINVAL maps to Python ValueError, PN_ETYPE to TypeError, every other non-zero
+ *    status to RuntimeError (the exception types pybind11 produces for the reference's throws).
+ *  - "host" pointers are ordinary CPU memory, borrowed for the duration of the call; `set` copies
+ *    the rows into device memory owned by the handle (the reference copies into the index too,
+ *    VPTree.hpp:96-104).
+ *  - "_device" entry points take device pointers and a hipStream_t (passed as void*, NULL = the
+ *    library's own stream) and do not synchronise: results are ready when the stream is.
+ *  - Result order is ascending by (distance, index); the Python layer reverses it where the
+ *    reference emits farthest-first (VPTree.hpp:621-631).
+ *  - Calls on one handle are serialised by the caller (the reference holds the GIL throughout);
+ *    handles are independent.
+ */
+#ifndef PYNEAR_AMD_H
+#define PYNEAR_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t pn_status;
+enum {
+    PN_OK = 0,
+    PN_EINVAL = 1,    /* bad argument value or shape  -> ValueError   */
+    PN_ERUNTIME = 2,  /* invalid state (empty index)  -> RuntimeError */
+    PN_ETYPE = 3,     /* wrong element type           -> TypeError    */
+    PN_EDEVICE = 4,   /* HIP runtime failure          -> RuntimeError */
+    PN_ENOMEM = 5     /* device allocation failure    -> RuntimeError */
+};
+
+typedef enum {
+    PN_METRIC_L2 = 0,        /* sqrt(sum (a-b)^2)   DistanceFunctions.hpp:304-345 */
+    PN_METRIC_L1 = 1,        /* sum |a-b|           DistanceFunctions.hpp:347-382 */
+    PN_METRIC_CHEBYSHEV = 2, /* max |a-b|           DistanceFunctions.hpp:384-420 */
+    PN_METRIC_HAMMING = 3    /* popcount(a xor b)   DistanceFunctions.hpp:431-462 */
+} pn_metric;
+
+/* Thread-local message of the last failed call on this thread. */
+const char *pn_last_error(void);
+
+/* Library version string. */
+const char *pn_version(void);
+
+/* Number of visible HIP devices (0 when none). */
+pn_status pn_device_count(int32_t *count);
+
+/* Measurement hook (no counterpart in the reference, which has no instrumentation): when enabled,
+ * every kernel stage records a pair of HIP events on its launch stream. pn_stage_times() waits for
+ * them and returns, per stage (0 scan, 1 select/merge, 2 coarse assignment, 3 rerank, 4 other),
+ * the summed milliseconds and launch counts since the previous call, then clears them. */
+pn_status pn_stage_timing(int32_t enable);
+pn_status pn_stage_times(double *ms /*[5]*/, int64_t *launches /*[5]*/);
+
+/* ---------------------------------------------------------------- distances
+ * Row-wise distances out[i] = dist(a[i], b[i]) for n row pairs of `dim` elements (float32 for
+ * L2/L1/Chebyshev, bytes for Hamming). Replaces the free functions dist_l2 / dist_l1 /
+ * dist_chebyshev / dist_hamming_{64,128,256,512} (PythonBindings.cpp:281-303, :332-338).
+ * out is float32[n] for float metrics, int64[n] for Hamming. Host pointers. */
+pn_status pn_pairwise_distance(int32_t metric, const void *a, const void *b, int64_t n, int64_t dim, void *out);
+
+/* ---------------------------------------------------------------- exact index
+ * Brute-force exact index; replaces the search side of vptree::VPTree<T, dist_t, dist>
+ * (VPTree.hpp:92-191) behind VPTreeL2Index / VPTreeL1Index / VPTreeChebyshevIndex /
+ * VPTreeBinaryIndex{64,128,256,512,} (PythonBindings.cpp:344-406). */
+typedef struct pn_exact pn_exact;
+
+/* VPTreeNumpyAdapter() (PythonBindings.cpp:33). */
+pn_status pn_exact_create(int32_t metric, pn_exact **out);
+void pn_exact_destroy(pn_exact *h);
+
+/* VPTree::set (VPTree.hpp:92-110) via the adapters' set (PythonBindings.cpp:36-47, :152).
+ * rows: host, n x dim (float32 elements, or bytes for Hamming), C-contiguous. n == 0 empties. */
+pn_status pn_exact_set(pn_exact *h, const void *rows, int64_t n, int64_t dim);
+
+/* Number of rows and row width of the indexed set (0, 0 when empty). */
+pn_status pn_exact_info(const pn_exact *h, int64_t *n, int64_t *dim);
+
+/* VPTree::searchKNN (VPTree.hpp:143-166) / search1NN (:169-191, k = 1).
+ * queries: host nq x dim. Writes kk = min(k, n) columns per query, ascending (dist, idx):
+ * out_idx int64[nq, kk]; out_dist float32[nq, kk] (float metrics, L2 as sqrt) or int64[nq, kk]
+ * (Hamming). An empty index returns PN_ERUNTIME (VPTree.hpp:145-147). */
+pn_status pn_exact_search(pn_exact *h, const void *queries, int64_t nq, int64_t dim, int32_t k,
+                          int64_t *out_idx, void *out_dist);
+
+/* Device-resident form of pn_exact_search: queries, out_idx (int32[nq, kk]) and out_dist
+ * (float32 or int32 [nq, kk]) are device pointers; enqueued on `stream`, no host sync. */
+pn_status pn_exact_search_device(pn_exact *h, const void *queries, int64_t nq, int32_t k, int32_t *out_idx,
+                                 void *out_dist, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYNEAR_AMD_H */

@@ -1,0 +1,154 @@
+"""ctypes binding of ``libpynear_amd.so`` (the C-ABI declared in ``include/pynear_amd.h``).
+
+This module is the Python side of the drop-in boundary: it plays the role of the reference's
+pybind11 module ``_pynear`` (pynear/src/PythonBindings.cpp:331-481). Status codes are mapped to
+the exception types pybind11 raises for the reference's C++ throws (runtime_error ->
+RuntimeError, invalid_argument -> ValueError).
+
+There is no fallback: if the library is missing or no HIP device is usable, calls raise.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpynear_amd.so")
+
+PN_OK, PN_EINVAL, PN_ERUNTIME, PN_ETYPE, PN_EDEVICE, PN_ENOMEM = range(6)
+METRIC_L2, METRIC_L1, METRIC_CHEBYSHEV, METRIC_HAMMING = range(4)
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_u32 = ctypes.c_uint32
+
+# name -> (restype, argtypes); every symbol declared in include/pynear_amd.h
+SIGNATURES = {
+    "pn_last_error": (ctypes.c_char_p, []),
+    "pn_version": (ctypes.c_char_p, []),
+    "pn_device_count": (_i32, [ctypes.POINTER(_i32)]),
+    "pn_stage_timing": (_i32, [_i32]),
+    "pn_stage_times": (_i32, [_vp, _vp]),
+    "pn_pairwise_distance": (_i32, [_i32, _vp, _vp, _i64, _i64, _vp]),
+    "pn_exact_create": (_i32, [_i32, ctypes.POINTER(_vp)]),
+    "pn_exact_destroy": (None, [_vp]),
+    "pn_exact_set": (_i32, [_vp, _vp, _i64, _i64]),
+    "pn_exact_info": (_i32, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "pn_exact_search": (_i32, [_vp, _vp, _i64, _i64, _i32, _vp, _vp]),
+    "pn_exact_search_device": (_i32, [_vp, _vp, _i64, _i32, _vp, _vp, _vp]),
+}
+
+_LIB = None
+
+
+class NativeLibraryError(RuntimeError):
+    """The HIP library could not be loaded; the GPU path has no CPU fallback."""
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise NativeLibraryError(
+            f"{LIB_PATH} not found: build it with `python -m pynear_amd.build` (hipcc, gfx950)")
+    try:
+        L = ctypes.CDLL(LIB_PATH)
+    except OSError as e:
+        raise NativeLibraryError(f"cannot load {LIB_PATH}: {e}") from e
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def check(status):
+    if status == PN_OK:
+        return
+    msg = lib().pn_last_error().decode(errors="replace")
+    if status == PN_EINVAL:
+        raise ValueError(msg)
+    if status == PN_ETYPE:
+        raise TypeError(msg)
+    raise RuntimeError(msg)
+
+
+def ptr(a):
+    """Address of a numpy array's data (None for an empty array)."""
+    return ctypes.c_void_p(a.ctypes.data) if a.size else None
+
+
+def device_count():
+    c = _i32(0)
+    check(lib().pn_device_count(ctypes.byref(c)))
+    return c.value
+
+
+class ExactHandle:
+    """Owner of a ``pn_exact`` handle (brute-force exact index on the current device)."""
+
+    def __init__(self, metric):
+        h = _vp()
+        check(lib().pn_exact_create(metric, ctypes.byref(h)))
+        self._h = h
+        self.metric = metric
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _LIB is not None:
+            _LIB.pn_exact_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set(self, rows):
+        n, d = rows.shape
+        check(lib().pn_exact_set(self._h, ptr(rows), n, d))
+
+    def info(self):
+        n, d = _i64(0), _i64(0)
+        check(lib().pn_exact_info(self._h, ctypes.byref(n), ctypes.byref(d)))
+        return n.value, d.value
+
+    def search(self, queries, k):
+        """Ascending (dist, idx) arrays of shape (nq, min(k, n))."""
+        nq, d = queries.shape
+        n, _ = self.info()
+        kk = max(0, min(int(k), n))
+        idx = np.empty((nq, kk), np.int64)
+        dist = np.empty((nq, kk), np.int64 if self.metric == METRIC_HAMMING else np.float32)
+        check(lib().pn_exact_search(self._h, ptr(queries), nq, d, int(k), ptr(idx), ptr(dist)))
+        return idx, dist
+
+    def search_device(self, q_ptr, nq, k, out_idx_ptr, out_dist_ptr, stream_ptr=None):
+        check(lib().pn_exact_search_device(self._h, _vp(q_ptr), nq, int(k), _vp(out_idx_ptr),
+                                           _vp(out_dist_ptr), _vp(stream_ptr) if stream_ptr else None))
+
+
+STAGES = ("scan", "select", "coarse", "rerank", "other")
+
+
+def stage_timing(enable):
+    check(lib().pn_stage_timing(1 if enable else 0))
+
+
+def stage_times():
+    """{stage: (total_ms, launches)} since the previous call (events resolved here)."""
+    ms = np.zeros(5, np.float64)
+    n = np.zeros(5, np.int64)
+    check(lib().pn_stage_times(ptr(ms), ptr(n)))
+    return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(STAGES)}
+
+
+def pairwise_distance(metric, a, b):
+    """Row-wise distances between equal-shape 2-D arrays a and b."""
+    n, d = a.shape
+    out = np.empty(n, np.int64 if metric == METRIC_HAMMING else np.float32)
+    check(lib().pn_pairwise_distance(metric, ptr(a), ptr(b), n, d, ptr(out)))
+    return out

@@ -1,0 +1,183 @@
+"""Index classes and free functions with the interface of the reference's ``_pynear`` module.
+
+Each class mirrors one pybind11 class of pynear/src/PythonBindings.cpp:331-481 (same names,
+methods, argument meaning, result layout and exception types) and runs on the GPU through the
+C-ABI in ``include/pynear_amd.h``:
+
+* ``VPTreeL2Index`` / ``VPTreeL1Index`` / ``VPTreeChebyshevIndex`` (:344-366): float32 rows,
+  ``searchKNN`` -> (list[list[int]], list[list[float]]) farthest-first (VPTree.hpp:621-631),
+  L2 distances are Euclidean (sqrt), ``search1NN`` -> (list[int], list[float]).
+* ``VPTreeBinaryIndex{512,256,128,64}`` and ``VPTreeBinaryIndex`` (:368-406): uint8 rows,
+  integer Hamming distances.
+
+The VP-tree walk is replaced by an exact brute-force scan, so the tree-shaped parts of the
+interface change meaning: ``to_string`` describes the scan index and pickling stores the rows
+(the reference pickles tree state, PythonBindings.cpp:97-142, :185-197). Neighbours come back in
+a total order: ascending (distance, index), reversed to farthest-first as in the reference.
+"""
+
+import numpy as np
+
+from . import _native as N
+
+
+def _as_2d_float(arr, what):
+    a = np.ascontiguousarray(np.asarray(arr), dtype=np.float32)  # py::array::forcecast
+    if a.ndim != 2:
+        raise RuntimeError(f"{what}() expects a 2D float32 array of shape (n, d)")
+    return a
+
+
+def _as_2d_u8(arr):
+    a = np.asarray(arr)
+    if a.ndim != 2:
+        raise RuntimeError("binary index expects a 2D array of uint8 rows")
+    if a.size and (a.dtype.kind not in "iub" or a.min() < 0 or a.max() > 255):
+        raise TypeError("binary index expects uint8 values")
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+class _ExactIndexBase:
+    _metric = None
+
+    def __init__(self):
+        self._h = N.ExactHandle(self._metric)
+        self._rows = None
+
+    # -- state
+    def _prepare(self, arr, what):
+        raise NotImplementedError
+
+    def set(self, vectors):
+        rows = self._prepare(vectors, "set")
+        self._h.set(rows)
+        self._rows = rows if rows.shape[0] else None
+
+    def _search(self, vectors, k, what):
+        q = self._prepare(vectors, what)
+        n, _ = self._h.info()
+        if n == 0:
+            raise RuntimeError("index must be first initialized with .set() function and non empty dataset")
+        return self._h.search(q, k)
+
+    def searchKNN(self, vectors, k):
+        """Batch find top-k vectors in index and return indices and distances (farthest-first)."""
+        idx, dist = self._search(vectors, k, "searchKNN")
+        return idx[:, ::-1].tolist(), dist[:, ::-1].tolist()
+
+    def search1NN(self, vectors):
+        """Batch find closest vectors in index and return indices and distances."""
+        idx, dist = self._search(vectors, 1, "search1NN")
+        return idx[:, 0].tolist(), dist[:, 0].tolist()
+
+    def to_string(self):
+        n, d = self._h.info()
+        return (f"####################\n# [{type(self).__name__} state]\nNum Data Points: {n}\n"
+                f"Dimension: {d}\nSearch: exact GPU scan\n####################\n")
+
+    # -- pickle: rows are stored and re-uploaded to the device on load
+    def __getstate__(self):
+        return {"rows": self._rows}
+
+    def __setstate__(self, state):
+        self.__init__()
+        if state.get("rows") is not None:
+            self.set(state["rows"])
+
+
+class _FloatIndex(_ExactIndexBase):
+    def _prepare(self, arr, what):
+        return _as_2d_float(arr, what)
+
+
+class _BinaryIndex(_ExactIndexBase):
+    _metric = N.METRIC_HAMMING
+    _width = None  # bytes per row for the fixed-width classes
+
+    def _prepare(self, arr, what):
+        a = _as_2d_u8(arr)
+        if self._width is not None and a.shape[0] and a.shape[1] != self._width:
+            raise ValueError(f"{type(self).__name__} expects rows of {self._width} bytes, got {a.shape[1]}")
+        return a
+
+
+class VPTreeL2Index(_FloatIndex):
+    _metric = N.METRIC_L2
+
+
+class VPTreeL1Index(_FloatIndex):
+    _metric = N.METRIC_L1
+
+
+class VPTreeChebyshevIndex(_FloatIndex):
+    _metric = N.METRIC_CHEBYSHEV
+
+
+class VPTreeBinaryIndex512(_BinaryIndex):
+    _width = 64
+
+
+class VPTreeBinaryIndex256(_BinaryIndex):
+    _width = 32
+
+
+class VPTreeBinaryIndex128(_BinaryIndex):
+    _width = 16
+
+
+class VPTreeBinaryIndex64(_BinaryIndex):
+    _width = 8
+
+
+class VPTreeBinaryIndex(_BinaryIndex):
+    _width = None
+
+
+# ---------------------------------------------------------------- free functions
+def _pair_float(metric, a, b):
+    a = np.ascontiguousarray(np.asarray(a), dtype=np.float32).reshape(1, -1)
+    b = np.ascontiguousarray(np.asarray(b), dtype=np.float32).reshape(1, -1)
+    if a.shape != b.shape:
+        raise ValueError("a and b must have the same number of elements")
+    return float(N.pairwise_distance(metric, a, b)[0])
+
+
+def dist_l2(a, b):
+    """Euclidean distance, reference arithmetic (DistanceFunctions.hpp:304-345)."""
+    return _pair_float(N.METRIC_L2, a, b)
+
+
+def dist_l1(a, b):
+    """Manhattan distance (DistanceFunctions.hpp:347-382)."""
+    return _pair_float(N.METRIC_L1, a, b)
+
+
+def dist_chebyshev(a, b):
+    """Chebyshev distance (DistanceFunctions.hpp:384-420)."""
+    return _pair_float(N.METRIC_CHEBYSHEV, a, b)
+
+
+def _hamming_fixed(a, b, nbytes):
+    a = np.asarray(a, dtype=np.uint8).reshape(1, -1)
+    b = np.asarray(b, dtype=np.uint8).reshape(1, -1)
+    if a.shape[1] < nbytes or b.shape[1] < nbytes:
+        raise ValueError(f"hamming distance on {nbytes * 8} bits needs {nbytes} bytes per operand")
+    a = np.ascontiguousarray(a[:, :nbytes])
+    b = np.ascontiguousarray(b[:, :nbytes])
+    return int(N.pairwise_distance(N.METRIC_HAMMING, a, b)[0])
+
+
+def dist_hamming_512(a, b):
+    return _hamming_fixed(a, b, 64)
+
+
+def dist_hamming_256(a, b):
+    return _hamming_fixed(a, b, 32)
+
+
+def dist_hamming_128(a, b):
+    return _hamming_fixed(a, b, 16)
+
+
+def dist_hamming_64(a, b):
+    return _hamming_fixed(a, b, 8)

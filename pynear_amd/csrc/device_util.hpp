@@ -1,0 +1,42 @@
+// Device-side helpers shared by the scan and select kernels.
+//
+// Ordering used everywhere: a candidate is the pair (distance, index) compared lexicographically,
+// which makes the selection a total order (lower index wins a distance tie). Inside a scan wave
+// the pair is packed into one unsigned key so that a single unsigned compare implements it:
+//   Hamming  u32 lane key  = d << 21 | row-in-slice         (d <= 1024 for rows up to 1024 bits)
+//   any      u64 list key  = monotone-bits(distance) << 32 | global index
+// Non-negative float distances map to monotone unsigned keys by their IEEE bit pattern.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pn {
+
+constexpr int kHamIdxBits = 21;                     // row-in-slice bits of the Hamming lane key
+constexpr uint32_t kHamIdxMask = (1u << kHamIdxBits) - 1u;
+constexpr int64_t kMaxRowsPerSlice = int64_t(1) << kHamIdxBits;
+constexpr uint32_t kKey32None = 0xFFFFFFFFu;
+constexpr uint64_t kKey64None = ~uint64_t(0);
+
+// Insert `key` into the ascending list L[0..K-1] (L[K-1] is dropped). With L sorted,
+// max(L[j-1], min(key, L[j])) == med3(L[j-1], key, L[j]), one v_med3_u32 per slot.
+template <int K> __device__ __forceinline__ void list_insert_u32(uint32_t (&L)[K], uint32_t key) {
+#pragma unroll
+    for (int j = K - 1; j > 0; --j) L[j] = max(L[j - 1], min(key, L[j]));
+    L[0] = min(L[0], key);
+}
+
+template <int K> __device__ __forceinline__ void list_insert_u64(uint64_t (&L)[K], uint64_t key) {
+#pragma unroll
+    for (int j = K - 1; j > 0; --j) {
+        uint64_t lo = key < L[j] ? key : L[j];
+        L[j] = L[j - 1] > lo ? L[j - 1] : lo;
+    }
+    L[0] = key < L[0] ? key : L[0];
+}
+
+__device__ __forceinline__ uint32_t f32_key(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float key_f32(uint32_t u) { return __uint_as_float(u); }
+
+} // namespace pn

@@ -1,0 +1,248 @@
+// pn_exact: brute-force exact index behind the VPTree*Index classes (C-ABI in pynear_amd.h).
+//
+// Replaces vptree::VPTree<T, dist_t, dist>::set/searchKNN/search1NN (VPTree.hpp:92-191) for the
+// float adapters (PythonBindings.cpp:33-145) and the binary adapters (:147-200). The tree walk
+// is replaced by a full scan; the result contract is kept: min(n, k) neighbours per query, exact
+// distances in the reference's arithmetic, an error on an empty index (VPTree.hpp:145-147).
+//
+// Device layout: rows padded to `row_elems` elements (32-bit words for Hamming, floats for the
+// float metrics; zero padding is neutral in every metric), contiguous, row-major.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "device_util.hpp"
+#include "kernels.hpp"
+
+struct pn_exact {
+    int32_t metric = PN_METRIC_L2;
+    int device = 0;
+    int64_t n = 0, dim = 0;      // rows; row length in floats (float metrics) or bytes (Hamming)
+    int64_t row_elems = 0;       // padded row length on the device (floats or 32-bit words)
+    bool scan_ok = false;        // the sliced scan kernel serves this row width
+    pn::DevBuf db, qbuf, part, oidx, odist, ws;
+    int cus = 256;
+};
+
+namespace pn {
+
+static int64_t padded_row_elems(int32_t metric, int64_t dim, bool *scan_ok) {
+    if (metric == PN_METRIC_HAMMING) {
+        int w = hamming_padded_words(dim);
+        *scan_ok = w > 0;
+        return w > 0 ? w : ceil_div(dim, 4);
+    }
+    *scan_ok = dim <= kFloatScanMaxDim;
+    if (!*scan_ok) return dim;
+    for (int p : {8, 16, 32, 64})
+        if (dim <= p) return p;
+    return dim;
+}
+
+static size_t elem_bytes(int32_t metric) { return 4; }
+
+static size_t row_bytes_in(int32_t metric, int64_t dim) {
+    return metric == PN_METRIC_HAMMING ? (size_t)dim : (size_t)dim * sizeof(float);
+}
+
+// Copies n host rows of `dim` elements into a zero-padded device array of `row_elems` elements.
+static void upload_padded(const pn_exact *h, const void *rows, int64_t n, int64_t dim, void *dst, hipStream_t st) {
+    size_t in_b = row_bytes_in(h->metric, dim);
+    size_t out_b = (size_t)h->row_elems * elem_bytes(h->metric);
+    if (in_b == out_b) {
+        PN_HIP(hipMemcpyAsync(dst, rows, in_b * n, hipMemcpyHostToDevice, st));
+        return;
+    }
+    std::vector<uint8_t> tmp(out_b * n, 0);
+    for (int64_t i = 0; i < n; ++i) std::memcpy(tmp.data() + i * out_b, (const uint8_t *)rows + i * in_b, in_b);
+    PN_HIP(hipMemcpyAsync(dst, tmp.data(), tmp.size(), hipMemcpyHostToDevice, st));
+    PN_HIP(hipStreamSynchronize(st)); // tmp is released on return
+}
+
+static int list_size_for(int kk) {
+    int K = 1;
+    while (K < kk) K <<= 1;
+    return K;
+}
+
+// Enqueues the search of nq padded device queries; out_idx int32[nq, kk], out_dist int32/f32.
+static void exact_search_enqueue(pn_exact *h, const void *dq, int64_t nq, int kk, int32_t *oi, void *od,
+                                 hipStream_t st) {
+    int K = list_size_for(kk);
+    int out_kind = h->metric == PN_METRIC_HAMMING ? OUT_HAMMING_I32
+                                                  : (h->metric == PN_METRIC_L2 ? OUT_L2_F32 : OUT_F32);
+    if (h->scan_ok && K <= 32) {
+        int64_t gx = ceil_div(nq, 256);
+        int64_t nq_pad = gx * 256;
+        int64_t target = 4LL * h->cus;
+        int64_t S = std::max<int64_t>(1, ceil_div(target, gx));
+        S = std::min<int64_t>(S, std::max<int64_t>(1, h->n / 64));
+        S = std::max<int64_t>(S, ceil_div(h->n, kMaxRowsPerSlice));
+        int64_t rps = ceil_div(h->n, S);
+        S = ceil_div(h->n, rps);
+        uint64_t *part = (uint64_t *)h->part.ensure((size_t)S * K * nq_pad * sizeof(uint64_t));
+        ScanArgs a{h->db.p, h->n, h->row_elems, h->dim, dq, nq, rps, S, part, nq_pad};
+        {
+            StageScope sc(st, STAGE_SCAN);
+            if (h->metric == PN_METRIC_HAMMING)
+                hamming_scan(a, (int)h->row_elems, K, st);
+            else
+                float_scan(a, h->metric, K, st);
+        }
+        StageScope sc(st, STAGE_SELECT);
+        merge_lists(part, S, K, nq, nq_pad, kk, out_kind, oi, od, st);
+        return;
+    }
+    // Distance-matrix path: bound the matrix to ~1 GiB of keys per chunk.
+    int64_t q_chunk = std::max<int64_t>(1, std::min<int64_t>({nq, (int64_t)(256 << 20) / std::max<int64_t>(h->n, 1),
+                                                                (int64_t)(INT32_MAX - 1) / std::max<int64_t>(h->n, 1),
+                                                                60000}));
+    size_t wsb = matrix_search_workspace(h->n, q_chunk);
+    void *ws = h->ws.ensure(wsb);
+    StageScope sc(st, STAGE_OTHER);
+    matrix_search(h->metric, h->db.p, h->n, h->row_elems, h->dim, dq, nq, kk, oi, od, ws, wsb, q_chunk, st);
+}
+
+} // namespace pn
+
+using namespace pn;
+
+extern "C" pn_status pn_exact_create(int32_t metric, pn_exact **out) {
+    return guard([&] {
+        PN_REQUIRE(out, PN_EINVAL, "out is null");
+        PN_REQUIRE(metric >= PN_METRIC_L2 && metric <= PN_METRIC_HAMMING, PN_EINVAL, "unknown metric");
+        auto *h = new pn_exact();
+        h->metric = metric;
+        h->device = current_device();
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess && cus > 0)
+            h->cus = cus;
+        *out = h;
+    });
+}
+
+extern "C" void pn_exact_destroy(pn_exact *h) {
+    if (!h) return;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(h->device);
+    delete h;
+    if (prev >= 0) (void)hipSetDevice(prev);
+}
+
+extern "C" pn_status pn_exact_set(pn_exact *h, const void *rows, int64_t n, int64_t dim) {
+    return guard([&] {
+        PN_REQUIRE(h, PN_EINVAL, "index is null");
+        PN_REQUIRE(n >= 0 && dim >= 0, PN_EINVAL, "negative shape");
+        DeviceScope ds(h->device);
+        if (n == 0) { // VPTree::set on an empty array leaves the index empty (VPTree.hpp:93-94)
+            h->n = 0;
+            h->dim = 0;
+            h->db.release();
+            return;
+        }
+        PN_REQUIRE(rows, PN_EINVAL, "rows is null");
+        PN_REQUIRE(dim > 0, PN_EINVAL, "rows must have at least one element");
+        PN_REQUIRE(n < (int64_t)INT32_MAX, PN_EINVAL, "too many rows (indices are 32-bit on the device)");
+        bool scan_ok = false;
+        int64_t re = padded_row_elems(h->metric, dim, &scan_ok);
+        h->n = 0;
+        h->dim = dim;
+        h->row_elems = re;
+        h->scan_ok = scan_ok;
+        hipStream_t st = lib_stream();
+        void *dst = h->db.ensure((size_t)n * re * elem_bytes(h->metric));
+        upload_padded(h, rows, n, dim, dst, st);
+        PN_HIP(hipStreamSynchronize(st));
+        h->n = n;
+    });
+}
+
+extern "C" pn_status pn_exact_info(const pn_exact *h, int64_t *n, int64_t *dim) {
+    return guard([&] {
+        PN_REQUIRE(h && n && dim, PN_EINVAL, "null argument");
+        *n = h->n;
+        *dim = h->dim;
+    });
+}
+
+extern "C" pn_status pn_exact_search(pn_exact *h, const void *queries, int64_t nq, int64_t dim, int32_t k,
+                                     int64_t *out_idx, void *out_dist) {
+    return guard([&] {
+        PN_REQUIRE(h, PN_EINVAL, "index is null");
+        PN_REQUIRE(h->n > 0, PN_ERUNTIME, "index must be first initialized with .set() function and non empty dataset");
+        PN_REQUIRE(nq >= 0, PN_EINVAL, "negative query count");
+        PN_REQUIRE(k >= 0, PN_EINVAL, "k must be non-negative");
+        PN_REQUIRE(dim == h->dim, PN_EINVAL,
+                   "invalid data dimension: index built data and query data dimensions must agree");
+        int kk = (int)std::min<int64_t>(k, h->n);
+        if (nq == 0 || kk == 0) return;
+        PN_REQUIRE(queries && out_idx && out_dist, PN_EINVAL, "null pointer");
+        DeviceScope ds(h->device);
+        hipStream_t st = lib_stream();
+        size_t qb = (size_t)nq * h->row_elems * 4;
+        void *dq = h->qbuf.ensure(qb);
+        upload_padded(h, queries, nq, dim, dq, st);
+        int32_t *oi = (int32_t *)h->oidx.ensure((size_t)nq * kk * 4);
+        void *od = h->odist.ensure((size_t)nq * kk * 4);
+        exact_search_enqueue(h, dq, nq, kk, oi, od, st);
+        std::vector<int32_t> hi((size_t)nq * kk), hd((size_t)nq * kk);
+        PN_HIP(hipMemcpyAsync(hi.data(), oi, hi.size() * 4, hipMemcpyDeviceToHost, st));
+        PN_HIP(hipMemcpyAsync(hd.data(), od, hd.size() * 4, hipMemcpyDeviceToHost, st));
+        PN_HIP(hipStreamSynchronize(st));
+        for (size_t i = 0; i < hi.size(); ++i) out_idx[i] = hi[i];
+        if (h->metric == PN_METRIC_HAMMING) {
+            int64_t *o = (int64_t *)out_dist;
+            for (size_t i = 0; i < hd.size(); ++i) o[i] = hd[i];
+        } else {
+            std::memcpy(out_dist, hd.data(), hd.size() * 4);
+        }
+    });
+}
+
+extern "C" pn_status pn_exact_search_device(pn_exact *h, const void *queries, int64_t nq, int32_t k,
+                                            int32_t *out_idx, void *out_dist, void *stream) {
+    return guard([&] {
+        PN_REQUIRE(h, PN_EINVAL, "index is null");
+        PN_REQUIRE(h->n > 0, PN_ERUNTIME, "index must be first initialized with .set() function and non empty dataset");
+        PN_REQUIRE(nq >= 0 && k >= 0, PN_EINVAL, "negative size");
+        int kk = (int)std::min<int64_t>(k, h->n);
+        if (nq == 0 || kk == 0) return;
+        PN_REQUIRE(queries && out_idx && out_dist, PN_EINVAL, "null pointer");
+        DeviceScope ds(h->device);
+        hipStream_t st = stream ? (hipStream_t)stream : lib_stream();
+        const void *dq = queries;
+        size_t in_b = row_bytes_in(h->metric, h->dim);
+        size_t pad_b = (size_t)h->row_elems * 4;
+        if (in_b != pad_b) { // pad into the workspace (zero tail per row)
+            void *tmp = h->qbuf.ensure((size_t)nq * pad_b);
+            PN_HIP(hipMemsetAsync(tmp, 0, (size_t)nq * pad_b, st));
+            PN_HIP(hipMemcpy2DAsync(tmp, pad_b, queries, in_b, in_b, nq, hipMemcpyDeviceToDevice, st));
+            dq = tmp;
+        }
+        exact_search_enqueue(h, dq, nq, kk, out_idx, out_dist, st);
+    });
+}
+
+extern "C" pn_status pn_pairwise_distance(int32_t metric, const void *a, const void *b, int64_t n, int64_t dim,
+                                          void *out) {
+    return guard([&] {
+        PN_REQUIRE(metric >= PN_METRIC_L2 && metric <= PN_METRIC_HAMMING, PN_EINVAL, "unknown metric");
+        PN_REQUIRE(n >= 0 && dim >= 0, PN_EINVAL, "negative shape");
+        if (n == 0) return;
+        PN_REQUIRE(a && b && out, PN_EINVAL, "null pointer");
+        hipStream_t st = lib_stream();
+        size_t rb = row_bytes_in(metric, dim) * n;
+        size_t ob = n * (metric == PN_METRIC_HAMMING ? 8 : 4);
+        DevBuf da, db, dout;
+        da.ensure(rb);
+        db.ensure(rb);
+        dout.ensure(ob);
+        PN_HIP(hipMemcpyAsync(da.p, a, rb, hipMemcpyHostToDevice, st));
+        PN_HIP(hipMemcpyAsync(db.p, b, rb, hipMemcpyHostToDevice, st));
+        pairwise_distance(metric, da.p, db.p, n, dim, dout.p, st);
+        PN_HIP(hipMemcpyAsync(out, dout.p, ob, hipMemcpyDeviceToHost, st));
+        PN_HIP(hipStreamSynchronize(st));
+    });
+}

@@ -1,0 +1,56 @@
+// Host-side launchers of the HIP kernels (one translation unit each) used by the index code.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pn {
+
+// Arguments of a sliced exact scan. The database and the queries are device arrays with a row
+// stride of `row_elems` elements (32-bit words for Hamming, floats for the float metrics).
+// part: [slices][K][nq_pad] u64 keys (see device_util.hpp), written by the scan.
+struct ScanArgs {
+    const void *db;
+    int64_t n;
+    int64_t row_elems;
+    int64_t dim;             // real row length (floats) for the float metrics
+    const void *queries;
+    int64_t nq;
+    int64_t rows_per_slice;
+    int64_t slices;
+    uint64_t *part;
+    int64_t nq_pad;
+};
+
+// Distance kinds for the select stage's output.
+enum OutKind { OUT_HAMMING_I32 = 0, OUT_L2_F32 = 1, OUT_F32 = 2 };
+
+// Hamming scan over rows of `words` 32-bit words; K in {1,2,4,8,16,32}.
+void hamming_scan(const ScanArgs &a, int words, int K, hipStream_t st);
+// Padded row width (words) served by the Hamming scan for `nbytes`-byte rows; 0 when too wide.
+int hamming_padded_words(int64_t nbytes);
+
+// Float scan (reference summation order) for dim <= kFloatScanMaxDim; K in {1,...,32}.
+constexpr int kFloatScanMaxDim = 64;
+void float_scan(const ScanArgs &a, int metric, int K, hipStream_t st);
+
+// Merge the per-slice lists into kk results per query, ascending (distance, index).
+// out_idx int32[nq, kk]; out_dist int32 or float32 [nq, kk] per `kind`.
+void merge_lists(const uint64_t *part, int64_t slices, int K, int64_t nq, int64_t nq_pad, int kk, int kind,
+                 int32_t *out_idx, void *out_dist, hipStream_t st);
+
+// Exact k-NN through a full distance matrix and a segmented radix sort; any k, any width.
+// metric: PN_METRIC_*; rows/queries padded to `row_elems` (words for Hamming, floats otherwise).
+struct MatrixSearch {
+    void *ws = nullptr;     // workspace, caller-owned
+    size_t ws_bytes = 0;
+};
+size_t matrix_search_workspace(int64_t n, int64_t q_chunk);
+void matrix_search(int metric, const void *db, int64_t n, int64_t row_elems, int64_t dim, const void *queries,
+                   int64_t nq, int kk, int32_t *out_idx, void *out_dist, void *ws, size_t ws_bytes,
+                   int64_t q_chunk, hipStream_t st);
+
+// Row-wise distances out[i] = dist(a[i], b[i]); device pointers.
+void pairwise_distance(int metric, const void *a, const void *b, int64_t n, int64_t dim, void *out, hipStream_t st);
+
+} // namespace pn

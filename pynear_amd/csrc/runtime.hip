@@ -1,0 +1,93 @@
+// Library-wide runtime pieces of the C-ABI: the thread-local error string, the library stream,
+// version and device queries.
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace pn {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+hipStream_t lib_stream() {
+    // One non-blocking stream per device, created on first use at default priority.
+    static std::mutex mu;
+    static hipStream_t streams[64] = {};
+    int dev = current_device();
+    PN_REQUIRE(dev >= 0 && dev < 64, PN_EDEVICE, "device id out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!streams[dev]) PN_HIP(hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking));
+    return streams[dev];
+}
+
+// Stage timer: begin/end events per launch, resolved by pn_stage_times().
+static std::mutex g_stage_mu;
+static bool g_stage_on = false;
+struct StageRec {
+    int stage;
+    hipEvent_t b, e;
+};
+static std::vector<StageRec> g_stage_recs;
+static hipEvent_t g_open[STAGE_COUNT] = {};
+
+void stage_mark(hipStream_t st, int stage, bool begin) {
+    if (!g_stage_on) return;
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    hipEvent_t ev;
+    if (hipEventCreate(&ev) != hipSuccess) return;
+    (void)hipEventRecord(ev, st);
+    if (begin) {
+        g_open[stage] = ev;
+    } else if (g_open[stage]) {
+        g_stage_recs.push_back({stage, g_open[stage], ev});
+        g_open[stage] = nullptr;
+    } else {
+        (void)hipEventDestroy(ev);
+    }
+}
+
+} // namespace pn
+
+extern "C" pn_status pn_stage_timing(int32_t enable) {
+    return pn::guard([&] {
+        std::lock_guard<std::mutex> lk(pn::g_stage_mu);
+        pn::g_stage_on = enable != 0;
+    });
+}
+
+extern "C" pn_status pn_stage_times(double *ms, int64_t *launches) {
+    return pn::guard([&] {
+        PN_REQUIRE(ms && launches, PN_EINVAL, "null argument");
+        std::lock_guard<std::mutex> lk(pn::g_stage_mu);
+        for (int s = 0; s < pn::STAGE_COUNT; ++s) {
+            ms[s] = 0.0;
+            launches[s] = 0;
+        }
+        for (auto &r : pn::g_stage_recs) {
+            PN_HIP(hipEventSynchronize(r.e));
+            float t = 0.f;
+            PN_HIP(hipEventElapsedTime(&t, r.b, r.e));
+            ms[r.stage] += t;
+            launches[r.stage] += 1;
+            (void)hipEventDestroy(r.b);
+            (void)hipEventDestroy(r.e);
+        }
+        pn::g_stage_recs.clear();
+    });
+}
+
+extern "C" const char *pn_last_error(void) { return pn::g_last_error.c_str(); }
+
+extern "C" const char *pn_version(void) { return "pynear_amd 0.1.0 (gfx950)"; }
+
+extern "C" pn_status pn_device_count(int32_t *count) {
+    return pn::guard([&] {
+        PN_REQUIRE(count, PN_EINVAL, "count is null");
+        int c = 0;
+        hipError_t e = hipGetDeviceCount(&c);
+        *count = e == hipSuccess ? c : 0;
+    });
+}

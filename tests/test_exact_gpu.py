@@ -1,0 +1,158 @@
+"""GPU parity of the exact search (pn_exact) against the CPU oracle.
+
+Hamming results must be bit-exact, indices included, with ties broken by lower index. Float
+results must match the oracle's distances bit for bit (the kernels use the reference's operation
+order) and its (distance, index) order.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from pynear_amd import _native as N
+
+pytestmark = pytest.mark.gpu
+
+
+def _hamming_case(n, nbytes, nq, seed, dup_frac=0.0):
+    rng = np.random.default_rng(seed)
+    X = rng.integers(0, 256, (n, nbytes), dtype=np.uint8)
+    if dup_frac:
+        m = int(n * dup_frac)
+        X[rng.integers(0, n, m)] = X[rng.integers(0, n, m)]
+    Q = X[rng.integers(0, n, nq)].copy()
+    flips = rng.integers(0, 256, Q.shape, dtype=np.uint8) & rng.integers(0, 256, Q.shape, dtype=np.uint8)
+    Q[: nq // 2] ^= flips[: nq // 2]
+    return X, Q
+
+
+@pytest.mark.parametrize("nbytes", [4, 8, 16, 20, 32, 64, 100, 130])
+@pytest.mark.parametrize("k", [1, 5, 10, 32])
+def test_hamming_widths_and_k(nbytes, k):
+    X, Q = _hamming_case(3001, nbytes, 70, seed=nbytes * 100 + k)
+    h = N.ExactHandle(N.METRIC_HAMMING)
+    h.set(X)
+    gi, gd = h.search(Q, k)
+    oi, od = oracle.knn_hamming(X, Q, k)
+    np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(gi, oi)
+
+
+@pytest.mark.parametrize("k", [1, 7, 33, 300])
+def test_hamming_ties_lower_index_wins(k):
+    # few distinct values -> massive ties at every rank
+    rng = np.random.default_rng(7)
+    X = (rng.integers(0, 4, (5000, 8)) * 3).astype(np.uint8)
+    Q = X[:40].copy()
+    h = N.ExactHandle(N.METRIC_HAMMING)
+    h.set(X)
+    gi, gd = h.search(Q, k)
+    oi, od = oracle.knn_hamming(X, Q, k)
+    np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(gi, oi)
+
+
+def test_hamming_config2_shape_sample():
+    """The 512-bit workload (uniform rows, Binomial(512, 0.1) bit flips) at 200k rows."""
+    import bench
+    X, Q = bench.hamming_workload(200_000, 2_000, seed=1)
+    h = N.ExactHandle(N.METRIC_HAMMING)
+    h.set(X)
+    gi, gd = h.search(Q, 10)
+    sub = np.arange(0, len(Q), 10)
+    oi, od = oracle.knn_hamming(X, Q[sub], 10)
+    np.testing.assert_array_equal(gd[sub], od)
+    np.testing.assert_array_equal(gi[sub], oi)
+    # size-independent properties on every query: sorted, source row first at its flip distance
+    assert (np.diff(gd, axis=1) >= 0).all()
+
+
+@pytest.mark.parametrize("metric", [N.METRIC_L2, N.METRIC_L1, N.METRIC_CHEBYSHEV])
+@pytest.mark.parametrize("d", [1, 2, 3, 5, 8, 13, 16, 31, 32, 33, 64, 65, 100])
+def test_float_bit_exact(metric, d):
+    rng = np.random.default_rng(1000 + d * 7 + metric)
+    X = rng.random((2500, d)).astype(np.float32)
+    Q = rng.random((45, d)).astype(np.float32)
+    h = N.ExactHandle(metric)
+    h.set(X)
+    for k in (1, 5, 16):
+        gi, gd = h.search(Q, k)
+        oi, od = oracle.knn_float(metric, X, Q, k)
+        np.testing.assert_array_equal(gd, od)
+        np.testing.assert_array_equal(gi, oi)
+
+
+@pytest.mark.parametrize("metric", [N.METRIC_L2, N.METRIC_L1, N.METRIC_CHEBYSHEV])
+def test_float_vs_fp64(metric):
+    """Distances within rtol 1e-5 of FP64 and equal index sets (no near-ties in this data)."""
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((4000, 24)).astype(np.float32)
+    Q = rng.standard_normal((30, 24)).astype(np.float32)
+    h = N.ExactHandle(metric)
+    h.set(X)
+    gi, gd = h.search(Q, 8)
+    fi, fd = oracle.knn_fp64(metric, X, Q, 8)
+    np.testing.assert_allclose(gd, fd, rtol=1e-5)
+    np.testing.assert_array_equal(gi, fi)
+
+
+@pytest.mark.parametrize("metric", [N.METRIC_L2, N.METRIC_L1, N.METRIC_CHEBYSHEV, N.METRIC_HAMMING])
+def test_k_at_and_above_n(metric):
+    rng = np.random.default_rng(11)
+    if metric == N.METRIC_HAMMING:
+        X = rng.integers(0, 256, (37, 8), dtype=np.uint8)
+        Q = rng.integers(0, 256, (5, 8), dtype=np.uint8)
+        ref = lambda k: oracle.knn_hamming(X, Q, k)  # noqa: E731
+    else:
+        X = rng.random((37, 6)).astype(np.float32)
+        Q = rng.random((5, 6)).astype(np.float32)
+        ref = lambda k: oracle.knn_float(metric, X, Q, k)  # noqa: E731
+    h = N.ExactHandle(metric)
+    h.set(X)
+    for k in (36, 37, 38, 100):
+        gi, gd = h.search(Q, k)
+        oi, od = ref(k)
+        assert gi.shape == (5, 37 if k >= 37 else k)
+        np.testing.assert_array_equal(gd, od)
+        np.testing.assert_array_equal(gi, oi)
+
+
+def test_empty_and_mismatch_errors():
+    h = N.ExactHandle(N.METRIC_L2)
+    with pytest.raises(RuntimeError):
+        h.search(np.zeros((1, 4), np.float32), 1)
+    h.set(np.zeros((0, 4), np.float32))
+    with pytest.raises(RuntimeError):
+        h.search(np.zeros((1, 4), np.float32), 1)
+    h.set(np.ones((10, 4), np.float32))
+    with pytest.raises(ValueError):
+        h.search(np.zeros((1, 5), np.float32), 1)
+
+
+def test_device_entry_point_matches_host():
+    torch = pytest.importorskip("torch")
+    X, Q = _hamming_case(50_000, 64, 600, seed=3)
+    h = N.ExactHandle(N.METRIC_HAMMING)
+    h.set(X)
+    hi, hd = h.search(Q, 10)
+    dq = torch.from_numpy(Q).cuda()
+    oi = torch.empty((600, 10), dtype=torch.int32, device="cuda")
+    od = torch.empty((600, 10), dtype=torch.int32, device="cuda")
+    h.search_device(dq.data_ptr(), 600, 10, oi.data_ptr(), od.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(oi.cpu().numpy(), hi)
+    np.testing.assert_array_equal(od.cpu().numpy(), hd)
+
+
+def test_pairwise_distances_bit_exact():
+    rng = np.random.default_rng(9)
+    for d in (1, 3, 8, 13, 32, 77):
+        a = rng.standard_normal((200, d)).astype(np.float32)
+        b = rng.standard_normal((200, d)).astype(np.float32)
+        for metric, m in ((N.METRIC_L2, oracle.L2), (N.METRIC_L1, oracle.L1), (N.METRIC_CHEBYSHEV, oracle.LINF)):
+            g = N.pairwise_distance(metric, a, b)
+            o = np.array([oracle.dist(m, a[i], b[i]) for i in range(200)], np.float32)
+            np.testing.assert_array_equal(g, o)
+    a = rng.integers(0, 256, (300, 23), dtype=np.uint8)
+    b = rng.integers(0, 256, (300, 23), dtype=np.uint8)
+    np.testing.assert_array_equal(N.pairwise_distance(N.METRIC_HAMMING, a, b), np.bitwise_count(a ^ b).sum(1))
