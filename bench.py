@@ -4,7 +4,8 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hamming|l2]
 
 A step is one search of the whole query batch, with the database and the queries already
-resident in HBM (the device entry point ``pn_exact_search_device`` on torch's current stream).
+resident in HBM (the device entry point ``pn_exact_search_device`` on a HIP stream of the
+library's own runtime, see pynear_amd/hipdev.py).
 Default workload: exact Hamming k-NN, N = 1,000,000 rows x 512 bits, 10,000 queries, k = 10
 (uniform random rows; each query is a random row with a Binomial(512, 0.1)-sized random set of
 bits flipped; seed 1). ``--workload l2``: exact L2 k-NN, N = 100,000 x 32 float32 U[0,1),
@@ -114,51 +115,50 @@ def main():
               "MI355X; multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     from pynear_amd import _native as N
+    from pynear_amd import hipdev
 
     W = WORKLOADS[args.workload]
     k = W["k"]
     if args.workload == "hamming":
         X, Q = hamming_workload(W["n"], W["nq"], seed=W["seed"])
         metric = N.METRIC_HAMMING
-        dist_dtype = torch.int32
+        dist_dtype = np.int32
     else:
         X, Q = l2_workload(W["n"], W["nq"], seed=W["seed"])
         metric = N.METRIC_L2
-        dist_dtype = torch.float32
+        dist_dtype = np.float32
     n, nq = len(X), len(Q)
 
     h = N.ExactHandle(metric)
     h.set(X)
-    dq = torch.from_numpy(Q).cuda()
-    oi = torch.empty((nq, k), dtype=torch.int32, device="cuda")
-    od = torch.empty((nq, k), dtype=dist_dtype, device="cuda")
-    stream = torch.cuda.current_stream()
-    sp = stream.cuda_stream
+    dq = hipdev.DeviceArray.from_numpy(Q)
+    oi = hipdev.DeviceArray((nq, k), np.int32)
+    od = hipdev.DeviceArray((nq, k), dist_dtype)
+    stream = hipdev.Stream()
 
     def step():
-        h.search_device(dq.data_ptr(), nq, k, oi.data_ptr(), od.data_ptr(), sp)
+        h.search_device(dq.ptr, nq, k, oi.ptr, od.ptr, stream.handle)
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    stream.synchronize()
 
     N.stage_timing(True)
     N.stage_times()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
+    ev0, ev1 = hipdev.Event(), hipdev.Event()
+    hipdev.synchronize()
     t0 = time.perf_counter()
     ev0.record(stream)
     for _ in range(args.steps):
         step()
     ev1.record(stream)
-    torch.cuda.synchronize()
+    ev1.synchronize()
+    hipdev.synchronize()
     wall = time.perf_counter() - t0
     N.stage_timing(False)
     stages = N.stage_times()
-    gpu_ms = ev0.elapsed_time(ev1)
+    gpu_ms = ev0.elapsed_ms(ev1)
     ms_per_step = gpu_ms / args.steps
     qps = nq * args.steps / (gpu_ms / 1e3)
 
@@ -201,9 +201,10 @@ def main():
         "roofline": roof,
         "stages_ms_per_step": {s: round(v[0] / max(args.steps, 1), 4) for s, v in stages.items() if v[1]},
         "wall_s_timed": round(wall, 4),
+        "tuning": {name: N.tune_get(name) for name in ("hamming.wg_per_cu", "float.wg_per_cu", "merge.wave_per_query")},
     }
     if not args.no_cpu_baseline:
-        got_d = od.cpu().numpy().astype(np.float64)
+        got_d = od.numpy().astype(np.float64)
         cb, ref_d = cpu_baseline(args.workload, X, Q, k, oi)
         cb["value"] = round(cb["value"], 2)
         result["cpu_baseline"] = cb

@@ -13,8 +13,9 @@
  *  - "host" pointers are ordinary CPU memory, borrowed for the duration of the call; `set` copies
  *    the rows into device memory owned by the handle (the reference copies into the index too,
  *    VPTree.hpp:96-104).
- *  - "_device" entry points take device pointers and a hipStream_t (passed as void*, NULL = the
- *    library's own stream) and do not synchronise: results are ready when the stream is.
+ *  - "_device" entry points take device pointers and a hipStream_t (passed as void*; NULL is the
+ *    device's null stream, as everywhere in HIP) and do not synchronise: results are ready when
+ *    the stream is.
  *  - Result order is ascending by (distance, index); the Python layer reverses it where the
  *    reference emits farthest-first (VPTree.hpp:621-631).
  *  - Calls on one handle are serialised by the caller (the reference holds the GIL throughout);
@@ -61,6 +62,12 @@ pn_status pn_device_count(int32_t *count);
  * the summed milliseconds and launch counts since the previous call, then clears them. */
 pn_status pn_stage_timing(int32_t enable);
 pn_status pn_stage_times(double *ms /*[5]*/, int64_t *launches /*[5]*/);
+
+/* Launch-shape tuning parameters (no counterpart in the reference): named integers such as
+ * "hamming.wg_per_cu" that the autotuner (pynear_amd/autotune.py) sets from its measured table.
+ * Unknown names return PN_EINVAL. */
+pn_status pn_tune_set(const char *name, int64_t value);
+pn_status pn_tune_get(const char *name, int64_t *value);
 
 /* ---------------------------------------------------------------- distances
  * Row-wise distances out[i] = dist(a[i], b[i]) for n row pairs of `dim` elements (float32 for

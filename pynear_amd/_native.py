@@ -31,6 +31,8 @@ SIGNATURES = {
     "pn_device_count": (_i32, [ctypes.POINTER(_i32)]),
     "pn_stage_timing": (_i32, [_i32]),
     "pn_stage_times": (_i32, [_vp, _vp]),
+    "pn_tune_set": (_i32, [ctypes.c_char_p, _i64]),
+    "pn_tune_get": (_i32, [ctypes.c_char_p, ctypes.POINTER(_i64)]),
     "pn_pairwise_distance": (_i32, [_i32, _vp, _vp, _i64, _i64, _vp]),
     "pn_exact_create": (_i32, [_i32, ctypes.POINTER(_vp)]),
     "pn_exact_destroy": (None, [_vp]),
@@ -129,6 +131,16 @@ class ExactHandle:
     def search_device(self, q_ptr, nq, k, out_idx_ptr, out_dist_ptr, stream_ptr=None):
         check(lib().pn_exact_search_device(self._h, _vp(q_ptr), nq, int(k), _vp(out_idx_ptr),
                                            _vp(out_dist_ptr), _vp(stream_ptr) if stream_ptr else None))
+
+
+def tune_set(name, value):
+    check(lib().pn_tune_set(name.encode(), int(value)))
+
+
+def tune_get(name):
+    v = _i64(0)
+    check(lib().pn_tune_get(name.encode(), ctypes.byref(v)))
+    return v.value
 
 
 STAGES = ("scan", "select", "coarse", "rerank", "other")

@@ -101,6 +101,10 @@ inline int current_device() {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Launch-shape tuning table: named integer parameters with defaults, overridable through the
+// PN_TUNE environment variable ("name=value,name=value") and pn_tune_set().
+int64_t tune_get(const char *name);
+
 // Optional per-stage timing with HIP events recorded on the launch stream (off by default).
 enum Stage { STAGE_SCAN = 0, STAGE_SELECT = 1, STAGE_COARSE = 2, STAGE_RERANK = 3, STAGE_OTHER = 4, STAGE_COUNT = 5 };
 void stage_mark(hipStream_t st, int stage, bool begin);

@@ -73,16 +73,19 @@ static void exact_search_enqueue(pn_exact *h, const void *dq, int64_t nq, int kk
     int out_kind = h->metric == PN_METRIC_HAMMING ? OUT_HAMMING_I32
                                                   : (h->metric == PN_METRIC_L2 ? OUT_L2_F32 : OUT_F32);
     if (h->scan_ok && K <= 32) {
-        int64_t gx = ceil_div(nq, 256);
-        int64_t nq_pad = gx * 256;
-        int64_t target = 4LL * h->cus;
-        int64_t S = std::max<int64_t>(1, ceil_div(target, gx));
+        const int qpl = (h->metric == PN_METRIC_HAMMING && tune_get("hamming.qpl") == 2 && nq > 256) ? 2 : 1;
+        int64_t gx = ceil_div(nq, 256 * qpl);
+        int64_t nq_pad = gx * 256 * qpl;
+        // Whole multiples of the CU count: every CU holds the same number of resident workgroups.
+        int64_t wpc = tune_get(h->metric == PN_METRIC_HAMMING ? "hamming.wg_per_cu" : "float.wg_per_cu");
+        int64_t target = std::max<int64_t>(1, wpc) * h->cus;
+        int64_t S = std::max<int64_t>(1, target / gx);
         S = std::min<int64_t>(S, std::max<int64_t>(1, h->n / 64));
         S = std::max<int64_t>(S, ceil_div(h->n, kMaxRowsPerSlice));
         int64_t rps = ceil_div(h->n, S);
         S = ceil_div(h->n, rps);
         uint64_t *part = (uint64_t *)h->part.ensure((size_t)S * K * nq_pad * sizeof(uint64_t));
-        ScanArgs a{h->db.p, h->n, h->row_elems, h->dim, dq, nq, rps, S, part, nq_pad};
+        ScanArgs a{h->db.p, h->n, h->row_elems, h->dim, dq, nq, rps, S, part, nq_pad, qpl};
         {
             StageScope sc(st, STAGE_SCAN);
             if (h->metric == PN_METRIC_HAMMING)
@@ -211,7 +214,7 @@ extern "C" pn_status pn_exact_search_device(pn_exact *h, const void *queries, in
         if (nq == 0 || kk == 0) return;
         PN_REQUIRE(queries && out_idx && out_dist, PN_EINVAL, "null pointer");
         DeviceScope ds(h->device);
-        hipStream_t st = stream ? (hipStream_t)stream : lib_stream();
+        hipStream_t st = (hipStream_t)stream;
         const void *dq = queries;
         size_t in_b = row_bytes_in(h->metric, h->dim);
         size_t pad_b = (size_t)h->row_elems * 4;

@@ -20,12 +20,14 @@ struct ScanArgs {
     int64_t slices;
     uint64_t *part;
     int64_t nq_pad;
+    int qpl = 1;             // queries per lane (Hamming scan)
 };
 
 // Distance kinds for the select stage's output.
 enum OutKind { OUT_HAMMING_I32 = 0, OUT_L2_F32 = 1, OUT_F32 = 2 };
 
-// Hamming scan over rows of `words` 32-bit words; K in {1,2,4,8,16,32}.
+// Hamming scan over rows of `words` 32-bit words; K in {1,2,4,8,16,32}; a.qpl in {1,2}
+// (nq_pad must be a multiple of 256 * qpl).
 void hamming_scan(const ScanArgs &a, int words, int K, hipStream_t st);
 // Padded row width (words) served by the Hamming scan for `nbytes`-byte rows; 0 when too wide.
 int hamming_padded_words(int64_t nbytes);

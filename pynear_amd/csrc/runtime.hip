@@ -1,5 +1,7 @@
 // Library-wide runtime pieces of the C-ABI: the thread-local error string, the library stream,
 // version and device queries.
+#include <cstdlib>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -21,6 +23,40 @@ hipStream_t lib_stream() {
     std::lock_guard<std::mutex> lk(mu);
     if (!streams[dev]) PN_HIP(hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking));
     return streams[dev];
+}
+
+// Tuning table. Defaults are the measured best on MI355X for the workloads in bench.py.
+static std::mutex g_tune_mu;
+static std::map<std::string, int64_t> &tune_table() {
+    static std::map<std::string, int64_t> t;
+    static bool init = false;
+    if (!init) {
+        init = true;
+        t["hamming.wg_per_cu"] = 12;  // scan workgroups per CU (4 waves each)
+        t["hamming.qpl"] = 2;         // queries per lane in the Hamming scan (1 or 2)
+        t["float.wg_per_cu"] = 16;
+        t["merge.wave_per_query"] = 1; // 1: one wave merges one query's slice lists
+        if (const char *env = std::getenv("PN_TUNE")) {
+            std::string e(env);
+            size_t pos = 0;
+            while (pos < e.size()) {
+                size_t comma = e.find(',', pos);
+                std::string item = e.substr(pos, comma == std::string::npos ? std::string::npos : comma - pos);
+                size_t eq = item.find('=');
+                if (eq != std::string::npos) t[item.substr(0, eq)] = std::strtoll(item.c_str() + eq + 1, nullptr, 10);
+                if (comma == std::string::npos) break;
+                pos = comma + 1;
+            }
+        }
+    }
+    return t;
+}
+
+int64_t tune_get(const char *name) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto &t = tune_table();
+    auto it = t.find(name);
+    return it == t.end() ? 0 : it->second;
 }
 
 // Stage timer: begin/end events per launch, resolved by pn_stage_times().
@@ -50,6 +86,26 @@ void stage_mark(hipStream_t st, int stage, bool begin) {
 }
 
 } // namespace pn
+
+extern "C" pn_status pn_tune_set(const char *name, int64_t value) {
+    return pn::guard([&] {
+        PN_REQUIRE(name, PN_EINVAL, "name is null");
+        std::lock_guard<std::mutex> lk(pn::g_tune_mu);
+        auto &t = pn::tune_table();
+        PN_REQUIRE(t.count(name), PN_EINVAL, std::string("unknown tuning parameter: ") + name);
+        t[name] = value;
+    });
+}
+
+extern "C" pn_status pn_tune_get(const char *name, int64_t *value) {
+    return pn::guard([&] {
+        PN_REQUIRE(name && value, PN_EINVAL, "null argument");
+        std::lock_guard<std::mutex> lk(pn::g_tune_mu);
+        auto &t = pn::tune_table();
+        PN_REQUIRE(t.count(name), PN_EINVAL, std::string("unknown tuning parameter: ") + name);
+        *value = t[name];
+    });
+}
 
 extern "C" pn_status pn_stage_timing(int32_t enable) {
     return pn::guard([&] {

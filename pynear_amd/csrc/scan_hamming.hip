@@ -19,76 +19,110 @@
 
 namespace pn {
 
-template <int W, int K, int RU>
+template <int W, int K, int QPL, int RU>
 __global__ __launch_bounds__(256) void hamming_scan_kernel(const uint32_t *__restrict__ db, int64_t n,
                                                            int64_t rows_per_slice, const uint32_t *__restrict__ qs,
                                                            int64_t nq, uint64_t *__restrict__ part, int64_t nq_pad) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // lane t of the workgroup serves queries base + t + 256 p, p < QPL (each wave stays dense)
+    const int64_t qbase = (int64_t)blockIdx.x * (256 * QPL) + threadIdx.x;
     const int64_t s = blockIdx.y;
     const int64_t r0 = s * rows_per_slice;
     const int64_t r1 = min(n, r0 + rows_per_slice);
 
-    uint32_t qv[W];
-    if (q < nq) {
+    uint32_t qv[QPL][W];
 #pragma unroll
-        for (int w = 0; w < W; ++w) qv[w] = qs[q * W + w];
-    } else {
+    for (int p = 0; p < QPL; ++p) {
+        const int64_t q = qbase + 256 * p;
 #pragma unroll
-        for (int w = 0; w < W; ++w) qv[w] = 0u;
+        for (int w = 0; w < W; ++w) qv[p][w] = q < nq ? qs[q * W + w] : 0u;
     }
-    uint32_t L[K];
+    uint32_t L[QPL][K], thr[QPL];
 #pragma unroll
-    for (int j = 0; j < K; ++j) L[j] = kKey32None;
-    uint32_t thr = kKey32None >> kHamIdxBits;
+    for (int p = 0; p < QPL; ++p) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) L[p][j] = kKey32None;
+        thr[p] = kKey32None >> kHamIdxBits;
+    }
 
-    int64_t r = r0;
-    for (; r + RU <= r1; r += RU) {
-        uint32_t d[RU];
+    // Rows are wave-uniform: the loads below become s_load_dwordx16 into SGPRs (scalar loads
+    // complete out of order, so each group of RU rows is waited for as a whole; QPL queries per
+    // lane give every loaded row QPL x 2W VALU operations). Rows past the slice end are clamped to
+    // its last row (valid memory) and never inserted.
+    // 32-bit row arithmetic inside the slice keeps the clamp on the scalar unit (s_min_i32).
+    const uint32_t *sdb = db + r0 * W;
+    const int nrows = (int)(r1 - r0);
+    auto load_rows = [&](uint32_t (&R)[RU][W], int lr) {
 #pragma unroll
         for (int u = 0; u < RU; ++u) {
-            const uint32_t *row = db + (r + u) * W;
-            uint32_t acc = 0;
+            const int rr = min(lr + u, nrows - 1);
 #pragma unroll
-            for (int w = 0; w < W; ++w) acc = __popc(qv[w] ^ row[w]) + acc;
-            d[u] = acc;
+            for (int w = 0; w < W; ++w) R[u][w] = sdb[rr * W + w];
         }
+    };
+    auto compute = [&](const uint32_t (&R)[RU][W], int lr) {
+        const int cnt = min(RU, nrows - lr);
+        uint32_t d[QPL][RU];
 #pragma unroll
-        for (int u = 0; u < RU; ++u) {
-            if (d[u] < thr) {
-                list_insert_u32<K>(L, (d[u] << kHamIdxBits) | (uint32_t)(r + u - r0));
-                thr = L[K - 1] >> kHamIdxBits;
+        for (int u = 0; u < RU; ++u)
+#pragma unroll
+            for (int p = 0; p < QPL; ++p) {
+                uint32_t acc = 0;
+#pragma unroll
+                for (int w = 0; w < W; ++w) acc = bcnt_acc(qv[p][w] ^ R[u][w], acc);
+                d[p][u] = acc;
+            }
+#pragma unroll
+        for (int p = 0; p < QPL; ++p) {
+            uint32_t mn = d[p][0];
+#pragma unroll
+            for (int u = 1; u < RU; ++u) mn = min(mn, d[p][u]);
+            if (mn < thr[p]) {
+#pragma unroll
+                for (int u = 0; u < RU; ++u) {
+                    if (u < cnt && d[p][u] < thr[p]) {
+                        list_insert_u32<K>(L[p], (d[p][u] << kHamIdxBits) | (uint32_t)(lr + u));
+                        thr[p] = L[p][K - 1] >> kHamIdxBits;
+                    }
+                }
             }
         }
-    }
-    for (; r < r1; ++r) {
-        const uint32_t *row = db + r * W;
-        uint32_t acc = 0;
-#pragma unroll
-        for (int w = 0; w < W; ++w) acc = __popc(qv[w] ^ row[w]) + acc;
-        if (acc < thr) {
-            list_insert_u32<K>(L, (acc << kHamIdxBits) | (uint32_t)(r - r0));
-            thr = L[K - 1] >> kHamIdxBits;
-        }
+    };
+
+    uint32_t R[RU][W];
+    for (int lr = 0; lr < nrows; lr += RU) {
+        load_rows(R, lr);
+        __builtin_amdgcn_sched_barrier(0); // all RU row loads in flight before the first use
+        compute(R, lr);
     }
 
-    if (q < nq) {
-        uint64_t *out = part + (s * K) * nq_pad + q;
 #pragma unroll
-        for (int j = 0; j < K; ++j) {
-            uint32_t key = L[j];
-            uint64_t g = kKey64None;
-            if (key != kKey32None)
-                g = ((uint64_t)(key >> kHamIdxBits) << 32) | (uint64_t)(r0 + (key & kHamIdxMask));
-            out[(int64_t)j * nq_pad] = g;
+    for (int p = 0; p < QPL; ++p) {
+        const int64_t q = qbase + 256 * p;
+        if (q < nq) {
+            uint64_t *out = part + (s * K) * nq_pad + q;
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                uint32_t key = L[p][j];
+                uint64_t g = kKey64None;
+                if (key != kKey32None)
+                    g = ((uint64_t)(key >> kHamIdxBits) << 32) | (uint64_t)(r0 + (key & kHamIdxMask));
+                out[(int64_t)j * nq_pad] = g;
+            }
         }
     }
 }
 
 template <int W, int K>
 static void launch_w_k(const ScanArgs &a, hipStream_t st) {
-    dim3 grid((unsigned)ceil_div(a.nq, 256), (unsigned)a.slices);
-    hipLaunchKernelGGL((hamming_scan_kernel<W, K, 4>), grid, dim3(256), 0, st, (const uint32_t *)a.db, a.n,
-                       a.rows_per_slice, (const uint32_t *)a.queries, a.nq, a.part, a.nq_pad);
+    dim3 grid((unsigned)ceil_div(a.nq, 256 * a.qpl), (unsigned)a.slices);
+    // SGPR budget: RU x W row words must fit next to the addressing registers.
+    constexpr int RU = W >= 32 ? 2 : (W >= 16 ? 4 : 8);
+    if (a.qpl == 2)
+        hipLaunchKernelGGL((hamming_scan_kernel<W, K, 2, RU>), grid, dim3(256), 0, st, (const uint32_t *)a.db, a.n,
+                           a.rows_per_slice, (const uint32_t *)a.queries, a.nq, a.part, a.nq_pad);
+    else
+        hipLaunchKernelGGL((hamming_scan_kernel<W, K, 1, RU>), grid, dim3(256), 0, st, (const uint32_t *)a.db, a.n,
+                           a.rows_per_slice, (const uint32_t *)a.queries, a.nq, a.part, a.nq_pad);
 }
 
 template <int W> static void launch_w(const ScanArgs &a, int K, hipStream_t st) {

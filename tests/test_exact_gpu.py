@@ -129,19 +129,63 @@ def test_empty_and_mismatch_errors():
         h.search(np.zeros((1, 5), np.float32), 1)
 
 
-def test_device_entry_point_matches_host():
-    torch = pytest.importorskip("torch")
+@pytest.mark.parametrize("use_null_stream", [False, True])
+def test_device_entry_point_matches_host(use_null_stream):
+    from pynear_amd import hipdev
     X, Q = _hamming_case(50_000, 64, 600, seed=3)
     h = N.ExactHandle(N.METRIC_HAMMING)
     h.set(X)
     hi, hd = h.search(Q, 10)
-    dq = torch.from_numpy(Q).cuda()
-    oi = torch.empty((600, 10), dtype=torch.int32, device="cuda")
-    od = torch.empty((600, 10), dtype=torch.int32, device="cuda")
-    h.search_device(dq.data_ptr(), 600, 10, oi.data_ptr(), od.data_ptr(), torch.cuda.current_stream().cuda_stream)
-    torch.cuda.synchronize()
-    np.testing.assert_array_equal(oi.cpu().numpy(), hi)
-    np.testing.assert_array_equal(od.cpu().numpy(), hd)
+    dq = hipdev.DeviceArray.from_numpy(Q)
+    oi = hipdev.DeviceArray((600, 10), np.int32)
+    od = hipdev.DeviceArray((600, 10), np.int32)
+    st = None if use_null_stream else hipdev.Stream()
+    h.search_device(dq.ptr, 600, 10, oi.ptr, od.ptr, st.handle if st else None)
+    if st:
+        st.synchronize()
+    hipdev.synchronize()
+    np.testing.assert_array_equal(oi.numpy(), hi)
+    np.testing.assert_array_equal(od.numpy(), hd)
+
+
+@pytest.mark.parametrize("metric", [N.METRIC_L2, N.METRIC_L1])
+def test_device_entry_point_pads_queries(metric):
+    """Queries with a row length that is not a padded width (d=13) go through the device API."""
+    from pynear_amd import hipdev
+    rng = np.random.default_rng(21)
+    X = rng.random((9000, 13)).astype(np.float32)
+    Q = rng.random((77, 13)).astype(np.float32)
+    h = N.ExactHandle(metric)
+    h.set(X)
+    dq = hipdev.DeviceArray.from_numpy(Q)
+    oi = hipdev.DeviceArray((77, 5), np.int32)
+    od = hipdev.DeviceArray((77, 5), np.float32)
+    st = hipdev.Stream()
+    h.search_device(dq.ptr, 77, 5, oi.ptr, od.ptr, st.handle)
+    st.synchronize()
+    ri, rd = oracle.knn_float(metric, X, Q, 5)
+    np.testing.assert_array_equal(oi.numpy(), ri)
+    np.testing.assert_array_equal(od.numpy(), rd)
+
+
+@pytest.mark.parametrize("wave", [0, 1])
+@pytest.mark.parametrize("wpc", [1, 8])
+def test_merge_and_slicing_variants(wave, wpc):
+    """Both merge kernels and different slice counts give identical results."""
+    X, Q = _hamming_case(40_000, 32, 300, seed=wpc * 10 + wave)
+    h = N.ExactHandle(N.METRIC_HAMMING)
+    h.set(X)
+    old = (N.tune_get("merge.wave_per_query"), N.tune_get("hamming.wg_per_cu"))
+    try:
+        N.tune_set("merge.wave_per_query", wave)
+        N.tune_set("hamming.wg_per_cu", wpc)
+        gi, gd = h.search(Q, 16)
+    finally:
+        N.tune_set("merge.wave_per_query", old[0])
+        N.tune_set("hamming.wg_per_cu", old[1])
+    oi, od = oracle.knn_hamming(X, Q, 16)
+    np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(gi, oi)
 
 
 def test_pairwise_distances_bit_exact():
