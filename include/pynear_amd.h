@@ -105,6 +105,40 @@ pn_status pn_exact_search(pn_exact *h, const void *queries, int64_t nq, int64_t 
 pn_status pn_exact_search_device(pn_exact *h, const void *queries, int64_t nq, int32_t k, int32_t *out_idx,
                                  void *out_dist, void *stream);
 
+/* ---------------------------------------------------------------- IVF binary index
+ * IVFFlatBinaryIndex (BinaryIVF.hpp:34-227) behind the pybind11 class of the same name
+ * (PythonBindings.cpp:231-251, :454-466). The build reproduces the reference's k-means++ and
+ * Lloyd iterations with the same random draws, so centroids and inverted lists are identical. */
+typedef struct pn_ivfbin pn_ivfbin;
+
+/* IVFFlatBinaryIndex(nlist, nprobe, max_iter, seed) (BinaryIVF.hpp:40-46). */
+pn_status pn_ivfbin_create(int32_t nlist, int32_t nprobe, int32_t max_iter, uint32_t seed, pn_ivfbin **out);
+void pn_ivfbin_destroy(pn_ivfbin *h);
+
+/* IVFFlatBinaryIndex::set (BinaryIVF.hpp:49-60): host rows n x nbytes; n == 0 clears. */
+pn_status pn_ivfbin_set(pn_ivfbin *h, const uint8_t *rows, int64_t n, int64_t nbytes);
+
+/* Rows, row width and number of clusters built (min(nlist, n)). */
+pn_status pn_ivfbin_info(const pn_ivfbin *h, int64_t *n, int64_t *nbytes, int32_t *n_clusters);
+
+/* nlist() / nprobe() / set_nprobe() (BinaryIVF.hpp:113-115). */
+pn_status pn_ivfbin_get_params(const pn_ivfbin *h, int32_t *nlist, int32_t *nprobe);
+pn_status pn_ivfbin_set_nprobe(pn_ivfbin *h, int32_t nprobe);
+
+/* Copies the built centroids (n_clusters x nbytes) and/or the final labels (n), either may be
+ * NULL. No counterpart in the reference (its build state is private); used to check the build. */
+pn_status pn_ivfbin_export(const pn_ivfbin *h, uint8_t *centroids, int32_t *labels);
+
+/* IVFFlatBinaryIndex::searchKNN (BinaryIVF.hpp:65-111): host queries nq x nbytes; per query
+ * out_count[q] <= k results, ascending (dist, id), in out_idx/out_dist [nq, k] (rest -1).
+ * An empty index yields empty results, as in the reference. k <= 2048. */
+pn_status pn_ivfbin_search(pn_ivfbin *h, const uint8_t *queries, int64_t nq, int64_t nbytes, int32_t k,
+                           int64_t *out_idx, int64_t *out_dist, int32_t *out_count);
+
+/* Device-resident form: device queries (nq x nbytes), int32 outputs [nq, k] and counts [nq]. */
+pn_status pn_ivfbin_search_device(pn_ivfbin *h, const void *queries, int64_t nq, int32_t k, int32_t *out_idx,
+                                  int32_t *out_dist, int32_t *out_count, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

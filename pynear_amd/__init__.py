@@ -12,6 +12,7 @@ from typing import List, Tuple
 import numpy as np
 
 from ._native import NativeLibraryError  # noqa: F401
+from ._pynear import IVFFlatBinaryIndex
 from ._pynear import VPTreeBinaryIndex as VPTreeBinaryIndexN
 from ._pynear import VPTreeBinaryIndex64, VPTreeBinaryIndex128, VPTreeBinaryIndex256, VPTreeBinaryIndex512
 from ._pynear import VPTreeChebyshevIndex, VPTreeL1Index, VPTreeL2Index
@@ -77,7 +78,7 @@ class VPTreeBinaryIndex:
 
 
 __all__ = [
-    "VPTreeL2Index", "VPTreeL1Index", "VPTreeChebyshevIndex", "VPTreeBinaryIndex", "VPTreeBinaryIndexN",
+    "IVFFlatBinaryIndex", "VPTreeL2Index", "VPTreeL1Index", "VPTreeChebyshevIndex", "VPTreeBinaryIndex", "VPTreeBinaryIndexN",
     "VPTreeBinaryIndex64", "VPTreeBinaryIndex128", "VPTreeBinaryIndex256", "VPTreeBinaryIndex512",
     "dist_l2", "dist_l1", "dist_chebyshev", "dist_hamming", "dist_hamming_64", "dist_hamming_128",
     "dist_hamming_256", "dist_hamming_512", "NativeLibraryError",

@@ -40,6 +40,15 @@ SIGNATURES = {
     "pn_exact_info": (_i32, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "pn_exact_search": (_i32, [_vp, _vp, _i64, _i64, _i32, _vp, _vp]),
     "pn_exact_search_device": (_i32, [_vp, _vp, _i64, _i32, _vp, _vp, _vp]),
+    "pn_ivfbin_create": (_i32, [_i32, _i32, _i32, _u32, ctypes.POINTER(_vp)]),
+    "pn_ivfbin_destroy": (None, [_vp]),
+    "pn_ivfbin_set": (_i32, [_vp, _vp, _i64, _i64]),
+    "pn_ivfbin_info": (_i32, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i32)]),
+    "pn_ivfbin_get_params": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "pn_ivfbin_set_nprobe": (_i32, [_vp, _i32]),
+    "pn_ivfbin_export": (_i32, [_vp, _vp, _vp]),
+    "pn_ivfbin_search": (_i32, [_vp, _vp, _i64, _i64, _i32, _vp, _vp, _vp]),
+    "pn_ivfbin_search_device": (_i32, [_vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp]),
 }
 
 _LIB = None
@@ -141,6 +150,58 @@ def tune_get(name):
     v = _i64(0)
     check(lib().pn_tune_get(name.encode(), ctypes.byref(v)))
     return v.value
+
+
+class IvfBinHandle:
+    """Owner of a ``pn_ivfbin`` handle (IVFFlatBinaryIndex on the current device)."""
+
+    def __init__(self, nlist, nprobe, max_iter, seed):
+        h = _vp()
+        check(lib().pn_ivfbin_create(int(nlist), int(nprobe), int(max_iter), int(seed), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _LIB is not None:
+            _LIB.pn_ivfbin_destroy(h)
+            self._h = None
+
+    def set(self, rows):
+        n, nb = rows.shape
+        check(lib().pn_ivfbin_set(self._h, ptr(rows), n, nb))
+
+    def info(self):
+        n, nb, nc = _i64(0), _i64(0), _i32(0)
+        check(lib().pn_ivfbin_info(self._h, ctypes.byref(n), ctypes.byref(nb), ctypes.byref(nc)))
+        return n.value, nb.value, nc.value
+
+    def params(self):
+        a, b = _i32(0), _i32(0)
+        check(lib().pn_ivfbin_get_params(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def set_nprobe(self, nprobe):
+        check(lib().pn_ivfbin_set_nprobe(self._h, int(nprobe)))
+
+    def export(self):
+        n, nb, nc = self.info()
+        cent = np.zeros((nc, nb), np.uint8)
+        labels = np.zeros(n, np.int32)
+        check(lib().pn_ivfbin_export(self._h, ptr(cent), ptr(labels)))
+        return cent, labels
+
+    def search(self, queries, k):
+        nq, nb = queries.shape
+        k = int(k)
+        idx = np.full((nq, max(k, 0)), -1, np.int64)
+        dist = np.full((nq, max(k, 0)), -1, np.int64)
+        cnt = np.zeros(nq, np.int32)
+        check(lib().pn_ivfbin_search(self._h, ptr(queries), nq, nb, k, ptr(idx), ptr(dist), ptr(cnt)))
+        return idx, dist, cnt
+
+    def search_device(self, q_ptr, nq, k, out_idx_ptr, out_dist_ptr, out_cnt_ptr, stream_ptr=None):
+        check(lib().pn_ivfbin_search_device(self._h, _vp(q_ptr), nq, int(k), _vp(out_idx_ptr), _vp(out_dist_ptr),
+                                            _vp(out_cnt_ptr), _vp(stream_ptr) if stream_ptr else None))
 
 
 STAGES = ("scan", "select", "coarse", "rerank", "other")

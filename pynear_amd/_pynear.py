@@ -133,6 +133,52 @@ class VPTreeBinaryIndex(_BinaryIndex):
     _width = None
 
 
+class IVFFlatBinaryIndex:
+    """Inverted-file index for binary descriptors (approximate Hamming k-NN) on the GPU.
+
+    Mirrors ``_pynear.IVFFlatBinaryIndex`` (PythonBindings.cpp:454-466, BinaryIVF.hpp:34-227):
+    ``IVFFlatBinaryIndex(nlist=256, nprobe=8, max_iter=20, seed=42)``, ``set``, ``searchKNN`` ->
+    (list[list[int]], list[list[int]]) ascending by distance, ``nlist()``, ``nprobe()``,
+    ``set_nprobe()``. The build reproduces the reference's clustering draw for draw, and the
+    search returns what the reference's heap returns, ties included. Unlike the reference, the
+    index can be pickled (the rows and parameters are stored and the index is rebuilt on load).
+    """
+
+    def __init__(self, nlist: int = 256, nprobe: int = 8, max_iter: int = 20, seed: int = 42):
+        self._args = (int(nlist), int(nprobe), int(max_iter), int(seed))
+        self._h = N.IvfBinHandle(*self._args)
+        self._rows = None
+
+    def set(self, vectors):
+        rows = _as_2d_u8(vectors) if np.asarray(vectors).size else np.zeros((0, 0), np.uint8)
+        self._h.set(rows)
+        self._rows = rows if rows.shape[0] else None
+
+    def searchKNN(self, vectors, k):
+        """Batch find top-k vectors in index and return indices and distances."""
+        q = _as_2d_u8(vectors)
+        idx, dist, cnt = self._h.search(q, k)
+        return ([idx[i, :c].tolist() for i, c in enumerate(cnt)], [dist[i, :c].tolist() for i, c in enumerate(cnt)])
+
+    def nlist(self):
+        return self._h.params()[0]
+
+    def nprobe(self):
+        return self._h.params()[1]
+
+    def set_nprobe(self, nprobe):
+        self._h.set_nprobe(nprobe)
+
+    def __getstate__(self):
+        nlist, nprobe = self._h.params()
+        return {"args": (nlist, nprobe) + self._args[2:], "rows": self._rows}
+
+    def __setstate__(self, state):
+        self.__init__(*state["args"])
+        if state.get("rows") is not None:
+            self.set(state["rows"])
+
+
 # ---------------------------------------------------------------- free functions
 def _pair_float(metric, a, b):
     a = np.ascontiguousarray(np.asarray(a), dtype=np.float32).reshape(1, -1)

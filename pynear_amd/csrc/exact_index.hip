@@ -14,20 +14,11 @@
 #include "common.hpp"
 #include "device_util.hpp"
 #include "kernels.hpp"
-
-struct pn_exact {
-    int32_t metric = PN_METRIC_L2;
-    int device = 0;
-    int64_t n = 0, dim = 0;      // rows; row length in floats (float metrics) or bytes (Hamming)
-    int64_t row_elems = 0;       // padded row length on the device (floats or 32-bit words)
-    bool scan_ok = false;        // the sliced scan kernel serves this row width
-    pn::DevBuf db, qbuf, part, oidx, odist, ws;
-    int cus = 256;
-};
+#include "exact_index.hpp"
 
 namespace pn {
 
-static int64_t padded_row_elems(int32_t metric, int64_t dim, bool *scan_ok) {
+int64_t padded_row_elems(int32_t metric, int64_t dim, bool *scan_ok) {
     if (metric == PN_METRIC_HAMMING) {
         int w = hamming_padded_words(dim);
         *scan_ok = w > 0;
@@ -42,14 +33,11 @@ static int64_t padded_row_elems(int32_t metric, int64_t dim, bool *scan_ok) {
 
 static size_t elem_bytes(int32_t metric) { return 4; }
 
-static size_t row_bytes_in(int32_t metric, int64_t dim) {
+size_t row_bytes_in(int32_t metric, int64_t dim) {
     return metric == PN_METRIC_HAMMING ? (size_t)dim : (size_t)dim * sizeof(float);
 }
 
-// Copies n host rows of `dim` elements into a zero-padded device array of `row_elems` elements.
-static void upload_padded(const pn_exact *h, const void *rows, int64_t n, int64_t dim, void *dst, hipStream_t st) {
-    size_t in_b = row_bytes_in(h->metric, dim);
-    size_t out_b = (size_t)h->row_elems * elem_bytes(h->metric);
+void upload_rows_padded(const void *rows, int64_t n, size_t in_b, size_t out_b, void *dst, hipStream_t st) {
     if (in_b == out_b) {
         PN_HIP(hipMemcpyAsync(dst, rows, in_b * n, hipMemcpyHostToDevice, st));
         return;
@@ -60,6 +48,39 @@ static void upload_padded(const pn_exact *h, const void *rows, int64_t n, int64_
     PN_HIP(hipStreamSynchronize(st)); // tmp is released on return
 }
 
+// Copies n host rows of `dim` elements into a zero-padded device array of `row_elems` elements.
+static void upload_padded(const pn_exact *h, const void *rows, int64_t n, int64_t dim, void *dst, hipStream_t st) {
+    upload_rows_padded(rows, n, row_bytes_in(h->metric, dim), (size_t)h->row_elems * elem_bytes(h->metric), dst, st);
+}
+
+void pad_rows_device(const void *src, int64_t n, size_t in_b, size_t out_b, void *dst, hipStream_t st) {
+    if (in_b == out_b) {
+        PN_HIP(hipMemcpyAsync(dst, src, in_b * n, hipMemcpyDeviceToDevice, st));
+        return;
+    }
+    PN_HIP(hipMemsetAsync(dst, 0, out_b * n, st));
+    PN_HIP(hipMemcpy2DAsync(dst, out_b, src, in_b, in_b, n, hipMemcpyDeviceToDevice, st));
+}
+
+void exact_init(pn_exact *h, int32_t metric) {
+    h->metric = metric;
+    h->device = current_device();
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess && cus > 0)
+        h->cus = cus;
+}
+
+void exact_set_device(pn_exact *h, const void *rows_padded, int64_t n, int64_t dim, hipStream_t st) {
+    bool scan_ok = false;
+    int64_t re = padded_row_elems(h->metric, dim, &scan_ok);
+    h->dim = dim;
+    h->row_elems = re;
+    h->scan_ok = scan_ok;
+    void *dst = h->db.ensure((size_t)std::max<int64_t>(n, 1) * re * 4);
+    if (n) PN_HIP(hipMemcpyAsync(dst, rows_padded, (size_t)n * re * 4, hipMemcpyDeviceToDevice, st));
+    h->n = n;
+}
+
 static int list_size_for(int kk) {
     int K = 1;
     while (K < kk) K <<= 1;
@@ -67,7 +88,7 @@ static int list_size_for(int kk) {
 }
 
 // Enqueues the search of nq padded device queries; out_idx int32[nq, kk], out_dist int32/f32.
-static void exact_search_enqueue(pn_exact *h, const void *dq, int64_t nq, int kk, int32_t *oi, void *od,
+void exact_search_enqueue(pn_exact *h, const void *dq, int64_t nq, int kk, int32_t *oi, void *od,
                                  hipStream_t st) {
     int K = list_size_for(kk);
     int out_kind = h->metric == PN_METRIC_HAMMING ? OUT_HAMMING_I32
@@ -116,11 +137,7 @@ extern "C" pn_status pn_exact_create(int32_t metric, pn_exact **out) {
         PN_REQUIRE(out, PN_EINVAL, "out is null");
         PN_REQUIRE(metric >= PN_METRIC_L2 && metric <= PN_METRIC_HAMMING, PN_EINVAL, "unknown metric");
         auto *h = new pn_exact();
-        h->metric = metric;
-        h->device = current_device();
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess && cus > 0)
-            h->cus = cus;
+        exact_init(h, metric);
         *out = h;
     });
 }
@@ -220,8 +237,7 @@ extern "C" pn_status pn_exact_search_device(pn_exact *h, const void *queries, in
         size_t pad_b = (size_t)h->row_elems * 4;
         if (in_b != pad_b) { // pad into the workspace (zero tail per row)
             void *tmp = h->qbuf.ensure((size_t)nq * pad_b);
-            PN_HIP(hipMemsetAsync(tmp, 0, (size_t)nq * pad_b, st));
-            PN_HIP(hipMemcpy2DAsync(tmp, pad_b, queries, in_b, in_b, nq, hipMemcpyDeviceToDevice, st));
+            pad_rows_device(queries, nq, in_b, pad_b, tmp, st);
             dq = tmp;
         }
         exact_search_enqueue(h, dq, nq, kk, out_idx, out_dist, st);

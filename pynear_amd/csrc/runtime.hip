@@ -32,8 +32,8 @@ static std::map<std::string, int64_t> &tune_table() {
     static bool init = false;
     if (!init) {
         init = true;
-        t["hamming.wg_per_cu"] = 12;  // scan workgroups per CU (4 waves each)
-        t["hamming.qpl"] = 2;         // queries per lane in the Hamming scan (1 or 2)
+        t["hamming.wg_per_cu"] = 8;   // scan workgroups per CU (4 waves each)
+        t["hamming.qpl"] = 1;         // queries per lane in the Hamming scan (1 or 2)
         t["float.wg_per_cu"] = 16;
         t["merge.wave_per_query"] = 1; // 1: one wave merges one query's slice lists
         if (const char *env = std::getenv("PN_TUNE")) {

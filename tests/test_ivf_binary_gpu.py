@@ -1,0 +1,135 @@
+"""GPU parity of IVFFlatBinaryIndex against the oracle's literal restatement of the reference.
+
+The build (k-means++ draws, Lloyd assignment, majority vote, empty-cluster reseeding, inverted
+lists) must give identical centroids and labels; the search must give identical result lists,
+which includes the reference heap's handling of distance ties.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import pynear_amd
+from pynear_amd import _native as N
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _case(n, nbytes, seed, distinct=None):
+    rng = np.random.default_rng(seed)
+    if distinct:
+        X = (rng.integers(0, distinct, (n, nbytes)) * 37 % 256).astype(np.uint8)
+    else:
+        X = rng.integers(0, 256, (n, nbytes), dtype=np.uint8)
+    Q = X[rng.integers(0, n, 25)].copy()
+    Q[::2] ^= rng.integers(0, 256, Q[::2].shape, dtype=np.uint8) & np.uint8(0x11)
+    Q = np.concatenate([Q, rng.integers(0, 256, (8, nbytes), dtype=np.uint8)])
+    return X, Q
+
+
+@pytest.mark.parametrize("n,nbytes,nlist,max_iter,seed", [
+    (500, 64, 16, 20, 42), (1500, 32, 40, 20, 7), (900, 8, 10, 5, 1), (700, 16, 64, 20, 3),
+    (300, 12, 9, 20, 42), (64, 64, 64, 20, 42), (20, 8, 50, 20, 42), (2000, 20, 33, 3, 11),
+    (1200, 4, 32, 20, 5),
+])
+def test_build_identical_to_reference_restatement(n, nbytes, nlist, max_iter, seed):
+    X, _ = _case(n, nbytes, seed)
+    h = N.IvfBinHandle(nlist, 8, max_iter, seed)
+    h.set(X)
+    cent, labels = h.export()
+    oc, ol = oracle.ivfbin_build(X, nlist, max_iter, seed)
+    np.testing.assert_array_equal(labels, ol)
+    np.testing.assert_array_equal(cent, oc)
+
+
+@pytest.mark.parametrize("nprobe", [0, 1, 3, 8, 16, 40])
+@pytest.mark.parametrize("k", [1, 5, 10, 64])
+@pytest.mark.parametrize("distinct", [None, 3])
+def test_search_identical_to_reference_restatement(nprobe, k, distinct):
+    X, Q = _case(1500, 32, 100 + k + nprobe, distinct=distinct)
+    idx = pynear_amd.IVFFlatBinaryIndex(nlist=16, nprobe=nprobe, max_iter=20, seed=42)
+    idx.set(X)
+    gi, gd = idx.searchKNN(Q, k)
+    cent, labels = oracle.ivfbin_build(X, 16, 20, 42)
+    off, ids = oracle.lists_from_labels(labels, len(cent))
+    oi, od = oracle.ivfbin_search(cent, off, ids, X, Q, nprobe, k)
+    assert gd == od
+    assert gi == oi
+
+
+@pytest.mark.parametrize("nprobe", [1, 4, 32])
+def test_search_matches_recorded_reference_outputs(nprobe):
+    g = np.load(os.path.join(GOLD, "ref_vectors.npz"))
+    idx = pynear_amd.IVFFlatBinaryIndex(nlist=32, nprobe=nprobe, max_iter=20, seed=42)
+    idx.set(g["ivfbin_X"])
+    gi, gd = idx.searchKNN(g["ivfbin_Q"], 10)
+    assert gi == g[f"ivfbin_idx_p{nprobe}"].tolist()
+    assert gd == g[f"ivfbin_dist_p{nprobe}"].tolist()
+
+
+def test_reference_api_behaviour():
+    """The behaviours pynear/tests/test_binary_approx.py:37-117 pins, through the same calls."""
+    rng = np.random.default_rng(0)
+    db = rng.integers(0, 256, size=(1000, 64), dtype=np.uint8)
+    true = list(range(0, 100, 10))
+    q = db[true].copy()
+    q[:, 3] ^= 0x0F  # 4 flipped bits
+    idx = pynear_amd.IVFFlatBinaryIndex(nlist=32, nprobe=32)
+    idx.set(db)
+    ri, rd = idx.searchKNN(q, k=10)
+    assert all(t in r for t, r in zip(true, ri))
+    assert all(d == sorted(d) for d in rd)
+    idx.set_nprobe(4)
+    assert idx.nprobe() == 4 and idx.nlist() == 32
+    small = pynear_amd.IVFFlatBinaryIndex(nlist=10, nprobe=1)
+    small.set(db[:50])
+    ri, rd = small.searchKNN(db[:3], k=100)
+    assert len(ri) == 3 and all(len(r) <= 100 for r in ri)
+    assert all(r[0] == i for i, r in enumerate(ri))  # a row finds itself at distance 0
+    empty = pynear_amd.IVFFlatBinaryIndex()
+    assert empty.searchKNN(db[:2], 5) == ([[], []], [[], []])
+
+
+def test_pickle_roundtrip():
+    import pickle
+    X, Q = _case(800, 32, 9)
+    idx = pynear_amd.IVFFlatBinaryIndex(nlist=12, nprobe=3)
+    idx.set(X)
+    a = idx.searchKNN(Q, 7)
+    idx2 = pickle.loads(pickle.dumps(idx))
+    assert idx2.nprobe() == 3 and idx2.nlist() == 12
+    assert idx2.searchKNN(Q, 7) == a
+
+
+def test_device_entry_point_matches_host():
+    from pynear_amd import hipdev
+    X, Q = _case(3000, 64, 13)
+    h = N.IvfBinHandle(24, 5, 20, 42)
+    h.set(X)
+    hi, hd, hc = h.search(Q, 10)
+    dq = hipdev.DeviceArray.from_numpy(Q)
+    oi = hipdev.DeviceArray((len(Q), 10), np.int32)
+    od = hipdev.DeviceArray((len(Q), 10), np.int32)
+    oc = hipdev.DeviceArray((len(Q),), np.int32)
+    st = hipdev.Stream()
+    h.search_device(dq.ptr, len(Q), 10, oi.ptr, od.ptr, oc.ptr, st.handle)
+    st.synchronize()
+    np.testing.assert_array_equal(oc.numpy(), hc)
+    np.testing.assert_array_equal(oi.numpy(), hi)
+    np.testing.assert_array_equal(od.numpy(), hd)
+
+
+def test_larger_index_sample():
+    """30k x 512-bit, nlist 128: identical build, identical search on every query."""
+    import bench
+    X, Q = bench.hamming_workload(30_000, 200, seed=4)
+    idx = pynear_amd.IVFFlatBinaryIndex(nlist=128, nprobe=16)
+    idx.set(X)
+    gi, gd = idx.searchKNN(Q, 10)
+    cent, labels = oracle.ivfbin_build(X, 128, 20, 42)
+    off, ids = oracle.lists_from_labels(labels, len(cent))
+    oi, od = oracle.ivfbin_search(cent, off, ids, X, Q, 16, 10)
+    assert gd == od and gi == oi
