@@ -78,6 +78,7 @@ void exact_set_device(pn_exact *h, const void *rows_padded, int64_t n, int64_t d
     h->scan_ok = scan_ok;
     void *dst = h->db.ensure((size_t)std::max<int64_t>(n, 1) * re * 4);
     if (n) PN_HIP(hipMemcpyAsync(dst, rows_padded, (size_t)n * re * 4, hipMemcpyDeviceToDevice, st));
+    if (h->metric == PN_METRIC_L2) l2_prepare(h->l2, (const float *)dst, n, dim, re, st);
     h->n = n;
 }
 
@@ -91,6 +92,11 @@ static int list_size_for(int kk) {
 void exact_search_enqueue(pn_exact *h, const void *dq, int64_t nq, int kk, int32_t *oi, void *od,
                                  hipStream_t st) {
     int K = list_size_for(kk);
+    if (h->metric == PN_METRIC_L2 && kk <= 32 && tune_get("l2.mfma") != 0) {
+        l2_mfma_search(h->l2, (const float *)h->db.p, h->row_elems, (const float *)dq, h->row_elems, nq, kk, oi,
+                       (float *)od, h->l2w, h->cus, st);
+        return;
+    }
     int out_kind = h->metric == PN_METRIC_HAMMING ? OUT_HAMMING_I32
                                                   : (h->metric == PN_METRIC_L2 ? OUT_L2_F32 : OUT_F32);
     if (h->scan_ok && K <= 32) {
@@ -174,6 +180,7 @@ extern "C" pn_status pn_exact_set(pn_exact *h, const void *rows, int64_t n, int6
         hipStream_t st = lib_stream();
         void *dst = h->db.ensure((size_t)n * re * elem_bytes(h->metric));
         upload_padded(h, rows, n, dim, dst, st);
+        if (h->metric == PN_METRIC_L2) l2_prepare(h->l2, (const float *)dst, n, dim, re, st);
         PN_HIP(hipStreamSynchronize(st));
         h->n = n;
     });

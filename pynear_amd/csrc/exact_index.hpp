@@ -2,6 +2,7 @@
 // k-means assignment are exact nearest-centroid searches).
 #pragma once
 #include "common.hpp"
+#include "l2_mfma.hpp"
 
 struct pn_exact {
     int32_t metric = PN_METRIC_L2;
@@ -11,6 +12,8 @@ struct pn_exact {
     bool scan_ok = false;        // the sliced scan kernel serves this row width
     pn::DevBuf db, qbuf, part, oidx, odist, ws;
     int cus = 256;
+    pn::L2Set l2;                // bf16 split of the rows (L2 metric)
+    pn::L2Work l2w;
 };
 
 

@@ -1,0 +1,396 @@
+// Exact L2 k-NN through a bf16 MFMA candidate scan and an exact rerank.
+//
+// The reference computes every distance as sqrt(sum (q-x)^2) (DistanceFunctions.hpp:304-345). A
+// full scan in that form costs two VALU operations per element; here the scan is the GEMM form
+// |q|^2 + |x|^2 - 2 q.x on the matrix cores, and only a few candidates per query are recomputed
+// in the reference's arithmetic:
+//   1. split: x = hi + lo with hi = bf16(x), lo = bf16(x - hi); |x|^2 accumulated in fp64.
+//   2. scan: v_mfma_f32_32x32x16_bf16 over (32 rows x 32 queries) tiles, three products per
+//      16-wide K step (hi.hi + hi.lo + lo.hi; relative error of the dot product ~2^-16), fp32
+//      accumulate. Each lane owns one query and a sorted list of its KL best (approx, row)
+//      candidates; lanes of every (wave, half-wave) form one partial list.
+//   3. merge: the partial lists give each query its K2 best candidates by the approximation.
+//   4. rerank: the K2 candidates are recomputed exactly (reference order) and ordered by
+//      (distance, index). A candidate set is proven complete when the smallest approximation any
+//      list or the merge could have dropped, minus a rigorous error bound, exceeds the k-th exact
+//      squared distance; otherwise the query is flagged.
+//   5. fallback: flagged queries (rare) are searched by an exact full scan on the device.
+//
+// MFMA layout (checked on the device): A 32x16 -- lane l holds row l%32, k = 8(l/32)..+7;
+// B 16x32 -- lane l holds column l%32, k = 8(l/32)..+7; D 32x32 -- lane l, register r holds
+// row 8(r/4) + 4(l/32) + r%4, column l%32.
+#include "common.hpp"
+#include "device_util.hpp"
+#include "kernels.hpp"
+#include "l2_mfma.hpp"
+#include "ref_arith.hpp"
+
+namespace pn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(256) void l2_split_kernel(const float *__restrict__ X, int64_t n, int64_t d,
+                                                       int64_t stride, int dp, __bf16 *__restrict__ H,
+                                                       __bf16 *__restrict__ Lo, float *__restrict__ nrm) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double acc = 0.0;
+    for (int e = lane; e < dp; e += 64) {
+        const float x = e < d ? X[row * stride + e] : 0.f;
+        const __bf16 h = (__bf16)x;
+        const __bf16 l = (__bf16)(x - (float)h);
+        H[row * dp + e] = h;
+        Lo[row * dp + e] = l;
+        acc += (double)x * (double)x;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) nrm[row] = (float)acc;
+}
+
+__global__ void max_f32_kernel(const float *__restrict__ v, int64_t n, unsigned int *__restrict__ out) {
+    __shared__ float red[256];
+    float m = 0.f;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) m = fmaxf(m, v[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, __float_as_uint(red[0])); // non-negative floats order as uints
+}
+
+template <int KL, int PREC>
+__global__ __launch_bounds__(256) void l2_mfma_scan_kernel(const __bf16 *__restrict__ Xh, const __bf16 *__restrict__ Xl,
+                                                           const float *__restrict__ xn, int64_t n, int dp,
+                                                           const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
+                                                           const float *__restrict__ qn, int64_t rows_per_slice,
+                                                           uint64_t *__restrict__ part, int64_t nq_pad) {
+    // A workgroup owns 128 queries (wave w: queries 32w..32w+31 of the block) and one slice of
+    // rows; the four waves walk the same 32-row tiles, so each tile is fetched from L2 once per
+    // workgroup and served to the other waves from L1. Row arrays are padded with zero rows to a
+    // multiple of 32 and slices start on 32-row boundaries: tile loads need no guard, rows >= r1
+    // are masked.
+    __shared__ uint64_t lds[4][32][KL];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
+    const int64_t q = (int64_t)blockIdx.x * 128 + 32 * w + col; // query buffers are padded to nq_pad rows
+    const int64_t s = blockIdx.y;
+    const int64_t r0 = s * rows_per_slice, r1 = min(n, r0 + rows_per_slice);
+    const float qnv = qn[q];
+    const __bf16 *qh = Qh + q * dp + 8 * half;
+    const __bf16 *ql = Ql + q * dp + 8 * half;
+
+    uint32_t Lk[KL], Li[KL];
+#pragma unroll
+    for (int j = 0; j < KL; ++j) {
+        Lk[j] = kKey32None;
+        Li[j] = 0u;
+    }
+    uint32_t thr = kKey32None;
+
+    for (int64_t t = r0; t < r1; t += 32) {
+        // |x|^2 of this lane's 16 output rows: rows t + 8g + 4 half + {0..3}
+        float4 xv[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) xv[g] = *(const float4 *)(xn + t + 8 * g + 4 * half);
+        const __bf16 *xh = Xh + (t + col) * dp + 8 * half;
+        const __bf16 *xl = Xl + (t + col) * dp + 8 * half;
+        f32x16 acc = {};
+        bf16x8 ah = *(const bf16x8 *)xh, bh = *(const bf16x8 *)qh, al = {}, bl = {};
+        if (PREC == 3) {
+            al = *(const bf16x8 *)xl;
+            bl = *(const bf16x8 *)ql;
+        }
+        for (int kk = 0; kk < dp; kk += 16) {
+            bf16x8 nah = ah, nbh = bh, nal = al, nbl = bl;
+            if (kk + 16 < dp) { // next step's fragments in flight during this step's products
+                nah = *(const bf16x8 *)(xh + kk + 16);
+                nbh = *(const bf16x8 *)(qh + kk + 16);
+                if (PREC == 3) {
+                    nal = *(const bf16x8 *)(xl + kk + 16);
+                    nbl = *(const bf16x8 *)(ql + kk + 16);
+                }
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+            if (PREC == 3) {
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+            }
+            ah = nah;
+            bh = nbh;
+            al = nal;
+            bl = nbl;
+        }
+        uint32_t key[16];
+        uint32_t mn = kKey32None;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int64_t i = t + 8 * (r >> 2) + 4 * half + (r & 3);
+            const float xr = (r & 3) == 0 ? xv[r >> 2].x : (r & 3) == 1 ? xv[r >> 2].y : (r & 3) == 2 ? xv[r >> 2].z : xv[r >> 2].w;
+            const float v = fmaxf(0.f, (qnv + xr) - 2.f * acc[r]);
+            key[r] = i < r1 ? f32_key(v) : kKey32None;
+            mn = min(mn, key[r]);
+        }
+        // Insert passing values one per lane per round; rounds run while any lane of the wave still
+        // has one (usually none or one round per tile). Values leave in increasing key order and,
+        // among equal keys, in increasing row order, so equal keys keep the lower row first.
+        while (__any(mn < thr)) {
+            if (mn < thr) {
+                uint32_t ir = 0;
+                bool found = false;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const bool hit = !found && key[r] == mn;
+                    ir = hit ? (uint32_t)(t + 8 * (r >> 2) + 4 * half + (r & 3)) : ir;
+                    key[r] = hit ? kKey32None : key[r];
+                    found = found || hit;
+                }
+#pragma unroll
+                for (int j = KL - 1; j > 0; --j) {
+                    const bool up = mn < Lk[j - 1];
+                    const bool here = mn < Lk[j];
+                    Li[j] = up ? Li[j - 1] : (here ? ir : Li[j]);
+                    Lk[j] = max(Lk[j - 1], min(mn, Lk[j]));
+                }
+                if (mn < Lk[0]) {
+                    Li[0] = ir;
+                    Lk[0] = mn;
+                }
+                thr = Lk[KL - 1];
+            }
+            mn = kKey32None;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) mn = min(mn, key[r]);
+        }
+    }
+    // Half-wave merge: lanes l and l+32 hold lists of the same query; lane l keeps the KL
+    // smallest of both. Keys a list dropped are >= its last key >= the merged last key, so the
+    // merged last key still bounds every dropped candidate from below.
+    if (half == 1) {
+#pragma unroll
+        for (int j = 0; j < KL; ++j)
+            lds[w][col][j] = Lk[j] == kKey32None ? kKey64None : (((uint64_t)Lk[j] << 32) | Li[j]);
+    }
+    __syncthreads();
+    if (half == 0) {
+        uint64_t *out = part + (s * KL) * nq_pad + q;
+        int pb = 0;
+        int pa = 0;
+        for (int j = 0; j < KL; ++j) {
+            uint64_t va = kKey64None;
+#pragma unroll
+            for (int e = 0; e < KL; ++e) // own list entry pa, selected without dynamic register indexing
+                if (e == pa) va = Lk[e] == kKey32None ? kKey64None : (((uint64_t)Lk[e] << 32) | Li[e]);
+            const uint64_t vb = pb < KL ? lds[w][col][pb] : kKey64None;
+            const bool takea = va <= vb;
+            out[(int64_t)j * nq_pad] = takea ? va : vb;
+            pa += takea ? 1 : 0;
+            pb += takea ? 0 : 1;
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint64_t o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// One wave per query: exact rerank of K2c candidates, (distance, index) order, completeness check.
+__global__ __launch_bounds__(256) void l2_rerank_kernel(const float *__restrict__ X, int64_t xstride, int64_t d,
+                                                        const float *__restrict__ Q, int64_t qstride,
+                                                        const float *__restrict__ qn, const unsigned int *__restrict__ xn_max_bits,
+                                                        const int32_t *__restrict__ cidx, const float *__restrict__ capx,
+                                                        int K2c, const uint64_t *__restrict__ part, int64_t nlists,
+                                                        int KL, int64_t nq_pad, int64_t nq, int kk, int64_t n,
+                                                        float eps_coef, int32_t *__restrict__ out_idx,
+                                                        float *__restrict__ out_dist, int32_t *__restrict__ flags) {
+    const int lane = threadIdx.x & 63;
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= nq) return;
+    uint32_t bmin = kKey32None;
+    for (int64_t l = lane; l < nlists; l += 64)
+        bmin = min(bmin, (uint32_t)(part[(l * KL + KL - 1) * nq_pad + q] >> 32));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) bmin = min(bmin, (uint32_t)__shfl_xor((int)bmin, off, 64));
+
+    uint64_t key = kKey64None;
+    float core = 0.f;
+    if (lane < K2c) {
+        const int32_t idx = cidx[q * K2c + lane];
+        if (idx >= 0 && idx < n) {
+            core = l2_core(Q + q * qstride, X + (int64_t)idx * xstride, d);
+            key = ((uint64_t)f32_key(sqrtf(core)) << 32) | (uint32_t)idx;
+        }
+    }
+    float dk_core = 0.f;
+    for (int r = 0; r < kk; ++r) {
+        const uint64_t m = wave_min_u64(key);
+        const unsigned long long own = __ballot(key == m);
+        const int owner = __ffsll((unsigned long long)own) - 1;
+        const float cr = __shfl(core, owner, 64);
+        if (key == m) key = kKey64None;
+        if (lane == 0) {
+            out_idx[q * kk + r] = (int32_t)(uint32_t)(m & 0xFFFFFFFFull);
+            out_dist[q * kk + r] = key_f32((uint32_t)(m >> 32));
+        }
+        dk_core = cr;
+    }
+    if (lane == 0) {
+        int flag = 0;
+        if (n > K2c) {
+            const float a_last = capx[q * K2c + K2c - 1];
+            const float bound = fminf(key_f32(bmin), a_last);
+            const float s = sqrtf(qn[q]) + sqrtf(__uint_as_float(*xn_max_bits));
+            const float eps = eps_coef * s * s;
+            flag = !(bound - eps > dk_core);
+        }
+        flags[q] = flag;
+    }
+}
+
+// Exact full scan for flagged queries (one workgroup each; unflagged workgroups exit at once).
+template <int K>
+__global__ __launch_bounds__(256) void l2_fallback_kernel(const float *__restrict__ X, int64_t xstride, int64_t n,
+                                                          int64_t d, const float *__restrict__ Q, int64_t qstride,
+                                                          const int32_t *__restrict__ flags, int kk,
+                                                          int32_t *__restrict__ out_idx, float *__restrict__ out_dist) {
+    __shared__ uint64_t wres[4][K];
+    const int64_t q = blockIdx.x;
+    if (!flags[q]) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t L[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) L[j] = kKey64None;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        const float dd = sqrtf(l2_core(Q + q * qstride, X + i * xstride, d));
+        const uint64_t key = ((uint64_t)f32_key(dd) << 32) | (uint32_t)i;
+        if (key < L[K - 1]) list_insert_u64<K>(L, key);
+    }
+    for (int r = 0; r < kk; ++r) { // wave-level extraction of the wave's kk best
+        const uint64_t m = wave_min_u64(L[0]);
+        if (L[0] == m && m != kKey64None) {
+#pragma unroll
+            for (int j = 0; j < K - 1; ++j) L[j] = L[j + 1];
+            L[K - 1] = kKey64None;
+        }
+        if (lane == 0) wres[w][r] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int p[4] = {0, 0, 0, 0};
+        auto head = [&](int ww) { return p[ww] < kk ? wres[ww][p[ww]] : kKey64None; };
+        for (int r = 0; r < kk; ++r) {
+            int bw = 0;
+            for (int ww = 1; ww < 4; ++ww)
+                if (head(ww) < head(bw)) bw = ww;
+            const uint64_t m = head(bw);
+            ++p[bw];
+            out_idx[q * kk + r] = (int32_t)(uint32_t)(m & 0xFFFFFFFFull);
+            out_dist[q * kk + r] = key_f32((uint32_t)(m >> 32));
+        }
+    }
+}
+
+// ---------------------------------------------------------------- host side
+void l2_prepare(L2Set &s, const float *X, int64_t n, int64_t d, int64_t stride, hipStream_t st) {
+    s.n = n;
+    s.d = d;
+    s.dp = (int)ceil_div(d, 16) * 16;
+    const int64_t npad = std::max<int64_t>(32, ceil_div(n, 32) * 32);
+    __bf16 *H = (__bf16 *)s.hi.ensure((size_t)npad * s.dp * 2);
+    __bf16 *L = (__bf16 *)s.lo.ensure((size_t)npad * s.dp * 2);
+    float *nr = (float *)s.norm.ensure((size_t)npad * 4);
+    unsigned int *mx = (unsigned int *)s.norm_max.ensure(4);
+    PN_HIP(hipMemsetAsync(mx, 0, 4, st));
+    if (npad > n) { // zero tail rows: tiles read them unguarded
+        PN_HIP(hipMemsetAsync(H + n * s.dp, 0, (size_t)(npad - n) * s.dp * 2, st));
+        PN_HIP(hipMemsetAsync(L + n * s.dp, 0, (size_t)(npad - n) * s.dp * 2, st));
+        PN_HIP(hipMemsetAsync(nr + n, 0, (size_t)(npad - n) * 4, st));
+    }
+    if (n == 0) return;
+    hipLaunchKernelGGL(l2_split_kernel, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st, X, n, d, stride, s.dp, H, L,
+                       nr);
+    hipLaunchKernelGGL(max_f32_kernel, dim3((unsigned)std::min<int64_t>(1024, ceil_div(n, 256))), dim3(256), 0, st, nr,
+                       n, mx);
+    PN_HIP(hipGetLastError());
+}
+
+static int kl_for(int kk) { return kk <= 4 ? 8 : (kk <= 12 ? 16 : 32); }
+
+void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const float *Q, int64_t qstride, int64_t nq,
+                    int kk, int32_t *out_idx, float *out_dist, L2Work &wk, int cus, hipStream_t st) {
+    const int64_t n = xs.n, d = xs.d;
+    const int dp = xs.dp;
+    const int KL = kl_for(kk);
+    const int K2 = 2 * KL;
+    const int K2c = (int)std::min<int64_t>(K2, n);
+    const int64_t gx = ceil_div(nq, 128);
+    const int64_t nq_pad = gx * 128;
+    // queries: split into hi/lo (padded to nq_pad rows of dp; padding rows are zero)
+    L2Set &qs = wk.qsplit;
+    qs.dp = dp;
+    __bf16 *Qh = (__bf16 *)qs.hi.ensure((size_t)nq_pad * dp * 2);
+    __bf16 *Ql = (__bf16 *)qs.lo.ensure((size_t)nq_pad * dp * 2);
+    float *qn = (float *)qs.norm.ensure((size_t)nq_pad * 4);
+    PN_HIP(hipMemsetAsync(Qh, 0, (size_t)nq_pad * dp * 2, st));
+    PN_HIP(hipMemsetAsync(Ql, 0, (size_t)nq_pad * dp * 2, st));
+    PN_HIP(hipMemsetAsync(qn, 0, (size_t)nq_pad * 4, st));
+    hipLaunchKernelGGL(l2_split_kernel, dim3((unsigned)ceil_div(nq, 4)), dim3(256), 0, st, Q, nq, d, qstride, dp, Qh,
+                       Ql, qn);
+    // slices: whole multiples of the CU count, at least 128 rows each
+    const int64_t wpc = std::max<int64_t>(1, tune_get("l2.wg_per_cu"));
+    int64_t S = std::max<int64_t>(1, wpc * cus / gx);
+    S = std::min<int64_t>(S, std::max<int64_t>(1, n / 128));
+    const int64_t rps = ceil_div(ceil_div(n, S), 32) * 32; // slices start on 32-row boundaries
+    S = ceil_div(n, rps);
+    const int64_t nlists = S;
+    uint64_t *part = (uint64_t *)wk.part.ensure((size_t)nlists * KL * nq_pad * 8);
+    const int prec = (int)tune_get("l2.mfma_terms");
+    {
+        StageScope sc(st, STAGE_SCAN);
+        dim3 grid((unsigned)gx, (unsigned)S);
+#define PN_L2SCAN(KLV, PV)                                                                                  \
+    hipLaunchKernelGGL((l2_mfma_scan_kernel<KLV, PV>), grid, dim3(256), 0, st, xs.hi.as<__bf16>(), xs.lo.as<__bf16>(), \
+                       xs.norm.as<float>(), n, dp, Qh, Ql, qn, rps, part, nq_pad)
+        if (prec == 1) {
+            if (KL == 8) PN_L2SCAN(8, 1); else if (KL == 16) PN_L2SCAN(16, 1); else PN_L2SCAN(32, 1);
+        } else {
+            if (KL == 8) PN_L2SCAN(8, 3); else if (KL == 16) PN_L2SCAN(16, 3); else PN_L2SCAN(32, 3);
+        }
+#undef PN_L2SCAN
+        PN_HIP(hipGetLastError());
+    }
+    int32_t *cidx = (int32_t *)wk.cidx.ensure((size_t)nq * K2c * 4);
+    float *capx = (float *)wk.capx.ensure((size_t)nq * K2c * 4);
+    int32_t *flags = (int32_t *)wk.flags.ensure((size_t)nq * 4);
+    {
+        StageScope sc(st, STAGE_SELECT);
+        merge_lists_wave(part, nlists, KL, nq, nq_pad, K2c, OUT_F32, cidx, capx, st);
+    }
+    {
+        StageScope sc(st, STAGE_RERANK);
+        const float eps_coef = (prec == 1 ? 0.0078125f : 6.103515625e-05f) + (float)dp * 4.76837158203125e-07f;
+        hipLaunchKernelGGL(l2_rerank_kernel, dim3((unsigned)ceil_div(nq, 4)), dim3(256), 0, st, X, xstride, d, Q,
+                           qstride, qn, xs.norm_max.as<unsigned int>(), cidx, capx, K2c, part, nlists, KL, nq_pad, nq,
+                           kk, n, eps_coef, out_idx, out_dist, flags);
+        PN_HIP(hipGetLastError());
+    }
+    {
+        StageScope sc(st, STAGE_OTHER);
+#define PN_L2FB(KV)                                                                                              \
+    hipLaunchKernelGGL((l2_fallback_kernel<KV>), dim3((unsigned)nq), dim3(256), 0, st, X, xstride, n, d, Q, qstride, \
+                       flags, kk, out_idx, out_dist)
+        if (kk <= 8) PN_L2FB(8); else if (kk <= 16) PN_L2FB(16); else PN_L2FB(32);
+#undef PN_L2FB
+        PN_HIP(hipGetLastError());
+    }
+}
+
+} // namespace pn

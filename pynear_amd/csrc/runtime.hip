@@ -36,6 +36,9 @@ static std::map<std::string, int64_t> &tune_table() {
         t["hamming.qpl"] = 1;         // queries per lane in the Hamming scan (1 or 2)
         t["float.wg_per_cu"] = 16;
         t["merge.wave_per_query"] = 1; // 1: one wave merges one query's slice lists
+        t["l2.mfma"] = 1;              // exact L2 (k <= 32) through the MFMA candidate scan + rerank
+        t["l2.wg_per_cu"] = 2;
+        t["l2.mfma_terms"] = 3;        // 3: hi.hi + hi.lo + lo.hi products; 1: hi.hi only
         if (const char *env = std::getenv("PN_TUNE")) {
             std::string e(env);
             size_t pos = 0;

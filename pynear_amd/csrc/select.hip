@@ -86,13 +86,13 @@ __global__ __launch_bounds__(256) void merge_lists_wave_kernel(const uint64_t *_
             uint64_t o = __shfl_xor(m, off, 64);
             m = o < m ? o : m;
         }
-        if (L[0] == m) { // the unique owner pops its head
+        if (L[0] == m && m != kKey64None) { // the unique owner pops its head
 #pragma unroll
             for (int j = 0; j < K - 1; ++j) L[j] = L[j + 1];
             L[K - 1] = kKey64None;
         }
         if (lane == 0) {
-            out_idx[q * kk + r] = (int32_t)(uint32_t)(m & 0xFFFFFFFFull);
+            out_idx[q * kk + r] = m == kKey64None ? -1 : (int32_t)(uint32_t)(m & 0xFFFFFFFFull);
             uint32_t hi = (uint32_t)(m >> 32);
             if (kind == OUT_HAMMING_I32)
                 ((int32_t *)out_dist)[q * kk + r] = (int32_t)hi;
@@ -125,6 +125,25 @@ void merge_lists(const uint64_t *part, int64_t slices, int K, int64_t nq, int64_
     default: fail(PN_EINVAL, "merge: unsupported list size");
     }
 #undef PN_MERGE
+    PN_HIP(hipGetLastError());
+}
+
+void merge_lists_wave(const uint64_t *part, int64_t slices, int K, int64_t nq, int64_t nq_pad, int kk, int kind,
+                      int32_t *out_idx, void *out_dist, hipStream_t st) {
+    dim3 grid((unsigned)ceil_div(nq, 4));
+#define PN_MERGEW(KV)                                                                                     \
+    hipLaunchKernelGGL((merge_lists_wave_kernel<KV>), grid, dim3(256), 0, st, part, slices, nq, nq_pad, kk, kind, \
+                       out_idx, out_dist)
+    switch (K) {
+    case 1: PN_MERGEW(1); break;
+    case 2: PN_MERGEW(2); break;
+    case 4: PN_MERGEW(4); break;
+    case 8: PN_MERGEW(8); break;
+    case 16: PN_MERGEW(16); break;
+    case 32: PN_MERGEW(32); break;
+    default: fail(PN_EINVAL, "merge: unsupported list size");
+    }
+#undef PN_MERGEW
     PN_HIP(hipGetLastError());
 }
 

@@ -200,3 +200,69 @@ def test_pairwise_distances_bit_exact():
     a = rng.integers(0, 256, (300, 23), dtype=np.uint8)
     b = rng.integers(0, 256, (300, 23), dtype=np.uint8)
     np.testing.assert_array_equal(N.pairwise_distance(N.METRIC_HAMMING, a, b), np.bitwise_count(a ^ b).sum(1))
+
+
+class _Tune:
+    def __init__(self, **kv):
+        self.kv = {k.replace("__", "."): v for k, v in kv.items()}
+
+    def __enter__(self):
+        self.old = {k: N.tune_get(k) for k in self.kv}
+        for k, v in self.kv.items():
+            N.tune_set(k, v)
+
+    def __exit__(self, *a):
+        for k, v in self.old.items():
+            N.tune_set(k, v)
+
+
+@pytest.mark.parametrize("d", [1, 3, 16, 32, 48, 100, 256])
+@pytest.mark.parametrize("k", [1, 5, 10, 32])
+def test_l2_mfma_path_bit_exact(d, k):
+    """The MFMA candidate scan + exact rerank returns the oracle's result bit for bit."""
+    rng = np.random.default_rng(d * 1000 + k)
+    X = rng.standard_normal((6000, d)).astype(np.float32)
+    Q = rng.standard_normal((130, d)).astype(np.float32)
+    h = N.ExactHandle(N.METRIC_L2)
+    h.set(X)
+    with _Tune(l2__mfma=1):
+        gi, gd = h.search(Q, k)
+    oi, od = oracle.knn_float(N.METRIC_L2, X, Q, k)
+    np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(gi, oi)
+
+
+@pytest.mark.parametrize("d", [8, 32, 64])
+def test_l2_fallback_path_exact(d):
+    """With single-product bf16 the error bound is wide, queries fall back to the exact scan,
+    and the result is still the oracle's (and identical to the VALU scan's)."""
+    rng = np.random.default_rng(d)
+    X = rng.random((20000, d)).astype(np.float32)
+    X[5000:5100] = X[:100]  # exact duplicates: ties broken by lower index
+    Q = np.concatenate([X[:40] + 1e-4, rng.random((40, d)).astype(np.float32)])
+    h = N.ExactHandle(N.METRIC_L2)
+    h.set(X)
+    with _Tune(l2__mfma_terms=1):
+        gi, gd = h.search(Q, 10)
+    with _Tune(l2__mfma=0):
+        vi, vd = h.search(Q, 10)
+    oi, od = oracle.knn_float(N.METRIC_L2, X, Q, 10)
+    np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(vd, od)
+    np.testing.assert_array_equal(vi, oi)
+
+
+def test_l2_config1_shape_sample():
+    """100k x 32 U[0,1), 1000 queries, k=5 (seed 0): every query vs the oracle."""
+    import bench
+    X, Q = bench.l2_workload()
+    h = N.ExactHandle(N.METRIC_L2)
+    h.set(X)
+    gi, gd = h.search(Q, 5)
+    sub = np.arange(0, 1000, 4)
+    oi, od = oracle.knn_float(N.METRIC_L2, X, Q[sub], 5)
+    np.testing.assert_array_equal(gd[sub], od)
+    np.testing.assert_array_equal(gi[sub], oi)
+    fi, fd = oracle.knn_fp64(N.METRIC_L2, X, Q[sub], 5)
+    np.testing.assert_allclose(gd[sub], fd, rtol=1e-6)
