@@ -105,6 +105,13 @@ pn_status pn_exact_search(pn_exact *h, const void *queries, int64_t nq, int64_t 
 pn_status pn_exact_search_device(pn_exact *h, const void *queries, int64_t nq, int32_t k, int32_t *out_idx,
                                  void *out_dist, void *stream);
 
+/* ---------------------------------------------------------------- k-means
+ * kmeans_l2(data, k, max_iter, seed) (PythonBindings.cpp:305-329, KMeans.hpp:25-126): host data
+ * n x d float32; writes labels int32[n] and centroids float32[k, d]. The result is identical to
+ * the reference's (same random draws, same arithmetic). k outside [1, n] returns PN_ERUNTIME. */
+pn_status pn_kmeans_l2(const float *data, int64_t n, int64_t d, int64_t k, int64_t max_iter, uint32_t seed,
+                       int32_t *labels, float *centroids);
+
 /* ---------------------------------------------------------------- IVF binary index
  * IVFFlatBinaryIndex (BinaryIVF.hpp:34-227) behind the pybind11 class of the same name
  * (PythonBindings.cpp:231-251, :454-466). The build reproduces the reference's k-means++ and
@@ -138,6 +145,41 @@ pn_status pn_ivfbin_search(pn_ivfbin *h, const uint8_t *queries, int64_t nq, int
 /* Device-resident form: device queries (nq x nbytes), int32 outputs [nq, k] and counts [nq]. */
 pn_status pn_ivfbin_search_device(pn_ivfbin *h, const void *queries, int64_t nq, int32_t k, int32_t *out_idx,
                                   int32_t *out_dist, int32_t *out_count, void *stream);
+
+/* ---------------------------------------------------------------- IVF L2 index
+ * The search of IVFFlatL2Index (pynear/forest.py:25-115), which is pure numpy in the reference:
+ * coarse top-nprobe clusters, then the probed clusters' rows; results ascending by squared L2
+ * (forest.py:107-109). Clusters without rows are left out, like the reference's per-cluster
+ * arrays (forest.py:51-59). */
+typedef struct pn_ivfl2 pn_ivfl2;
+
+pn_status pn_ivfl2_create(pn_ivfl2 **out);
+void pn_ivfl2_destroy(pn_ivfl2 *h);
+
+/* IVFFlatL2Index.set (forest.py:36-59) with its k-means (forest.py:136-143 -> kmeans_l2):
+ * host rows n x d; runs kmeans_l2(rows, n_clusters, max_iter, seed) on the device, returns its
+ * labels (int32[n]) and centroids (float32[n_clusters, d]) and builds the device index. */
+pn_status pn_ivfl2_build(pn_ivfl2 *h, const float *rows, int64_t n, int64_t d, int64_t n_clusters, int64_t max_iter,
+                         uint32_t seed, int32_t *labels_out, float *centroids_out);
+
+/* Builds the device index from given centroids (nc x d) and row labels (in [0, nc)). */
+pn_status pn_ivfl2_set(pn_ivfl2 *h, const float *rows, int64_t n, int64_t d, const float *centroids, int64_t nc,
+                       const int32_t *labels);
+
+/* Rows, dimension and number of non-empty clusters. */
+pn_status pn_ivfl2_info(const pn_ivfl2 *h, int64_t *n, int64_t *d, int32_t *n_lists);
+
+/* _IVFFlatIndex.searchKNN (forest.py:63-111): host queries nq x d; nprobe clusters per query
+ * (capped at the non-empty cluster count); k <= 64. precision 0: bf16 candidate distances,
+ * 1: three-product bf16 (~fp32) candidate distances; both are followed by an exact squared-L2
+ * rerank of 64 candidates. out_idx int64[nq, k], out_dist float32[nq, k] (squared L2), out_count. */
+pn_status pn_ivfl2_search(pn_ivfl2 *h, const float *queries, int64_t nq, int64_t d, int32_t nprobe, int32_t k,
+                          int32_t precision, int64_t *out_idx, float *out_dist, int32_t *out_count);
+
+/* Device-resident form: device queries (nq x d, contiguous), int32/float32 outputs [nq, k]. */
+pn_status pn_ivfl2_search_device(pn_ivfl2 *h, const float *queries, int64_t nq, int32_t nprobe, int32_t k,
+                                 int32_t precision, int32_t *out_idx, float *out_dist, int32_t *out_count,
+                                 void *stream);
 
 #ifdef __cplusplus
 }

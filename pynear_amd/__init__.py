@@ -17,8 +17,9 @@ from ._pynear import VPTreeBinaryIndex as VPTreeBinaryIndexN
 from ._pynear import VPTreeBinaryIndex64, VPTreeBinaryIndex128, VPTreeBinaryIndex256, VPTreeBinaryIndex512
 from ._pynear import VPTreeChebyshevIndex, VPTreeL1Index, VPTreeL2Index
 from ._pynear import dist_chebyshev, dist_hamming_64, dist_hamming_128, dist_hamming_256, dist_hamming_512
-from ._pynear import dist_l1, dist_l2
+from ._pynear import dist_l1, dist_l2, kmeans_l2
 from ._version import __version__  # noqa: F401
+from .forest import IVFFlatL2Index
 
 try:
     from .sklearn_adapter import PyNearKNeighborsClassifier, PyNearKNeighborsRegressor, PyNearNearestNeighbors
@@ -78,8 +79,8 @@ class VPTreeBinaryIndex:
 
 
 __all__ = [
-    "IVFFlatBinaryIndex", "VPTreeL2Index", "VPTreeL1Index", "VPTreeChebyshevIndex", "VPTreeBinaryIndex", "VPTreeBinaryIndexN",
+    "IVFFlatBinaryIndex", "IVFFlatL2Index", "VPTreeL2Index", "VPTreeL1Index", "VPTreeChebyshevIndex", "VPTreeBinaryIndex", "VPTreeBinaryIndexN",
     "VPTreeBinaryIndex64", "VPTreeBinaryIndex128", "VPTreeBinaryIndex256", "VPTreeBinaryIndex512",
     "dist_l2", "dist_l1", "dist_chebyshev", "dist_hamming", "dist_hamming_64", "dist_hamming_128",
-    "dist_hamming_256", "dist_hamming_512", "NativeLibraryError",
+    "dist_hamming_256", "dist_hamming_512", "kmeans_l2", "NativeLibraryError",
 ]

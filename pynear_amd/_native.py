@@ -40,6 +40,14 @@ SIGNATURES = {
     "pn_exact_info": (_i32, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "pn_exact_search": (_i32, [_vp, _vp, _i64, _i64, _i32, _vp, _vp]),
     "pn_exact_search_device": (_i32, [_vp, _vp, _i64, _i32, _vp, _vp, _vp]),
+    "pn_kmeans_l2": (_i32, [_vp, _i64, _i64, _i64, _i64, _u32, _vp, _vp]),
+    "pn_ivfl2_create": (_i32, [ctypes.POINTER(_vp)]),
+    "pn_ivfl2_destroy": (None, [_vp]),
+    "pn_ivfl2_build": (_i32, [_vp, _vp, _i64, _i64, _i64, _i64, _u32, _vp, _vp]),
+    "pn_ivfl2_set": (_i32, [_vp, _vp, _i64, _i64, _vp, _i64, _vp]),
+    "pn_ivfl2_info": (_i32, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i32)]),
+    "pn_ivfl2_search": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _i32, _vp, _vp, _vp]),
+    "pn_ivfl2_search_device": (_i32, [_vp, _vp, _i64, _i32, _i32, _i32, _vp, _vp, _vp, _vp]),
     "pn_ivfbin_create": (_i32, [_i32, _i32, _i32, _u32, ctypes.POINTER(_vp)]),
     "pn_ivfbin_destroy": (None, [_vp]),
     "pn_ivfbin_set": (_i32, [_vp, _vp, _i64, _i64]),
@@ -202,6 +210,65 @@ class IvfBinHandle:
     def search_device(self, q_ptr, nq, k, out_idx_ptr, out_dist_ptr, out_cnt_ptr, stream_ptr=None):
         check(lib().pn_ivfbin_search_device(self._h, _vp(q_ptr), nq, int(k), _vp(out_idx_ptr), _vp(out_dist_ptr),
                                             _vp(out_cnt_ptr), _vp(stream_ptr) if stream_ptr else None))
+
+
+class IvfL2Handle:
+    """Owner of a ``pn_ivfl2`` handle (the IVFFlatL2Index search on the current device)."""
+
+    PRECISION = {"bf16": 0, "bf16x3": 1}
+
+    def __init__(self):
+        h = _vp()
+        check(lib().pn_ivfl2_create(ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _LIB is not None:
+            _LIB.pn_ivfl2_destroy(h)
+            self._h = None
+
+    def build(self, rows, n_clusters, max_iter, seed):
+        n, d = rows.shape
+        labels = np.empty(n, np.int32)
+        cent = np.empty((int(n_clusters), d), np.float32)
+        check(lib().pn_ivfl2_build(self._h, ptr(rows), n, d, int(n_clusters), int(max_iter), int(seed),
+                                   ptr(labels), ptr(cent)))
+        return labels, cent
+
+    def set(self, rows, centroids, labels):
+        n, d = rows.shape
+        check(lib().pn_ivfl2_set(self._h, ptr(rows), n, d, ptr(centroids), len(centroids), ptr(labels)))
+
+    def info(self):
+        n, d, nl = _i64(0), _i64(0), _i32(0)
+        check(lib().pn_ivfl2_info(self._h, ctypes.byref(n), ctypes.byref(d), ctypes.byref(nl)))
+        return n.value, d.value, nl.value
+
+    def search(self, queries, nprobe, k, precision="bf16"):
+        nq, d = queries.shape
+        k = int(k)
+        idx = np.full((nq, k), -1, np.int64)
+        dist = np.full((nq, k), -1, np.float32)
+        cnt = np.zeros(nq, np.int32)
+        check(lib().pn_ivfl2_search(self._h, ptr(queries), nq, d, int(nprobe), k, self.PRECISION[precision],
+                                    ptr(idx), ptr(dist), ptr(cnt)))
+        return idx, dist, cnt
+
+    def search_device(self, q_ptr, nq, nprobe, k, out_idx_ptr, out_dist_ptr, out_cnt_ptr, precision="bf16",
+                      stream_ptr=None):
+        check(lib().pn_ivfl2_search_device(self._h, _vp(q_ptr), nq, int(nprobe), int(k), self.PRECISION[precision],
+                                           _vp(out_idx_ptr), _vp(out_dist_ptr), _vp(out_cnt_ptr),
+                                           _vp(stream_ptr) if stream_ptr else None))
+
+
+def kmeans_l2(data, k, max_iter, seed):
+    """(labels int32[n], centroids float32[k, d]) from the GPU k-means."""
+    n, d = data.shape
+    labels = np.empty(n, np.int32)
+    cent = np.empty((max(int(k), 0), d), np.float32)
+    check(lib().pn_kmeans_l2(ptr(data), n, d, int(k), int(max_iter), int(seed), ptr(labels), ptr(cent)))
+    return labels, cent
 
 
 STAGES = ("scan", "select", "coarse", "rerank", "other")

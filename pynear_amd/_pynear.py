@@ -188,6 +188,19 @@ def _pair_float(metric, a, b):
     return float(N.pairwise_distance(metric, a, b)[0])
 
 
+def kmeans_l2(data, k, max_iter=100, seed=42):
+    """Lloyd k-means with k-means++ initialisation on the GPU (PythonBindings.cpp:305-329).
+
+    Returns (labels int32[n], centroids float32[k, d]), identical to the reference's kmeans_l2.
+    """
+    a = np.ascontiguousarray(np.asarray(data), dtype=np.float32)
+    if a.ndim != 2:
+        raise RuntimeError("kmeans_l2: data must be a 2D float32 array (N, D)")
+    if k == 0 or k > a.shape[0]:
+        raise RuntimeError("kmeans_l2: k must be between 1 and N")
+    return N.kmeans_l2(a, k, max_iter, seed)
+
+
 def dist_l2(a, b):
     """Euclidean distance, reference arithmetic (DistanceFunctions.hpp:304-345)."""
     return _pair_float(N.METRIC_L2, a, b)

@@ -303,7 +303,8 @@ void l2_prepare(L2Set &s, const float *X, int64_t n, int64_t d, int64_t stride, 
     s.n = n;
     s.d = d;
     s.dp = (int)ceil_div(d, 16) * 16;
-    const int64_t npad = std::max<int64_t>(32, ceil_div(n, 32) * 32);
+    // 32 zero rows past the last full tile: tiles that start at any row < n stay in bounds
+    const int64_t npad = ceil_div(n, 32) * 32 + 32;
     __bf16 *H = (__bf16 *)s.hi.ensure((size_t)npad * s.dp * 2);
     __bf16 *L = (__bf16 *)s.lo.ensure((size_t)npad * s.dp * 2);
     float *nr = (float *)s.norm.ensure((size_t)npad * 4);
@@ -319,6 +320,14 @@ void l2_prepare(L2Set &s, const float *X, int64_t n, int64_t d, int64_t stride, 
                        nr);
     hipLaunchKernelGGL(max_f32_kernel, dim3((unsigned)std::min<int64_t>(1024, ceil_div(n, 256))), dim3(256), 0, st, nr,
                        n, mx);
+    PN_HIP(hipGetLastError());
+}
+
+void l2_split_rows(const float *X, int64_t n, int64_t d, int64_t stride, int dp, __bf16 *H, __bf16 *L, float *nrm,
+                   hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(l2_split_kernel, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st, X, n, d, stride, dp, H, L,
+                       nrm);
     PN_HIP(hipGetLastError());
 }
 

@@ -20,6 +20,10 @@ struct L2Work {
 // X: device rows (fp32, row stride `stride` floats).
 void l2_prepare(L2Set &s, const float *X, int64_t n, int64_t d, int64_t stride, hipStream_t st);
 
+// bf16 hi/lo split + squared norms of n rows into caller buffers (rows of dp elements).
+void l2_split_rows(const float *X, int64_t n, int64_t d, int64_t stride, int dp, __bf16 *H, __bf16 *L, float *nrm,
+                   hipStream_t st);
+
 // Exact k-NN (kk <= 32) of nq device queries Q (row stride qstride) against the rows X the split
 // `xs` was prepared from; out_idx int32[nq, kk], out_dist float32[nq, kk] (sqrt distances),
 // ascending (distance, index), bit-identical to the reference's dist_l2_f_avx2.

@@ -1,0 +1,160 @@
+"""GPU parity of kmeans_l2 and IVFFlatL2Index.
+
+kmeans_l2 must be identical to the reference (labels and centroids bit for bit). The IVF search
+returns exact squared L2 distances (reference kernel order) over the probed clusters: it must
+equal brute force when every cluster is probed, and agree with the numpy restatement of
+pynear/forest.py:63-111 on the same clusters (index sets, distances within 1e-4 relative).
+"""
+
+import os
+import pickle
+
+import numpy as np
+import pytest
+
+import oracle
+import pynear_amd
+from pynear_amd import IVFFlatL2Index
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.mark.parametrize("n,d,k,seed", [(500, 32, 20, 42), (1000, 8, 10, 1), (300, 37, 7, 3), (2000, 64, 40, 42),
+                                        (800, 16, 12, 42), (3000, 100, 30, 9), (64, 5, 64, 42)])
+def test_kmeans_identical_to_reference_restatement(n, d, k, seed):
+    rng = np.random.default_rng(n + d)
+    X = rng.random((n, d)).astype(np.float32)
+    gl, gc = pynear_amd.kmeans_l2(X, k, 100, seed)
+    ol, oc = oracle.kmeans_l2(X, k, 100, seed)
+    np.testing.assert_array_equal(gl, ol)
+    np.testing.assert_array_equal(gc, oc)
+
+
+def test_kmeans_matches_recorded_reference_outputs():
+    g = np.load(os.path.join(GOLD, "ref_vectors.npz"))
+    gl, gc = pynear_amd.kmeans_l2(g["km_X"], 12, 100, 42)
+    np.testing.assert_array_equal(gl, g["km_labels"])
+    np.testing.assert_array_equal(gc, g["km_centroids"])
+
+
+def test_kmeans_errors():
+    with pytest.raises(RuntimeError):
+        pynear_amd.kmeans_l2(np.zeros((5, 3), np.float32), 6)
+    with pytest.raises(RuntimeError):
+        pynear_amd.kmeans_l2(np.zeros(5, np.float32), 2)
+
+
+def _gmm(n, d, comps, seed, sigma=0.3):
+    rng = np.random.default_rng(seed)
+    centres = rng.standard_normal((comps, d)).astype(np.float32)
+    lab = rng.integers(0, comps, n)
+    return (centres[lab] + sigma * rng.standard_normal((n, d))).astype(np.float32)
+
+
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3"])
+@pytest.mark.parametrize("d", [8, 32, 100])
+def test_full_probe_is_exact(precision, d):
+    X = _gmm(4000, d, 30, seed=d)
+    Q = _gmm(60, d, 30, seed=d + 1)
+    idx = IVFFlatL2Index(n_clusters=25, n_probe=25)
+    idx.precision = precision
+    idx.set(X)
+    gi, gd = idx.searchKNN(Q, 10)
+    fi, fd = oracle.knn_fp64(oracle.L2, X, Q, 10)
+    np.testing.assert_allclose(np.array(gd), fd ** 2, rtol=2e-5, atol=1e-6)
+    assert (np.array(gi) == fi).mean() > 0.999  # equal up to float near-ties
+
+
+@pytest.mark.parametrize("n_probe", [1, 3, 8])
+def test_partial_probe_matches_numpy_restatement(n_probe):
+    X = _gmm(5000, 48, 40, seed=7)
+    Q = _gmm(50, 48, 40, seed=8)
+    idx = IVFFlatL2Index(n_clusters=40, n_probe=n_probe)
+    idx.set(X)
+    gi, gd = idx.searchKNN(Q, 10)
+    cent = idx._centroids[idx._list_centroids]
+    oi, od = oracle.ivf_l2_search(cent, idx._cluster_data, idx._cluster_norms, idx._orig_indices, Q, n_probe, 10)
+    same = np.mean([len(set(a) & set(b)) / max(len(b), 1) for a, b in zip(gi, oi)])
+    assert same > 0.995
+    for a, b, da, db in zip(gi, oi, gd, od):
+        common = set(a) & set(b)
+        for c in common:
+            assert da[a.index(c)] == pytest.approx(db[b.index(c)], rel=1e-4, abs=1e-3)
+
+
+def test_reference_ivfflat_behaviour():
+    """pynear/tests/test_ivfflat.py:44-161 through the same calls."""
+    rng = np.random.default_rng(0)
+    data = rng.random((500, 32)).astype(np.float32)
+    queries = rng.random((20, 32)).astype(np.float32)
+    index = IVFFlatL2Index(n_clusters=20, n_probe=20)
+    index.set(data)
+    ai, ad = index.searchKNN(queries, 5)
+    ei = [np.argsort(((data - q) ** 2).sum(1))[:5].tolist() for q in queries]
+    assert all(set(a) == set(e) for a, e in zip(ai, ei))
+    assert all(d == sorted(d) for d in ad)
+    # recall at n_probe 15 of 20 on low-dim data
+    rng = np.random.default_rng(1)
+    data = rng.random((2000, 8)).astype(np.float32)
+    queries = rng.random((50, 8)).astype(np.float32)
+    index = IVFFlatL2Index(n_clusters=20, n_probe=15)
+    index.set(data)
+    ai, _ = index.searchKNN(queries, 10)
+    ei = [np.argsort(((data - q) ** 2).sum(1))[:10] for q in queries]
+    assert np.mean([len(set(a) & set(e.tolist())) / 10 for a, e in zip(ai, ei)]) >= 0.9
+    # search1NN == searchKNN(k=1); 1-D query; k larger than a cluster; n_clusters capped at N
+    rng = np.random.default_rng(2)
+    data = rng.random((500, 16)).astype(np.float32)
+    q = rng.random((10, 16)).astype(np.float32)
+    index = IVFFlatL2Index(n_clusters=10, n_probe=10)
+    index.set(data)
+    assert index.search1NN(q) == index.searchKNN(q, 1)
+    i1, _ = index.searchKNN(q[0], 3)
+    assert len(i1) == 1 and len(i1[0]) == 3
+    small = IVFFlatL2Index(n_clusters=20, n_probe=20)
+    small.set(rng.random((50, 4)).astype(np.float32))
+    si, _ = small.searchKNN(rng.random((1, 4)).astype(np.float32), k=10)
+    assert len(si[0]) == 10
+    tiny = IVFFlatL2Index(n_clusters=100, n_probe=100)
+    tiny.set(rng.random((10, 4)).astype(np.float32))
+    assert tiny.n_clusters <= 10
+    wide = IVFFlatL2Index(n_clusters=40, n_probe=10)
+    wide.set(rng.random((2000, 1024)).astype(np.float32))
+    wi, _ = wide.searchKNN(rng.random((5, 1024)).astype(np.float32), k=5)
+    assert len(wi) == 5 and all(len(r) == 5 for r in wi)
+    with pytest.raises(RuntimeError):
+        IVFFlatL2Index().searchKNN(q, 3)
+
+
+def test_pickle_roundtrip_rebuilds_device_index():
+    rng = np.random.default_rng(8)
+    data = rng.random((500, 32)).astype(np.float32)
+    queries = rng.random((10, 32)).astype(np.float32)
+    index = IVFFlatL2Index(n_clusters=20, n_probe=7)
+    index.set(data)
+    before = index.searchKNN(queries, 5)
+    index2 = pickle.loads(pickle.dumps(index))
+    assert index2._dev is None
+    after = index2.searchKNN(queries, 5)
+    assert before[0] == after[0]
+    np.testing.assert_allclose(before[1], after[1])
+
+
+def test_device_entry_point_matches_host():
+    from pynear_amd import _native as N, hipdev
+    X = _gmm(6000, 64, 50, seed=3)
+    Q = _gmm(200, 64, 50, seed=4)
+    h = N.IvfL2Handle()
+    h.build(X, 50, 100, 42)
+    hi, hd, hc = h.search(Q, 8, 10)
+    dq = hipdev.DeviceArray.from_numpy(Q)
+    oi = hipdev.DeviceArray((200, 10), np.int32)
+    od = hipdev.DeviceArray((200, 10), np.float32)
+    oc = hipdev.DeviceArray((200,), np.int32)
+    st = hipdev.Stream()
+    h.search_device(dq.ptr, 200, 8, 10, oi.ptr, od.ptr, oc.ptr, "bf16", st.handle)
+    st.synchronize()
+    np.testing.assert_array_equal(oc.numpy(), hc)
+    np.testing.assert_array_equal(oi.numpy(), hi)
+    np.testing.assert_array_equal(od.numpy(), hd)
