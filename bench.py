@@ -1,20 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark of the pynear_amd search path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hamming|l2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ivf|hamming|l2] [--no-cpu-baseline]
 
-A step is one search of the whole query batch, with the database and the queries already
-resident in HBM (the device entry point ``pn_exact_search_device`` on a HIP stream of the
-library's own runtime, see pynear_amd/hipdev.py).
-Default workload: exact Hamming k-NN, N = 1,000,000 rows x 512 bits, 10,000 queries, k = 10
-(uniform random rows; each query is a random row with a Binomial(512, 0.1)-sized random set of
-bits flipped; seed 1). ``--workload l2``: exact L2 k-NN, N = 100,000 x 32 float32 U[0,1),
-1,000 queries, k = 5 (seed 0).
+A step is one search of the whole query batch with the database and the queries already resident
+in HBM (the device entry points on a HIP stream of the library's own runtime). Workloads:
 
-Prints one JSON line with the throughput (queries/s), the roofline of the dominant kernel
-(measured with HIP events around each launch on its stream) and a CPU baseline: the reference's
-own VP-tree (built from its sources into oracle/_ref) timed on a bounded query sample on this
-host, or the repository's C++ restatement when the reference module is absent.
+* ivf (default; the headline): IVFFlatL2Index, N = 1,000,000 x d = 768 float32 drawn from a
+  1,024-component Gaussian mixture (centres ~ N(0,1), sigma 0.3, seed 2), nlist = 1,024 (the
+  reference's kmeans_l2 with 100 iterations, seed 42, run on the GPU), 10,000 queries from the same
+  mixture, k = 10, n_probe in {8, 16, 32}. `value` is the QPS at the smallest n_probe whose
+  Recall@10 against the exact search is >= 0.95. The exact Hamming and exact L2 workloads below
+  are measured in the same run and reported under "also".
+* hamming: exact Hamming k-NN, N = 1,000,000 x 512 bits (uniform bytes), 10,000 queries (a random
+  row with a Binomial(512, 0.1)-sized random set of bits flipped), k = 10, seed 1.
+* l2: exact L2 k-NN, N = 100,000 x 32 float32 U[0,1), 1,000 queries, k = 5, seed 0.
+
+The roofline of the dominant kernel is measured with HIP events around each launch on its
+stream. The CPU baseline runs on this host: the reference's own VP-tree (built from its sources
+into oracle/_ref) for the exact workloads, and the numpy restatement of the reference's
+IVFFlatL2Index search over the same clusters for the IVF workload.
 """
 
 import argparse
@@ -30,6 +35,7 @@ METRIC = "QPS @Recall@10>=0.95 (IVF-L2 d768 N1M) + exact L2/Hamming QPS, % roofl
 
 # MI355X published peaks
 PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_INT32_VALU_TOPS = 39.3   # 256 CUs x 64 lanes x 2.4 GHz, one 32-bit op per lane per clock
 
@@ -58,47 +64,212 @@ def l2_workload(n=100_000, nq=1_000, d=32, seed=0):
     return X, Q
 
 
-WORKLOADS = {
-    "hamming": dict(desc="exact Hamming kNN", n=1_000_000, nq=10_000, k=10, width="512-bit", seed=1),
-    "l2": dict(desc="exact L2 kNN float32 U[0,1)", n=100_000, nq=1_000, k=5, width="d=32", seed=0),
-}
+def gmm_workload(n=1_000_000, nq=10_000, d=768, comps=1024, sigma=0.3, seed=2):
+    """Gaussian mixture: centres ~ N(0,1)^d, equal weights, per-component sigma; rows then queries."""
+    rng = np.random.default_rng(seed)
+    centres = rng.standard_normal((comps, d), dtype=np.float32)
+
+    def draw(m):
+        out = np.empty((m, d), np.float32)
+        for s in range(0, m, 65536):
+            e = min(m, s + 65536)
+            lab = rng.integers(0, comps, e - s)
+            out[s:e] = centres[lab] + np.float32(sigma) * rng.standard_normal((e - s, d), dtype=np.float32)
+        return out
+
+    return draw(n), draw(nq)
 
 
-# ---------------------------------------------------------------- CPU baseline
-def cpu_baseline(workload, X, Q, k, gpu_idx):
-    """Reference VP-tree (oracle/_ref) on a bounded query sample; falls back to the C++ port."""
-    sys.path.insert(0, ROOT)
-    import oracle
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    R = oracle.load_ref()
-    sample = 128 if workload == "hamming" else 1000
-    sample = min(sample, len(Q))
-    Qs = Q[:sample]
-    out = {"cores": cores, "unit": "queries/s"}
-    if R is not None:
-        idx = R.VPTreeBinaryIndex512() if workload == "hamming" else R.VPTreeL2Index()
-        t0 = time.perf_counter()
-        idx.set(X)
-        t_build = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        ri, rd = idx.searchKNN(Qs, k)
-        t = time.perf_counter() - t0
-        out.update(value=sample / t, kind="reference",
-                   sample=f"{sample} of {len(Q)} queries vs all {len(X)} rows; reference VPTree "
-                          f"{type(idx).__name__} (OpenMP, {cores} threads), search only; build {t_build:.1f} s excluded")
-        ref_d = np.sort(np.asarray(rd, dtype=np.float64), axis=1)
+# ---------------------------------------------------------------- helpers
+def _time_device(N, hipdev, step, steps, warmup, stream):
+    for _ in range(warmup):
+        step()
+    stream.synchronize()
+    N.stage_timing(True)
+    N.stage_times()
+    ev0, ev1 = hipdev.Event(), hipdev.Event()
+    hipdev.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        step()
+    ev1.record(stream)
+    ev1.synchronize()
+    hipdev.synchronize()
+    wall = time.perf_counter() - t0
+    N.stage_timing(False)
+    stages = N.stage_times()
+    return ev0.elapsed_ms(ev1), wall, stages
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _cores():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def run_exact(workload, steps, warmup, cpu):
+    from pynear_amd import _native as N
+    from pynear_amd import hipdev
+    if workload == "hamming":
+        X, Q = hamming_workload()
+        k, metric, dt = 10, N.METRIC_HAMMING, np.int32
+        desc = f"exact Hamming kNN, N={len(X):,} x 512-bit, Q={len(Q):,}, k={k}"
     else:
-        t0 = time.perf_counter()
-        if workload == "hamming":
-            oi, od = oracle.knn_hamming(X, Qs, k)
+        X, Q = l2_workload()
+        k, metric, dt = 5, N.METRIC_L2, np.float32
+        desc = f"exact L2 kNN float32 U[0,1), N={len(X):,} x d=32, Q={len(Q):,}, k={k}"
+    n, nq = len(X), len(Q)
+    _log(f"{workload}: data {X.shape}")
+    h = N.ExactHandle(metric)
+    h.set(X)
+    dq = hipdev.DeviceArray.from_numpy(Q)
+    oi = hipdev.DeviceArray((nq, k), np.int32)
+    od = hipdev.DeviceArray((nq, k), dt)
+    stream = hipdev.Stream()
+    gpu_ms, wall, stages = _time_device(
+        N, hipdev, lambda: h.search_device(dq.ptr, nq, k, oi.ptr, od.ptr, stream.handle), steps, warmup, stream)
+    scan_ms, scan_n = stages["scan"]
+    scan_s = scan_ms / max(scan_n, 1) / 1e3
+    if workload == "hamming":
+        ops = 2.0 * (X.shape[1] // 4) * nq * n
+        ach = ops / scan_s / 1e12
+        roof = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_INT32_VALU_TOPS, "unit": "TOP/s",
+                "frac": round(ach / PEAK_INT32_VALU_TOPS, 4), "traffic": None,
+                "kernel": "hamming_scan_kernel<16,16,1,4>", "kernel_ms": round(scan_s * 1e3, 4),
+                "work": f"{ops:.3e} int32 ops per launch: 16 words x (xor + popcount-accumulate) x Q x N"}
+    else:
+        flops = 2.0 * X.shape[1] * nq * n
+        ach = flops / scan_s / 1e12
+        roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_BF16_TFLOPS, 4), "traffic": None,
+                "kernel": "l2_mfma_scan_kernel<16,3>", "kernel_ms": round(scan_s * 1e3, 4),
+                "work": f"{flops:.3e} flop per launch (2 d Q N; the kernel issues 3 bf16 products per term)"}
+    out = {"workload": desc, "value": round(nq * steps / (gpu_ms / 1e3), 1), "unit": "queries/s",
+           "ms_per_step": round(gpu_ms / steps, 4), "roofline": roof,
+           "stages_ms_per_step": {s: round(v[0] / steps, 4) for s, v in stages.items() if v[1]}}
+    if cpu:
+        sys.path.insert(0, ROOT)
+        import oracle
+        R = oracle.load_ref()
+        sample = min(128 if workload == "hamming" else 1000, nq)
+        if R is not None:
+            idx = R.VPTreeBinaryIndex512() if workload == "hamming" else R.VPTreeL2Index()
+            t0 = time.perf_counter()
+            idx.set(X)
+            tb = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            _, rd = idx.searchKNN(Q[:sample], k)
+            t = time.perf_counter() - t0
+            kind = "reference"
+            samp = (f"{sample} of {nq} queries vs all {n} rows; reference {type(idx).__name__} (OpenMP, "
+                    f"{_cores()} threads), search only, build {tb:.1f} s excluded")
+            ref_d = np.sort(np.asarray(rd, np.float64), axis=1)
         else:
-            oi, od = oracle.knn_float(oracle.L2, X, Qs, k)
+            t0 = time.perf_counter()
+            _, rd = (oracle.knn_hamming(X, Q[:sample], k) if workload == "hamming"
+                     else oracle.knn_float(oracle.L2, X, Q[:sample], k))
+            t = time.perf_counter() - t0
+            kind, samp = "port", f"{sample} of {nq} queries vs all {n} rows; C++ brute-force restatement"
+            ref_d = np.asarray(rd, np.float64)
+        out["cpu_baseline"] = {"value": round(sample / t, 2), "unit": "queries/s", "cores": _cores(), "kind": kind,
+                               "sample": samp}
+        out["cpu_sample_distances_equal"] = bool(np.array_equal(od.numpy()[:sample].astype(np.float64), ref_d))
+    return out
+
+
+def run_ivf(steps, warmup, cpu, probes=(8, 16, 32)):
+    from pynear_amd import _native as N
+    from pynear_amd import hipdev
+    t0 = time.perf_counter()
+    X, Q = gmm_workload()
+    t_gen = time.perf_counter() - t0
+    _log(f"ivf: generated {X.shape} in {t_gen:.1f} s")
+    n, d = X.shape
+    nq, k, nlist = len(Q), 10, 1024
+    h = N.IvfL2Handle()
+    t0 = time.perf_counter()
+    labels, cent = h.build(X, nlist, 100, 42)
+    t_build = time.perf_counter() - t0
+    _log(f"ivf: built (kmeans + lists) in {t_build:.1f} s")
+    # ground truth: exact GPU search (bit-exact reference arithmetic)
+    ex = N.ExactHandle(N.METRIC_L2)
+    t0 = time.perf_counter()
+    ex.set(X)
+    gt, _ = ex.search(Q, k)
+    t_exact = time.perf_counter() - t0
+    _log(f"ivf: exact ground truth in {t_exact:.1f} s")
+    del ex
+    dq = hipdev.DeviceArray.from_numpy(Q)
+    oi = hipdev.DeviceArray((nq, k), np.int32)
+    od = hipdev.DeviceArray((nq, k), np.float32)
+    oc = hipdev.DeviceArray((nq,), np.int32)
+    stream = hipdev.Stream()
+    sizes = np.bincount(labels, minlength=nlist)
+    sweep = {}
+    for npb in probes:
+        hi, hd, hc = h.search(Q, npb, k)
+        recall = float(np.mean([len(set(hi[i, :hc[i]].tolist()) & set(gt[i].tolist())) / k for i in range(nq)]))
+        gpu_ms, wall, stages = _time_device(
+            N, hipdev, lambda: h.search_device(dq.ptr, nq, npb, k, oi.ptr, od.ptr, oc.ptr, "bf16", stream.handle),
+            steps, warmup, stream)
+        sweep[npb] = {"qps": round(nq * steps / (gpu_ms / 1e3), 1), "ms_per_step": round(gpu_ms / steps, 4),
+                      "recall_at_10": round(recall, 4),
+                      "stages_ms_per_step": {s: round(v[0] / steps, 4) for s, v in stages.items() if v[1]},
+                      "scan_launch_ms": stages["scan"][0] / max(stages["scan"][1], 1)}
+        _log(f"ivf: n_probe={npb} {sweep[npb]}")
+    ok = [p for p in probes if sweep[p]["recall_at_10"] >= 0.95]
+    chosen = ok[0] if ok else probes[-1]
+    S = sweep[chosen]
+    scan_s = S["scan_launch_ms"] / 1e3
+    dp = (d + 15) // 16 * 16
+    # algorithmic work of one scan launch: every probed cluster's bf16 rows once (at this batch
+    # size every cluster is probed), and 2 d flops per (query, probed row)
+    bytes_ = float(n * dp * 2)
+    mean_rows = float((sizes.astype(np.float64) ** 2).sum() / n)  # size-weighted list length
+    flops = 2.0 * d * nq * chosen * mean_rows
+    bw = bytes_ / scan_s / 1e9
+    tf = flops / scan_s / 1e12
+    if bw / PEAK_HBM_GBS >= tf / PEAK_BF16_TFLOPS:
+        roof = {"bound": "hbm", "achieved": round(bw, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(bw / PEAK_HBM_GBS, 4)}
+    else:
+        roof = {"bound": "mfma", "achieved": round(tf, 2), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tf / PEAK_BF16_TFLOPS, 4)}
+    roof.update({"traffic": None, "kernel": "ivfl2_scan_kernel<1>", "kernel_ms": round(scan_s * 1e3, 4),
+                 "work": f"{bytes_:.3e} B (bf16 rows of the probed clusters) and {flops:.3e} flop per launch",
+                 "other_bound": {"hbm_GBs": round(bw, 1), "mfma_TFLOPs": round(tf, 2)}})
+    out = {"workload": (f"IVFFlatL2Index GMM N={n:,} x d={d} (1,024 comps, sigma 0.3), nlist={nlist}, "
+                        f"Q={nq:,}, k={k}, n_probe={chosen} (smallest with Recall@10 >= 0.95)"),
+           "value": S["qps"], "unit": "queries/s", "ms_per_step": S["ms_per_step"], "roofline": roof,
+           "recall_at_10": S["recall_at_10"], "n_probe": chosen,
+           "stages_ms_per_step": S["stages_ms_per_step"],
+           "sweep": {str(p): {kk: v for kk, v in s.items() if kk != "scan_launch_ms"} for p, s in sweep.items()},
+           "setup_s": {"generate": round(t_gen, 1), "build_kmeans_lists": round(t_build, 1),
+                       "exact_ground_truth": round(t_exact, 1)}}
+    if cpu:
+        sys.path.insert(0, ROOT)
+        import oracle
+        data = [X[labels == c] for c in range(nlist)]
+        ids = [np.where(labels == c)[0] for c in range(nlist)]
+        keep = [c for c in range(nlist) if len(ids[c])]
+        data = [data[c] for c in keep]
+        ids = [ids[c] for c in keep]
+        norms = [(x * x).sum(1) for x in data]
+        sample = 100
+        t0 = time.perf_counter()
+        ci, cd = oracle.ivf_l2_search(cent[keep], data, norms, ids, Q[:sample], chosen, k)
         t = time.perf_counter() - t0
-        out.update(value=sample / t, kind="port",
-                   sample=f"{sample} of {len(Q)} queries vs all {len(X)} rows; C++ brute-force restatement "
-                          f"(OpenMP, {cores} threads)")
-        ref_d = np.asarray(od, dtype=np.float64)
-    return out, ref_d
+        out["cpu_baseline"] = {"value": round(sample / t, 2), "unit": "queries/s", "cores": _cores(), "kind": "port",
+                               "sample": (f"{sample} of {nq} queries; numpy restatement of the reference's "
+                                          f"IVFFlatL2Index.searchKNN over the same {len(keep)} clusters at "
+                                          f"n_probe={chosen} (BLAS threads = host cores)")}
+        hi, _, hc = h.search(Q[:sample], chosen, k)
+        out["cpu_sample_recall_vs_port"] = round(float(np.mean(
+            [len(set(hi[i, :hc[i]].tolist()) & set(ci[i])) / k for i in range(sample)])), 4)
+    return out
 
 
 # ---------------------------------------------------------------- main
@@ -107,108 +278,49 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="hamming", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="ivf", choices=["ivf", "hamming", "l2"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true", help="ivf only: skip the exact workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on a single "
               "MI355X; multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
-
+    cpu = not args.no_cpu_baseline
+    if args.workload == "ivf":
+        main_r = run_ivf(args.steps, args.warmup, cpu)
+        dtype = "bf16"
+    else:
+        main_r = run_exact(args.workload, args.steps, args.warmup, cpu)
+        dtype = "u32" if args.workload == "hamming" else "f32"
     from pynear_amd import _native as N
-    from pynear_amd import hipdev
-
-    W = WORKLOADS[args.workload]
-    k = W["k"]
-    if args.workload == "hamming":
-        X, Q = hamming_workload(W["n"], W["nq"], seed=W["seed"])
-        metric = N.METRIC_HAMMING
-        dist_dtype = np.int32
-    else:
-        X, Q = l2_workload(W["n"], W["nq"], seed=W["seed"])
-        metric = N.METRIC_L2
-        dist_dtype = np.float32
-    n, nq = len(X), len(Q)
-
-    h = N.ExactHandle(metric)
-    h.set(X)
-    dq = hipdev.DeviceArray.from_numpy(Q)
-    oi = hipdev.DeviceArray((nq, k), np.int32)
-    od = hipdev.DeviceArray((nq, k), dist_dtype)
-    stream = hipdev.Stream()
-
-    def step():
-        h.search_device(dq.ptr, nq, k, oi.ptr, od.ptr, stream.handle)
-
-    for _ in range(args.warmup):
-        step()
-    stream.synchronize()
-
-    N.stage_timing(True)
-    N.stage_times()
-    ev0, ev1 = hipdev.Event(), hipdev.Event()
-    hipdev.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        step()
-    ev1.record(stream)
-    ev1.synchronize()
-    hipdev.synchronize()
-    wall = time.perf_counter() - t0
-    N.stage_timing(False)
-    stages = N.stage_times()
-    gpu_ms = ev0.elapsed_ms(ev1)
-    ms_per_step = gpu_ms / args.steps
-    qps = nq * args.steps / (gpu_ms / 1e3)
-
-    scan_ms, scan_n = stages["scan"]
-    scan_avg_s = scan_ms / max(scan_n, 1) / 1e3
-    if args.workload == "hamming":
-        words = X.shape[1] // 4
-        ops = 2.0 * words * nq * n          # XOR + popcount-accumulate per 32-bit word per pair
-        achieved = ops / scan_avg_s / 1e12
-        roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT32_VALU_TOPS, "unit": "TOP/s",
-                "frac": round(achieved / PEAK_INT32_VALU_TOPS, 4), "traffic": None,
-                "kernel": "hamming_scan_kernel<16,16,4>", "kernel_ms": round(scan_avg_s * 1e3, 4),
-                "work": f"{ops:.3e} int32 ops per launch (16 words x (xor + popcount-add) x Q x N)"}
-        dtype = "u32"
-    else:
-        d = X.shape[1]
-        flops = 2.0 * d * nq * n
-        achieved = flops / scan_avg_s / 1e12
-        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                "kernel": "float_scan_kernel<32,8,L2,2>", "kernel_ms": round(scan_avg_s * 1e3, 4),
-                "work": f"{flops:.3e} flop per launch (2 x d x Q x N)"}
-        dtype = "f32"
-
     result = {
         "metric": METRIC,
-        "value": round(qps, 1),
+        "value": main_r["value"],
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": main_r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": dtype,
         "data": "synthetic",
-        "config": {"workload": f"{W['desc']}, N={n:,} x {W['width']}, Q={nq:,}, k={k}", "global_batch": nq,
+        "config": {"workload": main_r["workload"], "global_batch": 10_000 if args.workload != "l2" else 1_000,
                    "parallelism": "single GPU"},
-        "roofline": roof,
-        "stages_ms_per_step": {s: round(v[0] / max(args.steps, 1), 4) for s, v in stages.items() if v[1]},
-        "wall_s_timed": round(wall, 4),
-        "tuning": {name: N.tune_get(name) for name in ("hamming.wg_per_cu", "float.wg_per_cu", "merge.wave_per_query")},
+        "roofline": main_r["roofline"],
     }
-    if not args.no_cpu_baseline:
-        got_d = od.numpy().astype(np.float64)
-        cb, ref_d = cpu_baseline(args.workload, X, Q, k, oi)
-        cb["value"] = round(cb["value"], 2)
-        result["cpu_baseline"] = cb
-        result["cpu_sample_distances_equal"] = bool(np.array_equal(got_d[: len(ref_d)], ref_d))
+    for key in ("cpu_baseline", "recall_at_10", "n_probe", "stages_ms_per_step", "sweep", "setup_s",
+                "cpu_sample_recall_vs_port", "cpu_sample_distances_equal"):
+        if key in main_r:
+            result[key] = main_r[key]
+    if args.workload == "ivf" and not args.no_secondary:
+        result["also"] = {"exact_hamming": run_exact("hamming", args.steps, args.warmup, cpu),
+                          "exact_l2": run_exact("l2", max(args.steps, 50), args.warmup, cpu)}
+    result["tuning"] = {name: N.tune_get(name) for name in
+                        ("hamming.wg_per_cu", "hamming.qpl", "float.wg_per_cu", "l2.mfma", "l2.wg_per_cu",
+                         "l2.mfma_terms")}
     print(json.dumps(result))
 
 

@@ -6,8 +6,8 @@
 // Here one search is four device stages:
 //   1. coarse: exact L2 top-nprobe clusters per query (the exact search over the centroids).
 //   2. bucketing: (query, probe) pairs are grouped by cluster and cut into work items of at most
-//      32 queries of one cluster, so every probed cluster is streamed once per search.
-//   3. scan: per item, MFMA tiles of (32 cluster rows x 32 queries) in the GEMM form
+//      64 queries of one cluster; items of the longest lists come first.
+//   3. scan: per item, a GEMM of the item's queries against the cluster's rows in the form
 //      |q|^2 + |x|^2 - 2 q.x on a bf16 copy of the rows (hi only, or hi+lo for 3 products);
 //      each (query, probe) keeps its KP best candidates by the approximation.
 //   4. rerank: per query the K2 = 64 best candidates over all probes are recomputed exactly
@@ -29,8 +29,6 @@
 
 namespace pn {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kIvfKL = 16; // per-lane candidate list
 constexpr int kIvfKP = 32; // candidates kept per (query, probe)
@@ -47,7 +45,8 @@ struct pn_ivfl2 {
     pn::DevBuf X;            // fp32 rows, original order (rerank)
     pn::L2Set lrows;         // bf16 hi/lo of the rows in cluster order + norms
     pn::DevBuf lids, loff;   // cluster-order position -> original id; cluster row offsets [nl + 1]
-    pn::DevBuf qpad, qsplit_h, qsplit_l, qn, probes, pdist, lcount, loffq, cursor, pair_q, pair_p;
+    pn::DevBuf corder;       // cluster ids by decreasing list length (work-item order)
+    pn::DevBuf qpad, qsplit_h, qsplit_l, qn, probes, pdist, lcount, cursor, pair_q, pair_p;
     pn::DevBuf items, ntot, cand, oidx, odist, ocnt;
 };
 
@@ -58,41 +57,43 @@ __global__ void probe_count_kernel(const int32_t *__restrict__ probes, int64_t t
     if (t < total) atomicAdd(&lcount[probes[t]], 1);
 }
 
-// One workgroup: exclusive scans of the per-cluster pair counts and item counts; writes the
-// pair cursor, the items (cluster, first pair, count) and the item total.
-__global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restrict__ lcount, int nl,
-                                                          int32_t *__restrict__ loffq, int32_t *__restrict__ cursor,
-                                                          int4 *__restrict__ items, int32_t *__restrict__ ntot) {
+constexpr int kIvfIQ = 64;  // queries (pairs of one cluster) per work item
+constexpr int kIvfKC = 64;  // dims per K chunk
+constexpr int kIvfQS = 72;  // LDS row stride (bf16) of a staged query chunk: 16 B of pad spreads the banks
+
+// One workgroup: the work items, (cluster, first pair, count <= 64), cluster by cluster in the
+// order corder (lists by decreasing length, so the longest items are dispatched first), and per
+// cluster the first pair slot (pair cursor). Writes the item count to ntot.
+__global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restrict__ lcount,
+                                                          const int32_t *__restrict__ corder, int nl,
+                                                          int32_t *__restrict__ cursor, int4 *__restrict__ items,
+                                                          int32_t *__restrict__ ntot) {
     __shared__ int32_t s_pairs[256], s_items[256];
     const int tid = threadIdx.x;
     const int per = (nl + 255) / 256;
-    const int c0 = tid * per, c1 = min(nl, c0 + per);
-    int pairs = 0, its = 0;
-    for (int c = c0; c < c1; ++c) {
-        pairs += lcount[c];
-        its += (lcount[c] + 31) / 32;
+    const int o0 = min(nl, tid * per), o1 = min(nl, o0 + per);
+    int pairs = 0, nit = 0;
+    for (int o = o0; o < o1; ++o) {
+        const int m = lcount[corder[o]];
+        pairs += m;
+        nit += (m + kIvfIQ - 1) / kIvfIQ;
     }
     s_pairs[tid] = pairs;
-    s_items[tid] = its;
+    s_items[tid] = nit;
     __syncthreads();
-    if (tid == 0) {
-        int a = 0, b = 0;
-        for (int i = 0; i < 256; ++i) {
-            const int pa = s_pairs[i], pb = s_items[i];
-            s_pairs[i] = a;
-            s_items[i] = b;
-            a += pa;
-            b += pb;
-        }
-        *ntot = b;
+    for (int off = 1; off < 256; off <<= 1) { // inclusive scans
+        const int a = tid >= off ? s_pairs[tid - off] : 0, b = tid >= off ? s_items[tid - off] : 0;
+        __syncthreads();
+        s_pairs[tid] += a;
+        s_items[tid] += b;
+        __syncthreads();
     }
-    __syncthreads();
-    int pa = s_pairs[tid], ib = s_items[tid];
-    for (int c = c0; c < c1; ++c) {
-        const int m = lcount[c];
-        loffq[c] = pa;
+    if (tid == 255) *ntot = s_items[255];
+    int pa = s_pairs[tid] - pairs, ia = s_items[tid] - nit;
+    for (int o = o0; o < o1; ++o) {
+        const int c = corder[o], m = lcount[c];
         cursor[c] = pa;
-        for (int s = 0; s < m; s += 32) items[ib++] = make_int4(c, pa + s, min(32, m - s), 0);
+        for (int s = 0; s < m; s += kIvfIQ) items[ia++] = make_int4(c, pa + s, min(kIvfIQ, m - s), 0);
         pa += m;
     }
 }
@@ -108,130 +109,228 @@ __global__ void probe_scatter_kernel(const int32_t *__restrict__ probes, int64_t
     }
 }
 
+// Inserts the 16 keys of one query (rows row(r)) into the lane's sorted list (Lk, Li) of KL; the
+// loop runs while any lane of the wave still has a key below its list's last entry.
+template <int KL, typename RowOf>
+__device__ __forceinline__ void list_insert16(uint32_t (&key)[16], uint32_t (&Lk)[KL], uint32_t (&Li)[KL],
+                                              RowOf row) {
+    uint32_t thr = Lk[KL - 1];
+    uint32_t mn = kKey32None;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) mn = min(mn, key[r]);
+    while (__any(mn < thr)) {
+        if (mn < thr) {
+            uint32_t ir = 0;
+            bool found = false;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const bool hit = !found && key[r] == mn;
+                ir = hit ? row(r) : ir;
+                key[r] = hit ? kKey32None : key[r];
+                found = found || hit;
+            }
+#pragma unroll
+            for (int j = KL - 1; j > 0; --j) {
+                const bool up = mn < Lk[j - 1];
+                const bool here = mn < Lk[j];
+                Li[j] = up ? Li[j - 1] : (here ? ir : Li[j]);
+                Lk[j] = max(Lk[j - 1], min(mn, Lk[j]));
+            }
+            if (mn < Lk[0]) {
+                Li[0] = ir;
+                Lk[0] = mn;
+            }
+            thr = Lk[KL - 1];
+        }
+        mn = kKey32None;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) mn = min(mn, key[r]);
+    }
+}
+
+// The list scan. One workgroup (4 waves) per work item: up to 64 queries of one cluster against
+// all rows of that cluster. Rows go in blocks of 256 (64 per wave, two 32-row MFMA tiles); the
+// queries are two 32-query tiles, so a wave holds 2 x 2 accumulators of 32 x 32. The K loop runs
+// in chunks of 64 dims: the query chunk is staged in LDS (two buffers, the next chunk is loaded
+// into registers during the current chunk's products), the wave's row fragments are read straight
+// from HBM/L2 into registers, issued for the next chunk right after the current chunk's products
+// (for the next row block's first chunk: after this row block's keys).
+// dp is a multiple of the chunk (64). After a
+// row block, lane (c, h) of wave w turns its 32 accumulators of query c (+ 32 j) into keys
+// |q|^2 + |x|^2 - 2 q.x and keeps the KL smallest (key, row position) per query; at the end the
+// 8 lists (4 waves x 2 halves) of each query are merged into its KP best for this probe.
 template <int PREC>
-__global__ __launch_bounds__(256) void ivfl2_scan_kernel(const __bf16 *__restrict__ Lh, const __bf16 *__restrict__ Ll,
-                                                         const float *__restrict__ lxn, const int64_t *__restrict__ loff,
-                                                         int dp, const __bf16 *__restrict__ Qh,
-                                                         const __bf16 *__restrict__ Ql, const float *__restrict__ qn,
-                                                         const int4 *__restrict__ items, const int32_t *__restrict__ ntot,
-                                                         const int32_t *__restrict__ pair_q,
-                                                         const int32_t *__restrict__ pair_p, int np,
-                                                         uint64_t *__restrict__ cand) {
-    constexpr int KL = kIvfKL, KP = kIvfKP;
-    __shared__ uint64_t lds[8][32][KL];
-    __shared__ int ptr[8][32];
-    if ((int)blockIdx.x >= *ntot) return; // grid is an upper bound on the item count
+__global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
+    const __bf16 *__restrict__ Lh, const __bf16 *__restrict__ Ll, const float *__restrict__ lxn,
+    const int64_t *__restrict__ loff, int dp, const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
+    const float *__restrict__ qn, const int4 *__restrict__ items, const int32_t *__restrict__ ntot,
+    const int32_t *__restrict__ pair_q, const int32_t *__restrict__ pair_p, int np, uint64_t *__restrict__ cand) {
+    constexpr int KL = kIvfKL, KP = kIvfKP, QS = kIvfQS, IQ = kIvfIQ;
+    constexpr int NPL = PREC == 3 ? 2 : 1;      // staged planes: hi (and lo)
+    constexpr int PLANE = IQ * QS;              // bf16 elements of one staged plane
+    constexpr int BUF = PLANE * NPL;            // one query-chunk buffer
+    constexpr int SCAN_BYTES = 2 * BUF * 2;
+    constexpr int LIST_BYTES = 8 * IQ * KL * 8;
+    __shared__ __align__(16) unsigned char smem[SCAN_BYTES > LIST_BYTES ? SCAN_BYTES : LIST_BYTES];
+    __shared__ int32_t s_q[IQ];
+    __bf16 *sq = (__bf16 *)smem;
+
+    if ((int)blockIdx.x >= *ntot) return; // the grid is an upper bound on the item count
     const int4 it = items[blockIdx.x];
     const int c = it.x, start = it.y, cnt = it.z;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
-    const int pj = start + min(col, cnt - 1);
-    const int64_t q = pair_q[pj];
-    const int p = pair_p[pj];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5, col = lane & 31;
+    if (tid < IQ) s_q[tid] = pair_q[start + min(tid, cnt - 1)];
+    __syncthreads();
     const int64_t r0 = loff[c], r1 = loff[c + 1];
-    const float qnv = qn[q];
-    const __bf16 *qh = Qh + q * dp + 8 * half;
-    const __bf16 *ql = Ql + q * dp + 8 * half;
+    const int nch = dp / kIvfKC;
+    const int nsteps = (int)((r1 - r0 + 255) / 256) * nch;
+    // staging role: 16-byte units tid and tid + 256 -> (query su, su + 32; dims 8 sp .. 8 sp + 7)
+    const int su = tid >> 3, sp = tid & 7;
+    const int64_t sg0 = (int64_t)s_q[su] * dp + 8 * sp, sg1 = (int64_t)s_q[su + 32] * dp + 8 * sp;
+    const float qn0 = qn[s_q[col]], qn1 = qn[s_q[col + 32]];
 
-    uint32_t Lk[KL], Li[KL];
+    bf16x8 qreg[2][NPL];
+    auto load_q = [&](int kc) {
 #pragma unroll
-    for (int j = 0; j < KL; ++j) {
-        Lk[j] = kKey32None;
-        Li[j] = 0u;
-    }
-    uint32_t thr = kKey32None;
+        for (int pl = 0; pl < NPL; ++pl) {
+            const __bf16 *src = pl == 0 ? Qh : Ql;
+            qreg[0][pl] = *(const bf16x8 *)(src + sg0 + kc);
+            qreg[1][pl] = *(const bf16x8 *)(src + sg1 + kc);
+        }
+    };
+    auto store_q = [&](int buf) {
+#pragma unroll
+        for (int pl = 0; pl < NPL; ++pl) {
+            *(bf16x8 *)(sq + buf * BUF + pl * PLANE + su * QS + 8 * sp) = qreg[0][pl];
+            *(bf16x8 *)(sq + buf * BUF + pl * PLANE + (su + 32) * QS + 8 * sp) = qreg[1][pl];
+        }
+    };
+    // row fragments of (row block rb, chunk ch): rows r0 + 256 rb + 64 w + 32 i + col (clamped
+    // into the list), dims 64 ch + 16 s + 8 half
+    bf16x8 acur[2][4][NPL];
+    auto load_a = [&](int rb, int ch, bf16x8 (&a)[2][4][NPL]) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int64_t row = min(r0 + 256 * rb + 64 * w + 32 * i + col, r1 - 1);
+#pragma unroll
+            for (int pl = 0; pl < NPL; ++pl) {
+                const __bf16 *src = (pl == 0 ? Lh : Ll) + row * dp + ch * kIvfKC + 8 * half;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) a[i][s][pl] = *(const bf16x8 *)(src + 16 * s);
+            }
+        }
+    };
 
-    // Row arrays carry 32 zero rows past the end, so a tile that runs past r1 stays in bounds.
-    for (int64_t t = r0 + 32 * w; t < r1; t += 128) {
-        float xr[16];
+    uint32_t Lk[2][KL], Li[2][KL];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) xr[r] = lxn[t + 8 * (r >> 2) + 4 * half + (r & 3)];
-        const __bf16 *xh = Lh + (t + col) * dp + 8 * half;
-        const __bf16 *xl = Ll + (t + col) * dp + 8 * half;
-        f32x16 acc = {};
-        bf16x8 ah = *(const bf16x8 *)xh, bh = *(const bf16x8 *)qh, al = {}, bl = {};
-        if (PREC == 3) {
-            al = *(const bf16x8 *)xl;
-            bl = *(const bf16x8 *)ql;
-        }
-        for (int kk = 0; kk < dp; kk += 16) {
-            bf16x8 nah = ah, nbh = bh, nal = al, nbl = bl;
-            if (kk + 16 < dp) {
-                nah = *(const bf16x8 *)(xh + kk + 16);
-                nbh = *(const bf16x8 *)(qh + kk + 16);
-                if (PREC == 3) {
-                    nal = *(const bf16x8 *)(xl + kk + 16);
-                    nbl = *(const bf16x8 *)(ql + kk + 16);
-                }
-            }
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
-            if (PREC == 3) {
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
-            }
-            ah = nah;
-            bh = nbh;
-            al = nal;
-            bl = nbl;
-        }
-        uint32_t key[16];
-        uint32_t mn = kKey32None;
+    for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int64_t i = t + 8 * (r >> 2) + 4 * half + (r & 3);
-            const float v = fmaxf(0.f, (qnv + xr[r]) - 2.f * acc[r]);
-            key[r] = i < r1 ? f32_key(v) : kKey32None;
-            mn = min(mn, key[r]);
+        for (int e = 0; e < KL; ++e) {
+            Lk[j][e] = kKey32None;
+            Li[j][e] = 0u;
         }
-        while (__any(mn < thr)) {
-            if (mn < thr) {
-                uint32_t ir = 0;
-                bool found = false;
+    load_q(0);
+    store_q(0);
+    load_a(0, 0, acur);
+    __syncthreads();
+    const int nrb = (int)((r1 - r0 + 255) / 256);
+    int g = 0;
+    for (int rb = 0; rb < nrb; ++rb) {
+        f32x16 acc[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = f32x16{};
+        for (int ch = 0; ch < nch; ++ch, ++g) {
+            const int buf = g & 1;
+            const bool more = g + 1 < nsteps;
+            const int ch1 = ch + 1 == nch ? 0 : ch + 1;
+            if (more) load_q(ch1 * kIvfKC);
+            const __bf16 *qb = sq + buf * BUF;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                bf16x8 b[2][NPL];
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int pl = 0; pl < NPL; ++pl)
+                        b[j][pl] = *(const bf16x8 *)(qb + pl * PLANE + (32 * j + col) * QS + 16 * s + 8 * half);
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        acc[i][j] =
+                            __builtin_amdgcn_mfma_f32_32x32x16_bf16(acur[i][s][0], b[j][0], acc[i][j], 0, 0, 0);
+                        if (PREC == 3) {
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(acur[i][s][0], b[j][NPL - 1],
+                                                                                acc[i][j], 0, 0, 0);
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(acur[i][s][NPL - 1], b[j][0],
+                                                                                acc[i][j], 0, 0, 0);
+                        }
+                    }
+            }
+            if (more) store_q(buf ^ 1);
+            if (ch + 1 < nch) load_a(rb, ch + 1, acur); // the next row block's first chunk: after the keys
+            __syncthreads();
+        }
+        // row block done: keys |q|^2 + |x|^2 - 2 q.x and list insertion
+        const int64_t rbase = r0 + 256 * rb + 64 * w;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            float xr[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) xr[r] = lxn[min(rbase + 32 * i + 8 * (r >> 2) + 4 * half + (r & 3), r1 - 1)];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const float qv = j == 0 ? qn0 : qn1;
+                uint32_t key[16];
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    const bool hit = !found && key[r] == mn;
-                    ir = hit ? (uint32_t)(t + 8 * (r >> 2) + 4 * half + (r & 3)) : ir;
-                    key[r] = hit ? kKey32None : key[r];
-                    found = found || hit;
+                    const int64_t row = rbase + 32 * i + 8 * (r >> 2) + 4 * half + (r & 3);
+                    const float v = fmaxf(0.f, (qv + xr[r]) - 2.f * acc[i][j][r]);
+                    key[r] = row < r1 ? f32_key(v) : kKey32None;
                 }
-#pragma unroll
-                for (int j = KL - 1; j > 0; --j) {
-                    const bool up = mn < Lk[j - 1];
-                    const bool here = mn < Lk[j];
-                    Li[j] = up ? Li[j - 1] : (here ? ir : Li[j]);
-                    Lk[j] = max(Lk[j - 1], min(mn, Lk[j]));
-                }
-                if (mn < Lk[0]) {
-                    Li[0] = ir;
-                    Lk[0] = mn;
-                }
-                thr = Lk[KL - 1];
+                list_insert16<KL>(key, Lk[j], Li[j], [&](int r) {
+                    return (uint32_t)(rbase + 32 * i + 8 * (r >> 2) + 4 * half + (r & 3));
+                });
             }
-            mn = kKey32None;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) mn = min(mn, key[r]);
         }
+        if (rb + 1 < nrb) load_a(rb + 1, 0, acur);
     }
+    // candidate lists -> LDS (the query buffers are no longer read), then one lane per query merges
+    uint64_t *sl = (uint64_t *)smem; // [8 lists][64 queries][KL]
 #pragma unroll
-    for (int j = 0; j < KL; ++j)
-        lds[w * 2 + half][col][j] = Lk[j] == kKey32None ? kKey64None : (((uint64_t)Lk[j] << 32) | Li[j]);
-    if (threadIdx.x < 256 && lane < 32) ptr[w * 2][col] = 0, ptr[w * 2 + 1][col] = 0;
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int e = 0; e < KL; ++e)
+            sl[((w * 2 + half) * IQ + 32 * j + col) * KL + e] =
+                Lk[j][e] == kKey32None ? kKey64None : (((uint64_t)Lk[j][e] << 32) | Li[j][e]);
     __syncthreads();
-    if (threadIdx.x < 32 && col < cnt) { // 8-way merge of this query's lists -> its KP best for this probe
-        uint64_t *out = cand + ((int64_t)q * np + p) * KP;
+    if (tid < cnt) {
+        uint64_t *out = cand + ((int64_t)s_q[tid] * np + pair_p[start + tid]) * KP;
+        int p[8];
+        uint64_t head[8];
+#pragma unroll
+        for (int l = 0; l < 8; ++l) {
+            p[l] = 0;
+            head[l] = sl[(l * IQ + tid) * KL];
+        }
         for (int e = 0; e < KP; ++e) {
             int best = 0;
-            uint64_t bv = kKey64None;
+            uint64_t bv = head[0];
 #pragma unroll
-            for (int l = 0; l < 8; ++l) {
-                const int pl = ptr[l][col];
-                const uint64_t v = pl < KL ? lds[l][col][pl] : kKey64None;
-                if (v < bv) {
-                    bv = v;
+            for (int l = 1; l < 8; ++l)
+                if (head[l] < bv) {
+                    bv = head[l];
                     best = l;
                 }
-            }
-            if (bv != kKey64None) ptr[best][col] += 1;
             out[e] = bv;
+#pragma unroll
+            for (int l = 0; l < 8; ++l)
+                if (l == best && bv != kKey64None) {
+                    p[l] += 1;
+                    head[l] = p[l] < KL ? sl[(l * IQ + tid) * KL + p[l]] : kKey64None;
+                }
         }
     }
 }
@@ -245,8 +344,36 @@ __device__ __forceinline__ uint64_t wmin64(uint64_t v) {
     return v;
 }
 
-// One wave per query: the K2 best candidates by the approximation, exact squared L2 for them,
-// the k smallest (distance, id) out.
+// Squared-sum core of dist_l2_f_avx2 computed by the 8 lanes of a group (lane c runs FMA chain
+// c; fold, fused tails and horizontal adds as in the reference); valid in lane 0 of the group.
+// Each group reads 8 consecutive floats per step, so a wave's loads coalesce in 32-byte pieces.
+__device__ __forceinline__ float l2_core_g8(const float *a, const float *b, int64_t d, int c) {
+    const int64_t nb = d / 8 * 8;
+    float acc = 0.f;
+    for (int64_t i = c; i < nb; i += 8) {
+        const float t = a[i] - b[i];
+        acc = fmaf(t, t, acc);
+    }
+    float m = __shfl_down(acc, 4, 8) + acc;
+    if (c < 4) {
+        int64_t i = nb, rem = d - nb;
+        if (rem >= 4) {
+            const float t = a[i + c] - b[i + c];
+            m = fmaf(t, t, m);
+            i += 4;
+            rem -= 4;
+        }
+        if (rem > 0) {
+            const float t = c < rem ? a[i + c] - b[i + c] : 0.f;
+            m = fmaf(t, t, m);
+        }
+    }
+    const float p = m + __shfl_down(m, 1, 8);
+    return p + __shfl_down(p, 2, 8);
+}
+
+// One wave per query: the K2 best candidates by the approximation, exact squared L2 for them
+// (eight 8-lane groups, eight candidates per round), the k smallest (distance, id) out.
 template <int PER>
 __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restrict__ X, int64_t d,
                                                            const int32_t *__restrict__ lids, const float *__restrict__ Q,
@@ -275,11 +402,19 @@ __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restri
         }
         if (lane == r) mine = m;
     }
+    const int32_t myid = mine == kKey64None ? -1 : lids[(uint32_t)(mine & 0xFFFFFFFFull)];
+    const float *qrow = Q + q * qstride;
     uint64_t key = kKey64None;
-    if (mine != kKey64None) {
-        const int32_t id = lids[(uint32_t)(mine & 0xFFFFFFFFull)];
-        const float core = l2_core(Q + q * qstride, X + (int64_t)id * d, d);
-        key = ((uint64_t)f32_key(core) << 32) | (uint32_t)id;
+    const int g = lane >> 3, c = lane & 7;
+#pragma unroll 1
+    for (int round = 0; round < 8; ++round) {
+        // group g computes candidate 8*round + g; its core lands in the group's lane 0
+        const int32_t id = __shfl(myid, 8 * round + g, 64);
+        float core = 0.f;
+        if (id >= 0) core = l2_core_g8(qrow, X + (int64_t)id * d, d, c);
+        const bool owner = lane >= 8 * round && lane < 8 * round + 8;
+        const float cv = __shfl(core, owner ? 8 * (lane - 8 * round) : 0, 64);
+        if (owner && myid >= 0) key = ((uint64_t)f32_key(cv) << 32) | (uint32_t)myid;
     }
     int cnt = 0;
     for (int r = 0; r < k; ++r) {
@@ -340,11 +475,17 @@ static void ivfl2_set_device(pn_ivfl2 *h, const float *dX, int64_t n, int64_t d,
     int64_t *dof = (int64_t *)h->loff.ensure((size_t)(nl + 1) * 8);
     PN_HIP(hipMemcpyAsync(dl, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
     PN_HIP(hipMemcpyAsync(dof, off.data(), (size_t)(nl + 1) * 8, hipMemcpyHostToDevice, st));
+    std::vector<int32_t> order(nl);
+    for (int32_t j = 0; j < nl; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
+    int32_t *dco = (int32_t *)h->corder.ensure((size_t)nl * 4);
+    PN_HIP(hipMemcpyAsync(dco, order.data(), (size_t)nl * 4, hipMemcpyHostToDevice, st));
     // rows in cluster order -> bf16 split (+ zero rows past the end, see the scan kernel)
     DevBuf ordered;
     float *Xo = (float *)ordered.ensure((size_t)n * d * 4);
     hipLaunchKernelGGL(gather_rows_f32_kernel, dim3((unsigned)ceil_div(n * d, 256)), dim3(256), 0, st, X, d, dl, n, Xo);
-    l2_prepare(h->lrows, Xo, n, d, d, st);
+    l2_prepare(h->lrows, Xo, n, d, d, st, kIvfKC); // whole K chunks: the scan has no dim tail
     // coarse index over the kept centroids
     bool scan_ok = false;
     const int64_t re = padded_row_elems(PN_METRIC_L2, d, &scan_ok);
@@ -391,11 +532,11 @@ static void ivfl2_search_enqueue(pn_ivfl2 *h, const float *dq, int64_t qstride_i
     float *qn = (float *)h->qn.ensure((size_t)nq * 4);
     const int64_t total = nq * np;
     int32_t *lc = (int32_t *)h->lcount.ensure((size_t)h->nl * 4);
-    int32_t *lo = (int32_t *)h->loffq.ensure((size_t)h->nl * 4);
     int32_t *cur = (int32_t *)h->cursor.ensure((size_t)h->nl * 4);
     int32_t *pq = (int32_t *)h->pair_q.ensure((size_t)total * 4);
     int32_t *pp = (int32_t *)h->pair_p.ensure((size_t)total * 4);
-    const int64_t max_items = ceil_div(total, 32) + h->nl;
+    // each cluster has at most one partly filled item
+    const int64_t max_items = ceil_div(total, kIvfIQ) + h->nl;
     int4 *items = (int4 *)h->items.ensure((size_t)max_items * 16);
     int32_t *ntot = (int32_t *)h->ntot.ensure(4);
     uint64_t *cand = (uint64_t *)h->cand.ensure((size_t)total * kIvfKP * 8);
@@ -404,7 +545,8 @@ static void ivfl2_search_enqueue(pn_ivfl2 *h, const float *dq, int64_t qstride_i
         l2_split_rows(Q, nq, d, re, dp, Qh, Ql, qn, st);
         PN_HIP(hipMemsetAsync(lc, 0, (size_t)h->nl * 4, st));
         hipLaunchKernelGGL(probe_count_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, st, pr, total, lc);
-        hipLaunchKernelGGL(build_items_kernel, dim3(1), dim3(256), 0, st, lc, h->nl, lo, cur, items, ntot);
+        hipLaunchKernelGGL(build_items_kernel, dim3(1), dim3(256), 0, st, lc, h->corder.as<int32_t>(), h->nl, cur,
+                           items, ntot);
         hipLaunchKernelGGL(probe_scatter_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, st, pr, total, np,
                            cur, pq, pp);
     }

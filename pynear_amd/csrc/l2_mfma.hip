@@ -27,8 +27,6 @@
 
 namespace pn {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __global__ __launch_bounds__(256) void l2_split_kernel(const float *__restrict__ X, int64_t n, int64_t d,
                                                        int64_t stride, int dp, __bf16 *__restrict__ H,
@@ -99,31 +97,7 @@ __global__ __launch_bounds__(256) void l2_mfma_scan_kernel(const __bf16 *__restr
         const __bf16 *xh = Xh + (t + col) * dp + 8 * half;
         const __bf16 *xl = Xl + (t + col) * dp + 8 * half;
         f32x16 acc = {};
-        bf16x8 ah = *(const bf16x8 *)xh, bh = *(const bf16x8 *)qh, al = {}, bl = {};
-        if (PREC == 3) {
-            al = *(const bf16x8 *)xl;
-            bl = *(const bf16x8 *)ql;
-        }
-        for (int kk = 0; kk < dp; kk += 16) {
-            bf16x8 nah = ah, nbh = bh, nal = al, nbl = bl;
-            if (kk + 16 < dp) { // next step's fragments in flight during this step's products
-                nah = *(const bf16x8 *)(xh + kk + 16);
-                nbh = *(const bf16x8 *)(qh + kk + 16);
-                if (PREC == 3) {
-                    nal = *(const bf16x8 *)(xl + kk + 16);
-                    nbl = *(const bf16x8 *)(ql + kk + 16);
-                }
-            }
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
-            if (PREC == 3) {
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
-            }
-            ah = nah;
-            bh = nbh;
-            al = nal;
-            bl = nbl;
-        }
+        mfma_tile_bf16<PREC, PREC == 3 ? 4 : 8>(acc, xh, xl, qh, ql, dp);
         uint32_t key[16];
         uint32_t mn = kKey32None;
 #pragma unroll
@@ -299,10 +273,10 @@ __global__ __launch_bounds__(256) void l2_fallback_kernel(const float *__restric
 }
 
 // ---------------------------------------------------------------- host side
-void l2_prepare(L2Set &s, const float *X, int64_t n, int64_t d, int64_t stride, hipStream_t st) {
+void l2_prepare(L2Set &s, const float *X, int64_t n, int64_t d, int64_t stride, hipStream_t st, int align) {
     s.n = n;
     s.d = d;
-    s.dp = (int)ceil_div(d, 16) * 16;
+    s.dp = (int)(ceil_div(d, (int64_t)align) * align);
     // 32 zero rows past the last full tile: tiles that start at any row < n stay in bounds
     const int64_t npad = ceil_div(n, 32) * 32 + 32;
     __bf16 *H = (__bf16 *)s.hi.ensure((size_t)npad * s.dp * 2);

@@ -70,22 +70,30 @@ struct StageRec {
     hipEvent_t b, e;
 };
 static std::vector<StageRec> g_stage_recs;
-static hipEvent_t g_open[STAGE_COUNT] = {};
+static hipEvent_t g_open_ev = nullptr;
+static int g_open_stage = -1;
+static int g_depth = 0; // nested scopes belong to the outermost open stage
 
 void stage_mark(hipStream_t st, int stage, bool begin) {
     if (!g_stage_on) return;
     std::lock_guard<std::mutex> lk(g_stage_mu);
+    if (begin) {
+        if (g_depth++ > 0) return;
+        hipEvent_t ev;
+        if (hipEventCreate(&ev) != hipSuccess) return;
+        (void)hipEventRecord(ev, st);
+        g_open_ev = ev;
+        g_open_stage = stage;
+        return;
+    }
+    if (g_depth == 0 || --g_depth > 0) return;
+    if (!g_open_ev) return;
     hipEvent_t ev;
     if (hipEventCreate(&ev) != hipSuccess) return;
     (void)hipEventRecord(ev, st);
-    if (begin) {
-        g_open[stage] = ev;
-    } else if (g_open[stage]) {
-        g_stage_recs.push_back({stage, g_open[stage], ev});
-        g_open[stage] = nullptr;
-    } else {
-        (void)hipEventDestroy(ev);
-    }
+    g_stage_recs.push_back({g_open_stage, g_open_ev, ev});
+    g_open_ev = nullptr;
+    g_open_stage = -1;
 }
 
 } // namespace pn
