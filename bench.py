@@ -283,8 +283,8 @@ def main():
     ap.add_argument("--no-secondary", action="store_true", help="ivf only: skip the exact workloads")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on a single "
-              "MI355X; multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the search path runs on a single "
+              "MI355X and the multi-GPU sharding is a design, not built", file=sys.stderr)
         sys.exit(2)
     cpu = not args.no_cpu_baseline
     if args.workload == "ivf":

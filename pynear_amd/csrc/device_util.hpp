@@ -45,6 +45,24 @@ __device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t acc) {
     return r;
 }
 
+// Ascending bitonic sorting network over 16 keys held in registers (80 compare-exchanges).
+__device__ __forceinline__ void sort16_u32(uint32_t (&k)[16]) {
+#pragma unroll
+    for (int size = 2; size <= 16; size <<= 1)
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const uint32_t lo = min(k[i], k[j]), hi = max(k[i], k[j]);
+                    const bool asc = (i & size) == 0;
+                    k[i] = asc ? lo : hi;
+                    k[j] = asc ? hi : lo;
+                }
+            }
+}
+
 __device__ __forceinline__ uint32_t f32_key(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float key_f32(uint32_t u) { return __uint_as_float(u); }
 

@@ -17,6 +17,7 @@
 // clusters are left out, as the reference's per-cluster arrays leave them out (forest.py:51-59).
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -52,9 +53,26 @@ struct pn_ivfl2 {
 
 namespace pn {
 
-__global__ void probe_count_kernel(const int32_t *__restrict__ probes, int64_t total, int32_t *__restrict__ lcount) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < total) atomicAdd(&lcount[probes[t]], 1);
+// Pairs are bucketed by cluster in chunks of kPairChunk per workgroup: with at most kLdsBins
+// clusters the counts of a chunk are gathered in LDS first, so a cluster that many queries probe
+// takes one global atomic per chunk instead of one per pair.
+constexpr int kPairChunk = 1024;
+constexpr int kLdsBins = 8192;
+
+__global__ __launch_bounds__(256) void probe_count_kernel(const int32_t *__restrict__ probes, int64_t total, int nl,
+                                                          int32_t *__restrict__ lcount) {
+    __shared__ int32_t hist[kLdsBins];
+    const int64_t c0 = (int64_t)blockIdx.x * kPairChunk, c1 = min(total, c0 + kPairChunk);
+    if (nl > kLdsBins) {
+        for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) atomicAdd(&lcount[probes[t]], 1);
+        return;
+    }
+    for (int b = threadIdx.x; b < nl; b += 256) hist[b] = 0;
+    __syncthreads();
+    for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) atomicAdd(&hist[probes[t]], 1);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nl; b += 256)
+        if (hist[b]) atomicAdd(&lcount[b], hist[b]);
 }
 
 constexpr int kIvfIQ = 64;  // queries (pairs of one cluster) per work item
@@ -98,37 +116,62 @@ __global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restr
     }
 }
 
-__global__ void probe_scatter_kernel(const int32_t *__restrict__ probes, int64_t total, int np,
-                                     int32_t *__restrict__ cursor, int32_t *__restrict__ pair_q,
-                                     int32_t *__restrict__ pair_p) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < total) {
-        const int pos = atomicAdd(&cursor[probes[t]], 1);
+// Writes each pair (query, probe rank) to its cluster's next free slot. Per chunk, one global
+// atomic per cluster reserves the chunk's slots (LDS path); pairs then take ranks inside them.
+__global__ __launch_bounds__(256) void probe_scatter_kernel(const int32_t *__restrict__ probes, int64_t total, int np,
+                                                            int nl, int32_t *__restrict__ cursor,
+                                                            int32_t *__restrict__ pair_q, int32_t *__restrict__ pair_p) {
+    __shared__ int32_t hist[kLdsBins];
+    const int64_t c0 = (int64_t)blockIdx.x * kPairChunk, c1 = min(total, c0 + kPairChunk);
+    if (nl > kLdsBins) {
+        for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) {
+            const int pos = atomicAdd(&cursor[probes[t]], 1);
+            pair_q[pos] = (int32_t)(t / np);
+            pair_p[pos] = (int32_t)(t % np);
+        }
+        return;
+    }
+    for (int b = threadIdx.x; b < nl; b += 256) hist[b] = 0;
+    __syncthreads();
+    for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) atomicAdd(&hist[probes[t]], 1);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nl; b += 256) // count -> first global slot of this chunk's run
+        if (hist[b]) hist[b] = atomicAdd(&cursor[b], hist[b]);
+    __syncthreads();
+    for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) {
+        const int pos = atomicAdd(&hist[probes[t]], 1);
         pair_q[pos] = (int32_t)(t / np);
         pair_p[pos] = (int32_t)(t % np);
     }
 }
 
-// Inserts the 16 keys of one query (rows row(r)) into the lane's sorted list (Lk, Li) of KL; the
-// loop runs while any lane of the wave still has a key below its list's last entry.
-template <int KL, typename RowOf>
+// Inserts one tile's 16 keys of one query into the lane's sorted list (Lk, Li) of KL. A key is
+// the approximation's float bits with the low 5 bits replaced by the row's offset in the tile, so
+// a minimum names its row (base + (key & 31)). Keys at or above thr are dropped; thr may be below
+// the list's last key when it comes from another list of the same query (the lists together
+// then still hold KL keys below it). The loop runs while any lane of the wave has a key below
+// its thr. FIRST: the list is empty and takes the tile's keys through a sorting network.
+template <int KL, bool FIRST>
 __device__ __forceinline__ void list_insert16(uint32_t (&key)[16], uint32_t (&Lk)[KL], uint32_t (&Li)[KL],
-                                              RowOf row) {
-    uint32_t thr = Lk[KL - 1];
+                                              uint32_t base, uint32_t &thr) {
+    if (FIRST) {
+        sort16_u32(key);
+#pragma unroll
+        for (int j = 0; j < KL; ++j) {
+            Lk[j] = j < 16 && key[j] < thr ? key[j] : kKey32None;
+            Li[j] = j < 16 ? base + (key[j] & 31u) : 0u;
+        }
+        thr = min(thr, Lk[KL - 1]);
+        return;
+    }
     uint32_t mn = kKey32None;
 #pragma unroll
     for (int r = 0; r < 16; ++r) mn = min(mn, key[r]);
     while (__any(mn < thr)) {
         if (mn < thr) {
-            uint32_t ir = 0;
-            bool found = false;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const bool hit = !found && key[r] == mn;
-                ir = hit ? row(r) : ir;
-                key[r] = hit ? kKey32None : key[r];
-                found = found || hit;
-            }
+            for (int r = 0; r < 16; ++r) key[r] = key[r] == mn ? kKey32None : key[r];
+            const uint32_t ir = base + (mn & 31u);
 #pragma unroll
             for (int j = KL - 1; j > 0; --j) {
                 const bool up = mn < Lk[j - 1];
@@ -140,7 +183,7 @@ __device__ __forceinline__ void list_insert16(uint32_t (&key)[16], uint32_t (&Lk
                 Li[0] = ir;
                 Lk[0] = mn;
             }
-            thr = Lk[KL - 1];
+            thr = min(thr, Lk[KL - 1]);
         }
         mn = kKey32None;
 #pragma unroll
@@ -173,6 +216,7 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
     constexpr int LIST_BYTES = 8 * IQ * KL * 8;
     __shared__ __align__(16) unsigned char smem[SCAN_BYTES > LIST_BYTES ? SCAN_BYTES : LIST_BYTES];
     __shared__ int32_t s_q[IQ];
+    __shared__ uint32_t s_thr[8][IQ]; // per (wave, half) list: its query's prune threshold
     __bf16 *sq = (__bf16 *)smem;
 
     if ((int)blockIdx.x >= *ntot) return; // the grid is an upper bound on the item count
@@ -180,6 +224,7 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
     const int c = it.x, start = it.y, cnt = it.z;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5, col = lane & 31;
     if (tid < IQ) s_q[tid] = pair_q[start + min(tid, cnt - 1)];
+    for (int e = tid; e < 8 * IQ; e += 256) s_thr[e / IQ][e % IQ] = kKey32None;
     __syncthreads();
     const int64_t r0 = loff[c], r1 = loff[c + 1];
     const int nch = dp / kIvfKC;
@@ -273,8 +318,18 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
             if (ch + 1 < nch) load_a(rb, ch + 1, acur); // the next row block's first chunk: after the keys
             __syncthreads();
         }
-        // row block done: keys |q|^2 + |x|^2 - 2 q.x and list insertion
+        // row block done: keys |q|^2 + |x|^2 - 2 q.x and list insertion. Thresholds are shared by
+        // the 8 lists of a query: the other half-wave's at once, the other waves' as posted after
+        // their previous row block.
         const int64_t rbase = r0 + 256 * rb + 64 * w;
+        uint32_t thr[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            uint32_t m = min(Lk[j][KL - 1], s_thr[0][32 * j + col]);
+#pragma unroll
+            for (int l = 1; l < 8; ++l) m = min(m, s_thr[l][32 * j + col]);
+            thr[j] = m;
+        }
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             float xr[16];
@@ -286,15 +341,20 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
                 uint32_t key[16];
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    const int64_t row = rbase + 32 * i + 8 * (r >> 2) + 4 * half + (r & 3);
+                    const int off = 8 * (r >> 2) + 4 * half + (r & 3);
                     const float v = fmaxf(0.f, (qv + xr[r]) - 2.f * acc[i][j][r]);
-                    key[r] = row < r1 ? f32_key(v) : kKey32None;
+                    key[r] = rbase + 32 * i + off < r1 ? ((f32_key(v) & ~31u) | (uint32_t)off) : kKey32None;
                 }
-                list_insert16<KL>(key, Lk[j], Li[j], [&](int r) {
-                    return (uint32_t)(rbase + 32 * i + 8 * (r >> 2) + 4 * half + (r & 3));
-                });
+                thr[j] = min(thr[j], (uint32_t)__shfl_xor((int)thr[j], 32, 64));
+                const uint32_t base = (uint32_t)(rbase + 32 * i);
+                if (rb == 0 && i == 0)
+                    list_insert16<KL, true>(key, Lk[j], Li[j], base, thr[j]);
+                else
+                    list_insert16<KL, false>(key, Lk[j], Li[j], base, thr[j]);
             }
         }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) s_thr[w * 2 + half][32 * j + col] = thr[j];
         if (rb + 1 < nrb) load_a(rb + 1, 0, acur);
     }
     // candidate lists -> LDS (the query buffers are no longer read), then one lane per query merges
@@ -344,37 +404,11 @@ __device__ __forceinline__ uint64_t wmin64(uint64_t v) {
     return v;
 }
 
-// Squared-sum core of dist_l2_f_avx2 computed by the 8 lanes of a group (lane c runs FMA chain
-// c; fold, fused tails and horizontal adds as in the reference); valid in lane 0 of the group.
-// Each group reads 8 consecutive floats per step, so a wave's loads coalesce in 32-byte pieces.
-__device__ __forceinline__ float l2_core_g8(const float *a, const float *b, int64_t d, int c) {
-    const int64_t nb = d / 8 * 8;
-    float acc = 0.f;
-    for (int64_t i = c; i < nb; i += 8) {
-        const float t = a[i] - b[i];
-        acc = fmaf(t, t, acc);
-    }
-    float m = __shfl_down(acc, 4, 8) + acc;
-    if (c < 4) {
-        int64_t i = nb, rem = d - nb;
-        if (rem >= 4) {
-            const float t = a[i + c] - b[i + c];
-            m = fmaf(t, t, m);
-            i += 4;
-            rem -= 4;
-        }
-        if (rem > 0) {
-            const float t = c < rem ? a[i + c] - b[i + c] : 0.f;
-            m = fmaf(t, t, m);
-        }
-    }
-    const float p = m + __shfl_down(m, 1, 8);
-    return p + __shfl_down(p, 2, 8);
-}
-
-// One wave per query: the K2 best candidates by the approximation, exact squared L2 for them
-// (eight 8-lane groups, eight candidates per round), the k smallest (distance, id) out.
-template <int PER>
+// One wave per query: the K2 best candidates by the approximation are recomputed in fp32 as
+// sum (q - x)^2 by 16 lanes each (four candidates per round; with VEC, 16-byte loads, so a wave
+// reads four rows per step in 256-byte pieces), then the k smallest (distance, id) are written
+// ascending.
+template <int PER, bool VEC>
 __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restrict__ X, int64_t d,
                                                            const int32_t *__restrict__ lids, const float *__restrict__ Q,
                                                            int64_t qstride, const uint64_t *__restrict__ cand, int ncand,
@@ -405,16 +439,37 @@ __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restri
     const int32_t myid = mine == kKey64None ? -1 : lids[(uint32_t)(mine & 0xFFFFFFFFull)];
     const float *qrow = Q + q * qstride;
     uint64_t key = kKey64None;
-    const int g = lane >> 3, c = lane & 7;
+    const int g = lane >> 4, c = lane & 15;
 #pragma unroll 1
-    for (int round = 0; round < 8; ++round) {
-        // group g computes candidate 8*round + g; its core lands in the group's lane 0
-        const int32_t id = __shfl(myid, 8 * round + g, 64);
-        float core = 0.f;
-        if (id >= 0) core = l2_core_g8(qrow, X + (int64_t)id * d, d, c);
-        const bool owner = lane >= 8 * round && lane < 8 * round + 8;
-        const float cv = __shfl(core, owner ? 8 * (lane - 8 * round) : 0, 64);
-        if (owner && myid >= 0) key = ((uint64_t)f32_key(cv) << 32) | (uint32_t)myid;
+    for (int round = 0; round < kIvfK2 / 4; ++round) {
+        const int32_t id = __shfl(myid, 4 * round + g, 64); // group g: candidate 4 round + g
+        float part = 0.f;
+        if (id >= 0) {
+            const float *xr = X + (int64_t)id * d;
+            if (VEC) {
+                float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
+#pragma unroll 4
+                for (int64_t i = 4 * c; i < d; i += 64) {
+                    const float4 a = *(const float4 *)(qrow + i), b = *(const float4 *)(xr + i);
+                    const float t0 = a.x - b.x, t1 = a.y - b.y, t2 = a.z - b.z, t3 = a.w - b.w;
+                    p0 = fmaf(t0, t0, p0);
+                    p1 = fmaf(t1, t1, p1);
+                    p2 = fmaf(t2, t2, p2);
+                    p3 = fmaf(t3, t3, p3);
+                }
+                part = (p0 + p1) + (p2 + p3);
+            } else {
+                for (int64_t i = c; i < d; i += 16) {
+                    const float t = qrow[i] - xr[i];
+                    part = fmaf(t, t, part);
+                }
+            }
+        }
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) part += __shfl_xor(part, off, 16);
+        const bool owner = lane >= 4 * round && lane < 4 * round + 4;
+        const float v = __shfl(part, owner ? 16 * (lane - 4 * round) : 0, 64);
+        if (owner && myid >= 0) key = ((uint64_t)f32_key(v) << 32) | (uint32_t)myid;
     }
     int cnt = 0;
     for (int r = 0; r < k; ++r) {
@@ -544,11 +599,11 @@ static void ivfl2_search_enqueue(pn_ivfl2 *h, const float *dq, int64_t qstride_i
         StageScope sc(st, STAGE_OTHER);
         l2_split_rows(Q, nq, d, re, dp, Qh, Ql, qn, st);
         PN_HIP(hipMemsetAsync(lc, 0, (size_t)h->nl * 4, st));
-        hipLaunchKernelGGL(probe_count_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, st, pr, total, lc);
+        const unsigned pchunks = (unsigned)ceil_div(total, (int64_t)kPairChunk);
+        hipLaunchKernelGGL(probe_count_kernel, dim3(pchunks), dim3(256), 0, st, pr, total, h->nl, lc);
         hipLaunchKernelGGL(build_items_kernel, dim3(1), dim3(256), 0, st, lc, h->corder.as<int32_t>(), h->nl, cur,
                            items, ntot);
-        hipLaunchKernelGGL(probe_scatter_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, st, pr, total, np,
-                           cur, pq, pp);
+        hipLaunchKernelGGL(probe_scatter_kernel, dim3(pchunks), dim3(256), 0, st, pr, total, np, h->nl, cur, pq, pp);
     }
     {
         StageScope sc(st, STAGE_SCAN);
@@ -565,16 +620,22 @@ static void ivfl2_search_enqueue(pn_ivfl2 *h, const float *dq, int64_t qstride_i
     {
         StageScope sc(st, STAGE_RERANK);
         const int ncand = np * kIvfKP;
+        const bool vec = d % 4 == 0 && re % 4 == 0;
         dim3 g((unsigned)ceil_div(nq, 4));
+        auto launch = [&](auto per, auto v) {
+            hipLaunchKernelGGL((ivfl2_rerank_kernel<decltype(per)::value, decltype(v)::value>), g, dim3(256), 0, st,
+                               h->X.as<float>(), d, h->lids.as<int32_t>(), Q, re, cand, ncand, nq, k, oi, od, oc);
+        };
+        using T = std::true_type;
+        using F = std::false_type;
         if (ncand <= 64 * 8)
-            hipLaunchKernelGGL((ivfl2_rerank_kernel<8>), g, dim3(256), 0, st, h->X.as<float>(), d, h->lids.as<int32_t>(),
-                               Q, re, cand, ncand, nq, k, oi, od, oc);
+            vec ? launch(std::integral_constant<int, 8>{}, T{}) : launch(std::integral_constant<int, 8>{}, F{});
         else if (ncand <= 64 * 16)
-            hipLaunchKernelGGL((ivfl2_rerank_kernel<16>), g, dim3(256), 0, st, h->X.as<float>(), d,
-                               h->lids.as<int32_t>(), Q, re, cand, ncand, nq, k, oi, od, oc);
-        else
-            hipLaunchKernelGGL((ivfl2_rerank_kernel<32>), g, dim3(256), 0, st, h->X.as<float>(), d,
-                               h->lids.as<int32_t>(), Q, re, cand, ncand, nq, k, oi, od, oc);
+            vec ? launch(std::integral_constant<int, 16>{}, T{}) : launch(std::integral_constant<int, 16>{}, F{});
+        else if (ncand <= 64 * 32)
+            vec ? launch(std::integral_constant<int, 32>{}, T{}) : launch(std::integral_constant<int, 32>{}, F{});
+        else // a lane may hold all K2 = 64 best
+            vec ? launch(std::integral_constant<int, 64>{}, T{}) : launch(std::integral_constant<int, 64>{}, F{});
         PN_HIP(hipGetLastError());
     }
 }

@@ -19,6 +19,8 @@
 // MFMA layout (checked on the device): A 32x16 -- lane l holds row l%32, k = 8(l/32)..+7;
 // B 16x32 -- lane l holds column l%32, k = 8(l/32)..+7; D 32x32 -- lane l, register r holds
 // row 8(r/4) + 4(l/32) + r%4, column l%32.
+#include <type_traits>
+
 #include "common.hpp"
 #include "device_util.hpp"
 #include "kernels.hpp"
@@ -28,15 +30,16 @@
 namespace pn {
 
 
-__global__ __launch_bounds__(256) void l2_split_kernel(const float *__restrict__ X, int64_t n, int64_t d,
-                                                       int64_t stride, int dp, __bf16 *__restrict__ H,
+// Rows [n, n_out) are written as zero rows (padding that later kernels read unguarded).
+__global__ __launch_bounds__(256) void l2_split_kernel(const float *__restrict__ X, int64_t n, int64_t n_out,
+                                                       int64_t d, int64_t stride, int dp, __bf16 *__restrict__ H,
                                                        __bf16 *__restrict__ Lo, float *__restrict__ nrm) {
     const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (row >= n) return;
+    if (row >= n_out) return;
     double acc = 0.0;
     for (int e = lane; e < dp; e += 64) {
-        const float x = e < d ? X[row * stride + e] : 0.f;
+        const float x = e < d && row < n ? X[row * stride + e] : 0.f;
         const __bf16 h = (__bf16)x;
         const __bf16 l = (__bf16)(x - (float)h);
         H[row * dp + e] = h;
@@ -59,6 +62,89 @@ __global__ void max_f32_kernel(const float *__restrict__ v, int64_t n, unsigned 
         __syncthreads();
     }
     if (threadIdx.x == 0) atomicMax(out, __float_as_uint(red[0])); // non-negative floats order as uints
+}
+
+// Keys of one 32 x 32 tile for this lane's query (rows t + 8g + 4 half + {0..3}, g < 4) into
+// the lane's sorted list (Lk, Li) of KL. A key is the approximation's float bits with the low 5
+// bits replaced by the row's offset in the tile: keys stay unique inside a tile, the minimum names
+// its row, and (key & ~31) is a lower bound of the approximation. Passing keys (below thr) are
+// inserted one per lane per round; rounds run while any lane of the wave still has one. thr may
+// come from the other half-wave's list of the same query: rows at or above it cannot reach the
+// KL best of the two lists together.
+template <int KL, bool MASK, bool FIRST>
+__device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4 (&xv)[4], float qnv, int64_t t,
+                                                 int64_t r1, int half, uint32_t (&Lk)[KL], uint32_t (&Li)[KL],
+                                                 uint32_t &thr) {
+    uint32_t key[16];
+    uint32_t mn = kKey32None;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int off = 8 * (r >> 2) + 4 * half + (r & 3);
+        const float xr = (r & 3) == 0 ? xv[r >> 2].x : (r & 3) == 1 ? xv[r >> 2].y : (r & 3) == 2 ? xv[r >> 2].z : xv[r >> 2].w;
+        const float v = fmaxf(0.f, (qnv + xr) - 2.f * acc[r]);
+        key[r] = !MASK || t + off < r1 ? ((f32_key(v) & ~31u) | (uint32_t)off) : kKey32None;
+        mn = min(mn, key[r]);
+    }
+    if (FIRST) { // empty list: the tile's KL smallest keys, sorted, by a network instead of KL rounds
+        sort16_u32(key);
+#pragma unroll
+        for (int j = 0; j < KL; ++j) {
+            Lk[j] = j < 16 ? key[j] : kKey32None;
+            Li[j] = j < 16 ? (uint32_t)t + (key[j] & 31u) : 0u;
+        }
+        thr = min(thr, Lk[KL - 1]);
+        return;
+    }
+    while (__any(mn < thr)) {
+        if (mn < thr) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) key[r] = key[r] == mn ? kKey32None : key[r];
+            const uint32_t ir = (uint32_t)t + (mn & 31u);
+#pragma unroll
+            for (int j = KL - 1; j > 0; --j) {
+                const bool up = mn < Lk[j - 1];
+                const bool here = mn < Lk[j];
+                Li[j] = up ? Li[j - 1] : (here ? ir : Li[j]);
+                Lk[j] = max(Lk[j - 1], min(mn, Lk[j]));
+            }
+            if (mn < Lk[0]) {
+                Li[0] = ir;
+                Lk[0] = mn;
+            }
+            thr = min(thr, Lk[KL - 1]);
+        }
+        mn = kKey32None;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) mn = min(mn, key[r]);
+    }
+}
+
+// Half-wave merge: lanes l and l+32 hold lists of the same query; lane l writes the KL smallest
+// of both to out[j * nq_pad]. Keys a list dropped are >= its last key >= the merged last key, so
+// the merged last key still bounds every dropped candidate from below.
+template <int KL>
+__device__ __forceinline__ void half_merge_out(uint64_t (*lds)[KL], int half, const uint32_t (&Lk)[KL],
+                                               const uint32_t (&Li)[KL], uint64_t *out, int64_t nq_pad) {
+    if (half == 1) {
+#pragma unroll
+        for (int j = 0; j < KL; ++j) lds[0][j] = Lk[j] == kKey32None ? kKey64None : (((uint64_t)Lk[j] << 32) | Li[j]);
+    }
+    __syncthreads();
+    if (half == 0) {
+        int pb = 0;
+        int pa = 0;
+        for (int j = 0; j < KL; ++j) {
+            uint64_t va = kKey64None;
+#pragma unroll
+            for (int e = 0; e < KL; ++e) // own list entry pa, selected without dynamic register indexing
+                if (e == pa) va = Lk[e] == kKey32None ? kKey64None : (((uint64_t)Lk[e] << 32) | Li[e]);
+            const uint64_t vb = pb < KL ? lds[0][pb] : kKey64None;
+            const bool takea = va <= vb;
+            out[(int64_t)j * nq_pad] = takea ? va : vb;
+            pa += takea ? 1 : 0;
+            pb += takea ? 0 : 1;
+        }
+    }
 }
 
 template <int KL, int PREC>
@@ -98,73 +184,84 @@ __global__ __launch_bounds__(256) void l2_mfma_scan_kernel(const __bf16 *__restr
         const __bf16 *xl = Xl + (t + col) * dp + 8 * half;
         f32x16 acc = {};
         mfma_tile_bf16<PREC, PREC == 3 ? 4 : 8>(acc, xh, xl, qh, ql, dp);
-        uint32_t key[16];
-        uint32_t mn = kKey32None;
+        thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64)); // the other half-wave's list of this query
+        if (t == r0) // only a slice's last tile can hold rows past its end
+            t + 32 <= r1 ? tile_keys_insert<KL, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
+                         : tile_keys_insert<KL, true, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+        else
+            t + 32 <= r1 ? tile_keys_insert<KL, false, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
+                         : tile_keys_insert<KL, true, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+    }
+    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + (s * KL) * nq_pad + q, nq_pad);
+}
+
+// The same scan for dp = 16 NS (NS <= 4): the lane's query fragments stay in registers for the
+// whole slice, and the next tile's row fragments and norms are loaded before the current tile's
+// products and keys, so the memory latency of one tile hides behind the work of the previous one.
+template <int KL, int PREC, int NS>
+__global__ __launch_bounds__(256) void l2_mfma_scan_small_kernel(
+    const __bf16 *__restrict__ Xh, const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
+    const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql, const float *__restrict__ qn,
+    int64_t rows_per_slice, uint64_t *__restrict__ part, int64_t nq_pad) {
+    constexpr int dp = 16 * NS;
+    constexpr int NPL = PREC == 3 ? 2 : 1;
+    __shared__ uint64_t lds[4][32][KL];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
+    const int64_t q = (int64_t)blockIdx.x * 128 + 32 * w + col;
+    const int64_t s = blockIdx.y;
+    const int64_t r0 = s * rows_per_slice, r1 = min(n, r0 + rows_per_slice);
+    const float qnv = qn[q];
+    bf16x8 b[NS][NPL];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int64_t i = t + 8 * (r >> 2) + 4 * half + (r & 3);
-            const float xr = (r & 3) == 0 ? xv[r >> 2].x : (r & 3) == 1 ? xv[r >> 2].y : (r & 3) == 2 ? xv[r >> 2].z : xv[r >> 2].w;
-            const float v = fmaxf(0.f, (qnv + xr) - 2.f * acc[r]);
-            key[r] = i < r1 ? f32_key(v) : kKey32None;
-            mn = min(mn, key[r]);
+    for (int k = 0; k < NS; ++k) {
+        b[k][0] = *(const bf16x8 *)(Qh + q * dp + 16 * k + 8 * half);
+        if (PREC == 3) b[k][NPL - 1] = *(const bf16x8 *)(Ql + q * dp + 16 * k + 8 * half);
+    }
+    uint32_t Lk[KL], Li[KL];
+#pragma unroll
+    for (int j = 0; j < KL; ++j) {
+        Lk[j] = kKey32None;
+        Li[j] = 0u;
+    }
+    uint32_t thr = kKey32None;
+    bf16x8 a[NS][NPL], an[NS][NPL];
+    float4 xv[4], xvn[4];
+    auto load_tile = [&](int64_t t, bf16x8 (&fa)[NS][NPL], float4 (&fx)[4]) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            fa[k][0] = *(const bf16x8 *)(Xh + (t + col) * dp + 16 * k + 8 * half);
+            if (PREC == 3) fa[k][NPL - 1] = *(const bf16x8 *)(Xl + (t + col) * dp + 16 * k + 8 * half);
         }
-        // Insert passing values one per lane per round; rounds run while any lane of the wave still
-        // has one (usually none or one round per tile). Values leave in increasing key order and,
-        // among equal keys, in increasing row order, so equal keys keep the lower row first.
-        while (__any(mn < thr)) {
-            if (mn < thr) {
-                uint32_t ir = 0;
-                bool found = false;
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const bool hit = !found && key[r] == mn;
-                    ir = hit ? (uint32_t)(t + 8 * (r >> 2) + 4 * half + (r & 3)) : ir;
-                    key[r] = hit ? kKey32None : key[r];
-                    found = found || hit;
-                }
+        for (int g = 0; g < 4; ++g) fx[g] = *(const float4 *)(xn + t + 8 * g + 4 * half);
+    };
+    if (r0 < r1) load_tile(r0, a, xv);
+    for (int64_t t = r0; t < r1; t += 32) {
+        if (t + 32 < r1) load_tile(t + 32, an, xvn); // padded arrays: in bounds
+        f32x16 acc = {};
 #pragma unroll
-                for (int j = KL - 1; j > 0; --j) {
-                    const bool up = mn < Lk[j - 1];
-                    const bool here = mn < Lk[j];
-                    Li[j] = up ? Li[j - 1] : (here ? ir : Li[j]);
-                    Lk[j] = max(Lk[j - 1], min(mn, Lk[j]));
-                }
-                if (mn < Lk[0]) {
-                    Li[0] = ir;
-                    Lk[0] = mn;
-                }
-                thr = Lk[KL - 1];
+        for (int k = 0; k < NS; ++k) {
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][0], b[k][0], acc, 0, 0, 0);
+            if (PREC == 3) {
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][0], b[k][NPL - 1], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][NPL - 1], b[k][0], acc, 0, 0, 0);
             }
-            mn = kKey32None;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) mn = min(mn, key[r]);
         }
-    }
-    // Half-wave merge: lanes l and l+32 hold lists of the same query; lane l keeps the KL
-    // smallest of both. Keys a list dropped are >= its last key >= the merged last key, so the
-    // merged last key still bounds every dropped candidate from below.
-    if (half == 1) {
+        thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64)); // the other half-wave's list of this query
+        if (t == r0) // only a slice's last tile can hold rows past its end
+            t + 32 <= r1 ? tile_keys_insert<KL, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
+                         : tile_keys_insert<KL, true, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+        else
+            t + 32 <= r1 ? tile_keys_insert<KL, false, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
+                         : tile_keys_insert<KL, true, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
 #pragma unroll
-        for (int j = 0; j < KL; ++j)
-            lds[w][col][j] = Lk[j] == kKey32None ? kKey64None : (((uint64_t)Lk[j] << 32) | Li[j]);
-    }
-    __syncthreads();
-    if (half == 0) {
-        uint64_t *out = part + (s * KL) * nq_pad + q;
-        int pb = 0;
-        int pa = 0;
-        for (int j = 0; j < KL; ++j) {
-            uint64_t va = kKey64None;
+        for (int k = 0; k < NS; ++k)
 #pragma unroll
-            for (int e = 0; e < KL; ++e) // own list entry pa, selected without dynamic register indexing
-                if (e == pa) va = Lk[e] == kKey32None ? kKey64None : (((uint64_t)Lk[e] << 32) | Li[e]);
-            const uint64_t vb = pb < KL ? lds[w][col][pb] : kKey64None;
-            const bool takea = va <= vb;
-            out[(int64_t)j * nq_pad] = takea ? va : vb;
-            pa += takea ? 1 : 0;
-            pb += takea ? 0 : 1;
-        }
+            for (int pl = 0; pl < NPL; ++pl) a[k][pl] = an[k][pl];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) xv[g] = xvn[g];
     }
+    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + (s * KL) * nq_pad + q, nq_pad);
 }
 
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
@@ -188,9 +285,12 @@ __global__ __launch_bounds__(256) void l2_rerank_kernel(const float *__restrict_
     const int lane = threadIdx.x & 63;
     const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (q >= nq) return;
+    // the smallest approximation any full list dropped: its last key with the row bits cleared
     uint32_t bmin = kKey32None;
-    for (int64_t l = lane; l < nlists; l += 64)
-        bmin = min(bmin, (uint32_t)(part[(l * KL + KL - 1) * nq_pad + q] >> 32));
+    for (int64_t l = lane; l < nlists; l += 64) {
+        const uint32_t last = (uint32_t)(part[(l * KL + KL - 1) * nq_pad + q] >> 32);
+        bmin = min(bmin, last == kKey32None ? kKey32None : (last & ~31u));
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) bmin = min(bmin, (uint32_t)__shfl_xor((int)bmin, off, 64));
 
@@ -219,8 +319,9 @@ __global__ __launch_bounds__(256) void l2_rerank_kernel(const float *__restrict_
     if (lane == 0) {
         int flag = 0;
         if (n > K2c) {
-            const float a_last = capx[q * K2c + K2c - 1];
-            const float bound = fminf(key_f32(bmin), a_last);
+            const uint32_t al = __float_as_uint(capx[q * K2c + K2c - 1]);
+            const uint32_t bnd = min(bmin, al == kKey32None ? kKey32None : (al & ~31u));
+            const float bound = bnd == kKey32None ? INFINITY : key_f32(bnd);
             const float s = sqrtf(qn[q]) + sqrtf(__uint_as_float(*xn_max_bits));
             const float eps = eps_coef * s * s;
             flag = !(bound - eps > dk_core);
@@ -284,14 +385,10 @@ void l2_prepare(L2Set &s, const float *X, int64_t n, int64_t d, int64_t stride, 
     float *nr = (float *)s.norm.ensure((size_t)npad * 4);
     unsigned int *mx = (unsigned int *)s.norm_max.ensure(4);
     PN_HIP(hipMemsetAsync(mx, 0, 4, st));
-    if (npad > n) { // zero tail rows: tiles read them unguarded
-        PN_HIP(hipMemsetAsync(H + n * s.dp, 0, (size_t)(npad - n) * s.dp * 2, st));
-        PN_HIP(hipMemsetAsync(L + n * s.dp, 0, (size_t)(npad - n) * s.dp * 2, st));
-        PN_HIP(hipMemsetAsync(nr + n, 0, (size_t)(npad - n) * 4, st));
-    }
     if (n == 0) return;
-    hipLaunchKernelGGL(l2_split_kernel, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st, X, n, d, stride, s.dp, H, L,
-                       nr);
+    // zero tail rows: tiles read them unguarded
+    hipLaunchKernelGGL(l2_split_kernel, dim3((unsigned)ceil_div(npad, 4)), dim3(256), 0, st, X, n, npad, d, stride,
+                       s.dp, H, L, nr);
     hipLaunchKernelGGL(max_f32_kernel, dim3((unsigned)std::min<int64_t>(1024, ceil_div(n, 256))), dim3(256), 0, st, nr,
                        n, mx);
     PN_HIP(hipGetLastError());
@@ -300,12 +397,12 @@ void l2_prepare(L2Set &s, const float *X, int64_t n, int64_t d, int64_t stride, 
 void l2_split_rows(const float *X, int64_t n, int64_t d, int64_t stride, int dp, __bf16 *H, __bf16 *L, float *nrm,
                    hipStream_t st) {
     if (n == 0) return;
-    hipLaunchKernelGGL(l2_split_kernel, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st, X, n, d, stride, dp, H, L,
+    hipLaunchKernelGGL(l2_split_kernel, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st, X, n, n, d, stride, dp, H, L,
                        nrm);
     PN_HIP(hipGetLastError());
 }
 
-static int kl_for(int kk) { return kk <= 4 ? 8 : (kk <= 12 ? 16 : 32); }
+static int kl_for(int kk) { return kk <= 6 ? 8 : (kk <= 12 ? 16 : 32); }
 
 void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const float *Q, int64_t qstride, int64_t nq,
                     int kk, int32_t *out_idx, float *out_dist, L2Work &wk, int cus, hipStream_t st) {
@@ -322,11 +419,8 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
     __bf16 *Qh = (__bf16 *)qs.hi.ensure((size_t)nq_pad * dp * 2);
     __bf16 *Ql = (__bf16 *)qs.lo.ensure((size_t)nq_pad * dp * 2);
     float *qn = (float *)qs.norm.ensure((size_t)nq_pad * 4);
-    PN_HIP(hipMemsetAsync(Qh, 0, (size_t)nq_pad * dp * 2, st));
-    PN_HIP(hipMemsetAsync(Ql, 0, (size_t)nq_pad * dp * 2, st));
-    PN_HIP(hipMemsetAsync(qn, 0, (size_t)nq_pad * 4, st));
-    hipLaunchKernelGGL(l2_split_kernel, dim3((unsigned)ceil_div(nq, 4)), dim3(256), 0, st, Q, nq, d, qstride, dp, Qh,
-                       Ql, qn);
+    hipLaunchKernelGGL(l2_split_kernel, dim3((unsigned)ceil_div(nq_pad, 4)), dim3(256), 0, st, Q, nq, nq_pad, d,
+                       qstride, dp, Qh, Ql, qn);
     // slices: whole multiples of the CU count, at least 128 rows each
     const int64_t wpc = std::max<int64_t>(1, tune_get("l2.wg_per_cu"));
     int64_t S = std::max<int64_t>(1, wpc * cus / gx);
@@ -339,15 +433,42 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
     {
         StageScope sc(st, STAGE_SCAN);
         dim3 grid((unsigned)gx, (unsigned)S);
-#define PN_L2SCAN(KLV, PV)                                                                                  \
-    hipLaunchKernelGGL((l2_mfma_scan_kernel<KLV, PV>), grid, dim3(256), 0, st, xs.hi.as<__bf16>(), xs.lo.as<__bf16>(), \
-                       xs.norm.as<float>(), n, dp, Qh, Ql, qn, rps, part, nq_pad)
+        auto launch = [&](auto klc, auto pc) {
+            constexpr int KLV = decltype(klc)::value, PV = decltype(pc)::value;
+            const __bf16 *xh = xs.hi.as<__bf16>(), *xl = xs.lo.as<__bf16>();
+            const float *xnr = xs.norm.as<float>();
+            switch (dp) {
+            case 16:
+                hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 1>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
+                                   Ql, qn, rps, part, nq_pad);
+                break;
+            case 32:
+                hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 2>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
+                                   Ql, qn, rps, part, nq_pad);
+                break;
+            case 48:
+                hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 3>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
+                                   Ql, qn, rps, part, nq_pad);
+                break;
+            case 64:
+                hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 4>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
+                                   Ql, qn, rps, part, nq_pad);
+                break;
+            default:
+                hipLaunchKernelGGL((l2_mfma_scan_kernel<KLV, PV>), grid, dim3(256), 0, st, xh, xl, xnr, n, dp, Qh, Ql, qn,
+                                   rps, part, nq_pad);
+            }
+        };
+        using I8 = std::integral_constant<int, 8>;
+        using I16 = std::integral_constant<int, 16>;
+        using I32 = std::integral_constant<int, 32>;
+        using P1 = std::integral_constant<int, 1>;
+        using P3 = std::integral_constant<int, 3>;
         if (prec == 1) {
-            if (KL == 8) PN_L2SCAN(8, 1); else if (KL == 16) PN_L2SCAN(16, 1); else PN_L2SCAN(32, 1);
+            if (KL == 8) launch(I8{}, P1{}); else if (KL == 16) launch(I16{}, P1{}); else launch(I32{}, P1{});
         } else {
-            if (KL == 8) PN_L2SCAN(8, 3); else if (KL == 16) PN_L2SCAN(16, 3); else PN_L2SCAN(32, 3);
+            if (KL == 8) launch(I8{}, P3{}); else if (KL == 16) launch(I16{}, P3{}); else launch(I32{}, P3{});
         }
-#undef PN_L2SCAN
         PN_HIP(hipGetLastError());
     }
     int32_t *cidx = (int32_t *)wk.cidx.ensure((size_t)nq * K2c * 4);
