@@ -14,7 +14,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpynear_amd.so")
+LIB_PATH = os.environ.get("PYNEAR_AMD_LIB") or os.path.join(HERE, "libpynear_amd.so")
 
 PN_OK, PN_EINVAL, PN_ERUNTIME, PN_ETYPE, PN_EDEVICE, PN_ENOMEM = range(6)
 METRIC_L2, METRIC_L1, METRIC_CHEBYSHEV, METRIC_HAMMING = range(4)

@@ -10,9 +10,10 @@
 //   3. scan: per item, a GEMM of the item's queries against the cluster's rows in the form
 //      |q|^2 + |x|^2 - 2 q.x on a bf16 copy of the rows (hi only, or hi+lo for 3 products);
 //      each (query, probe) keeps its KP best candidates by the approximation.
-//   4. rerank: per query the K2 = 64 best candidates over all probes are recomputed exactly
-//      (squared L2 in the reference kernel's operation order, fp32 rows) and the k smallest
-//      (distance, id) are returned.
+//   4. rerank: per query the K2 = 64 best candidates over all probes are recomputed in fp32 as
+//      sum (q - x)^2 from the fp32 rows (the reference's value, ||q||^2 + ||x||^2 - 2 q.x through
+//      BLAS, has no fixed summation order to reproduce) and the k smallest (distance, id) are
+//      returned.
 // Clusters are stored contiguously in cluster order (ids ascending within a cluster); empty
 // clusters are left out, as the reference's per-cluster arrays leave them out (forest.py:51-59).
 #include <algorithm>

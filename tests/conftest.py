@@ -12,15 +12,23 @@ if ROOT not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X); runs the HIP kernels")
     lib = os.path.join(ROOT, "oracle", "liboracle.so")
-    if not os.path.exists(lib):  # the CPU oracle is cheap to build; the product library is not built here
+    if not os.path.exists(lib):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    if not os.path.exists(os.path.join(ROOT, "pynear_amd", "libpynear_amd.so")):  # fresh checkout
+        subprocess.run([sys.executable, "-m", "pynear_amd.build"], cwd=ROOT, check=True)
 
 
 def _have_gpu():
+    from pynear_amd import _native
     try:
-        from pynear_amd import _native
+        _native.lib()
+    except _native.NativeLibraryError:
+        if os.path.exists("/dev/kfd"):  # a GPU machine without the HIP library: fail, never skip
+            raise
+        return False
+    try:
         return _native.device_count() > 0
-    except Exception:
+    except RuntimeError:
         return False
 
 
