@@ -137,7 +137,7 @@ def run_exact(workload, steps, warmup, cpu):
         ops = 2.0 * (X.shape[1] // 4) * nq * n
         ach = ops / scan_s / 1e12
         roof = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_INT32_VALU_TOPS, "unit": "TOP/s",
-                "frac": round(ach / PEAK_INT32_VALU_TOPS, 4), "traffic": None,
+                "frac": round(ach / PEAK_INT32_VALU_TOPS, 4), "traffic": measured_traffic("hamming_scan_kernel<16,16,1,4>"),
                 "kernel": "hamming_scan_kernel<16,16,1,4>", "kernel_ms": round(scan_s * 1e3, 4),
                 "work": f"{ops:.3e} int32 ops per launch: 16 words x (xor + popcount-accumulate) x Q x N"}
     else:
@@ -146,7 +146,7 @@ def run_exact(workload, steps, warmup, cpu):
         # the result is fp32-exact (bit-identical to the reference), so the work is priced as fp32
         # flops against the fp32 peak; the kernel gets there with 3 bf16 MFMA products per term
         roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic("l2_mfma_scan_small_kernel<8,3,2>"),
                 "kernel": "l2_mfma_scan_small_kernel<8,3,2>", "kernel_ms": round(scan_s * 1e3, 4),
                 "work": f"{flops:.3e} flop per launch (2 d Q N)"}
     out = {"workload": desc, "value": round(nq * steps / (gpu_ms / 1e3), 1), "unit": "queries/s",
@@ -245,7 +245,7 @@ def run_ivf(steps, warmup, cpu, probes=(8, 16, 32)):
     else:
         roof = {"bound": "mfma", "achieved": round(tf, 2), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(tf / PEAK_BF16_TFLOPS, 4)}
-    roof.update({"traffic": None, "kernel": "ivfl2_scan_kernel<1>", "kernel_ms": round(scan_s * 1e3, 4),
+    roof.update({"traffic": measured_traffic("ivfl2_scan_kernel<1>"), "kernel": "ivfl2_scan_kernel<1>", "kernel_ms": round(scan_s * 1e3, 4),
                  "work": f"{bytes_:.3e} B (bf16 rows of the probed clusters) and {flops:.3e} flop per launch",
                  "other_bound": {"hbm_GBs": round(bw, 1), "mfma_TFLOPs": round(tf, 2)}})
     out = {"workload": (f"IVFFlatL2Index GMM N={n:,} x d={d} (1,024 comps, sigma 0.3), nlist={nlist}, "
@@ -288,6 +288,32 @@ def run_ivf(steps, warmup, cpu, probes=(8, 16, 32)):
     return out
 
 
+# ---------------------------------------------------------------- HBM traffic
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """Device-to-device copies of a known size: FETCH_SIZE / WRITE_SIZE of these dispatches give the
+    counters' scale (bytes moved per KiB counted)."""
+    from pynear_amd import hipdev
+    a = hipdev.DeviceArray((nbytes,), np.uint8)
+    b = hipdev.DeviceArray((nbytes,), np.uint8)
+    for _ in range(reps):
+        hipdev._ok(hipdev.hip().hipMemcpy(hipdev._vp(b.ptr), hipdev._vp(a.ptr), nbytes, 3))
+    hipdev.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return t["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 # ---------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
@@ -297,7 +323,12 @@ def main():
     ap.add_argument("--workload", default="ivf", choices=["ivf", "hamming", "l2"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="ivf only: skip the exact workloads")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device-to-device copies of 1 GiB (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the search path runs on a single "
               "MI355X and the multi-GPU sharding is a design, not built", file=sys.stderr)
