@@ -49,7 +49,7 @@ struct pn_ivfl2 {
     pn::DevBuf lids, loff;   // cluster-order position -> original id; cluster row offsets [nl + 1]
     pn::DevBuf corder;       // cluster ids by decreasing list length (work-item order)
     pn::DevBuf qpad, qsplit_h, qsplit_l, qn, probes, pdist, lcount, cursor, pair_q, pair_p;
-    pn::DevBuf items, ntot, cand, oidx, odist, ocnt;
+    pn::DevBuf items, ntot /* XCD item ranges */, cand, oidx, odist, ocnt;
 };
 
 namespace pn {
@@ -80,40 +80,79 @@ constexpr int kIvfIQ = 64;  // queries (pairs of one cluster) per work item
 constexpr int kIvfKC = 64;  // dims per K chunk
 constexpr int kIvfQS = 72;  // LDS row stride (bf16) of a staged query chunk: 16 B of pad spreads the banks
 
-// One workgroup: the work items, (cluster, first pair, count <= 64), cluster by cluster in the
-// order corder (lists by decreasing length, so the longest items are dispatched first), and per
-// cluster the first pair slot (pair cursor). Writes the item count to ntot.
+// One workgroup: the work items (cluster, first pair, count <= 64), cluster by cluster in the
+// order corder (lists by decreasing length), and per cluster its first pair slot (pair cursor).
+// Items are then split into nranges contiguous ranges of about equal work (an item's work is its
+// list length + 64). With nranges = kXcds, range b is run by the workgroups with id % kXcds == b,
+// which round-robin dispatch places on one XCD, so a long list that many queries probe is read
+// into one or two L2s instead of all eight; with nranges = 1 (the default: measured faster, the
+// per-XCD ranges balance worse than one queue) every workgroup takes from one range. Inside a
+// range the longest items start first. Writes per range its first item, its item count, and a
+// zeroed work counter.
+constexpr int kXcds = 8;
 __global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restrict__ lcount,
-                                                          const int32_t *__restrict__ corder, int nl,
+                                                          const int32_t *__restrict__ corder,
+                                                          const int64_t *__restrict__ loff, int nl,
                                                           int32_t *__restrict__ cursor, int4 *__restrict__ items,
-                                                          int32_t *__restrict__ ntot) {
+                                                          int32_t *__restrict__ bins /*[3][kXcds]*/, int nranges) {
     __shared__ int32_t s_pairs[256], s_items[256];
+    __shared__ int64_t s_work[256];
+    __shared__ int32_t s_cnt[kXcds];
     const int tid = threadIdx.x;
     const int per = (nl + 255) / 256;
     const int o0 = min(nl, tid * per), o1 = min(nl, o0 + per);
     int pairs = 0, nit = 0;
+    int64_t work = 0;
     for (int o = o0; o < o1; ++o) {
-        const int m = lcount[corder[o]];
+        const int c = corder[o], m = lcount[c];
+        const int it = (m + kIvfIQ - 1) / kIvfIQ;
         pairs += m;
-        nit += (m + kIvfIQ - 1) / kIvfIQ;
+        nit += it;
+        work += (int64_t)it * (loff[c + 1] - loff[c] + 64);
     }
     s_pairs[tid] = pairs;
     s_items[tid] = nit;
+    s_work[tid] = work;
+    if (tid < kXcds) s_cnt[tid] = 0;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) { // inclusive scans
         const int a = tid >= off ? s_pairs[tid - off] : 0, b = tid >= off ? s_items[tid - off] : 0;
+        const int64_t w = tid >= off ? s_work[tid - off] : 0;
         __syncthreads();
         s_pairs[tid] += a;
         s_items[tid] += b;
+        s_work[tid] += w;
         __syncthreads();
     }
-    if (tid == 255) *ntot = s_items[255];
+    const double to_range = (double)nranges / (double)max<int64_t>(s_work[255], 1);
     int pa = s_pairs[tid] - pairs, ia = s_items[tid] - nit;
+    int64_t wa = s_work[tid] - work;
+    int cnt[kXcds] = {};
     for (int o = o0; o < o1; ++o) {
         const int c = corder[o], m = lcount[c];
+        const int64_t wi = loff[c + 1] - loff[c] + 64;
         cursor[c] = pa;
-        for (int s = 0; s < m; s += kIvfIQ) items[ia++] = make_int4(c, pa + s, min(kIvfIQ, m - s), 0);
+        for (int s = 0; s < m; s += kIvfIQ) {
+            items[ia++] = make_int4(c, pa + s, min(kIvfIQ, m - s), 0);
+            const int b = min(nranges - 1, (int)((double)(wa + wi / 2) * to_range)); // monotone in item order
+#pragma unroll
+            for (int x = 0; x < kXcds; ++x) cnt[x] += x == b;
+            wa += wi;
+        }
         pa += m;
+    }
+#pragma unroll
+    for (int x = 0; x < kXcds; ++x)
+        if (cnt[x]) atomicAdd(&s_cnt[x], cnt[x]);
+    __syncthreads();
+    if (tid == 0) {
+        int start = 0;
+        for (int x = 0; x < kXcds; ++x) {
+            bins[x] = start;             // first item of range x
+            bins[kXcds + x] = s_cnt[x];  // items in range x
+            bins[2 * kXcds + x] = 0;     // next item to take
+            start += s_cnt[x];
+        }
     }
 }
 
@@ -207,7 +246,7 @@ template <int PREC>
 __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
     const __bf16 *__restrict__ Lh, const __bf16 *__restrict__ Ll, const float *__restrict__ lxn,
     const int64_t *__restrict__ loff, int dp, const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
-    const float *__restrict__ qn, const int4 *__restrict__ items, const int32_t *__restrict__ ntot,
+    const float *__restrict__ qn, const int4 *__restrict__ items, int32_t *__restrict__ bins, int nranges,
     const int32_t *__restrict__ pair_q, const int32_t *__restrict__ pair_p, int np, uint64_t *__restrict__ cand) {
     constexpr int KL = kIvfKL, KP = kIvfKP, QS = kIvfQS, IQ = kIvfIQ;
     constexpr int NPL = PREC == 3 ? 2 : 1;      // staged planes: hi (and lo)
@@ -220,10 +259,19 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
     __shared__ uint32_t s_thr[8][IQ]; // per (wave, half) list: its query's prune threshold
     __bf16 *sq = (__bf16 *)smem;
 
-    if ((int)blockIdx.x >= *ntot) return; // the grid is an upper bound on the item count
-    const int4 it = items[blockIdx.x];
-    const int c = it.x, start = it.y, cnt = it.z;
+    __shared__ int s_slot;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5, col = lane & 31;
+    // workgroups take the items of their range (per XCD: id % kXcds; else the one range) in order
+    // until it is empty
+    const int xb = nranges == 1 ? 0 : blockIdx.x % kXcds;
+    const int first = bins[xb], count = bins[kXcds + xb];
+    for (;;) {
+    if (tid == 0) s_slot = atomicAdd(&bins[2 * kXcds + xb], 1);
+    __syncthreads();
+    const int slot = s_slot;
+    if (slot >= count) break;
+    const int4 it = items[first + slot];
+    const int c = it.x, start = it.y, cnt = it.z;
     if (tid < IQ) s_q[tid] = pair_q[start + min(tid, cnt - 1)];
     for (int e = tid; e < 8 * IQ; e += 256) s_thr[e / IQ][e % IQ] = kKey32None;
     __syncthreads();
@@ -393,6 +441,8 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
                     head[l] = p[l] < KL ? sl[(l * IQ + tid) * KL + p[l]] : kKey64None;
                 }
         }
+    }
+    __syncthreads(); // LDS (lists, query chunks, s_q, s_thr, s_slot) is reused by the next item
     }
 }
 
@@ -594,7 +644,8 @@ static void ivfl2_search_enqueue(pn_ivfl2 *h, const float *dq, int64_t qstride_i
     // each cluster has at most one partly filled item
     const int64_t max_items = ceil_div(total, kIvfIQ) + h->nl;
     int4 *items = (int4 *)h->items.ensure((size_t)max_items * 16);
-    int32_t *ntot = (int32_t *)h->ntot.ensure(4);
+    int32_t *bins = (int32_t *)h->ntot.ensure(3 * kXcds * 4);
+    const int nranges = tune_get("ivfl2.xcd_ranges") ? kXcds : 1;
     uint64_t *cand = (uint64_t *)h->cand.ensure((size_t)total * kIvfKP * 8);
     {
         StageScope sc(st, STAGE_OTHER);
@@ -602,20 +653,24 @@ static void ivfl2_search_enqueue(pn_ivfl2 *h, const float *dq, int64_t qstride_i
         PN_HIP(hipMemsetAsync(lc, 0, (size_t)h->nl * 4, st));
         const unsigned pchunks = (unsigned)ceil_div(total, (int64_t)kPairChunk);
         hipLaunchKernelGGL(probe_count_kernel, dim3(pchunks), dim3(256), 0, st, pr, total, h->nl, lc);
-        hipLaunchKernelGGL(build_items_kernel, dim3(1), dim3(256), 0, st, lc, h->corder.as<int32_t>(), h->nl, cur,
-                           items, ntot);
+        hipLaunchKernelGGL(build_items_kernel, dim3(1), dim3(256), 0, st, lc, h->corder.as<int32_t>(),
+                           h->loff.as<int64_t>(), h->nl, cur, items, bins, nranges);
         hipLaunchKernelGGL(probe_scatter_kernel, dim3(pchunks), dim3(256), 0, st, pr, total, np, h->nl, cur, pq, pp);
     }
     {
         StageScope sc(st, STAGE_SCAN);
+        // a persistent grid of wg_per_cu workgroups per CU (a multiple of kXcds) drains the item ranges
+        const int64_t per_xcd = std::max<int64_t>(1, ceil_div((int64_t)h->cus, (int64_t)kXcds) *
+                                                         std::max<int64_t>(1, tune_get("ivfl2.wg_per_cu")));
+        const dim3 grid((unsigned)std::min<int64_t>(per_xcd, max_items) * kXcds);
         if (prec == 3)
-            hipLaunchKernelGGL((ivfl2_scan_kernel<3>), dim3((unsigned)max_items), dim3(256), 0, st,
-                               h->lrows.hi.as<__bf16>(), h->lrows.lo.as<__bf16>(), h->lrows.norm.as<float>(),
-                               h->loff.as<int64_t>(), dp, Qh, Ql, qn, items, ntot, pq, pp, np, cand);
+            hipLaunchKernelGGL((ivfl2_scan_kernel<3>), grid, dim3(256), 0, st, h->lrows.hi.as<__bf16>(),
+                               h->lrows.lo.as<__bf16>(), h->lrows.norm.as<float>(), h->loff.as<int64_t>(), dp, Qh, Ql,
+                               qn, items, bins, nranges, pq, pp, np, cand);
         else
-            hipLaunchKernelGGL((ivfl2_scan_kernel<1>), dim3((unsigned)max_items), dim3(256), 0, st,
-                               h->lrows.hi.as<__bf16>(), h->lrows.lo.as<__bf16>(), h->lrows.norm.as<float>(),
-                               h->loff.as<int64_t>(), dp, Qh, Ql, qn, items, ntot, pq, pp, np, cand);
+            hipLaunchKernelGGL((ivfl2_scan_kernel<1>), grid, dim3(256), 0, st, h->lrows.hi.as<__bf16>(),
+                               h->lrows.lo.as<__bf16>(), h->lrows.norm.as<float>(), h->loff.as<int64_t>(), dp, Qh, Ql,
+                               qn, items, bins, nranges, pq, pp, np, cand);
         PN_HIP(hipGetLastError());
     }
     {

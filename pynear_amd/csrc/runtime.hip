@@ -39,6 +39,8 @@ static std::map<std::string, int64_t> &tune_table() {
         t["l2.mfma"] = 1;              // exact L2 (k <= 32) through the MFMA candidate scan + rerank
         t["l2.wg_per_cu"] = 2;
         t["l2.mfma_terms"] = 3;        // 3: hi.hi + hi.lo + lo.hi products; 1: hi.hi only
+        t["ivfl2.wg_per_cu"] = 2;      // IVF-L2 list scan: persistent workgroups per CU
+        t["ivfl2.xcd_ranges"] = 0;     // 1: one item range per XCD; 0: one queue for all workgroups
         if (const char *env = std::getenv("PN_TUNE")) {
             std::string e(env);
             size_t pos = 0;
