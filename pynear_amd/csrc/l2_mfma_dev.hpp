@@ -1,0 +1,333 @@
+// Device bodies of the exact L2 search (bf16 MFMA candidate scan, exact rerank, fallback), shared
+// by the single-index kernels of l2_mfma.hip and the batched kernels of batch.hip.
+#pragma once
+#include "common.hpp"
+#include "device_util.hpp"
+#include "l2_mfma.hpp"
+#include "ref_arith.hpp"
+
+namespace pn {
+
+// bf16 hi/lo split and fp64-accumulated squared norm of one row by one wave; rows [n, n_out)
+// are written as zero rows (padding that later kernels read unguarded).
+__device__ __forceinline__ void split_row(const float *__restrict__ X, int64_t n, int64_t d, int64_t stride, int dp,
+                                          __bf16 *__restrict__ H, __bf16 *__restrict__ Lo, float *__restrict__ nrm,
+                                          int64_t row) {
+    const int lane = threadIdx.x & 63;
+    double acc = 0.0;
+    for (int e = lane; e < dp; e += 64) {
+        const float x = e < d && row < n ? X[row * stride + e] : 0.f;
+        const __bf16 h = (__bf16)x;
+        const __bf16 l = (__bf16)(x - (float)h);
+        H[row * dp + e] = h;
+        Lo[row * dp + e] = l;
+        acc += (double)x * (double)x;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) nrm[row] = (float)acc;
+}
+
+
+// Keys of one 32 x 32 tile for this lane's query (rows t + 8g + 4 half + {0..3}, g < 4) into
+// the lane's sorted list (Lk, Li) of KL. A key is the approximation's float bits with the low 5
+// bits replaced by the row's offset in the tile: keys stay unique inside a tile, the minimum names
+// its row, and (key & ~31) is a lower bound of the approximation. Passing keys (below thr) are
+// inserted one per lane per round; rounds run while any lane of the wave still has one. thr may
+// come from the other half-wave's list of the same query: rows at or above it cannot reach the
+// KL best of the two lists together.
+template <int KL, bool MASK, bool FIRST>
+__device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4 (&xv)[4], float qnv, int64_t t,
+                                                 int64_t r1, int half, uint32_t (&Lk)[KL], uint32_t (&Li)[KL],
+                                                 uint32_t &thr) {
+    uint32_t key[16];
+    uint32_t mn = kKey32None;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int off = 8 * (r >> 2) + 4 * half + (r & 3);
+        const float xr = (r & 3) == 0 ? xv[r >> 2].x : (r & 3) == 1 ? xv[r >> 2].y : (r & 3) == 2 ? xv[r >> 2].z : xv[r >> 2].w;
+        const float v = fmaxf(0.f, (qnv + xr) - 2.f * acc[r]);
+        key[r] = !MASK || t + off < r1 ? ((f32_key(v) & ~31u) | (uint32_t)off) : kKey32None;
+        mn = min(mn, key[r]);
+    }
+    if (FIRST) { // empty list: the tile's KL smallest keys, sorted, by a network instead of KL rounds
+        sort16_u32(key);
+#pragma unroll
+        for (int j = 0; j < KL; ++j) {
+            Lk[j] = j < 16 ? key[j] : kKey32None;
+            Li[j] = j < 16 ? (uint32_t)t + (key[j] & 31u) : 0u;
+        }
+        thr = min(thr, Lk[KL - 1]);
+        return;
+    }
+    while (__any(mn < thr)) {
+        if (mn < thr) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) key[r] = key[r] == mn ? kKey32None : key[r];
+            const uint32_t ir = (uint32_t)t + (mn & 31u);
+#pragma unroll
+            for (int j = KL - 1; j > 0; --j) {
+                const bool up = mn < Lk[j - 1];
+                const bool here = mn < Lk[j];
+                Li[j] = up ? Li[j - 1] : (here ? ir : Li[j]);
+                Lk[j] = max(Lk[j - 1], min(mn, Lk[j]));
+            }
+            if (mn < Lk[0]) {
+                Li[0] = ir;
+                Lk[0] = mn;
+            }
+            thr = min(thr, Lk[KL - 1]);
+        }
+        mn = kKey32None;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) mn = min(mn, key[r]);
+    }
+}
+
+// Half-wave merge: lanes l and l+32 hold lists of the same query; lane l writes the KL smallest
+// of both to out[j * nq_pad]. Keys a list dropped are >= its last key >= the merged last key, so
+// the merged last key still bounds every dropped candidate from below.
+template <int KL>
+__device__ __forceinline__ void half_merge_out(uint64_t (*lds)[KL], int half, const uint32_t (&Lk)[KL],
+                                               const uint32_t (&Li)[KL], uint64_t *out, int64_t nq_pad) {
+    if (half == 1) {
+#pragma unroll
+        for (int j = 0; j < KL; ++j) lds[0][j] = Lk[j] == kKey32None ? kKey64None : (((uint64_t)Lk[j] << 32) | Li[j]);
+    }
+    __syncthreads();
+    if (half == 0) {
+        int pb = 0;
+        int pa = 0;
+        for (int j = 0; j < KL; ++j) {
+            uint64_t va = kKey64None;
+#pragma unroll
+            for (int e = 0; e < KL; ++e) // own list entry pa, selected without dynamic register indexing
+                if (e == pa) va = Lk[e] == kKey32None ? kKey64None : (((uint64_t)Lk[e] << 32) | Li[e]);
+            const uint64_t vb = pb < KL ? lds[0][pb] : kKey64None;
+            const bool takea = va <= vb;
+            out[(int64_t)j * nq_pad] = takea ? va : vb;
+            pa += takea ? 1 : 0;
+            pb += takea ? 0 : 1;
+        }
+    }
+}
+
+template <int KL, int PREC>
+__device__ __forceinline__ void scan_generic(uint64_t (*lds)[32][KL], const __bf16 *__restrict__ Xh,
+                                             const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
+                                             int dp, const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
+                                             const float *__restrict__ qn, int64_t rows_per_slice,
+                                             uint64_t *__restrict__ part, int64_t nq_pad, int64_t qb, int64_t s) {
+    // A workgroup owns 128 queries (wave w: queries 32w..32w+31 of the block) and one slice of
+    // rows; the four waves walk the same 32-row tiles, so each tile is fetched from L2 once per
+    // workgroup and served to the other waves from L1. Row arrays are padded with zero rows to a
+    // multiple of 32 and slices start on 32-row boundaries: tile loads need no guard, rows >= r1
+    // are masked.
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
+    const int64_t q = qb * 128 + 32 * w + col; // query buffers are padded to nq_pad rows
+    const int64_t r0 = s * rows_per_slice, r1 = min(n, r0 + rows_per_slice);
+    const float qnv = qn[q];
+    const __bf16 *qh = Qh + q * dp + 8 * half;
+    const __bf16 *ql = Ql + q * dp + 8 * half;
+
+    uint32_t Lk[KL], Li[KL];
+#pragma unroll
+    for (int j = 0; j < KL; ++j) {
+        Lk[j] = kKey32None;
+        Li[j] = 0u;
+    }
+    uint32_t thr = kKey32None;
+
+    for (int64_t t = r0; t < r1; t += 32) {
+        // |x|^2 of this lane's 16 output rows: rows t + 8g + 4 half + {0..3}
+        float4 xv[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) xv[g] = *(const float4 *)(xn + t + 8 * g + 4 * half);
+        const __bf16 *xh = Xh + (t + col) * dp + 8 * half;
+        const __bf16 *xl = Xl + (t + col) * dp + 8 * half;
+        f32x16 acc = {};
+        mfma_tile_bf16<PREC, PREC == 3 ? 4 : 8>(acc, xh, xl, qh, ql, dp);
+        thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64)); // the other half-wave's list of this query
+        if (t == r0) // only a slice's last tile can hold rows past its end
+            t + 32 <= r1 ? tile_keys_insert<KL, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
+                         : tile_keys_insert<KL, true, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+        else
+            t + 32 <= r1 ? tile_keys_insert<KL, false, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
+                         : tile_keys_insert<KL, true, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+    }
+    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + (s * KL) * nq_pad + q, nq_pad);
+}
+
+// The same scan for dp = 16 NS (NS <= 4): the lane's query fragments stay in registers for the
+// whole slice, and the next tile's row fragments and norms are loaded before the current tile's
+// products and keys, so the memory latency of one tile hides behind the work of the previous one.
+template <int KL, int PREC, int NS>
+__device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16 *__restrict__ Xh,
+                                           const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
+                                           const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
+                                           const float *__restrict__ qn, int64_t rows_per_slice,
+                                           uint64_t *__restrict__ part, int64_t nq_pad, int64_t qb, int64_t s) {
+    constexpr int dp = 16 * NS;
+    constexpr int NPL = PREC == 3 ? 2 : 1;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
+    const int64_t q = qb * 128 + 32 * w + col;
+    const int64_t r0 = s * rows_per_slice, r1 = min(n, r0 + rows_per_slice);
+    const float qnv = qn[q];
+    bf16x8 b[NS][NPL];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        b[k][0] = *(const bf16x8 *)(Qh + q * dp + 16 * k + 8 * half);
+        if (PREC == 3) b[k][NPL - 1] = *(const bf16x8 *)(Ql + q * dp + 16 * k + 8 * half);
+    }
+    uint32_t Lk[KL], Li[KL];
+#pragma unroll
+    for (int j = 0; j < KL; ++j) {
+        Lk[j] = kKey32None;
+        Li[j] = 0u;
+    }
+    uint32_t thr = kKey32None;
+    bf16x8 a[NS][NPL], an[NS][NPL];
+    float4 xv[4], xvn[4];
+    auto load_tile = [&](int64_t t, bf16x8 (&fa)[NS][NPL], float4 (&fx)[4]) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            fa[k][0] = *(const bf16x8 *)(Xh + (t + col) * dp + 16 * k + 8 * half);
+            if (PREC == 3) fa[k][NPL - 1] = *(const bf16x8 *)(Xl + (t + col) * dp + 16 * k + 8 * half);
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) fx[g] = *(const float4 *)(xn + t + 8 * g + 4 * half);
+    };
+    if (r0 < r1) load_tile(r0, a, xv);
+    for (int64_t t = r0; t < r1; t += 32) {
+        if (t + 32 < r1) load_tile(t + 32, an, xvn); // padded arrays: in bounds
+        f32x16 acc = {};
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][0], b[k][0], acc, 0, 0, 0);
+            if (PREC == 3) {
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][0], b[k][NPL - 1], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][NPL - 1], b[k][0], acc, 0, 0, 0);
+            }
+        }
+        thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64)); // the other half-wave's list of this query
+        if (t == r0) // only a slice's last tile can hold rows past its end
+            t + 32 <= r1 ? tile_keys_insert<KL, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
+                         : tile_keys_insert<KL, true, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+        else
+            t + 32 <= r1 ? tile_keys_insert<KL, false, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
+                         : tile_keys_insert<KL, true, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+#pragma unroll
+            for (int pl = 0; pl < NPL; ++pl) a[k][pl] = an[k][pl];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) xv[g] = xvn[g];
+    }
+    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + (s * KL) * nq_pad + q, nq_pad);
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint64_t o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// One wave per query q: exact rerank of K2c candidates, (distance, index) order, completeness check.
+__device__ __forceinline__ void rerank_query(const float *__restrict__ X, int64_t xstride, int64_t d,
+                                             const float *__restrict__ Q, int64_t qstride,
+                                             const float *__restrict__ qn, const unsigned int *__restrict__ xn_max_bits,
+                                             const int32_t *__restrict__ cidx, const float *__restrict__ capx, int K2c,
+                                             const uint64_t *__restrict__ part, int64_t nlists, int KL, int64_t nq_pad,
+                                             int kk, int64_t n, float eps_coef, int32_t *__restrict__ out_idx,
+                                             float *__restrict__ out_dist, int32_t *__restrict__ flags, int64_t q) {
+    const int lane = threadIdx.x & 63;
+    // the smallest approximation any full list dropped: its last key with the row bits cleared
+    uint32_t bmin = kKey32None;
+    for (int64_t l = lane; l < nlists; l += 64) {
+        const uint32_t last = (uint32_t)(part[(l * KL + KL - 1) * nq_pad + q] >> 32);
+        bmin = min(bmin, last == kKey32None ? kKey32None : (last & ~31u));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) bmin = min(bmin, (uint32_t)__shfl_xor((int)bmin, off, 64));
+
+    uint64_t key = kKey64None;
+    float core = 0.f;
+    if (lane < K2c) {
+        const int32_t idx = cidx[q * K2c + lane];
+        if (idx >= 0 && idx < n) {
+            core = l2_core(Q + q * qstride, X + (int64_t)idx * xstride, d);
+            key = ((uint64_t)f32_key(sqrtf(core)) << 32) | (uint32_t)idx;
+        }
+    }
+    float dk_core = 0.f;
+    for (int r = 0; r < kk; ++r) {
+        const uint64_t m = wave_min_u64(key);
+        const unsigned long long own = __ballot(key == m);
+        const int owner = __ffsll((unsigned long long)own) - 1;
+        const float cr = __shfl(core, owner, 64);
+        if (key == m) key = kKey64None;
+        if (lane == 0) {
+            out_idx[q * kk + r] = (int32_t)(uint32_t)(m & 0xFFFFFFFFull);
+            out_dist[q * kk + r] = key_f32((uint32_t)(m >> 32));
+        }
+        dk_core = cr;
+    }
+    if (lane == 0) {
+        int flag = 0;
+        if (n > K2c) {
+            const uint32_t al = __float_as_uint(capx[q * K2c + K2c - 1]);
+            const uint32_t bnd = min(bmin, al == kKey32None ? kKey32None : (al & ~31u));
+            const float bound = bnd == kKey32None ? INFINITY : key_f32(bnd);
+            const float s = sqrtf(qn[q]) + sqrtf(__uint_as_float(*xn_max_bits));
+            const float eps = eps_coef * s * s;
+            flag = !(bound - eps > dk_core);
+        }
+        flags[q] = flag;
+    }
+}
+
+// Exact full scan of query q by one workgroup (called for flagged queries only).
+template <int K>
+__device__ __forceinline__ void fallback_query(uint64_t (*wres)[K], const float *__restrict__ X, int64_t xstride,
+                                               int64_t n, int64_t d, const float *__restrict__ Q, int64_t qstride,
+                                               int kk, int32_t *__restrict__ out_idx, float *__restrict__ out_dist,
+                                               int64_t q) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t L[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) L[j] = kKey64None;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        const float dd = sqrtf(l2_core(Q + q * qstride, X + i * xstride, d));
+        const uint64_t key = ((uint64_t)f32_key(dd) << 32) | (uint32_t)i;
+        if (key < L[K - 1]) list_insert_u64<K>(L, key);
+    }
+    for (int r = 0; r < kk; ++r) { // wave-level extraction of the wave's kk best
+        const uint64_t m = wave_min_u64(L[0]);
+        if (L[0] == m && m != kKey64None) {
+#pragma unroll
+            for (int j = 0; j < K - 1; ++j) L[j] = L[j + 1];
+            L[K - 1] = kKey64None;
+        }
+        if (lane == 0) wres[w][r] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int p[4] = {0, 0, 0, 0};
+        auto head = [&](int ww) { return p[ww] < kk ? wres[ww][p[ww]] : kKey64None; };
+        for (int r = 0; r < kk; ++r) {
+            int bw = 0;
+            for (int ww = 1; ww < 4; ++ww)
+                if (head(ww) < head(bw)) bw = ww;
+            const uint64_t m = head(bw);
+            ++p[bw];
+            out_idx[q * kk + r] = (int32_t)(uint32_t)(m & 0xFFFFFFFFull);
+            out_dist[q * kk + r] = key_f32((uint32_t)(m >> 32));
+        }
+    }
+}
+
+// ---------------------------------------------------------------- host side
+} // namespace pn

@@ -12,6 +12,7 @@
 #include "common.hpp"
 #include "device_util.hpp"
 #include "kernels.hpp"
+#include "merge_dev.hpp"
 #include "ref_arith.hpp"
 
 namespace pn {
@@ -52,54 +53,14 @@ __global__ __launch_bounds__(256) void merge_lists_kernel(const uint64_t *__rest
     }
 }
 
-// One wave per query: lane l merges slices l, l+64, ... into a private sorted list, then the
-// wave extracts the kk smallest keys by kk rounds of a wave-wide minimum (keys are unique since
-// indices are). Used when queries are few and slices many, where a lane per query would serialise.
 template <int K>
 __global__ __launch_bounds__(256) void merge_lists_wave_kernel(const uint64_t *__restrict__ part, int64_t slices,
-                                                               int64_t nq, int64_t nq_pad, int kk, int kind,
-                                                               int32_t *__restrict__ out_idx,
+                                                               int klist, int64_t nq, int64_t nq_pad, int kk,
+                                                               int kind, int32_t *__restrict__ out_idx,
                                                                void *__restrict__ out_dist) {
-    const int lane = threadIdx.x & 63;
     const int64_t q = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (q >= nq) return; // whole wave exits together
-    uint64_t L[K];
-#pragma unroll
-    for (int j = 0; j < K; ++j) L[j] = kKey64None;
-    for (int64_t s = lane; s < slices; s += 64) {
-        const uint64_t *src = part + (s * K) * nq_pad + q;
-        for (int j = 0; j < K; ++j) {
-            uint64_t key = src[(int64_t)j * nq_pad];
-            if (key == kKey64None) break;
-            if (kind == OUT_L2_F32) {
-                float dd = sqrtf(key_f32((uint32_t)(key >> 32)));
-                key = ((uint64_t)f32_key(dd) << 32) | (key & 0xFFFFFFFFull);
-            }
-            if (key >= L[K - 1]) break;
-            list_insert_u64<K>(L, key);
-        }
-    }
-    for (int r = 0; r < kk; ++r) {
-        uint64_t m = L[0];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            uint64_t o = __shfl_xor(m, off, 64);
-            m = o < m ? o : m;
-        }
-        if (L[0] == m && m != kKey64None) { // the unique owner pops its head
-#pragma unroll
-            for (int j = 0; j < K - 1; ++j) L[j] = L[j + 1];
-            L[K - 1] = kKey64None;
-        }
-        if (lane == 0) {
-            out_idx[q * kk + r] = m == kKey64None ? -1 : (int32_t)(uint32_t)(m & 0xFFFFFFFFull);
-            uint32_t hi = (uint32_t)(m >> 32);
-            if (kind == OUT_HAMMING_I32)
-                ((int32_t *)out_dist)[q * kk + r] = (int32_t)hi;
-            else
-                ((float *)out_dist)[q * kk + r] = key_f32(hi);
-        }
-    }
+    merge_wave_query<K>(part, slices, klist, nq_pad, q, kk, kind, out_idx, out_dist);
 }
 
 void merge_lists(const uint64_t *part, int64_t slices, int K, int64_t nq, int64_t nq_pad, int kk, int kind,
@@ -109,8 +70,8 @@ void merge_lists(const uint64_t *part, int64_t slices, int K, int64_t nq, int64_
 #define PN_MERGE(KV)                                                                                   \
     do {                                                                                               \
         if (wave)                                                                                      \
-            hipLaunchKernelGGL((merge_lists_wave_kernel<KV>), grid, dim3(256), 0, st, part, slices, nq, nq_pad, \
-                               kk, kind, out_idx, out_dist);                                           \
+            hipLaunchKernelGGL((merge_lists_wave_kernel<KV>), grid, dim3(256), 0, st, part, slices, KV, nq,    \
+                               nq_pad, kk, kind, out_idx, out_dist);                                   \
         else                                                                                           \
             hipLaunchKernelGGL((merge_lists_kernel<KV>), grid, dim3(256), 0, st, part, slices, nq, nq_pad, kk, \
                                kind, out_idx, out_dist);                                               \
@@ -128,20 +89,21 @@ void merge_lists(const uint64_t *part, int64_t slices, int K, int64_t nq, int64_
     PN_HIP(hipGetLastError());
 }
 
-void merge_lists_wave(const uint64_t *part, int64_t slices, int K, int64_t nq, int64_t nq_pad, int kk, int kind,
-                      int32_t *out_idx, void *out_dist, hipStream_t st) {
+void merge_lists_wave(const uint64_t *part, int64_t slices, int klist, int cap, int64_t nq, int64_t nq_pad, int kk,
+                      int kind, int32_t *out_idx, void *out_dist, hipStream_t st) {
     dim3 grid((unsigned)ceil_div(nq, 4));
-#define PN_MERGEW(KV)                                                                                     \
-    hipLaunchKernelGGL((merge_lists_wave_kernel<KV>), grid, dim3(256), 0, st, part, slices, nq, nq_pad, kk, kind, \
-                       out_idx, out_dist)
-    switch (K) {
+#define PN_MERGEW(KV)                                                                                        \
+    hipLaunchKernelGGL((merge_lists_wave_kernel<KV>), grid, dim3(256), 0, st, part, slices, klist, nq, nq_pad, kk, \
+                       kind, out_idx, out_dist)
+    switch (cap) {
     case 1: PN_MERGEW(1); break;
     case 2: PN_MERGEW(2); break;
     case 4: PN_MERGEW(4); break;
     case 8: PN_MERGEW(8); break;
     case 16: PN_MERGEW(16); break;
     case 32: PN_MERGEW(32); break;
-    default: fail(PN_EINVAL, "merge: unsupported list size");
+    case 64: PN_MERGEW(64); break;
+    default: fail(PN_EINVAL, "merge: unsupported list capacity");
     }
 #undef PN_MERGEW
     PN_HIP(hipGetLastError());
