@@ -181,6 +181,33 @@ pn_status pn_ivfl2_search_device(pn_ivfl2 *h, const float *queries, int64_t nq, 
                                  int32_t precision, int32_t *out_idx, float *out_dist, int32_t *out_count,
                                  void *stream);
 
+/* ---------------------------------------------------------------- batched exact search
+ * Many exact L2 searches over independent indices of uneven sizes served together. No single
+ * call of the reference corresponds: each sub-search has the semantics of pn_exact_search_device
+ * on its index (VPTree::searchKNN, VPTree.hpp:143-166) and returns bit-identical results. */
+typedef struct pn_batch pn_batch;
+
+/* Plans n sub-searches: indices[i] (exact L2 indices on one device), device queries[i]
+ * (nq[i] x dim_i float32, contiguous), 1 <= k[i] <= 32, device outputs out_idx[i]
+ * (int32[nq[i], min(k[i], n_i)]) and out_dist[i] (float32, sqrt L2, same shape). The plan binds
+ * these pointers and captures the search (one split, one persistent scan over all tiles of all
+ * indices, one merge, rerank and fallback launch) in a HIP graph; its schedule comes from the
+ * tuning table (batch.slice_rows, batch.wg_per_cu, batch.order). */
+pn_status pn_batch_create(int32_t n, pn_exact *const *indices, const void *const *queries, const int64_t *nq,
+                          const int32_t *k, int32_t *const *out_idx, float *const *out_dist, pn_batch **out);
+void pn_batch_destroy(pn_batch *b);
+
+/* Replays the plan's graph on `stream` (NULL = the null stream); no host synchronisation. */
+pn_status pn_batch_run(pn_batch *b, void *stream);
+
+/* Autotuner: times a set of schedules (slice length, persistent grid size, tile order; `reps`
+ * replays each) and re-plans with the fastest. With table_path, a schedule stored there for
+ * this batch shape is used without measuring, and a measured one is appended. */
+pn_status pn_batch_tune(pn_batch *b, const char *table_path, int32_t reps);
+
+/* Number of scan tiles and the schedule in use: cfg = {slice_rows, wg_per_cu, order}. */
+pn_status pn_batch_info(const pn_batch *b, int64_t *tiles, int32_t *cfg);
+
 #ifdef __cplusplus
 }
 #endif

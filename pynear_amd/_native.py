@@ -57,6 +57,11 @@ SIGNATURES = {
     "pn_ivfbin_export": (_i32, [_vp, _vp, _vp]),
     "pn_ivfbin_search": (_i32, [_vp, _vp, _i64, _i64, _i32, _vp, _vp, _vp]),
     "pn_ivfbin_search_device": (_i32, [_vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp]),
+    "pn_batch_create": (_i32, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    "pn_batch_destroy": (None, [_vp]),
+    "pn_batch_run": (_i32, [_vp, _vp]),
+    "pn_batch_tune": (_i32, [_vp, ctypes.c_char_p, _i32]),
+    "pn_batch_info": (_i32, [_vp, ctypes.POINTER(_i64), _vp]),
 }
 
 _LIB = None
@@ -292,3 +297,42 @@ def pairwise_distance(metric, a, b):
     out = np.empty(n, np.int64 if metric == METRIC_HAMMING else np.float32)
     check(lib().pn_pairwise_distance(metric, ptr(a), ptr(b), n, d, ptr(out)))
     return out
+
+
+class BatchHandle:
+    """Owner of a ``pn_batch`` plan: exact L2 searches over many indices, replayed as one graph.
+
+    ``indices`` are ExactHandle objects (L2); ``queries``/``out_idx``/``out_dist`` are device
+    pointers (ints), one per index; the plan binds them."""
+
+    def __init__(self, indices, queries, nq, k, out_idx, out_dist):
+        n = len(indices)
+        P = _vp * n
+        self._keep = indices
+        hs = P(*[h.handle for h in indices])
+        qs = P(*[_vp(int(p)) for p in queries])
+        oi = P(*[_vp(int(p)) for p in out_idx])
+        od = P(*[_vp(int(p)) for p in out_dist])
+        nqa = (_i64 * n)(*[int(v) for v in nq])
+        ka = (_i32 * n)(*[int(v) for v in k])
+        h = _vp()
+        check(lib().pn_batch_create(n, hs, qs, nqa, ka, oi, od, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _LIB is not None:
+            _LIB.pn_batch_destroy(h)
+            self._h = None
+
+    def run(self, stream_ptr=None):
+        check(lib().pn_batch_run(self._h, _vp(stream_ptr) if stream_ptr else None))
+
+    def tune(self, table_path=None, reps=3):
+        check(lib().pn_batch_tune(self._h, table_path.encode() if table_path else None, int(reps)))
+
+    def info(self):
+        t = _i64(0)
+        cfg = (_i32 * 3)()
+        check(lib().pn_batch_info(self._h, ctypes.byref(t), cfg))
+        return t.value, {"slice_rows": cfg[0], "wg_per_cu": cfg[1], "order": cfg[2]}

@@ -1,0 +1,421 @@
+// pn_batch: exact L2 searches over many independent indices of uneven sizes, served together.
+//
+// Every sub-search is pn_exact_search_device on its own index (VPTree.hpp:143-166 semantics:
+// min(n, k) neighbours, ascending (distance, index), distances bit-identical to the reference's
+// dist_l2_f_avx2). A plan lays the sub-searches out as flat work lists and runs them as
+//   1. one split launch (bf16 hi/lo + norms of every query of every index),
+//   2. one persistent scan launch: all (index, 128-query block, row slice) tiles of all indices in
+//      one queue, longest tiles first; each tile runs the same device body as the single search,
+//   3. one merge, one rerank (with the completeness proof) and one fallback launch over all queries,
+// captured once in a HIP graph that every run replays. Slice length, persistent grid size and
+// tile order are the plan's schedule; pn_batch_tune measures a set of schedules on the device and
+// keeps the fastest (optionally persisted in a text table keyed by the batch's shape).
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <numeric>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "device_util.hpp"
+#include "exact_index.hpp"
+#include "l2_mfma.hpp"
+#include "l2_mfma_dev.hpp"
+#include "merge_dev.hpp"
+
+namespace pn {
+
+// One sub-search of a batch, as the device kernels see it.
+struct BatchEntry {
+    const __bf16 *Xh, *Xl;
+    const float *xn, *X;
+    const unsigned int *xn_max;
+    int64_t xstride, n, d;
+    int dp, kk, KL, K2, K2c;
+    float eps_coef;
+    const float *Q;
+    int64_t qstride, nq, nq_pad, gx, S, rps;
+    __bf16 *Qh, *Ql;
+    float *qn;
+    uint64_t *part;
+    int32_t *cidx, *flags;
+    float *capx;
+    int32_t *out_idx;
+    float *out_dist;
+    int64_t row_base, q_base; // prefixes over the entries: padded query rows, queries
+};
+
+struct BatchCfg {
+    int slice_rows = 512; // target rows per slice (rounded to 32-row tiles)
+    int wg_per_cu = 2;    // persistent scan workgroups per CU
+    int order = 1;        // 1: tiles by decreasing cost; 0: index order
+};
+
+__device__ __forceinline__ int find_entry(const int64_t *base, int n, int64_t x) {
+    int lo = 0, hi = n - 1; // last entry with base[e] <= x
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= x) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void batch_split_kernel(const BatchEntry *__restrict__ E, const int64_t *__restrict__ row_base,
+                                                          int ne, int64_t rows, int32_t *__restrict__ counter) {
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0; // the scan queue of this run
+    if (r >= rows) return;
+    const BatchEntry &e = E[find_entry(row_base, ne, r)];
+    split_row(e.Q, e.nq, e.d, e.qstride, e.dp, e.Qh, e.Ql, e.qn, r - e.row_base);
+}
+
+template <int KL>
+__device__ __forceinline__ void batch_tile(uint64_t *lds_raw, const BatchEntry &e, int64_t qb, int64_t s) {
+    auto lds = reinterpret_cast<uint64_t (*)[32][KL]>(lds_raw);
+    switch (e.dp) {
+    case 16: scan_small<KL, 3, 1>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s); break;
+    case 32: scan_small<KL, 3, 2>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s); break;
+    case 48: scan_small<KL, 3, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s); break;
+    case 64: scan_small<KL, 3, 4>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s); break;
+    default:
+        scan_generic<KL, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.dp, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
+    }
+}
+
+// Persistent: workgroups take tiles (entry, query block, slice) from one queue until it is empty.
+__global__ __launch_bounds__(256) void batch_scan_kernel(const BatchEntry *__restrict__ E,
+                                                         const int2 *__restrict__ tiles, int64_t ntiles,
+                                                         int32_t *__restrict__ counter) {
+    __shared__ uint64_t lds_raw[4 * 32 * 32];
+    __shared__ int64_t s_slot;
+    for (;;) {
+        if (threadIdx.x == 0) s_slot = atomicAdd(counter, 1);
+        __syncthreads();
+        const int64_t slot = s_slot;
+        if (slot >= ntiles) break;
+        const int2 t = tiles[slot];
+        const BatchEntry &e = E[t.x];
+        const int64_t qb = t.y / e.S, s = t.y % e.S;
+        if (e.KL == 8) batch_tile<8>(lds_raw, e, qb, s);
+        else if (e.KL == 16) batch_tile<16>(lds_raw, e, qb, s);
+        else batch_tile<32>(lds_raw, e, qb, s);
+        __syncthreads(); // LDS and s_slot are reused by the next tile
+    }
+}
+
+__global__ __launch_bounds__(256) void batch_merge_kernel(const BatchEntry *__restrict__ E,
+                                                          const int64_t *__restrict__ q_base, int ne, int64_t nq) {
+    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= nq) return;
+    const BatchEntry &e = E[find_entry(q_base, ne, g)];
+    const int64_t q = g - e.q_base;
+    if (e.K2 <= 16)
+        merge_wave_query<16>(e.part, e.S, e.KL, e.nq_pad, q, e.K2c, OUT_F32, e.cidx, e.capx);
+    else if (e.K2 <= 32)
+        merge_wave_query<32>(e.part, e.S, e.KL, e.nq_pad, q, e.K2c, OUT_F32, e.cidx, e.capx);
+    else
+        merge_wave_query<64>(e.part, e.S, e.KL, e.nq_pad, q, e.K2c, OUT_F32, e.cidx, e.capx);
+}
+
+__global__ __launch_bounds__(256) void batch_rerank_kernel(const BatchEntry *__restrict__ E,
+                                                           const int64_t *__restrict__ q_base, int ne, int64_t nq) {
+    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= nq) return;
+    const BatchEntry &e = E[find_entry(q_base, ne, g)];
+    rerank_query(e.X, e.xstride, e.d, e.Q, e.qstride, e.qn, e.xn_max, e.cidx, e.capx, e.K2c, e.part, e.S, e.KL,
+                 e.nq_pad, e.kk, e.n, e.eps_coef, e.out_idx, e.out_dist, e.flags, g - e.q_base);
+}
+
+__global__ __launch_bounds__(256) void batch_fallback_kernel(const BatchEntry *__restrict__ E,
+                                                             const int64_t *__restrict__ q_base, int ne) {
+    __shared__ uint64_t wres[4][32];
+    const BatchEntry &e = E[find_entry(q_base, ne, blockIdx.x)];
+    const int64_t q = blockIdx.x - e.q_base;
+    if (e.flags[q]) fallback_query<32>(wres, e.X, e.xstride, e.n, e.d, e.Q, e.qstride, e.kk, e.out_idx, e.out_dist, q);
+}
+
+static int kl_of(int kk) { return kk <= 6 ? 8 : (kk <= 12 ? 16 : 32); } // as the single search
+
+} // namespace pn
+
+struct pn_batch {
+    int device = 0, cus = 256;
+    std::vector<pn_exact *> idx;
+    std::vector<const float *> q;
+    std::vector<int64_t> nq;
+    std::vector<int32_t> k;
+    std::vector<int32_t *> oi;
+    std::vector<float *> od;
+    pn::BatchCfg cfg;
+    int64_t ntiles = 0, rows = 0, queries = 0;
+    pn::DevBuf entries, row_base, q_base, tiles, counter, scratch;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~pn_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    }
+};
+
+namespace pn {
+
+static std::string batch_signature(const pn_batch *b) {
+    std::ostringstream s;
+    s << "b" << b->idx.size();
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < b->idx.size(); ++i)
+        for (int64_t v : {b->idx[i]->n, b->idx[i]->dim, b->nq[i], (int64_t)b->k[i]}) {
+            h ^= (uint64_t)v;
+            h *= 1099511628211ull;
+        }
+    s << "_" << std::hex << h;
+    return s.str();
+}
+
+// Lays out entries, scratch and tiles for cfg, and captures the whole search in a graph.
+static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
+    const int ne = (int)b->idx.size();
+    std::vector<BatchEntry> E(ne);
+    std::vector<int64_t> rbase(ne), qbase(ne), off(ne);
+    size_t bytes = 0;
+    auto take = [&](size_t n) { // 256-byte aligned sub-allocations of one scratch buffer
+        const size_t o = bytes;
+        bytes += (n + 255) / 256 * 256;
+        return o;
+    };
+    struct Off { size_t qh, ql, qn, part, cidx, capx, flags; };
+    std::vector<Off> O(ne);
+    int64_t rows = 0, queries = 0;
+    std::vector<std::pair<double, int2>> tl;
+    for (int i = 0; i < ne; ++i) {
+        pn_exact *h = b->idx[i];
+        BatchEntry &e = E[i];
+        e.n = h->n;
+        e.d = h->dim;
+        e.dp = h->l2.dp;
+        e.xstride = h->row_elems;
+        e.kk = (int)std::min<int64_t>(b->k[i], h->n);
+        e.KL = kl_of(e.kk);
+        e.K2 = 2 * e.KL;
+        e.K2c = (int)std::min<int64_t>(e.K2, e.n);
+        e.eps_coef = 6.103515625e-05f + (float)e.dp * 4.76837158203125e-07f; // three products per term
+        e.nq = b->nq[i];
+        e.qstride = e.d;
+        e.gx = ceil_div(e.nq, (int64_t)128);
+        e.nq_pad = e.gx * 128;
+        int64_t S = std::max<int64_t>(1, ceil_div(e.n, (int64_t)std::max(32, cfg.slice_rows)));
+        e.rps = ceil_div(ceil_div(e.n, S), (int64_t)32) * 32;
+        e.S = ceil_div(e.n, e.rps);
+        e.row_base = rows;
+        e.q_base = queries;
+        rbase[i] = rows;
+        qbase[i] = queries;
+        rows += e.nq_pad;
+        queries += e.nq;
+        O[i].qh = take((size_t)e.nq_pad * e.dp * 2);
+        O[i].ql = take((size_t)e.nq_pad * e.dp * 2);
+        O[i].qn = take((size_t)e.nq_pad * 4);
+        O[i].part = take((size_t)e.S * e.KL * e.nq_pad * 8);
+        O[i].cidx = take((size_t)e.nq * e.K2c * 4);
+        O[i].capx = take((size_t)e.nq * e.K2c * 4);
+        O[i].flags = take((size_t)e.nq * 4);
+        const double cost = (double)e.rps * (e.dp / 16 + 4); // a tile: rows x (products + keys)
+        for (int64_t t = 0; t < e.gx * e.S; ++t) tl.push_back({cost, make_int2(i, (int)t)});
+    }
+    if (cfg.order == 1)
+        std::stable_sort(tl.begin(), tl.end(), [](const auto &a, const auto &c) { return a.first > c.first; });
+    char *base = (char *)b->scratch.ensure(bytes);
+    for (int i = 0; i < ne; ++i) {
+        pn_exact *h = b->idx[i];
+        BatchEntry &e = E[i];
+        e.Xh = h->l2.hi.as<__bf16>();
+        e.Xl = h->l2.lo.as<__bf16>();
+        e.xn = h->l2.norm.as<float>();
+        e.xn_max = h->l2.norm_max.as<unsigned int>();
+        e.X = (const float *)h->db.p;
+        e.Q = b->q[i];
+        e.Qh = (__bf16 *)(base + O[i].qh);
+        e.Ql = (__bf16 *)(base + O[i].ql);
+        e.qn = (float *)(base + O[i].qn);
+        e.part = (uint64_t *)(base + O[i].part);
+        e.cidx = (int32_t *)(base + O[i].cidx);
+        e.capx = (float *)(base + O[i].capx);
+        e.flags = (int32_t *)(base + O[i].flags);
+        e.out_idx = b->oi[i];
+        e.out_dist = b->od[i];
+    }
+    std::vector<int2> tiles(tl.size());
+    for (size_t t = 0; t < tl.size(); ++t) tiles[t] = tl[t].second;
+    b->ntiles = (int64_t)tiles.size();
+    b->rows = rows;
+    b->queries = queries;
+    hipStream_t st = b->cap_stream;
+    PN_HIP(hipMemcpyAsync(b->entries.ensure(ne * sizeof(BatchEntry)), E.data(), ne * sizeof(BatchEntry),
+                          hipMemcpyHostToDevice, st));
+    PN_HIP(hipMemcpyAsync(b->row_base.ensure(ne * 8), rbase.data(), ne * 8, hipMemcpyHostToDevice, st));
+    PN_HIP(hipMemcpyAsync(b->q_base.ensure(ne * 8), qbase.data(), ne * 8, hipMemcpyHostToDevice, st));
+    PN_HIP(hipMemcpyAsync(b->tiles.ensure(std::max<size_t>(1, tiles.size()) * 8), tiles.data(), tiles.size() * 8,
+                          hipMemcpyHostToDevice, st));
+    b->counter.ensure(8);
+    PN_HIP(hipStreamSynchronize(st));
+    b->cfg = cfg;
+
+    // capture: no memsets inside (the split kernel clears the tile counter)
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    const BatchEntry *dE = b->entries.as<BatchEntry>();
+    const int64_t *dR = b->row_base.as<int64_t>(), *dQ = b->q_base.as<int64_t>();
+    int32_t *ctr = b->counter.as<int32_t>();
+    const unsigned grid_scan =
+        (unsigned)std::max<int64_t>(1, std::min<int64_t>(b->ntiles, (int64_t)b->cus * std::max(1, cfg.wg_per_cu)));
+    PN_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(batch_split_kernel, dim3((unsigned)ceil_div(rows, (int64_t)4)), dim3(256), 0, st, dE, dR, ne,
+                       rows, ctr);
+    hipLaunchKernelGGL(batch_scan_kernel, dim3(grid_scan), dim3(256), 0, st, dE, b->tiles.as<int2>(), b->ntiles, ctr);
+    hipLaunchKernelGGL(batch_merge_kernel, dim3((unsigned)ceil_div(queries, (int64_t)4)), dim3(256), 0, st, dE, dQ, ne,
+                       queries);
+    hipLaunchKernelGGL(batch_rerank_kernel, dim3((unsigned)ceil_div(queries, (int64_t)4)), dim3(256), 0, st, dE, dQ,
+                       ne, queries);
+    hipLaunchKernelGGL(batch_fallback_kernel, dim3((unsigned)queries), dim3(256), 0, st, dE, dQ, ne);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(st, &g);
+    PN_HIP(ce);
+    b->graph = g;
+    PN_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+}
+
+static float batch_time(pn_batch *b, int reps) {
+    hipEvent_t e0, e1;
+    PN_HIP(hipEventCreate(&e0));
+    PN_HIP(hipEventCreate(&e1));
+    PN_HIP(hipGraphLaunch(b->exec, b->cap_stream)); // warm
+    PN_HIP(hipEventRecord(e0, b->cap_stream));
+    for (int r = 0; r < reps; ++r) PN_HIP(hipGraphLaunch(b->exec, b->cap_stream));
+    PN_HIP(hipEventRecord(e1, b->cap_stream));
+    PN_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / std::max(1, reps);
+}
+
+} // namespace pn
+
+using namespace pn;
+
+extern "C" pn_status pn_batch_create(int32_t n, pn_exact *const *indices, const void *const *queries,
+                                     const int64_t *nq, const int32_t *k, int32_t *const *out_idx,
+                                     float *const *out_dist, pn_batch **out) {
+    return guard([&] {
+        PN_REQUIRE(out && n > 0 && indices && queries && nq && k && out_idx && out_dist, PN_EINVAL, "null pointer");
+        auto *b = new pn_batch();
+        std::unique_ptr<pn_batch> hold(b);
+        b->device = indices[0]->device;
+        for (int i = 0; i < n; ++i) {
+            pn_exact *h = indices[i];
+            PN_REQUIRE(h && h->metric == PN_METRIC_L2, PN_EINVAL, "a batch holds exact L2 indices");
+            PN_REQUIRE(h->device == b->device, PN_EINVAL, "a batch's indices live on one device");
+            PN_REQUIRE(h->n > 0, PN_ERUNTIME, "index must be first initialized with .set() function and non empty dataset");
+            PN_REQUIRE(k[i] >= 1 && k[i] <= 32, PN_EINVAL, "batched search serves 1 <= k <= 32");
+            PN_REQUIRE(nq[i] >= 1 && queries[i] && out_idx[i] && out_dist[i], PN_EINVAL, "empty or null sub-search");
+            b->idx.push_back(h);
+            b->q.push_back((const float *)queries[i]);
+            b->nq.push_back(nq[i]);
+            b->k.push_back(k[i]);
+            b->oi.push_back(out_idx[i]);
+            b->od.push_back(out_dist[i]);
+        }
+        DeviceScope ds(b->device);
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) == hipSuccess && cus > 0)
+            b->cus = cus;
+        PN_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+        BatchCfg cfg;
+        cfg.slice_rows = (int)tune_get("batch.slice_rows");
+        cfg.wg_per_cu = (int)tune_get("batch.wg_per_cu");
+        cfg.order = (int)tune_get("batch.order");
+        batch_plan(b, cfg);
+        *out = hold.release();
+    });
+}
+
+extern "C" void pn_batch_destroy(pn_batch *b) {
+    if (!b) return;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(b->device);
+    delete b;
+    if (prev >= 0) (void)hipSetDevice(prev);
+}
+
+extern "C" pn_status pn_batch_run(pn_batch *b, void *stream) {
+    return guard([&] {
+        PN_REQUIRE(b && b->exec, PN_EINVAL, "batch is not planned");
+        DeviceScope ds(b->device);
+        PN_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    });
+}
+
+extern "C" pn_status pn_batch_info(const pn_batch *b, int64_t *tiles, int32_t *cfg /*[3]*/) {
+    return guard([&] {
+        PN_REQUIRE(b, PN_EINVAL, "batch is null");
+        if (tiles) *tiles = b->ntiles;
+        if (cfg) {
+            cfg[0] = b->cfg.slice_rows;
+            cfg[1] = b->cfg.wg_per_cu;
+            cfg[2] = b->cfg.order;
+        }
+    });
+}
+
+extern "C" pn_status pn_batch_tune(pn_batch *b, const char *table_path, int32_t reps) {
+    return guard([&] {
+        PN_REQUIRE(b, PN_EINVAL, "batch is null");
+        DeviceScope ds(b->device);
+        const std::string sig = batch_signature(b);
+        if (table_path && *table_path) { // a stored schedule for this shape wins
+            std::ifstream f(table_path);
+            std::string line;
+            while (std::getline(f, line)) {
+                std::istringstream ls(line);
+                std::string key;
+                BatchCfg c;
+                if ((ls >> key >> c.slice_rows >> c.wg_per_cu >> c.order) && key == sig) {
+                    batch_plan(b, c);
+                    return;
+                }
+            }
+        }
+        BatchCfg best = b->cfg;
+        float best_ms = batch_time(b, std::max(1, (int)reps));
+        for (int rows : {128, 256, 512, 1024, 2048, 4096})
+            for (int w : {1, 2, 4})
+                for (int o : {0, 1}) {
+                    BatchCfg c{rows, w, o};
+                    batch_plan(b, c);
+                    const float ms = batch_time(b, std::max(1, (int)reps));
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = c;
+                    }
+                }
+        batch_plan(b, best);
+        if (table_path && *table_path) {
+            std::ofstream f(table_path, std::ios::app);
+            f << sig << " " << best.slice_rows << " " << best.wg_per_cu << " " << best.order << "\n";
+        }
+    });
+}

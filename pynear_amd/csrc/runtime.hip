@@ -41,6 +41,9 @@ static std::map<std::string, int64_t> &tune_table() {
         t["l2.mfma_terms"] = 3;        // 3: hi.hi + hi.lo + lo.hi products; 1: hi.hi only
         t["ivfl2.wg_per_cu"] = 2;      // IVF-L2 list scan: persistent workgroups per CU
         t["ivfl2.xcd_ranges"] = 0;     // 1: one item range per XCD; 0: one queue for all workgroups
+        t["batch.slice_rows"] = 512;   // batched exact search: target rows per slice
+        t["batch.wg_per_cu"] = 2;      // batched exact search: persistent scan workgroups per CU
+        t["batch.order"] = 1;          // batched exact search: 1 = tiles by decreasing cost
         if (const char *env = std::getenv("PN_TUNE")) {
             std::string e(env);
             size_t pos = 0;

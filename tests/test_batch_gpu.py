@@ -1,0 +1,83 @@
+"""Batched exact L2 search over uneven indices: every sub-search must equal the plain search of its
+index bit for bit (indices and distances), before and after autotuning, on every graph replay."""
+
+import numpy as np
+import pytest
+
+import oracle
+from pynear_amd import VPTreeL2Index
+from pynear_amd.batch import BatchSearch
+
+
+def _make(seed, specs):
+    rng = np.random.default_rng(seed)
+    out = []
+    for n, d, nq, k in specs:
+        X = rng.standard_normal((n, d), dtype=np.float32)
+        Q = rng.standard_normal((nq, d), dtype=np.float32)
+        idx = VPTreeL2Index()
+        idx.set(X)
+        out.append((idx, X, Q, k))
+    return out
+
+
+SPECS = [(1000, 64, 7, 1), (5000, 128, 300, 10), (20000, 256, 40, 32), (3000, 64, 1, 10), (777, 100, 129, 5),
+         (64, 16, 20, 32), (40, 32, 5, 32), (12000, 128, 257, 1), (2500, 8, 33, 12)]
+
+
+def _plain(idx, Q, k):
+    i, d = idx._h.search(Q, k)
+    return i, d
+
+
+@pytest.mark.gpu
+def test_batch_equals_plain_searches():
+    subs = _make(0, SPECS)
+    b = BatchSearch([(idx, Q, k) for idx, X, Q, k in subs])
+    for rep in range(3):  # graph replays
+        b.run()
+        res = b.results()
+        for (idx, X, Q, k), (bi, bd) in zip(subs, res):
+            pi, pd = _plain(idx, Q, k)
+            assert np.array_equal(bi, pi), (rep, X.shape, Q.shape, k)
+            assert np.array_equal(bd, pd)
+
+
+@pytest.mark.gpu
+def test_batch_matches_oracle():
+    subs = _make(1, [(3000, 64, 50, 10), (800, 128, 17, 32), (1500, 20, 9, 1)])
+    b = BatchSearch([(idx, Q, k) for idx, X, Q, k in subs])
+    b.run()
+    for (idx, X, Q, k), (bi, bd) in zip(subs, b.results()):
+        ri, rd = oracle.knn_float(oracle.L2, X, Q, k)
+        assert np.array_equal(bi, ri)
+        assert np.array_equal(bd, rd)
+
+
+@pytest.mark.gpu
+def test_batch_tuned_schedule_keeps_results(tmp_path):
+    subs = _make(2, SPECS[:5])
+    b = BatchSearch([(idx, Q, k) for idx, X, Q, k in subs])
+    b.run()
+    before = b.results()
+    table = str(tmp_path / "tune.txt")
+    b._plan.tune(table, reps=2)
+    tiles, cfg = b.info()
+    assert tiles > 0 and cfg["slice_rows"] in (128, 256, 512, 1024, 2048, 4096)
+    assert open(table).read().split()[1:4] == [str(cfg["slice_rows"]), str(cfg["wg_per_cu"]), str(cfg["order"])]
+    b.run()
+    for (bi, bd), (ai, ad) in zip(b.results(), before):
+        assert np.array_equal(bi, ai) and np.array_equal(bd, ad)
+    # a second plan of the same shape takes the stored schedule without measuring
+    b2 = BatchSearch([(idx, Q, k) for idx, X, Q, k in subs], tune=True, table_path=table)
+    assert b2.info()[1] == cfg
+
+
+@pytest.mark.gpu
+def test_batch_rejects_unsupported():
+    subs = _make(3, [(100, 8, 3, 5)])
+    idx, X, Q, _ = subs[0]
+    with pytest.raises(ValueError):
+        BatchSearch([(idx, Q, 33)])
+    with pytest.raises(ValueError):
+        BatchSearch([(idx, Q[:, :4], 3)])
