@@ -15,6 +15,9 @@ in HBM (the device entry points on a HIP stream of the library's own runtime). W
 * hamming: exact Hamming k-NN, N = 1,000,000 x 512 bits (uniform bytes), 10,000 queries (a random
   row with a Binomial(512, 0.1)-sized random set of bits flipped), k = 10, seed 1.
 * l2: exact L2 k-NN, N = 100,000 x 32 float32 U[0,1), 1,000 queries, k = 5, seed 0.
+* batch: 64 independent exact L2 indices (N ~ LogU[1k, 200k], d in {64, 128, 256}, N(0, 1) rows,
+  Q ~ LogU[1, 4096], k in {1, 10, 32}, seed 3) served by one planned graph (autotuned schedule);
+  results are checked equal to the per-index plain searches.
 
 The roofline of the dominant kernel is measured with HIP events around each launch on its
 stream. The CPU baseline runs on this host: the reference's own VP-tree (built from its sources
@@ -78,6 +81,22 @@ def gmm_workload(n=1_000_000, nq=10_000, d=768, comps=1024, sigma=0.3, seed=2):
         return out
 
     return draw(n), draw(nq)
+
+
+def batch_workload(n_idx=64, seed=3):
+    """64 independent exact L2 indices: N ~ LogU[1k, 200k], d in {64, 128, 256}, rows ~ N(0, 1),
+    Q ~ LogU[1, 4096] queries, k in {1, 10, 32}."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_idx):
+        n = int(round(np.exp(rng.uniform(np.log(1e3), np.log(2e5)))))
+        d = int(rng.choice([64, 128, 256]))
+        nq = max(1, int(round(np.exp(rng.uniform(0.0, np.log(4096))))))
+        k = int(rng.choice([1, 10, 32]))
+        X = rng.standard_normal((n, d), dtype=np.float32)
+        Q = rng.standard_normal((nq, d), dtype=np.float32)
+        out.append((X, Q, k))
+    return out
 
 
 # ---------------------------------------------------------------- helpers
@@ -288,6 +307,74 @@ def run_ivf(steps, warmup, cpu, probes=(8, 16, 32)):
     return out
 
 
+def run_batch(steps, warmup, cpu):
+    from pynear_amd import _native as N
+    from pynear_amd import hipdev, VPTreeL2Index
+    from pynear_amd.batch import BatchSearch
+    subs = batch_workload()
+    _log(f"batch: {len(subs)} indices, {sum(len(X) for X, _, _ in subs):,} rows, "
+         f"{sum(len(Q) for _, Q, _ in subs):,} queries")
+    idx = []
+    for X, Q, k in subs:
+        h = VPTreeL2Index()
+        h.set(X)
+        idx.append(h)
+    b = BatchSearch([(h, Q, k) for h, (X, Q, k) in zip(idx, subs)])
+    t0 = time.perf_counter()
+    b._plan.tune(None, reps=3)  # untimed: the autotuner picks the schedule
+    t_tune = time.perf_counter() - t0
+    tiles, cfg = b.info()
+    stream = hipdev.Stream()
+    gpu_ms, wall, _ = _time_device(N, hipdev, lambda: b.run(stream), steps, warmup, stream)
+    ms = gpu_ms / steps
+    nq_tot = sum(len(Q) for _, Q, _ in subs)
+    # the same 64 searches as separate plain device searches, back to back on one stream
+    dev = [(hipdev.DeviceArray.from_numpy(Q), hipdev.DeviceArray((len(Q), min(k, len(X))), np.int32),
+            hipdev.DeviceArray((len(Q), min(k, len(X))), np.float32)) for X, Q, k in subs]
+
+    def sequential():
+        for h, (X, Q, k), (dq, di, dd) in zip(idx, subs, dev):
+            h._h.search_device(dq.ptr, len(Q), k, di.ptr, dd.ptr, stream.handle)
+    seq_ms, _, _ = _time_device(N, hipdev, sequential, max(3, steps // 4), 1, stream)
+    seq_ms /= max(3, steps // 4)
+    # results must equal the per-index plain searches
+    res = b.results()
+    equal = all(np.array_equal(bi, h._h.search(Q, k)[0]) and np.array_equal(bd, h._h.search(Q, k)[1])
+                for h, (X, Q, k), (bi, bd) in zip(idx, subs, res))
+    flops = sum(2.0 * len(Q) * len(X) * X.shape[1] for X, Q, _ in subs)
+    ideal_ms = sum(max(2.0 * len(Q) * len(X) * X.shape[1] / (PEAK_FP32_TFLOPS * 1e12),
+                       X.nbytes / (PEAK_HBM_GBS * 1e9)) for X, Q, _ in subs) * 1e3
+    out = {"workload": "64 exact L2 indices, N~LogU[1k,200k], d in {64,128,256}, Q~LogU[1,4096], k in {1,10,32}",
+           "value": round(nq_tot / (ms / 1e3), 1), "unit": "queries/s", "ms_per_step": round(ms, 4),
+           "roofline": {"bound": "mfma", "achieved": round(flops / (ms / 1e3) / 1e12, 3), "peak": PEAK_FP32_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(ideal_ms / ms, 4), "traffic": None,
+                        "kernel": "whole batch graph (split, persistent scan, merge, rerank, fallback)",
+                        "kernel_ms": round(ms, 4),
+                        "work": (f"{flops:.3e} flop (sum 2 Q N d); frac = sum over indices of max(flops/fp32 peak, "
+                                 f"fp32 rows/HBM) = {ideal_ms:.4f} ms over the measured time")},
+           "schedule": cfg, "scan_tiles": tiles, "tune_s": round(t_tune, 2), "equal_to_plain_searches": bool(equal),
+           "plain_searches_ms": round(seq_ms, 4), "queries": nq_tot}
+    if cpu:
+        sys.path.insert(0, ROOT)
+        import oracle
+        R = oracle.load_ref()
+        if R is not None:
+            t_total, q_done = 0.0, 0
+            for X, Q, k in subs:
+                m = min(len(Q), 16)
+                r = R.VPTreeL2Index()
+                r.set(X)
+                t0 = time.perf_counter()
+                r.searchKNN(Q[:m], k)
+                t_total += (time.perf_counter() - t0) * len(Q) / m
+                q_done += m
+            out["cpu_baseline"] = {"value": round(nq_tot / t_total, 2), "unit": "queries/s", "cores": _cores(),
+                                   "kind": "reference",
+                                   "sample": (f"{q_done} queries (up to 16 per index) on each index's reference "
+                                              f"VPTreeL2Index (OpenMP), time scaled to all {nq_tot} queries")}
+    return out
+
+
 # ---------------------------------------------------------------- HBM traffic
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
@@ -320,7 +407,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ivf", choices=["ivf", "hamming", "l2"])
+    ap.add_argument("--workload", default="ivf", choices=["ivf", "hamming", "l2", "batch"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="ivf only: skip the exact workloads")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -337,6 +424,9 @@ def main():
     if args.workload == "ivf":
         main_r = run_ivf(args.steps, args.warmup, cpu)
         dtype = "bf16"
+    elif args.workload == "batch":
+        main_r = run_batch(args.steps, args.warmup, cpu)
+        dtype = "f32"
     else:
         main_r = run_exact(args.workload, args.steps, args.warmup, cpu)
         dtype = "u32" if args.workload == "hamming" else "f32"
@@ -354,17 +444,20 @@ def main():
         "vs_baseline": None,
         "dtype": dtype,
         "data": "synthetic",
-        "config": {"workload": main_r["workload"], "global_batch": 10_000 if args.workload != "l2" else 1_000,
+        "config": {"workload": main_r["workload"],
+                   "global_batch": {"l2": 1_000, "batch": main_r.get("queries")}.get(args.workload, 10_000),
                    "parallelism": "single GPU"},
         "roofline": main_r["roofline"],
     }
     for key in ("cpu_baseline", "recall_at_10", "n_probe", "fp64_rel_err", "stages_ms_per_step", "sweep", "setup_s",
-                "cpu_sample_recall_vs_port", "cpu_sample_distances_equal"):
+                "cpu_sample_recall_vs_port", "cpu_sample_distances_equal", "schedule", "scan_tiles",
+                "equal_to_plain_searches", "plain_searches_ms"):
         if key in main_r:
             result[key] = main_r[key]
     if args.workload == "ivf" and not args.no_secondary:
         result["also"] = {"exact_hamming": run_exact("hamming", args.steps, args.warmup, cpu),
-                          "exact_l2": run_exact("l2", max(args.steps, 50), args.warmup, cpu)}
+                          "exact_l2": run_exact("l2", max(args.steps, 50), args.warmup, cpu),
+                          "batched_exact_l2": run_batch(args.steps, args.warmup, cpu)}
     result["tuning"] = {name: N.tune_get(name) for name in
                         ("hamming.wg_per_cu", "hamming.qpl", "float.wg_per_cu", "l2.mfma", "l2.wg_per_cu",
                          "l2.mfma_terms")}
