@@ -181,6 +181,48 @@ pn_status pn_ivfl2_search_device(pn_ivfl2 *h, const float *queries, int64_t nq, 
                                  int32_t precision, int32_t *out_idx, float *out_dist, int32_t *out_count,
                                  void *stream);
 
+/* The list scan and rerank of pn_ivfl2_search_device for probes chosen by the caller (no coarse
+ * stage; used with a learned probe predictor): probes int32[nq, np_max] are k-means cluster ids
+ * (ids of clusters without rows, and slots j >= counts[q], are skipped). No reference counterpart:
+ * the reference always probes the n_probe nearest centroids (forest.py:76-80). */
+pn_status pn_ivfl2_search_probes_device(pn_ivfl2 *h, const float *queries, int64_t nq, const int32_t *probes,
+                                        const int32_t *counts, int32_t np_max, int32_t k, int32_t precision,
+                                        int32_t *out_idx, float *out_dist, int32_t *out_count, void *stream);
+
+/* ---------------------------------------------------------------- learned probe predictor
+ * An MLP d -> hidden (ReLU) -> nlist in bf16 (fp32 accumulation and masters) that ranks a query's
+ * IVF clusters, trained on the device on exact neighbours; its softmax picks an adaptive number
+ * of probes for pn_ivfl2_search_probes_device. No reference counterpart (the reference probes
+ * the n_probe nearest centroids, forest.py:76-80). Parameters are one fp32 vector
+ * [W1 (hidden x d) | b1 (hidden) | W2 (nlist x hidden) | b2 (nlist)]. d, hidden and nlist must be
+ * multiples of 16; training batches too. */
+typedef struct pn_probe pn_probe;
+
+/* He-initialised weights (std::mt19937(seed)), zero biases, Adam state zero. */
+pn_status pn_probe_create(int32_t d, int32_t hidden, int32_t nlist, uint32_t seed, pn_probe **out);
+void pn_probe_destroy(pn_probe *p);
+pn_status pn_probe_get_params(const pn_probe *p, float *params);
+pn_status pn_probe_set_params(pn_probe *p, const float *params);
+
+/* Forward + backward on device inputs x (B x d fp32) and labels (B x L cluster ids: the clusters of
+ * each query's exact top-L neighbours; the target puts 1/L on each). Writes the batch-mean
+ * cross-entropy to *loss_out and the gradient (parameter layout) to grads_out (device; either may
+ * be NULL). */
+pn_status pn_probe_grads_device(pn_probe *p, const float *x, const int32_t *labels, int32_t L, int64_t B,
+                                float *loss_out, float *grads_out, void *stream);
+
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8, learning rate lr) on the batch's gradient. */
+pn_status pn_probe_train_step_device(pn_probe *p, const float *x, const int32_t *labels, int32_t L, int64_t B,
+                                     float lr, float *loss_out, void *stream);
+
+/* logits_out (device, B x nlist fp32) for device queries x (B x d). */
+pn_status pn_probe_forward_device(pn_probe *p, const float *x, int64_t B, float *logits_out, void *stream);
+
+/* Adaptive probes: per query the clusters of highest predicted probability, the fewest whose mass
+ * reaches `mass`, between np_min and np_max; probes int32[B x np_max], counts int32[B] (device). */
+pn_status pn_probe_select_device(pn_probe *p, const float *x, int64_t B, float mass, int32_t np_min, int32_t np_max,
+                                 int32_t *probes, int32_t *counts, void *stream);
+
 /* ---------------------------------------------------------------- batched exact search
  * Many exact L2 searches over independent indices of uneven sizes served together. No single
  * call of the reference corresponds: each sub-search has the semantics of pn_exact_search_device

@@ -57,6 +57,15 @@ SIGNATURES = {
     "pn_ivfbin_export": (_i32, [_vp, _vp, _vp]),
     "pn_ivfbin_search": (_i32, [_vp, _vp, _i64, _i64, _i32, _vp, _vp, _vp]),
     "pn_ivfbin_search_device": (_i32, [_vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp]),
+    "pn_ivfl2_search_probes_device": (_i32, [_vp, _vp, _i64, _vp, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp]),
+    "pn_probe_create": (_i32, [_i32, _i32, _i32, _u32, ctypes.POINTER(_vp)]),
+    "pn_probe_destroy": (None, [_vp]),
+    "pn_probe_get_params": (_i32, [_vp, _vp]),
+    "pn_probe_set_params": (_i32, [_vp, _vp]),
+    "pn_probe_grads_device": (_i32, [_vp, _vp, _vp, _i32, _i64, _vp, _vp, _vp]),
+    "pn_probe_train_step_device": (_i32, [_vp, _vp, _vp, _i32, _i64, ctypes.c_float, _vp, _vp]),
+    "pn_probe_forward_device": (_i32, [_vp, _vp, _i64, _vp, _vp]),
+    "pn_probe_select_device": (_i32, [_vp, _vp, _i64, ctypes.c_float, _i32, _i32, _vp, _vp, _vp]),
     "pn_batch_create": (_i32, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
     "pn_batch_destroy": (None, [_vp]),
     "pn_batch_run": (_i32, [_vp, _vp]),
@@ -260,6 +269,13 @@ class IvfL2Handle:
                                     ptr(idx), ptr(dist), ptr(cnt)))
         return idx, dist, cnt
 
+    def search_probes_device(self, q_ptr, nq, probes_ptr, counts_ptr, np_max, k, out_idx_ptr, out_dist_ptr,
+                             out_cnt_ptr, precision="bf16", stream_ptr=None):
+        check(lib().pn_ivfl2_search_probes_device(self._h, _vp(q_ptr), nq, _vp(probes_ptr), _vp(counts_ptr),
+                                                  int(np_max), int(k), self.PRECISION[precision], _vp(out_idx_ptr),
+                                                  _vp(out_dist_ptr), _vp(out_cnt_ptr),
+                                                  _vp(stream_ptr) if stream_ptr else None))
+
     def search_device(self, q_ptr, nq, nprobe, k, out_idx_ptr, out_dist_ptr, out_cnt_ptr, precision="bf16",
                       stream_ptr=None):
         check(lib().pn_ivfl2_search_device(self._h, _vp(q_ptr), nq, int(nprobe), int(k), self.PRECISION[precision],
@@ -336,3 +352,50 @@ class BatchHandle:
         cfg = (_i32 * 3)()
         check(lib().pn_batch_info(self._h, ctypes.byref(t), cfg))
         return t.value, {"slice_rows": cfg[0], "wg_per_cu": cfg[1], "order": cfg[2]}
+
+
+class ProbeHandle:
+    """Owner of a ``pn_probe`` (learned probe MLP d -> hidden -> nlist on the current device)."""
+
+    def __init__(self, d, hidden, nlist, seed=0):
+        h = _vp()
+        check(lib().pn_probe_create(int(d), int(hidden), int(nlist), int(seed), ctypes.byref(h)))
+        self._h = h
+        self.d, self.hidden, self.nlist = int(d), int(hidden), int(nlist)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _LIB is not None:
+            _LIB.pn_probe_destroy(h)
+            self._h = None
+
+    @property
+    def n_params(self):
+        return self.hidden * self.d + self.hidden + self.nlist * self.hidden + self.nlist
+
+    def get_params(self):
+        out = np.empty(self.n_params, np.float32)
+        check(lib().pn_probe_get_params(self._h, ptr(out)))
+        return out
+
+    def set_params(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        assert params.size == self.n_params
+        check(lib().pn_probe_set_params(self._h, ptr(params)))
+
+    def grads_device(self, x_ptr, labels_ptr, L, B, loss_ptr, grads_ptr, stream_ptr=None):
+        check(lib().pn_probe_grads_device(self._h, _vp(x_ptr), _vp(labels_ptr), int(L), int(B), _vp(loss_ptr),
+                                          _vp(grads_ptr), _vp(stream_ptr) if stream_ptr else None))
+
+    def train_step_device(self, x_ptr, labels_ptr, L, B, lr, loss_ptr=None, stream_ptr=None):
+        check(lib().pn_probe_train_step_device(self._h, _vp(x_ptr), _vp(labels_ptr), int(L), int(B), float(lr),
+                                               _vp(loss_ptr) if loss_ptr else None,
+                                               _vp(stream_ptr) if stream_ptr else None))
+
+    def forward_device(self, x_ptr, B, logits_ptr, stream_ptr=None):
+        check(lib().pn_probe_forward_device(self._h, _vp(x_ptr), int(B), _vp(logits_ptr),
+                                            _vp(stream_ptr) if stream_ptr else None))
+
+    def select_device(self, x_ptr, B, mass, np_min, np_max, probes_ptr, counts_ptr, stream_ptr=None):
+        check(lib().pn_probe_select_device(self._h, _vp(x_ptr), int(B), float(mass), int(np_min), int(np_max),
+                                           _vp(probes_ptr), _vp(counts_ptr), _vp(stream_ptr) if stream_ptr else None))

@@ -48,6 +48,9 @@ struct pn_ivfl2 {
     pn::L2Set lrows;         // bf16 hi/lo of the rows in cluster order + norms
     pn::DevBuf lids, loff;   // cluster-order position -> original id; cluster row offsets [nl + 1]
     pn::DevBuf corder;       // cluster ids by decreasing list length (work-item order)
+    pn::DevBuf clist;        // k-means cluster id -> list index (-1: empty cluster), nc entries
+    int32_t nc = 0;          // k-means clusters (empty ones included)
+    pn::DevBuf pmap;         // per search: given probes mapped to list indices
     pn::DevBuf qpad, qsplit_h, qsplit_l, qn, probes, pdist, lcount, cursor, pair_q, pair_p;
     pn::DevBuf items, ntot /* XCD item ranges */, cand, oidx, odist, ocnt;
 };
@@ -65,12 +68,14 @@ __global__ __launch_bounds__(256) void probe_count_kernel(const int32_t *__restr
     __shared__ int32_t hist[kLdsBins];
     const int64_t c0 = (int64_t)blockIdx.x * kPairChunk, c1 = min(total, c0 + kPairChunk);
     if (nl > kLdsBins) {
-        for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) atomicAdd(&lcount[probes[t]], 1);
+        for (int64_t t = c0 + threadIdx.x; t < c1; t += 256)
+            if (probes[t] >= 0) atomicAdd(&lcount[probes[t]], 1);
         return;
     }
     for (int b = threadIdx.x; b < nl; b += 256) hist[b] = 0;
     __syncthreads();
-    for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) atomicAdd(&hist[probes[t]], 1);
+    for (int64_t t = c0 + threadIdx.x; t < c1; t += 256)
+        if (probes[t] >= 0) atomicAdd(&hist[probes[t]], 1); // -1: no probe in this slot
     __syncthreads();
     for (int b = threadIdx.x; b < nl; b += 256)
         if (hist[b]) atomicAdd(&lcount[b], hist[b]);
@@ -165,6 +170,7 @@ __global__ __launch_bounds__(256) void probe_scatter_kernel(const int32_t *__res
     const int64_t c0 = (int64_t)blockIdx.x * kPairChunk, c1 = min(total, c0 + kPairChunk);
     if (nl > kLdsBins) {
         for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) {
+            if (probes[t] < 0) continue;
             const int pos = atomicAdd(&cursor[probes[t]], 1);
             pair_q[pos] = (int32_t)(t / np);
             pair_p[pos] = (int32_t)(t % np);
@@ -173,16 +179,35 @@ __global__ __launch_bounds__(256) void probe_scatter_kernel(const int32_t *__res
     }
     for (int b = threadIdx.x; b < nl; b += 256) hist[b] = 0;
     __syncthreads();
-    for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) atomicAdd(&hist[probes[t]], 1);
+    for (int64_t t = c0 + threadIdx.x; t < c1; t += 256)
+        if (probes[t] >= 0) atomicAdd(&hist[probes[t]], 1);
     __syncthreads();
     for (int b = threadIdx.x; b < nl; b += 256) // count -> first global slot of this chunk's run
         if (hist[b]) hist[b] = atomicAdd(&cursor[b], hist[b]);
     __syncthreads();
     for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) {
+        if (probes[t] < 0) continue;
         const int pos = atomicAdd(&hist[probes[t]], 1);
         pair_q[pos] = (int32_t)(t / np);
         pair_p[pos] = (int32_t)(t % np);
     }
+}
+
+// Candidate slots of (query, probe) pairs without a probe stay empty.
+__global__ void fill_u64_kernel(uint64_t *__restrict__ p, int64_t n, uint64_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Given probes (cluster ids, counts[q] valid per row of np_max) -> list indices, -1 for none.
+__global__ void map_probes_kernel(const int32_t *__restrict__ probes, const int32_t *__restrict__ counts,
+                                  int64_t nq, int np_max, const int32_t *__restrict__ clist, int nc,
+                                  int32_t *__restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq * np_max) return;
+    const int64_t q = t / np_max;
+    const int j = (int)(t - q * np_max), c = probes[t];
+    out[t] = j < counts[q] && c >= 0 && c < nc ? clist[c] : -1;
 }
 
 // Inserts one tile's 16 keys of one query into the lane's sorted list (Lk, Li) of KL. A key is
@@ -587,6 +612,10 @@ static void ivfl2_set_device(pn_ivfl2 *h, const float *dX, int64_t n, int64_t d,
                      [&](int32_t a, int32_t b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
     int32_t *dco = (int32_t *)h->corder.ensure((size_t)nl * 4);
     PN_HIP(hipMemcpyAsync(dco, order.data(), (size_t)nl * 4, hipMemcpyHostToDevice, st));
+    std::vector<int32_t> clist(nc, -1);
+    for (int32_t j = 0; j < nl; ++j) clist[keep[j]] = j;
+    h->nc = (int32_t)nc;
+    PN_HIP(hipMemcpyAsync(h->clist.ensure((size_t)nc * 4), clist.data(), (size_t)nc * 4, hipMemcpyHostToDevice, st));
     // rows in cluster order -> bf16 split (+ zero rows past the end, see the scan kernel)
     DevBuf ordered;
     float *Xo = (float *)ordered.ensure((size_t)n * d * 4);
@@ -604,6 +633,9 @@ static void ivfl2_set_device(pn_ivfl2 *h, const float *dX, int64_t n, int64_t d,
     exact_set_device(&h->cent, cpp, nl, d, st);
     PN_HIP(hipStreamSynchronize(st)); // host vectors and temporaries are released on return
 }
+
+static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t nq, const int32_t *pr, int np,
+                                bool sparse, int k, int prec, int32_t *oi, float *od, int32_t *oc, hipStream_t st);
 
 static void ivfl2_search_enqueue(pn_ivfl2 *h, const float *dq, int64_t qstride_in, int64_t nq, int nprobe, int k,
                                  int prec, int32_t *oi, float *od, int32_t *oc, hipStream_t st) {
@@ -631,6 +663,14 @@ static void ivfl2_search_enqueue(pn_ivfl2 *h, const float *dq, int64_t qstride_i
         StageScope sc(st, STAGE_COARSE);
         exact_search_enqueue(&h->cent, Q, nq, np, pr, pd, st);
     }
+    ivfl2_lists_enqueue(h, Q, re, nq, pr, np, false, k, prec, oi, od, oc, st);
+}
+
+// The list scan and rerank for given probes pr[nq x np] (list indices; with `sparse`, -1 marks an
+// empty slot). Q has row stride re.
+static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t nq, const int32_t *pr, int np,
+                                bool sparse, int k, int prec, int32_t *oi, float *od, int32_t *oc, hipStream_t st) {
+    const int64_t d = h->d;
     // bf16 split of the queries
     const int dp = h->lrows.dp;
     __bf16 *Qh = (__bf16 *)h->qsplit_h.ensure((size_t)nq * dp * 2);
@@ -649,6 +689,9 @@ static void ivfl2_search_enqueue(pn_ivfl2 *h, const float *dq, int64_t qstride_i
     uint64_t *cand = (uint64_t *)h->cand.ensure((size_t)total * kIvfKP * 8);
     {
         StageScope sc(st, STAGE_OTHER);
+        if (sparse)
+            hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)ceil_div(total * kIvfKP, (int64_t)256)), dim3(256), 0, st,
+                               cand, total * kIvfKP, kKey64None);
         l2_split_rows(Q, nq, d, re, dp, Qh, Ql, qn, st);
         PN_HIP(hipMemsetAsync(lc, 0, (size_t)h->nl * 4, st));
         const unsigned pchunks = (unsigned)ceil_div(total, (int64_t)kPairChunk);
@@ -813,5 +856,35 @@ extern "C" pn_status pn_ivfl2_search_device(pn_ivfl2 *h, const float *queries, i
         DeviceScope ds(h->device);
         ivfl2_search_enqueue(h, queries, h->d, nq, nprobe, k, precision == 1 ? 3 : 1, out_idx, out_dist, out_count,
                              (hipStream_t)stream);
+    });
+}
+
+extern "C" pn_status pn_ivfl2_search_probes_device(pn_ivfl2 *h, const float *queries, int64_t nq,
+                                                   const int32_t *probes, const int32_t *counts, int32_t np_max,
+                                                   int32_t k, int32_t precision, int32_t *out_idx, float *out_dist,
+                                                   int32_t *out_count, void *stream) {
+    return guard([&] {
+        PN_REQUIRE(h, PN_EINVAL, "index is null");
+        PN_REQUIRE(h->n > 0, PN_ERUNTIME, "Index is empty - call set() first");
+        PN_REQUIRE(nq >= 0 && k > 0 && k <= 64 && np_max > 0, PN_EINVAL, "bad size");
+        if (nq == 0) return;
+        PN_REQUIRE(queries && probes && counts && out_idx && out_dist && out_count, PN_EINVAL, "null pointer");
+        DeviceScope ds(h->device);
+        hipStream_t st = (hipStream_t)stream;
+        bool scan_ok = false;
+        const int64_t re = padded_row_elems(PN_METRIC_L2, h->d, &scan_ok);
+        const float *Q = queries;
+        if (re != h->d) {
+            float *qp = (float *)h->qpad.ensure((size_t)nq * re * 4);
+            PN_HIP(hipMemsetAsync(qp, 0, (size_t)nq * re * 4, st));
+            PN_HIP(hipMemcpy2DAsync(qp, (size_t)re * 4, queries, (size_t)h->d * 4, (size_t)h->d * 4, nq,
+                                    hipMemcpyDeviceToDevice, st));
+            Q = qp;
+        }
+        int32_t *pm = (int32_t *)h->pmap.ensure((size_t)nq * np_max * 4);
+        hipLaunchKernelGGL(map_probes_kernel, dim3((unsigned)ceil_div(nq * np_max, (int64_t)256)), dim3(256), 0, st,
+                           probes, counts, nq, np_max, h->clist.as<int32_t>(), h->nc, pm);
+        ivfl2_lists_enqueue(h, Q, re, nq, pm, np_max, true, k, precision == 1 ? 3 : 1, out_idx, out_dist, out_count,
+                            st);
     });
 }
