@@ -18,6 +18,10 @@ in HBM (the device entry points on a HIP stream of the library's own runtime). W
 * batch: 64 independent exact L2 indices (N ~ LogU[1k, 200k], d in {64, 128, 256}, N(0, 1) rows,
   Q ~ LogU[1, 4096], k in {1, 10, 32}, seed 3) served by one planned graph (autotuned schedule);
   results are checked equal to the per-index plain searches.
+* learned: IVF-L2 with a learned probe predictor (MLP 512 -> 256 -> 4096, bf16, trained on the GPU
+  on 200,000 queries' exact top-10) and adaptive n_probe, N = 2,000,000 x 512 from a Zipf(1.1)
+  weighted 4,096-component mixture (sigma 0.25), nlist = 4,096, 10,000 held-out queries, k = 10;
+  compared with fixed-n_probe IVF (bf16x3 ~ fp32 candidate distances, and bf16) at Recall@10 >= 0.95.
 
 The roofline of the dominant kernel is measured with HIP events around each launch on its
 stream. The CPU baseline runs on this host: the reference's own VP-tree (built from its sources
@@ -97,6 +101,26 @@ def batch_workload(n_idx=64, seed=3):
         Q = rng.standard_normal((nq, d), dtype=np.float32)
         out.append((X, Q, k))
     return out
+
+
+def learned_workload(n=2_000_000, d=512, comps=4096, sigma=0.25, n_train=200_000, n_test=10_000):
+    """4096-component Gaussian mixture in d = 512 with Zipf(1.1) component weights; rows (seed 4),
+    training queries (seed 5) and held-out queries (seed 6) from the same mixture."""
+    crng = np.random.default_rng(4)
+    centres = crng.standard_normal((comps, d), dtype=np.float32)
+    w = 1.0 / np.arange(1, comps + 1) ** 1.1
+    w /= w.sum()
+
+    def draw(m, seed):
+        rng = np.random.default_rng(seed)
+        out = np.empty((m, d), np.float32)
+        for s in range(0, m, 65536):
+            e = min(m, s + 65536)
+            lab = rng.choice(comps, e - s, p=w)
+            out[s:e] = centres[lab] + np.float32(sigma) * rng.standard_normal((e - s, d), dtype=np.float32)
+        return out
+
+    return draw(n, 4), draw(n_train, 5), draw(n_test, 6)
 
 
 # ---------------------------------------------------------------- helpers
@@ -375,6 +399,114 @@ def run_batch(steps, warmup, cpu):
     return out
 
 
+def run_learned(steps, warmup, cpu, target=0.95):
+    """Learned-probe IVF-L2 against fixed-nprobe IVF with ~fp32 candidate distances (bf16x3), both at
+    the smallest setting reaching Recall@10 >= target on held-out queries."""
+    from pynear_amd import _native as N
+    from pynear_amd import hipdev
+    from pynear_amd.learned import LearnedProbe
+    k, nlist = 10, 4096
+    t0 = time.perf_counter()
+    X, QT, QE = learned_workload()
+    _log(f"learned: data {X.shape}, train {QT.shape}, test {QE.shape} in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    h = N.IvfL2Handle()
+    labels, cent = h.build(X, nlist, 20, 42)
+    t_build = time.perf_counter() - t0
+    _log(f"learned: ivf built in {t_build:.1f} s")
+    t0 = time.perf_counter()
+    ex = N.ExactHandle(N.METRIC_L2)
+    ex.set(X)
+    gt_train = np.concatenate([ex.search(QT[s:s + 20000], k)[0] for s in range(0, len(QT), 20000)])
+    gt_test = ex.search(QE, k)[0]
+    del ex
+    t_gt = time.perf_counter() - t0
+    _log(f"learned: exact ground truth for {len(QT) + len(QE):,} queries in {t_gt:.1f} s")
+    t0 = time.perf_counter()
+    probe = LearnedProbe(X.shape[1], nlist, hidden=256, seed=0)
+    hist = probe.fit(QT, labels[gt_train].astype(np.int32), epochs=8, batch=1024, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    _log(f"learned: trained in {t_train:.1f} s, loss {hist}")
+    nq = len(QE)
+    dq = hipdev.DeviceArray.from_numpy(QE)
+    oi = hipdev.DeviceArray((nq, k), np.int32)
+    od = hipdev.DeviceArray((nq, k), np.float32)
+    oc = hipdev.DeviceArray((nq,), np.int32)
+    stream = hipdev.Stream()
+
+    def recall():
+        got = oi.numpy()
+        return float(np.mean([len(set(got[i].tolist()) & set(gt_test[i].tolist())) / k for i in range(nq)]))
+
+    def fixed(npb, prec):
+        return lambda: h.search_device(dq.ptr, nq, npb, k, oi.ptr, od.ptr, oc.ptr, prec, stream.handle)
+
+    def sweep_fixed(prec):
+        for npb in (1, 2, 4, 8, 16, 32, 48, 64, 80, 96, 112, 128, 144, 160, 192, 224, 256):
+            fixed(npb, prec)()
+            stream.synchronize()
+            r = recall()
+            if r >= target:
+                ms = _time_device(N, hipdev, fixed(npb, prec), steps, warmup, stream)[0] / steps
+                return {"n_probe": npb, "recall_at_10": round(r, 4), "ms_per_step": round(ms, 4),
+                        "qps": round(nq / (ms / 1e3), 1)}
+        return None
+
+    base = sweep_fixed("bf16x3")
+    fast = sweep_fixed("bf16")
+    _log(f"learned: fixed n_probe bf16x3 {base}, bf16 {fast}")
+    np_max = 256
+
+    def learned(mass):
+        def step():
+            probes, counts = probe.select(dq.ptr, nq, mass, 1, np_max, stream)
+            h.search_probes_device(dq.ptr, nq, probes.ptr, counts.ptr, np_max, k, oi.ptr, od.ptr, oc.ptr, "bf16",
+                                   stream.handle)
+        return step
+
+    lrn = None
+    for mass in (0.8, 0.85, 0.9, 0.92, 0.94, 0.95, 0.96, 0.97, 0.98, 0.99, 0.995, 0.998, 0.999, 0.9995, 0.9999):
+        learned(mass)()
+        stream.synchronize()
+        r = recall()
+        if r >= target:
+            _, counts = probe.select(dq.ptr, nq, mass, 1, np_max)
+            ms = _time_device(N, hipdev, learned(mass), steps, warmup, stream)[0] / steps
+            lrn = {"mass": mass, "mean_n_probe": round(float(counts.numpy().mean()), 2), "recall_at_10": round(r, 4),
+                   "ms_per_step": round(ms, 4), "qps": round(nq / (ms / 1e3), 1)}
+            break
+    _log(f"learned: adaptive {lrn}")
+    out = {"workload": (f"learned-probe IVF-L2, Zipf(1.1) GMM N={len(X):,} x d={X.shape[1]}, nlist={nlist}, "
+                        f"MLP {X.shape[1]}->256->{nlist} bf16 trained on {len(QT):,} queries, {nq:,} held-out, k={k}"),
+           "value": lrn["qps"] if lrn else None, "unit": "queries/s",
+           "ms_per_step": lrn["ms_per_step"] if lrn else None,
+           "learned": lrn, "fixed_nprobe_fp32": base, "fixed_nprobe_bf16": fast,
+           "speedup_vs_fixed_fp32": round(lrn["qps"] / base["qps"], 3) if lrn and base else None,
+           "speedup_vs_fixed_bf16": round(lrn["qps"] / fast["qps"], 3) if lrn and fast else None,
+           "train": {"epochs": len(hist), "loss": [round(v, 4) for v in hist], "seconds": round(t_train, 2)},
+           "setup_s": {"build": round(t_build, 1), "exact_ground_truth": round(t_gt, 1)}}
+    if cpu and base:
+        sys.path.insert(0, ROOT)
+        import oracle
+        sizes = np.bincount(labels, minlength=nlist)
+        keep = np.flatnonzero(sizes)
+        order = np.argsort(labels, kind="stable")
+        starts = np.concatenate([[0], np.cumsum(sizes)])
+        data = [X[order[starts[c]:starts[c + 1]]] for c in keep]
+        ids = [order[starts[c]:starts[c + 1]] for c in keep]
+        norms = [(x * x).sum(1) for x in data]
+        sample = 50
+        t0 = time.perf_counter()
+        oracle.ivf_l2_search(cent[keep], data, norms, ids, QE[:sample], base["n_probe"], k)
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(sample / t, 2), "unit": "queries/s", "cores": _cores(), "kind": "port",
+                               "sample": (f"{sample} of {nq} held-out queries; numpy restatement of the reference's "
+                                          f"IVFFlatL2Index.searchKNN over the same lists at the fixed "
+                                          f"n_probe={base['n_probe']} (the reference has no learned probe)")}
+    del X
+    return out
+
+
 # ---------------------------------------------------------------- HBM traffic
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
@@ -407,7 +539,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ivf", choices=["ivf", "hamming", "l2", "batch"])
+    ap.add_argument("--workload", default="ivf", choices=["ivf", "hamming", "l2", "batch", "learned"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="ivf only: skip the exact workloads")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -427,6 +559,10 @@ def main():
     elif args.workload == "batch":
         main_r = run_batch(args.steps, args.warmup, cpu)
         dtype = "f32"
+    elif args.workload == "learned":
+        main_r = run_learned(args.steps, args.warmup, cpu)
+        main_r.setdefault("roofline", None)
+        dtype = "bf16"
     else:
         main_r = run_exact(args.workload, args.steps, args.warmup, cpu)
         dtype = "u32" if args.workload == "hamming" else "f32"
@@ -451,13 +587,21 @@ def main():
     }
     for key in ("cpu_baseline", "recall_at_10", "n_probe", "fp64_rel_err", "stages_ms_per_step", "sweep", "setup_s",
                 "cpu_sample_recall_vs_port", "cpu_sample_distances_equal", "schedule", "scan_tiles",
-                "equal_to_plain_searches", "plain_searches_ms"):
+                "equal_to_plain_searches", "plain_searches_ms", "learned", "fixed_nprobe_fp32", "fixed_nprobe_bf16",
+                "speedup_vs_fixed_fp32", "speedup_vs_fixed_bf16", "train"):
         if key in main_r:
             result[key] = main_r[key]
     if args.workload == "ivf" and not args.no_secondary:
-        result["also"] = {"exact_hamming": run_exact("hamming", args.steps, args.warmup, cpu),
-                          "exact_l2": run_exact("l2", max(args.steps, 50), args.warmup, cpu),
-                          "batched_exact_l2": run_batch(args.steps, args.warmup, cpu)}
+        also = {}
+        for name, fn in (("exact_hamming", lambda: run_exact("hamming", args.steps, args.warmup, cpu)),
+                         ("exact_l2", lambda: run_exact("l2", max(args.steps, 50), args.warmup, cpu)),
+                         ("batched_exact_l2", lambda: run_batch(args.steps, args.warmup, cpu)),
+                         ("learned_probe_ivf", lambda: run_learned(args.steps, args.warmup, cpu))):
+            try:  # a secondary workload that fails is reported, not fatal to the headline line
+                also[name] = fn()
+            except Exception as e:  # noqa: BLE001
+                also[name] = {"error": f"{type(e).__name__}: {e}"}
+        result["also"] = also
     result["tuning"] = {name: N.tune_get(name) for name in
                         ("hamming.wg_per_cu", "hamming.qpl", "float.wg_per_cu", "l2.mfma", "l2.wg_per_cu",
                          "l2.mfma_terms")}
