@@ -46,7 +46,7 @@ __device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t acc) {
 }
 
 // Ascending bitonic sorting network over 16 keys held in registers (80 compare-exchanges).
-__device__ __forceinline__ void sort16_u32(uint32_t (&k)[16]) {
+template <typename T> __device__ __forceinline__ void sort16(T (&k)[16]) {
 #pragma unroll
     for (int size = 2; size <= 16; size <<= 1)
 #pragma unroll
@@ -55,7 +55,7 @@ __device__ __forceinline__ void sort16_u32(uint32_t (&k)[16]) {
             for (int i = 0; i < 16; ++i) {
                 const int j = i ^ stride;
                 if (j > i) {
-                    const uint32_t lo = min(k[i], k[j]), hi = max(k[i], k[j]);
+                    const T lo = k[i] < k[j] ? k[i] : k[j], hi = k[i] < k[j] ? k[j] : k[i];
                     const bool asc = (i & size) == 0;
                     k[i] = asc ? lo : hi;
                     k[j] = asc ? hi : lo;

@@ -220,7 +220,7 @@ template <int KL, bool FIRST>
 __device__ __forceinline__ void list_insert16(uint32_t (&key)[16], uint32_t (&Lk)[KL], uint32_t (&Li)[KL],
                                               uint32_t base, uint32_t &thr) {
     if (FIRST) {
-        sort16_u32(key);
+        sort16(key);
 #pragma unroll
         for (int j = 0; j < KL; ++j) {
             Lk[j] = j < 16 && key[j] < thr ? key[j] : kKey32None;

@@ -21,37 +21,10 @@
 #include "exact_index.hpp"
 #include "kernels.hpp"
 #include "kmeans_l2.hpp"
+#include "ref_arith.hpp"
 
 namespace pn {
 
-// Squared-sum core of dist_l2_f_avx2 with 8 lanes per row: lane c runs FMA chain c (elements
-// c, c+8, ...), then the fold, the fused tails and the horizontal adds; the result is valid in
-// lane 0 of the group.
-__device__ __forceinline__ float l2_core_group8(const float *a, const float *b, int64_t d, int c) {
-    const int64_t nb = d / 8 * 8;
-    float acc = 0.f;
-    for (int64_t i = c; i < nb; i += 8) {
-        const float t = a[i] - b[i];
-        acc = fmaf(t, t, acc);
-    }
-    const float hi = __shfl_down(acc, 4, 8);
-    float m = hi + acc; // lanes 0..3: m[c] = acc[c + 4] + acc[c]
-    if (c < 4) {
-        int64_t i = nb, rem = d - nb;
-        if (rem >= 4) {
-            const float t = a[i + c] - b[i + c];
-            m = fmaf(t, t, m);
-            i += 4;
-            rem -= 4;
-        }
-        if (rem > 0) {
-            const float t = c < rem ? a[i + c] - b[i + c] : 0.f;
-            m = fmaf(t, t, m);
-        }
-    }
-    const float p = m + __shfl_down(m, 1, 8); // lane 0: m0 + m1; lane 2: m2 + m3
-    return p + __shfl_down(p, 2, 8);
-}
 
 __global__ __launch_bounds__(256) void kpp_l2_kernel(const float *__restrict__ X, int64_t n, int64_t d, int64_t stride,
                                                      const float *__restrict__ c, float *__restrict__ min_sq) {

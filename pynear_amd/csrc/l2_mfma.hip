@@ -96,6 +96,151 @@ __global__ __launch_bounds__(256) void l2_fallback_kernel(const float *__restric
     if (flags[blockIdx.x]) fallback_query<K>(wres, X, xstride, n, d, Q, qstride, kk, out_idx, out_dist, blockIdx.x);
 }
 
+// ---------------------------------------------------------------- small-database path
+// For a few thousand rows of large d (IVF coarse centroids, k-means centroids) the approximate
+// distances of a query chunk to all rows are one product (bf16, three terms per K step) written
+// as a key matrix; a wave per query then takes the K2c smallest (key, row) pairs, and the exact
+// rerank, proof and fallback are the scan path's. A workgroup computes a 128 x 128 block (wave:
+// 64 x 64 as 2 x 2 MFMA tiles); rows are clamped on load and masked on store.
+template <int PREC>
+__global__ __launch_bounds__(256) void l2_matrix_keys_kernel(const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
+                                                             const float *__restrict__ qn, int M,
+                                                             const __bf16 *__restrict__ Xh, const __bf16 *__restrict__ Xl,
+                                                             const float *__restrict__ xn, int N, int dp,
+                                                             uint32_t *__restrict__ keys) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
+    const int m0 = blockIdx.y * 128 + (w >> 1) * 64, n0 = blockIdx.x * 128 + (w & 1) * 64;
+    int64_t ao[2], bo[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        ao[i] = (int64_t)min(m0 + 32 * i + col, M - 1) * dp + 8 * half;
+        bo[i] = (int64_t)min(n0 + 32 * i + col, N - 1) * dp + 8 * half;
+    }
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x16{};
+    // K steps in groups of CH: the group's loads are issued before its products
+    constexpr int CH = PREC == 3 ? 2 : 4;
+    int k0 = 0;
+    for (; k0 + 16 * CH <= dp; k0 += 16 * CH) {
+        bf16x8 a[CH][2], b[CH][2], al[CH][2], bl[CH][2];
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                a[u][i] = *(const bf16x8 *)(Qh + ao[i] + k0 + 16 * u);
+                b[u][i] = *(const bf16x8 *)(Xh + bo[i] + k0 + 16 * u);
+                if (PREC == 3) {
+                    al[u][i] = *(const bf16x8 *)(Ql + ao[i] + k0 + 16 * u);
+                    bl[u][i] = *(const bf16x8 *)(Xl + bo[i] + k0 + 16 * u);
+                }
+            }
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u][i], b[u][j], acc[i][j], 0, 0, 0);
+                    if (PREC == 3) {
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u][i], bl[u][j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[u][i], b[u][j], acc[i][j], 0, 0, 0);
+                    }
+                }
+    }
+    for (; k0 < dp; k0 += 16) {
+        bf16x8 a[2], b[2], al[2], bl[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            a[i] = *(const bf16x8 *)(Qh + ao[i] + k0);
+            b[i] = *(const bf16x8 *)(Xh + bo[i] + k0);
+            if (PREC == 3) {
+                al[i] = *(const bf16x8 *)(Ql + ao[i] + k0);
+                bl[i] = *(const bf16x8 *)(Xl + bo[i] + k0);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                if (PREC == 3) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bl[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], b[j], acc[i][j], 0, 0, 0);
+                }
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int n = n0 + 32 * j + col;
+        if (n >= N) continue;
+        const float xv = xn[n];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + 32 * i + 8 * (r >> 2) + 4 * half + (r & 3);
+                if (m < M) keys[(int64_t)m * N + n] = f32_key(fmaxf(0.f, (qn[m] + xv) - 2.f * acc[i][j][r]));
+            }
+    }
+}
+
+// One wave per query: the K2c smallest (key, row) of its key row, ascending, into cidx / capx.
+// A lane sorts its keys 16 at a time (rows lane, lane + 64, ...) and keeps its 16 smallest; the
+// wave then takes the K2c smallest of the 64 lane lists. capx[K2c - 1] is lowered to the smallest
+// key any lane dropped, so it stays a lower bound of every approximation left out.
+__global__ __launch_bounds__(256) void l2_matrix_select_kernel(const uint32_t *__restrict__ keys, int N, int64_t M,
+                                                               int K2c, int32_t *__restrict__ cidx,
+                                                               float *__restrict__ capx) {
+    const int lane = threadIdx.x & 63;
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= M) return;
+    const uint32_t *row = keys + q * N;
+    uint64_t L[16], C[16];
+    uint64_t dropped = kKey64None;
+    for (int c0 = 0; c0 < N; c0 += 16 * 64) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int c = c0 + 64 * j + lane;
+            C[j] = c < N ? (((uint64_t)row[c] << 32) | (uint32_t)c) : kKey64None;
+        }
+        sort16(C);
+        if (c0 == 0) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) L[j] = C[j];
+        } else { // keep the 16 smallest of L and C: bitonic split, then sort the lower half
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const uint64_t a = L[j], b = C[15 - j];
+                L[j] = a < b ? a : b;
+                const uint64_t hi = a < b ? b : a;
+                dropped = hi < dropped ? hi : dropped;
+            }
+            sort16(L);
+        }
+    }
+    int head = 0;
+    for (int r = 0; r < K2c; ++r) {
+        uint64_t mine = kKey64None;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) mine = j == head ? L[j] : mine;
+        const uint64_t m = wave_min_u64(mine);
+        if (mine == m && m != kKey64None) ++head;
+        if (lane == 0) {
+            cidx[q * K2c + r] = m == kKey64None ? -1 : (int32_t)(uint32_t)(m & 0xFFFFFFFFull);
+            capx[q * K2c + r] = key_f32((uint32_t)(m >> 32));
+        }
+    }
+    const uint64_t dmin = wave_min_u64(dropped);
+    if (lane == 0 && dmin != kKey64None) {
+        const float dv = key_f32((uint32_t)(dmin >> 32));
+        const float last = capx[q * K2c + K2c - 1];
+        capx[q * K2c + K2c - 1] = dv < last ? dv : last;
+    }
+}
+
 void l2_prepare(L2Set &s, const float *X, int64_t n, int64_t d, int64_t stride, hipStream_t st, int align) {
     s.n = n;
     s.d = d;
@@ -143,6 +288,60 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
     float *qn = (float *)qs.norm.ensure((size_t)nq_pad * 4);
     hipLaunchKernelGGL(l2_split_kernel, dim3((unsigned)ceil_div(nq_pad, 4)), dim3(256), 0, st, Q, nq, nq_pad, d,
                        qstride, dp, Qh, Ql, qn);
+    const int prec_m = (int)tune_get("l2.mfma_terms");
+    if (n <= 4096 && dp > 64 && tune_get("l2.matrix") != 0) {
+        // small database of wide rows: key matrix per query chunk (<= 256 MB), then selection
+        int32_t *cidx = (int32_t *)wk.cidx.ensure((size_t)nq * K2c * 4);
+        float *capx = (float *)wk.capx.ensure((size_t)nq * K2c * 4);
+        int32_t *flags = (int32_t *)wk.flags.ensure((size_t)nq * 4);
+        const int64_t chunk = std::max<int64_t>(128, std::min<int64_t>(nq_pad, ((int64_t)1 << 26) / n / 128 * 128));
+        uint32_t *keys = (uint32_t *)wk.part.ensure((size_t)chunk * n * 4);
+        for (int64_t c0 = 0; c0 < nq; c0 += chunk) {
+            const int64_t m = std::min<int64_t>(chunk, nq - c0);
+            {
+                StageScope sc(st, STAGE_SCAN);
+                dim3 grid((unsigned)ceil_div(n, (int64_t)128), (unsigned)ceil_div(m, (int64_t)128));
+                if (prec_m == 1)
+                    hipLaunchKernelGGL((l2_matrix_keys_kernel<1>), grid, dim3(256), 0, st, Qh + c0 * dp, Ql + c0 * dp,
+                                       qn + c0, (int)m, xs.hi.as<__bf16>(), xs.lo.as<__bf16>(), xs.norm.as<float>(),
+                                       (int)n, dp, keys);
+                else
+                    hipLaunchKernelGGL((l2_matrix_keys_kernel<3>), grid, dim3(256), 0, st, Qh + c0 * dp, Ql + c0 * dp,
+                                       qn + c0, (int)m, xs.hi.as<__bf16>(), xs.lo.as<__bf16>(), xs.norm.as<float>(),
+                                       (int)n, dp, keys);
+            }
+            {
+                StageScope sc(st, STAGE_SELECT);
+                dim3 g((unsigned)ceil_div(m, (int64_t)4));
+                hipLaunchKernelGGL(l2_matrix_select_kernel, g, dim3(256), 0, st, keys, (int)n, m, K2c, cidx + c0 * K2c,
+                                   capx + c0 * K2c);
+            }
+            PN_HIP(hipGetLastError());
+        }
+        {
+            StageScope sc(st, STAGE_RERANK);
+            const float eps_coef = (prec_m == 1 ? 0.0078125f : 6.103515625e-05f) + (float)dp * 4.76837158203125e-07f;
+            hipLaunchKernelGGL(l2_rerank_kernel, dim3((unsigned)ceil_div(nq, (int64_t)4)), dim3(256), 0, st, X, xstride,
+                               d, Q, qstride, qn, xs.norm_max.as<unsigned int>(), cidx, capx, K2c,
+                               (const uint64_t *)nullptr, (int64_t)0, KL, nq_pad, nq, kk, n, eps_coef, out_idx,
+                               out_dist, flags);
+            PN_HIP(hipGetLastError());
+        }
+        {
+            StageScope sc(st, STAGE_OTHER);
+            if (kk <= 8)
+                hipLaunchKernelGGL((l2_fallback_kernel<8>), dim3((unsigned)nq), dim3(256), 0, st, X, xstride, n, d, Q,
+                                   qstride, flags, kk, out_idx, out_dist);
+            else if (kk <= 16)
+                hipLaunchKernelGGL((l2_fallback_kernel<16>), dim3((unsigned)nq), dim3(256), 0, st, X, xstride, n, d, Q,
+                                   qstride, flags, kk, out_idx, out_dist);
+            else
+                hipLaunchKernelGGL((l2_fallback_kernel<32>), dim3((unsigned)nq), dim3(256), 0, st, X, xstride, n, d, Q,
+                                   qstride, flags, kk, out_idx, out_dist);
+            PN_HIP(hipGetLastError());
+        }
+        return;
+    }
     // slices: whole multiples of the CU count, at least 128 rows each
     const int64_t wpc = std::max<int64_t>(1, tune_get("l2.wg_per_cu"));
     int64_t S = std::max<int64_t>(1, wpc * cus / gx);
@@ -198,7 +397,8 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
     int32_t *flags = (int32_t *)wk.flags.ensure((size_t)nq * 4);
     {
         StageScope sc(st, STAGE_SELECT);
-        merge_lists_wave(part, nlists, KL, K2, nq, nq_pad, K2c, OUT_F32, cidx, capx, st); // capacity >= K2c
+        // a lane that merges one slice holds at most KL keys; with more slices per lane it needs K2c
+        merge_lists_wave(part, nlists, KL, nlists <= 64 ? KL : K2, nq, nq_pad, K2c, OUT_F32, cidx, capx, st);
     }
     {
         StageScope sc(st, STAGE_RERANK);

@@ -51,7 +51,7 @@ __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4
         mn = min(mn, key[r]);
     }
     if (FIRST) { // empty list: the tile's KL smallest keys, sorted, by a network instead of KL rounds
-        sort16_u32(key);
+        sort16(key);
 #pragma unroll
         for (int j = 0; j < KL; ++j) {
             Lk[j] = j < 16 ? key[j] : kKey32None;
@@ -253,15 +253,24 @@ __device__ __forceinline__ void rerank_query(const float *__restrict__ X, int64_
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) bmin = min(bmin, (uint32_t)__shfl_xor((int)bmin, off, 64));
 
-    uint64_t key = kKey64None;
+    // exact squared sums of the K2c candidates: 8-lane groups (lane c of a group runs the
+    // reference's FMA chain c, reading 8 consecutive floats with its group), 8 candidates a round;
+    // candidate j's sum ends in lane j
+    const int32_t myidx = lane < K2c ? cidx[q * K2c + lane] : -1;
+    const int grp = lane >> 3, c8 = lane & 7;
     float core = 0.f;
-    if (lane < K2c) {
-        const int32_t idx = cidx[q * K2c + lane];
-        if (idx >= 0 && idx < n) {
-            core = l2_core(Q + q * qstride, X + (int64_t)idx * xstride, d);
-            key = ((uint64_t)f32_key(sqrtf(core)) << 32) | (uint32_t)idx;
-        }
+    for (int base = 0; base < K2c; base += 8) {
+        const int cand = base + grp;
+        const int32_t idx = __shfl(myidx, cand & 63, 64);
+        float v = 0.f;
+        if (cand < K2c && idx >= 0 && idx < n) // uniform over the group
+            v = l2_core_group8(Q + q * qstride, X + (int64_t)idx * xstride, d, c8);
+        const bool owner = lane >= base && lane < base + 8;
+        const float got = __shfl(v, owner ? 8 * (lane - base) : 0, 64);
+        if (owner) core = got;
     }
+    uint64_t key = kKey64None;
+    if (lane < K2c && myidx >= 0 && myidx < n) key = ((uint64_t)f32_key(sqrtf(core)) << 32) | (uint32_t)myidx;
     float dk_core = 0.f;
     for (int r = 0; r < kk; ++r) {
         const uint64_t m = wave_min_u64(key);

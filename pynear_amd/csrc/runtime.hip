@@ -39,6 +39,7 @@ static std::map<std::string, int64_t> &tune_table() {
         t["l2.mfma"] = 1;              // exact L2 (k <= 32) through the MFMA candidate scan + rerank
         t["l2.wg_per_cu"] = 2;
         t["l2.mfma_terms"] = 3;        // 3: hi.hi + hi.lo + lo.hi products; 1: hi.hi only
+        t["l2.matrix"] = 1;            // exact L2 over <= 4096 rows of d > 64: key matrix + selection
         t["ivfl2.wg_per_cu"] = 2;      // IVF-L2 list scan: persistent workgroups per CU
         t["ivfl2.xcd_ranges"] = 0;     // 1: one item range per XCD; 0: one queue for all workgroups
         t["batch.slice_rows"] = 512;   // batched exact search: target rows per slice
