@@ -52,7 +52,7 @@ struct pn_ivfl2 {
     int32_t nc = 0;          // k-means clusters (empty ones included)
     pn::DevBuf pmap;         // per search: given probes mapped to list indices
     pn::DevBuf qpad, qsplit_h, qsplit_l, qn, probes, pdist, lcount, cursor, pair_q, pair_p;
-    pn::DevBuf items, ntot /* XCD item ranges */, cand, oidx, odist, ocnt;
+    pn::DevBuf items, ntot /* item ranges */, itoff, woff, cand, oidx, odist, ocnt;
 };
 
 namespace pn {
@@ -85,24 +85,26 @@ constexpr int kIvfIQ = 64;  // queries (pairs of one cluster) per work item
 constexpr int kIvfKC = 64;  // dims per K chunk
 constexpr int kIvfQS = 72;  // LDS row stride (bf16) of a staged query chunk: 16 B of pad spreads the banks
 
-// One workgroup: the work items (cluster, first pair, count <= 64), cluster by cluster in the
-// order corder (lists by decreasing length), and per cluster its first pair slot (pair cursor).
-// Items are then split into nranges contiguous ranges of about equal work (an item's work is its
-// list length + 64). With nranges = kXcds, range b is run by the workgroups with id % kXcds == b,
-// which round-robin dispatch places on one XCD, so a long list that many queries probe is read
-// into one or two L2s instead of all eight; with nranges = 1 (the default: measured faster, the
-// per-XCD ranges balance worse than one queue) every workgroup takes from one range. Inside a
-// range the longest items start first. Writes per range its first item, its item count, and a
-// zeroed work counter.
+// Work items are (cluster, first pair, count <= 64), cluster by cluster in the order corder
+// (lists by decreasing length, so the longest items come first). build_items_kernel (one
+// workgroup) scans the per-cluster pair, item and work counts (an item's work is its list length
+// + 64) and sets each cluster's first pair slot (pair cursor); expand_items_kernel writes the items
+// in parallel. Items are split into nranges contiguous ranges of about equal work. With
+// nranges = kXcds, range b is run by the workgroups with id % kXcds == b, which round-robin
+// dispatch places on one XCD, so a long list that many queries probe is read into one or two L2s
+// instead of all eight; with nranges = 1 (the default: measured faster, the per-XCD ranges
+// balance worse than one queue) every workgroup takes from one range. bins holds per range its
+// first item, its item count and a zeroed work counter; bins[3 kXcds] the item total and
+// bins[3 kXcds + 1 ..] (as int64) the total work.
 constexpr int kXcds = 8;
 __global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restrict__ lcount,
                                                           const int32_t *__restrict__ corder,
                                                           const int64_t *__restrict__ loff, int nl,
-                                                          int32_t *__restrict__ cursor, int4 *__restrict__ items,
-                                                          int32_t *__restrict__ bins /*[3][kXcds]*/, int nranges) {
+                                                          int32_t *__restrict__ cursor, int32_t *__restrict__ itoff,
+                                                          int64_t *__restrict__ woff, int32_t *__restrict__ bins,
+                                                          int nranges) {
     __shared__ int32_t s_pairs[256], s_items[256];
     __shared__ int64_t s_work[256];
-    __shared__ int32_t s_cnt[kXcds];
     const int tid = threadIdx.x;
     const int per = (nl + 255) / 256;
     const int o0 = min(nl, tid * per), o1 = min(nl, o0 + per);
@@ -118,7 +120,6 @@ __global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restr
     s_pairs[tid] = pairs;
     s_items[tid] = nit;
     s_work[tid] = work;
-    if (tid < kXcds) s_cnt[tid] = 0;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) { // inclusive scans
         const int a = tid >= off ? s_pairs[tid - off] : 0, b = tid >= off ? s_items[tid - off] : 0;
@@ -129,35 +130,53 @@ __global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restr
         s_work[tid] += w;
         __syncthreads();
     }
-    const double to_range = (double)nranges / (double)max<int64_t>(s_work[255], 1);
     int pa = s_pairs[tid] - pairs, ia = s_items[tid] - nit;
     int64_t wa = s_work[tid] - work;
-    int cnt[kXcds] = {};
     for (int o = o0; o < o1; ++o) {
         const int c = corder[o], m = lcount[c];
-        const int64_t wi = loff[c + 1] - loff[c] + 64;
+        const int it = (m + kIvfIQ - 1) / kIvfIQ;
         cursor[c] = pa;
-        for (int s = 0; s < m; s += kIvfIQ) {
-            items[ia++] = make_int4(c, pa + s, min(kIvfIQ, m - s), 0);
-            const int b = min(nranges - 1, (int)((double)(wa + wi / 2) * to_range)); // monotone in item order
-#pragma unroll
-            for (int x = 0; x < kXcds; ++x) cnt[x] += x == b;
-            wa += wi;
-        }
+        itoff[o] = ia;
+        woff[o] = wa;
         pa += m;
+        ia += it;
+        wa += (int64_t)it * (loff[c + 1] - loff[c] + 64);
     }
-#pragma unroll
-    for (int x = 0; x < kXcds; ++x)
-        if (cnt[x]) atomicAdd(&s_cnt[x], cnt[x]);
-    __syncthreads();
-    if (tid == 0) {
-        int start = 0;
-        for (int x = 0; x < kXcds; ++x) {
-            bins[x] = start;             // first item of range x
-            bins[kXcds + x] = s_cnt[x];  // items in range x
-            bins[2 * kXcds + x] = 0;     // next item to take
-            start += s_cnt[x];
-        }
+    if (tid < kXcds) { // one thread per range writes its three words
+        const bool all = nranges == 1 && tid == 0; // a single range holds every item
+        bins[tid] = all ? 0 : INT32_MAX;              // first item (else lowered by expand_items_kernel)
+        bins[kXcds + tid] = all ? s_items[255] : 0;   // items in the range
+        bins[2 * kXcds + tid] = 0;                    // next item to take
+    }
+    if (tid == 255) {
+        bins[3 * kXcds] = s_items[255];
+        *(int64_t *)(bins + 3 * kXcds + 2) = s_work[255];
+    }
+}
+
+// One thread per item g: its cluster (the last corder position o with itoff[o] <= g), its pairs
+// and its range.
+__global__ void expand_items_kernel(const int32_t *__restrict__ lcount, const int32_t *__restrict__ corder,
+                                    const int64_t *__restrict__ loff, int nl, const int32_t *__restrict__ cursor,
+                                    const int32_t *__restrict__ itoff, const int64_t *__restrict__ woff,
+                                    int64_t max_items, int nranges, int4 *__restrict__ items,
+                                    int32_t *__restrict__ bins) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= max_items || g >= bins[3 * kXcds]) return;
+    int lo = 0, hi = nl - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (itoff[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const int c = corder[lo], m = lcount[c];
+    const int s = (int)(g - itoff[lo]);
+    items[g] = make_int4(c, cursor[c] + kIvfIQ * s, min(kIvfIQ, m - kIvfIQ * s), 0);
+    if (nranges > 1) { // (with one range, build_items_kernel has set it to all items)
+        const int64_t W = max<int64_t>(*(const int64_t *)(bins + 3 * kXcds + 2), 1);
+        const int64_t wi = loff[c + 1] - loff[c] + 64;
+        const int b = min(nranges - 1, (int)((double)(woff[lo] + s * wi + wi / 2) * nranges / (double)W));
+        atomicMin(&bins[b], (int)g); // ranges are contiguous: b is monotone in g
+        atomicAdd(&bins[kXcds + b], 1);
     }
 }
 
@@ -289,12 +308,12 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
     // workgroups take the items of their range (per XCD: id % kXcds; else the one range) in order
     // until it is empty
     const int xb = nranges == 1 ? 0 : blockIdx.x % kXcds;
-    const int first = bins[xb], count = bins[kXcds + xb];
+    const int first = bins[xb], count = bins[kXcds + xb], total_items = bins[3 * kXcds];
     for (;;) {
     if (tid == 0) s_slot = atomicAdd(&bins[2 * kXcds + xb], 1);
     __syncthreads();
     const int slot = s_slot;
-    if (slot >= count) break;
+    if (slot >= count || (int64_t)first + slot >= total_items) break; // (the second test never fires)
     const int4 it = items[first + slot];
     const int c = it.x, start = it.y, cnt = it.z;
     if (tid < IQ) s_q[tid] = pair_q[start + min(tid, cnt - 1)];
@@ -684,7 +703,9 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     // each cluster has at most one partly filled item
     const int64_t max_items = ceil_div(total, kIvfIQ) + h->nl;
     int4 *items = (int4 *)h->items.ensure((size_t)max_items * 16);
-    int32_t *bins = (int32_t *)h->ntot.ensure(3 * kXcds * 4);
+    int32_t *bins = (int32_t *)h->ntot.ensure((3 * kXcds + 4) * 4);
+    int32_t *itoff = (int32_t *)h->itoff.ensure((size_t)h->nl * 4);
+    int64_t *woff = (int64_t *)h->woff.ensure((size_t)h->nl * 8);
     const int nranges = tune_get("ivfl2.xcd_ranges") ? kXcds : 1;
     uint64_t *cand = (uint64_t *)h->cand.ensure((size_t)total * kIvfKP * 8);
     {
@@ -697,7 +718,10 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         const unsigned pchunks = (unsigned)ceil_div(total, (int64_t)kPairChunk);
         hipLaunchKernelGGL(probe_count_kernel, dim3(pchunks), dim3(256), 0, st, pr, total, h->nl, lc);
         hipLaunchKernelGGL(build_items_kernel, dim3(1), dim3(256), 0, st, lc, h->corder.as<int32_t>(),
-                           h->loff.as<int64_t>(), h->nl, cur, items, bins, nranges);
+                           h->loff.as<int64_t>(), h->nl, cur, itoff, woff, bins, nranges);
+        hipLaunchKernelGGL(expand_items_kernel, dim3((unsigned)ceil_div(max_items, (int64_t)256)), dim3(256), 0, st, lc,
+                           h->corder.as<int32_t>(), h->loff.as<int64_t>(), h->nl, cur, itoff, woff, max_items, nranges,
+                           items, bins);
         hipLaunchKernelGGL(probe_scatter_kernel, dim3(pchunks), dim3(256), 0, st, pr, total, np, h->nl, cur, pq, pp);
     }
     {

@@ -48,7 +48,21 @@ __device__ inline float l2_core(const float *a, const float *b, int64_t d) {
 __device__ __forceinline__ float l2_core_group8(const float *a, const float *b, int64_t d, int c) {
     const int64_t nb = d / 8 * 8;
     float acc = 0.f;
-    for (int64_t i = c; i < nb; i += 8) {
+    int64_t i = c;
+    for (; i + 56 < nb; i += 64) { // the loads of 8 steps first, then the chain in order
+        float ta[8], tb[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            ta[u] = a[i + 8 * u];
+            tb[u] = b[i + 8 * u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const float t = ta[u] - tb[u];
+            acc = fmaf(t, t, acc);
+        }
+    }
+    for (; i < nb; i += 8) {
         const float t = a[i] - b[i];
         acc = fmaf(t, t, acc);
     }
