@@ -32,7 +32,8 @@
 namespace pn {
 
 
-constexpr int kIvfKL = 16; // per-lane candidate list
+// per-lane candidate list of the list scan: 8 for k <= 16, else 16 (a (query, list) pair keeps the
+// best of 8 such lists, each over an eighth of the list's rows)
 constexpr int kIvfKP = 32; // candidates kept per (query, probe)
 constexpr int kIvfK2 = 64; // candidates reranked per query
 
@@ -286,13 +287,13 @@ __device__ __forceinline__ void list_insert16(uint32_t (&key)[16], uint32_t (&Lk
 // row block, lane (c, h) of wave w turns its 32 accumulators of query c (+ 32 j) into keys
 // |q|^2 + |x|^2 - 2 q.x and keeps the KL smallest (key, row position) per query; at the end the
 // 8 lists (4 waves x 2 halves) of each query are merged into its KP best for this probe.
-template <int PREC>
+template <int PREC, int KL>
 __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
     const __bf16 *__restrict__ Lh, const __bf16 *__restrict__ Ll, const float *__restrict__ lxn,
     const int64_t *__restrict__ loff, int dp, const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
     const float *__restrict__ qn, const int4 *__restrict__ items, int32_t *__restrict__ bins, int nranges,
     const int32_t *__restrict__ pair_q, const int32_t *__restrict__ pair_p, int np, uint64_t *__restrict__ cand) {
-    constexpr int KL = kIvfKL, KP = kIvfKP, QS = kIvfQS, IQ = kIvfIQ;
+    constexpr int KP = kIvfKP, QS = kIvfQS, IQ = kIvfIQ;
     constexpr int NPL = PREC == 3 ? 2 : 1;      // staged planes: hi (and lo)
     constexpr int PLANE = IQ * QS;              // bf16 elements of one staged plane
     constexpr int BUF = PLANE * NPL;            // one query-chunk buffer
@@ -730,14 +731,19 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         const int64_t per_xcd = std::max<int64_t>(1, ceil_div((int64_t)h->cus, (int64_t)kXcds) *
                                                          std::max<int64_t>(1, tune_get("ivfl2.wg_per_cu")));
         const dim3 grid((unsigned)std::min<int64_t>(per_xcd, max_items) * kXcds);
+        auto launch = [&](auto pc, auto kc) {
+            hipLaunchKernelGGL((ivfl2_scan_kernel<decltype(pc)::value, decltype(kc)::value>), grid, dim3(256), 0, st,
+                               h->lrows.hi.as<__bf16>(), h->lrows.lo.as<__bf16>(), h->lrows.norm.as<float>(),
+                               h->loff.as<int64_t>(), dp, Qh, Ql, qn, items, bins, nranges, pq, pp, np, cand);
+        };
+        using P1 = std::integral_constant<int, 1>;
+        using P3 = std::integral_constant<int, 3>;
+        using K8 = std::integral_constant<int, 8>;
+        using K16 = std::integral_constant<int, 16>;
         if (prec == 3)
-            hipLaunchKernelGGL((ivfl2_scan_kernel<3>), grid, dim3(256), 0, st, h->lrows.hi.as<__bf16>(),
-                               h->lrows.lo.as<__bf16>(), h->lrows.norm.as<float>(), h->loff.as<int64_t>(), dp, Qh, Ql,
-                               qn, items, bins, nranges, pq, pp, np, cand);
+            k <= 16 ? launch(P3{}, K8{}) : launch(P3{}, K16{});
         else
-            hipLaunchKernelGGL((ivfl2_scan_kernel<1>), grid, dim3(256), 0, st, h->lrows.hi.as<__bf16>(),
-                               h->lrows.lo.as<__bf16>(), h->lrows.norm.as<float>(), h->loff.as<int64_t>(), dp, Qh, Ql,
-                               qn, items, bins, nranges, pq, pp, np, cand);
+            k <= 16 ? launch(P1{}, K8{}) : launch(P1{}, K16{});
         PN_HIP(hipGetLastError());
     }
     {
