@@ -416,6 +416,7 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
         // the 8 lists of a query: the other half-wave's at once, the other waves' as posted after
         // their previous row block.
         const int64_t rbase = r0 + 256 * rb + 64 * w;
+        const int rlim = (int)min<int64_t>(r1 - rbase, 64); // rows of this wave's block inside the list
         uint32_t thr[2];
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
@@ -426,9 +427,12 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
+            // 32-bit offsets from the tile's first row; rows past the list end are clamped / masked
+            const float *xb = lxn + rbase + 32 * i;
+            const int lim = rlim - 32 * i; // rows of this tile inside the list (may be <= 0)
             float xr[16];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) xr[r] = lxn[min(rbase + 32 * i + 8 * (r >> 2) + 4 * half + (r & 3), r1 - 1)];
+            for (int r = 0; r < 16; ++r) xr[r] = xb[max(0, min(8 * (r >> 2) + 4 * half + (r & 3), lim - 1))];
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 const float qv = j == 0 ? qn0 : qn1;
@@ -437,7 +441,7 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
                 for (int r = 0; r < 16; ++r) {
                     const int off = 8 * (r >> 2) + 4 * half + (r & 3);
                     const float v = fmaxf(0.f, (qv + xr[r]) - 2.f * acc[i][j][r]);
-                    key[r] = rbase + 32 * i + off < r1 ? ((f32_key(v) & ~31u) | (uint32_t)off) : kKey32None;
+                    key[r] = off < lim ? ((f32_key(v) & ~31u) | (uint32_t)off) : kKey32None;
                 }
                 thr[j] = min(thr[j], (uint32_t)__shfl_xor((int)thr[j], 32, 64));
                 const uint32_t base = (uint32_t)(rbase + 32 * i);
