@@ -43,6 +43,7 @@ METRIC = "QPS @Recall@10>=0.95 (IVF-L2 d768 N1M) + exact L2/Hamming QPS, % roofl
 # MI355X published peaks
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
+PEAK_FP8_TFLOPS = 5000.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_INT32_VALU_TOPS = 39.3   # 256 CUs x 64 lanes x 2.4 GHz, one 32-bit op per lane per clock
 
@@ -253,54 +254,71 @@ def run_ivf(steps, warmup, cpu, probes=(8, 16, 32)):
     oc = hipdev.DeviceArray((nq,), np.int32)
     stream = hipdev.Stream()
     sizes = np.bincount(labels, minlength=nlist)
-    sweep, results = {}, {}
-    for npb in probes:
-        hi, hd, hc = h.search(Q, npb, k)
-        results[npb] = (hi, hd, hc)
-        recall = float(np.mean([len(set(hi[i, :hc[i]].tolist()) & set(gt[i].tolist())) / k for i in range(nq)]))
-        gpu_ms, wall, stages = _time_device(
-            N, hipdev, lambda: h.search_device(dq.ptr, nq, npb, k, oi.ptr, od.ptr, oc.ptr, "bf16", stream.handle),
-            steps, warmup, stream)
-        sweep[npb] = {"qps": round(nq * steps / (gpu_ms / 1e3), 1), "ms_per_step": round(gpu_ms / steps, 4),
-                      "recall_at_10": round(recall, 4),
-                      "stages_ms_per_step": {s: round(v[0] / steps, 4) for s, v in stages.items() if v[1]},
-                      "scan_launch_ms": stages["scan"][0] / max(stages["scan"][1], 1)}
-        _log(f"ivf: n_probe={npb} {sweep[npb]}")
-    ok = [p for p in probes if sweep[p]["recall_at_10"] >= 0.95]
-    chosen = ok[0] if ok else probes[-1]
-    S = sweep[chosen]
+    sweeps, results = {}, {}
+    for prec in ("bf16", "fp8"):  # candidate distances: bf16 or fp8 e4m3 products, fp32 rerank either way
+        sweep = sweeps[prec] = {}
+        for npb in probes:
+            hi, hd, hc = h.search(Q, npb, k, prec)
+            results[prec, npb] = (hi, hd, hc)
+            recall = float(np.mean([len(set(hi[i, :hc[i]].tolist()) & set(gt[i].tolist())) / k for i in range(nq)]))
+            gpu_ms, wall, stages = _time_device(
+                N, hipdev, lambda: h.search_device(dq.ptr, nq, npb, k, oi.ptr, od.ptr, oc.ptr, prec, stream.handle),
+                steps, warmup, stream)
+            sweep[npb] = {"qps": round(nq * steps / (gpu_ms / 1e3), 1), "ms_per_step": round(gpu_ms / steps, 4),
+                          "recall_at_10": round(recall, 4),
+                          "stages_ms_per_step": {s: round(v[0] / steps, 4) for s, v in stages.items() if v[1]},
+                          "scan_launch_ms": stages["scan"][0] / max(stages["scan"][1], 1)}
+            _log(f"ivf: {prec} n_probe={npb} {sweep[npb]}")
+    # per precision the smallest n_probe reaching Recall@10 >= 0.95; the faster of the two is reported
+    best = {}
+    for prec, sweep in sweeps.items():
+        ok = [p for p in probes if sweep[p]["recall_at_10"] >= 0.95]
+        if ok:
+            best[prec] = ok[0]
+    if not best:
+        best = {"bf16": probes[-1]}
+    prec = max(best, key=lambda pr_: sweeps[pr_][best[pr_]]["qps"])
+    chosen = best[prec]
+    S = sweeps[prec][chosen]
     scan_s = S["scan_launch_ms"] / 1e3
-    dp = (d + 15) // 16 * 16
-    # algorithmic work of one scan launch: the bf16 rows of every probed cluster once, and 2 d
-    # flops per (query, probed row); the probes are recomputed here in fp64 (exact top-n_probe)
+    dp = (d + 63) // 64 * 64
+    ebytes = 1 if prec == "fp8" else 2
+    kname = "ivfl2_scan_fp8_kernel<8>" if prec == "fp8" else "ivfl2_scan_kernel<1, 8>"
+    peak_mm = PEAK_FP8_TFLOPS if prec == "fp8" else PEAK_BF16_TFLOPS
+    # algorithmic work of one scan launch: the rows of every probed cluster once (bf16: 2 B, fp8: 1 B
+    # per dim) and 2 d flops per (query, probed row); the probes are recomputed here in fp64
+    # (exact top-n_probe)
     keep = np.flatnonzero(sizes)
     C64 = cent[keep].astype(np.float64)
     pr = np.argpartition((C64 * C64).sum(1)[None, :] - 2.0 * (Q.astype(np.float64) @ C64.T), chosen - 1,
                          axis=1)[:, :chosen]
     probed_rows = sizes[keep][pr]
-    bytes_ = float(sizes[keep][np.unique(pr)].sum() * dp * 2)
+    bytes_ = float(sizes[keep][np.unique(pr)].sum() * dp * ebytes)
     flops = 2.0 * d * float(probed_rows.sum())
     bw = bytes_ / scan_s / 1e9
     tf = flops / scan_s / 1e12
-    if bw / PEAK_HBM_GBS >= tf / PEAK_BF16_TFLOPS:
+    if bw / PEAK_HBM_GBS >= tf / peak_mm:
         roof = {"bound": "hbm", "achieved": round(bw, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(bw / PEAK_HBM_GBS, 4)}
     else:
-        roof = {"bound": "mfma", "achieved": round(tf, 2), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(tf / PEAK_BF16_TFLOPS, 4)}
-    roof.update({"traffic": measured_traffic("ivfl2_scan_kernel<1>"), "kernel": "ivfl2_scan_kernel<1>", "kernel_ms": round(scan_s * 1e3, 4),
-                 "work": f"{bytes_:.3e} B (bf16 rows of the probed clusters) and {flops:.3e} flop per launch",
+        roof = {"bound": "mfma", "achieved": round(tf, 2), "peak": peak_mm, "unit": "TFLOP/s",
+                "frac": round(tf / peak_mm, 4)}
+    roof.update({"traffic": measured_traffic(kname), "kernel": kname, "kernel_ms": round(scan_s * 1e3, 4),
+                 "work": f"{bytes_:.3e} B ({prec} rows of the probed clusters) and {flops:.3e} flop per launch",
                  "other_bound": {"hbm_GBs": round(bw, 1), "mfma_TFLOPs": round(tf, 2)}})
     out = {"workload": (f"IVFFlatL2Index GMM N={n:,} x d={d} (1,024 comps, sigma 0.3), nlist={nlist}, "
-                        f"Q={nq:,}, k={k}, n_probe={chosen} (smallest with Recall@10 >= 0.95)"),
+                        f"Q={nq:,}, k={k}, n_probe={chosen} (smallest with Recall@10 >= 0.95), "
+                        f"{prec} candidate distances + fp32 rerank of 64"),
            "value": S["qps"], "unit": "queries/s", "ms_per_step": S["ms_per_step"], "roofline": roof,
-           "recall_at_10": S["recall_at_10"], "n_probe": chosen,
+           "dtype": "fp8e4m3+f32" if prec == "fp8" else "bf16+f32",
+           "precision": prec, "recall_at_10": S["recall_at_10"], "n_probe": chosen,
            "stages_ms_per_step": S["stages_ms_per_step"],
-           "sweep": {str(p): {kk: v for kk, v in s.items() if kk != "scan_launch_ms"} for p, s in sweep.items()},
+           "sweep": {pr_: {str(p): {kk: v for kk, v in s.items() if kk != "scan_launch_ms"} for p, s in sw.items()}
+                     for pr_, sw in sweeps.items()},
            "setup_s": {"generate": round(t_gen, 1), "build_kmeans_lists": round(t_build, 1),
                        "exact_ground_truth": round(t_exact, 1)}}
     # error of the returned squared distances against fp64 recomputation
-    hi, hd, hc = results[chosen]
+    hi, hd, hc = results[prec, chosen]
     ok = hi >= 0
     rows = X[np.where(ok, hi, 0)].astype(np.float64)
     ref = ((rows - Q[:, None, :].astype(np.float64)) ** 2).sum(-1)
@@ -325,7 +343,7 @@ def run_ivf(steps, warmup, cpu, probes=(8, 16, 32)):
                                "sample": (f"{sample} of {nq} queries; numpy restatement of the reference's "
                                           f"IVFFlatL2Index.searchKNN over the same {len(keep)} clusters at "
                                           f"n_probe={chosen} (BLAS threads = host cores)")}
-        hi, _, hc = h.search(Q[:sample], chosen, k)
+        hi, _, hc = h.search(Q[:sample], chosen, k, prec)
         out["cpu_sample_recall_vs_port"] = round(float(np.mean(
             [len(set(hi[i, :hc[i]].tolist()) & set(ci[i])) / k for i in range(sample)])), 4)
     return out
@@ -555,7 +573,7 @@ def main():
     cpu = not args.no_cpu_baseline
     if args.workload == "ivf":
         main_r = run_ivf(args.steps, args.warmup, cpu)
-        dtype = "bf16"
+        dtype = main_r["dtype"]
     elif args.workload == "batch":
         main_r = run_batch(args.steps, args.warmup, cpu)
         dtype = "f32"
@@ -585,9 +603,9 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": main_r["roofline"],
     }
-    for key in ("cpu_baseline", "recall_at_10", "n_probe", "fp64_rel_err", "stages_ms_per_step", "sweep", "setup_s",
+    for key in ("cpu_baseline", "precision", "recall_at_10", "n_probe", "fp64_rel_err", "stages_ms_per_step", "sweep", "setup_s",
                 "cpu_sample_recall_vs_port", "cpu_sample_distances_equal", "schedule", "scan_tiles",
-                "equal_to_plain_searches", "plain_searches_ms", "learned", "fixed_nprobe_fp32", "fixed_nprobe_bf16",
+                "equal_to_plain_searches", "plain_searches_ms", "sweep_fp8", "learned", "fixed_nprobe_fp32", "fixed_nprobe_bf16",
                 "speedup_vs_fixed_fp32", "speedup_vs_fixed_bf16", "train"):
         if key in main_r:
             result[key] = main_r[key]

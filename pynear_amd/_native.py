@@ -229,7 +229,7 @@ class IvfBinHandle:
 class IvfL2Handle:
     """Owner of a ``pn_ivfl2`` handle (the IVFFlatL2Index search on the current device)."""
 
-    PRECISION = {"bf16": 0, "bf16x3": 1}
+    PRECISION = {"bf16": 0, "bf16x3": 1, "fp8": 2}
 
     def __init__(self):
         h = _vp()

@@ -17,6 +17,7 @@
 // Clusters are stored contiguously in cluster order (ids ascending within a cluster); empty
 // clusters are left out, as the reference's per-cluster arrays leave them out (forest.py:51-59).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -52,6 +53,10 @@ struct pn_ivfl2 {
     pn::DevBuf clist;        // k-means cluster id -> list index (-1: empty cluster), nc entries
     int32_t nc = 0;          // k-means clusters (empty ones included)
     pn::DevBuf pmap;         // per search: given probes mapped to list indices
+    pn::DevBuf l8;           // fp8 e4m3 residuals x - c (cluster order) scaled by sx8, dp bytes per row
+    pn::DevBuf rn8;          // |x - c|^2 per row (cluster order)
+    pn::DevBuf lcent;        // list centroids, fp32 nl x d (list order)
+    float sx8 = 1.f;
     pn::DevBuf qpad, qsplit_h, qsplit_l, qn, probes, pdist, lcount, cursor, pair_q, pair_p;
     pn::DevBuf items, ntot /* item ranges */, itoff, woff, cand, oidx, odist, ocnt;
 };
@@ -276,6 +281,270 @@ __device__ __forceinline__ void list_insert16(uint32_t (&key)[16], uint32_t (&Lk
     }
 }
 
+// Row block done: keys |q|^2 + |x|^2 - 2 q.x (q.x = acc * dscale of the query) of the wave's
+// 64 rows (from row rbase; rows at or past r1 are masked) for its two queries per lane, into the
+// lane's lists. Thresholds are shared by the 8 lists of a query: the other half-wave's at once,
+// the other waves' as posted after their previous row block.
+template <int KL>
+__device__ __forceinline__ void rowblock_keys(const f32x16 (&acc)[2][2], float dscale0, float dscale1,
+                                              const float *__restrict__ lxn, int64_t rbase, int64_t r1, bool first,
+                                              float qn0, float qn1, uint32_t (&Lk)[2][KL], uint32_t (&Li)[2][KL],
+                                              uint32_t (*s_thr)[kIvfIQ], int w, int half, int col) {
+    const int rlim = (int)min<int64_t>(r1 - rbase, 64); // rows of this wave's block inside the list
+    uint32_t thr[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        uint32_t m = min(Lk[j][KL - 1], s_thr[0][32 * j + col]);
+#pragma unroll
+        for (int l = 1; l < 8; ++l) m = min(m, s_thr[l][32 * j + col]);
+        thr[j] = m;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        // 32-bit offsets from the tile's first row; rows past the list end are clamped / masked
+        const int lim = rlim - 32 * i; // rows of this tile inside the list (may be <= 0)
+        const float *xb = lim > 0 ? lxn + rbase + 32 * i : lxn + r1 - 1; // (no read past the list)
+        float xr[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) xr[r] = xb[max(0, min(8 * (r >> 2) + 4 * half + (r & 3), lim - 1))];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const float qv = j == 0 ? qn0 : qn1;
+            const float m2 = -2.f * (j == 0 ? dscale0 : dscale1);
+            uint32_t key[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int off = 8 * (r >> 2) + 4 * half + (r & 3);
+                const float v = fmaxf(0.f, fmaf(m2, acc[i][j][r], qv + xr[r]));
+                key[r] = off < lim ? ((f32_key(v) & ~31u) | (uint32_t)off) : kKey32None;
+            }
+            thr[j] = min(thr[j], (uint32_t)__shfl_xor((int)thr[j], 32, 64));
+            const uint32_t base = (uint32_t)(rbase + 32 * i);
+            if (first && i == 0)
+                list_insert16<KL, true>(key, Lk[j], Li[j], base, thr[j]);
+            else
+                list_insert16<KL, false>(key, Lk[j], Li[j], base, thr[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) s_thr[w * 2 + half][32 * j + col] = thr[j];
+}
+
+// Item done: the 8 lists of each query (4 waves x 2 halves) go to LDS (smem; the query buffers
+// are no longer read), then one lane per query merges its KP best into cand[query][probe].
+template <int KL>
+__device__ __forceinline__ void merge_item_lists(unsigned char *smem, const uint32_t (&Lk)[2][KL],
+                                                 const uint32_t (&Li)[2][KL], const int32_t *s_q,
+                                                 const int32_t *__restrict__ pair_p, int cnt, int np,
+                                                 uint64_t *__restrict__ cand, int w, int half, int col) {
+    constexpr int IQ = kIvfIQ, KP = kIvfKP;
+    const int tid = threadIdx.x;
+    uint64_t *sl = (uint64_t *)smem; // [8 lists][64 queries][KL]
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int e = 0; e < KL; ++e)
+            sl[((w * 2 + half) * IQ + 32 * j + col) * KL + e] =
+                Lk[j][e] == kKey32None ? kKey64None : (((uint64_t)Lk[j][e] << 32) | Li[j][e]);
+    __syncthreads();
+    if (tid < cnt) {
+        uint64_t *out = cand + ((int64_t)s_q[tid] * np + pair_p[tid]) * KP;
+        int p[8];
+        uint64_t head[8];
+#pragma unroll
+        for (int l = 0; l < 8; ++l) {
+            p[l] = 0;
+            head[l] = sl[(l * IQ + tid) * KL];
+        }
+        for (int e = 0; e < KP; ++e) {
+            int best = 0;
+            uint64_t bv = head[0];
+#pragma unroll
+            for (int l = 1; l < 8; ++l)
+                if (head[l] < bv) {
+                    bv = head[l];
+                    best = l;
+                }
+            out[e] = bv;
+#pragma unroll
+            for (int l = 0; l < 8; ++l)
+                if (l == best && bv != kKey64None) {
+                    p[l] += 1;
+                    head[l] = p[l] < KL ? sl[(l * IQ + tid) * KL + p[l]] : kKey64None;
+                }
+        }
+    }
+}
+
+typedef int v8i_t __attribute__((ext_vector_type(8)));
+union V8 {
+    v8i_t v;
+    uint4 u[2];
+};
+
+// Rows as residuals to their list's centroid, r = x - c (cluster order; the row's list is found in
+// loff). MODE 0: the largest |r| over all rows into *amax (float bits). MODE 1: |r|^2 into rn[row]
+// and r * scale as fp8 e4m3 (OCP, round to nearest even, clamped to +-448) into L8 (dp bytes per
+// row, zero padding). One wave per row.
+template <int MODE>
+__global__ __launch_bounds__(256) void residual_kernel(const float *__restrict__ X, int64_t n, int d, int dp,
+                                                       const int64_t *__restrict__ loff, int nl,
+                                                       const float *__restrict__ lcent, float scale,
+                                                       unsigned int *__restrict__ amax, uint8_t *__restrict__ L8,
+                                                       float *__restrict__ rn) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    int lo = 0, hi = nl - 1; // last list with loff[list] <= row
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (loff[mid] <= row) lo = mid; else hi = mid - 1;
+    }
+    const float *x = X + row * d, *c = lcent + (int64_t)lo * d;
+    if (MODE == 0) {
+        float m = 0.f;
+        for (int e = lane; e < d; e += 64) m = fmaxf(m, fabsf(x[e] - c[e]));
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        if (lane == 0) atomicMax(amax, __float_as_uint(m));
+        return;
+    }
+    float ss = 0.f;
+    for (int e = 4 * lane; e < dp; e += 256) {
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float r = e + u < d ? x[e + u] - c[e + u] : 0.f;
+            ss = fmaf(r, r, ss);
+            v[u] = fminf(fmaxf(r * scale, -448.f), 448.f);
+        }
+        int w = 0;
+        w = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], w, false);
+        w = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], w, true);
+        *(int *)(L8 + row * dp + e) = w;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+    if (lane == 0) rn[row] = ss;
+}
+
+// The list scan in fp8 e4m3 on residuals to the list's centroid c: |q - x|^2 =
+// |q - c|^2 + |x - c|^2 - 2 (q - c).(x - c), so the quantised vectors are the short residuals.
+// Per item the workgroup builds its 64 query residuals (q - c) * scale in LDS once (and their
+// |q - c|^2 in fp32); the K loop then needs no staging: per 64-dim chunk one
+// v_mfma_scale_f32_32x32x64_f8f6f4 (unit scales) per (row tile, query tile), lane l holding bytes
+// 32 (l/32) .. +31 of row / query l % 32 of the chunk; the products are scaled back by
+// 1 / scale^2. Row fragments are loaded one chunk ahead as in the bf16 scan.
+template <int KL>
+__global__ __launch_bounds__(256, 2) void ivfl2_scan_fp8_kernel(
+    const uint8_t *__restrict__ L8, const float *__restrict__ rn, const int64_t *__restrict__ loff, int d, int dp,
+    const float *__restrict__ lcent, float scale, const float *__restrict__ Q, int64_t qstride,
+    const int4 *__restrict__ items, int32_t *__restrict__ bins, int nranges, const int32_t *__restrict__ pair_q,
+    const int32_t *__restrict__ pair_p, int np, uint64_t *__restrict__ cand) {
+    constexpr int IQ = kIvfIQ;
+    extern __shared__ __align__(16) unsigned char dsm[]; // max(IQ x (dp + 16), lists) bytes
+    __shared__ int32_t s_q[IQ];
+    __shared__ uint32_t s_thr[8][IQ];
+    __shared__ float s_qr[IQ];
+    __shared__ int s_slot;
+    const int QS = dp + 16; // LDS row stride of the residual block (16 B of pad spreads the banks)
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5, col = lane & 31;
+    const float dscale = 1.f / (scale * scale);
+    const int xb = nranges == 1 ? 0 : blockIdx.x % kXcds;
+    const int first = bins[xb], count = bins[kXcds + xb], total_items = bins[3 * kXcds];
+    for (;;) {
+        if (tid == 0) s_slot = atomicAdd(&bins[2 * kXcds + xb], 1);
+        __syncthreads();
+        const int slot = s_slot;
+        if (slot >= count || (int64_t)first + slot >= total_items) break;
+        const int4 it = items[first + slot];
+        const int c = it.x, start = it.y, cnt = it.z;
+        if (tid < IQ) s_q[tid] = pair_q[start + min(tid, cnt - 1)];
+        for (int e = tid; e < 8 * IQ; e += 256) s_thr[e / IQ][e % IQ] = kKey32None;
+        __syncthreads();
+        { // query residuals, one query per wave at a time: lane l converts dims 4 l + 256 t .. +3
+            const float *cc = lcent + (int64_t)c * d;
+            for (int qq = w; qq < IQ; qq += 4) {
+                const float *qrow = Q + (int64_t)s_q[qq] * qstride;
+                float ss = 0.f;
+                for (int e0 = 4 * lane; e0 < dp; e0 += 256) {
+                    float v[4];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const float r = e0 + t < d ? qrow[e0 + t] - cc[e0 + t] : 0.f;
+                        ss = fmaf(r, r, ss);
+                        v[t] = fminf(fmaxf(r * scale, -448.f), 448.f);
+                    }
+                    int wq = 0;
+                    wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], wq, false);
+                    wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], wq, true);
+                    *(int *)(dsm + qq * QS + e0) = wq;
+                }
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+                if (lane == 0) s_qr[qq] = ss;
+            }
+        }
+        __syncthreads();
+        const int64_t r0 = loff[c], r1 = loff[c + 1];
+        const int nch = dp / kIvfKC;
+        const int nrb = (int)((r1 - r0 + 255) / 256);
+        const float qr0 = s_qr[col], qr1 = s_qr[col + 32];
+        V8 acur[2], anext[2];
+        auto load_a = [&](int rb, int ch, V8 (&a)[2]) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int64_t row = min(r0 + 256 * rb + 64 * w + 32 * i + col, r1 - 1);
+                const uint4 *src = (const uint4 *)(L8 + row * dp + ch * kIvfKC + 32 * half);
+                a[i].u[0] = src[0];
+                a[i].u[1] = src[1];
+            }
+        };
+        uint32_t Lk[2][KL], Li[2][KL];
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < KL; ++e) {
+                Lk[j][e] = kKey32None;
+                Li[j][e] = 0u;
+            }
+        load_a(0, 0, acur);
+        for (int rb = 0; rb < nrb; ++rb) {
+            f32x16 acc[2][2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = f32x16{};
+            for (int ch = 0; ch < nch; ++ch) {
+                if (ch + 1 < nch) load_a(rb, ch + 1, anext);
+                V8 b[2];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const uint4 *sb = (const uint4 *)(dsm + (32 * j + col) * QS + ch * kIvfKC + 32 * half);
+                    b[j].u[0] = sb[0];
+                    b[j].u[1] = sb[1];
+                }
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(acur[i].v, b[j].v, acc[i][j], 0, 0,
+                                                                                   0, 127, 0, 127);
+                if (ch + 1 < nch) {
+                    acur[0] = anext[0];
+                    acur[1] = anext[1];
+                }
+            }
+            rowblock_keys<KL>(acc, dscale, dscale, rn, r0 + 256 * rb + 64 * w, r1, rb == 0, qr0, qr1, Lk, Li, s_thr, w,
+                              half, col);
+            if (rb + 1 < nrb) load_a(rb + 1, 0, acur);
+        }
+        __syncthreads(); // the residual block is read by every wave until here
+        merge_item_lists<KL>(dsm, Lk, Li, s_q, pair_p + start, cnt, np, cand, w, half, col);
+        __syncthreads(); // LDS is reused by the next item
+    }
+}
+
 // The list scan. One workgroup (4 waves) per work item: up to 64 queries of one cluster against
 // all rows of that cluster. Rows go in blocks of 256 (64 per wave, two 32-row MFMA tiles); the
 // queries are two 32-query tiles, so a wave holds 2 x 2 accumulators of 32 x 32. The K loop runs
@@ -327,6 +596,7 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
     const int su = tid >> 3, sp = tid & 7;
     const int64_t sg0 = (int64_t)s_q[su] * dp + 8 * sp, sg1 = (int64_t)s_q[su + 32] * dp + 8 * sp;
     const float qn0 = qn[s_q[col]], qn1 = qn[s_q[col + 32]];
+    const float dscale0 = 1.f, dscale1 = 1.f; // the products are the dot products themselves
 
     bf16x8 qreg[2][NPL];
     auto load_q = [&](int kc) {
@@ -412,85 +682,11 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
             if (ch + 1 < nch) load_a(rb, ch + 1, acur); // the next row block's first chunk: after the keys
             __syncthreads();
         }
-        // row block done: keys |q|^2 + |x|^2 - 2 q.x and list insertion. Thresholds are shared by
-        // the 8 lists of a query: the other half-wave's at once, the other waves' as posted after
-        // their previous row block.
-        const int64_t rbase = r0 + 256 * rb + 64 * w;
-        const int rlim = (int)min<int64_t>(r1 - rbase, 64); // rows of this wave's block inside the list
-        uint32_t thr[2];
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            uint32_t m = min(Lk[j][KL - 1], s_thr[0][32 * j + col]);
-#pragma unroll
-            for (int l = 1; l < 8; ++l) m = min(m, s_thr[l][32 * j + col]);
-            thr[j] = m;
-        }
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            // 32-bit offsets from the tile's first row; rows past the list end are clamped / masked
-            const float *xb = lxn + rbase + 32 * i;
-            const int lim = rlim - 32 * i; // rows of this tile inside the list (may be <= 0)
-            float xr[16];
-#pragma unroll
-            for (int r = 0; r < 16; ++r) xr[r] = xb[max(0, min(8 * (r >> 2) + 4 * half + (r & 3), lim - 1))];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const float qv = j == 0 ? qn0 : qn1;
-                uint32_t key[16];
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int off = 8 * (r >> 2) + 4 * half + (r & 3);
-                    const float v = fmaxf(0.f, (qv + xr[r]) - 2.f * acc[i][j][r]);
-                    key[r] = off < lim ? ((f32_key(v) & ~31u) | (uint32_t)off) : kKey32None;
-                }
-                thr[j] = min(thr[j], (uint32_t)__shfl_xor((int)thr[j], 32, 64));
-                const uint32_t base = (uint32_t)(rbase + 32 * i);
-                if (rb == 0 && i == 0)
-                    list_insert16<KL, true>(key, Lk[j], Li[j], base, thr[j]);
-                else
-                    list_insert16<KL, false>(key, Lk[j], Li[j], base, thr[j]);
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 2; ++j) s_thr[w * 2 + half][32 * j + col] = thr[j];
+        rowblock_keys<KL>(acc, dscale0, dscale1, lxn, r0 + 256 * rb + 64 * w, r1, rb == 0, qn0, qn1, Lk, Li, s_thr,
+                          w, half, col);
         if (rb + 1 < nrb) load_a(rb + 1, 0, acur);
     }
-    // candidate lists -> LDS (the query buffers are no longer read), then one lane per query merges
-    uint64_t *sl = (uint64_t *)smem; // [8 lists][64 queries][KL]
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int e = 0; e < KL; ++e)
-            sl[((w * 2 + half) * IQ + 32 * j + col) * KL + e] =
-                Lk[j][e] == kKey32None ? kKey64None : (((uint64_t)Lk[j][e] << 32) | Li[j][e]);
-    __syncthreads();
-    if (tid < cnt) {
-        uint64_t *out = cand + ((int64_t)s_q[tid] * np + pair_p[start + tid]) * KP;
-        int p[8];
-        uint64_t head[8];
-#pragma unroll
-        for (int l = 0; l < 8; ++l) {
-            p[l] = 0;
-            head[l] = sl[(l * IQ + tid) * KL];
-        }
-        for (int e = 0; e < KP; ++e) {
-            int best = 0;
-            uint64_t bv = head[0];
-#pragma unroll
-            for (int l = 1; l < 8; ++l)
-                if (head[l] < bv) {
-                    bv = head[l];
-                    best = l;
-                }
-            out[e] = bv;
-#pragma unroll
-            for (int l = 0; l < 8; ++l)
-                if (l == best && bv != kKey64None) {
-                    p[l] += 1;
-                    head[l] = p[l] < KL ? sl[(l * IQ + tid) * KL + p[l]] : kKey64None;
-                }
-        }
-    }
+    merge_item_lists<KL>(smem, Lk, Li, s_q, pair_p + start, cnt, np, cand, w, half, col);
     __syncthreads(); // LDS (lists, query chunks, s_q, s_thr, s_slot) is reused by the next item
     }
 }
@@ -655,11 +851,38 @@ static void ivfl2_set_device(pn_ivfl2 *h, const float *dX, int64_t n, int64_t d,
         PN_HIP(hipMemcpyAsync(cpp + (int64_t)j * re, dC + (int64_t)keep[j] * d, (size_t)d * 4, hipMemcpyDeviceToDevice,
                               st));
     exact_set_device(&h->cent, cpp, nl, d, st);
+    { // fp8 residuals to the list centroids, one power-of-two scale mapping the largest |x - c| to <= 224
+        float *lc = (float *)h->lcent.ensure((size_t)nl * d * 4);
+        PN_HIP(hipMemcpy2DAsync(lc, (size_t)d * 4, cpp, (size_t)re * 4, (size_t)d * 4, nl, hipMemcpyDeviceToDevice, st));
+        DevBuf mx;
+        unsigned int *dm = (unsigned int *)mx.ensure(4);
+        PN_HIP(hipMemsetAsync(dm, 0, 4, st));
+        const dim3 g((unsigned)ceil_div(n, (int64_t)4));
+        hipLaunchKernelGGL(residual_kernel<0>, g, dim3(256), 0, st, Xo, n, (int)d, h->lrows.dp, dof, nl, lc, 1.f, dm,
+                           (uint8_t *)nullptr, (float *)nullptr);
+        unsigned int bits = 0;
+        PN_HIP(hipMemcpyAsync(&bits, dm, 4, hipMemcpyDeviceToHost, st));
+        PN_HIP(hipStreamSynchronize(st));
+        float m = 0.f;
+        std::memcpy(&m, &bits, 4);
+        h->sx8 = m > 0.f ? std::exp2(std::floor(std::log2(224.f / m))) : 1.f;
+        const int dp = h->lrows.dp;
+        uint8_t *L8 = (uint8_t *)h->l8.ensure((size_t)n * dp);
+        float *rn = (float *)h->rn8.ensure((size_t)n * 4);
+        hipLaunchKernelGGL(residual_kernel<1>, g, dim3(256), 0, st, Xo, n, (int)d, dp, dof, nl, lc, h->sx8, dm, L8, rn);
+        PN_HIP(hipGetLastError());
+    }
     PN_HIP(hipStreamSynchronize(st)); // host vectors and temporaries are released on return
 }
 
 static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t nq, const int32_t *pr, int np,
                                 bool sparse, int k, int prec, int32_t *oi, float *od, int32_t *oc, hipStream_t st);
+
+// ABI precision -> scan variant: 0 bf16 (1 product), 1 bf16 hi/lo (3 products), 2 fp8 e4m3
+static int prec_of(int32_t precision) {
+    PN_REQUIRE(precision >= 0 && precision <= 2, PN_EINVAL, "precision must be 0 (bf16), 1 (bf16x3) or 2 (fp8)");
+    return precision == 1 ? 3 : (precision == 2 ? 8 : 1);
+}
 
 static void ivfl2_search_enqueue(pn_ivfl2 *h, const float *dq, int64_t qstride_in, int64_t nq, int nprobe, int k,
                                  int prec, int32_t *oi, float *od, int32_t *oc, hipStream_t st) {
@@ -718,7 +941,8 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         if (sparse)
             hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)ceil_div(total * kIvfKP, (int64_t)256)), dim3(256), 0, st,
                                cand, total * kIvfKP, kKey64None);
-        l2_split_rows(Q, nq, d, re, dp, Qh, Ql, qn, st);
+        if (prec != 8) l2_split_rows(Q, nq, d, re, dp, Qh, Ql, qn, st); // (the fp8 scan converts in LDS)
+
         PN_HIP(hipMemsetAsync(lc, 0, (size_t)h->nl * 4, st));
         const unsigned pchunks = (unsigned)ceil_div(total, (int64_t)kPairChunk);
         hipLaunchKernelGGL(probe_count_kernel, dim3(pchunks), dim3(256), 0, st, pr, total, h->nl, lc);
@@ -744,7 +968,20 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         using P3 = std::integral_constant<int, 3>;
         using K8 = std::integral_constant<int, 8>;
         using K16 = std::integral_constant<int, 16>;
-        if (prec == 3)
+        auto launch8 = [&](auto kc) {
+            constexpr int KLV = decltype(kc)::value;
+            const size_t lds = std::max<size_t>((size_t)kIvfIQ * (dp + 16), (size_t)8 * kIvfIQ * KLV * 8);
+            PN_REQUIRE(lds <= 96 * 1024, PN_EINVAL, "fp8 precision serves d up to 1472");
+            if (lds > 64 * 1024)
+                PN_HIP(hipFuncSetAttribute((const void *)ivfl2_scan_fp8_kernel<KLV>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL((ivfl2_scan_fp8_kernel<KLV>), grid, dim3(256), lds, st, h->l8.as<uint8_t>(),
+                               h->rn8.as<float>(), h->loff.as<int64_t>(), (int)d, dp, h->lcent.as<float>(), h->sx8, Q,
+                               re, items, bins, nranges, pq, pp, np, cand);
+        };
+        if (prec == 8)
+            k <= 16 ? launch8(K8{}) : launch8(K16{});
+        else if (prec == 3)
             k <= 16 ? launch(P3{}, K8{}) : launch(P3{}, K16{});
         else
             k <= 16 ? launch(P1{}, K8{}) : launch(P1{}, K16{});
@@ -869,7 +1106,7 @@ extern "C" pn_status pn_ivfl2_search(pn_ivfl2 *h, const float *queries, int64_t 
         int32_t *i32 = (int32_t *)oi.ensure((size_t)nq * k * 4);
         float *f32 = (float *)od.ensure((size_t)nq * k * 4);
         int32_t *c32 = (int32_t *)oc.ensure((size_t)nq * 4);
-        ivfl2_search_enqueue(h, q, d, nq, nprobe, k, precision == 1 ? 3 : 1, i32, f32, c32, st);
+        ivfl2_search_enqueue(h, q, d, nq, nprobe, k, prec_of(precision), i32, f32, c32, st);
         std::vector<int32_t> hi((size_t)nq * k);
         PN_HIP(hipMemcpyAsync(hi.data(), i32, hi.size() * 4, hipMemcpyDeviceToHost, st));
         PN_HIP(hipMemcpyAsync(out_dist, f32, (size_t)nq * k * 4, hipMemcpyDeviceToHost, st));
@@ -888,7 +1125,7 @@ extern "C" pn_status pn_ivfl2_search_device(pn_ivfl2 *h, const float *queries, i
         PN_REQUIRE(nq >= 0 && k > 0 && k <= 64, PN_EINVAL, "bad size");
         if (nq == 0) return;
         DeviceScope ds(h->device);
-        ivfl2_search_enqueue(h, queries, h->d, nq, nprobe, k, precision == 1 ? 3 : 1, out_idx, out_dist, out_count,
+        ivfl2_search_enqueue(h, queries, h->d, nq, nprobe, k, prec_of(precision), out_idx, out_dist, out_count,
                              (hipStream_t)stream);
     });
 }
@@ -918,7 +1155,7 @@ extern "C" pn_status pn_ivfl2_search_probes_device(pn_ivfl2 *h, const float *que
         int32_t *pm = (int32_t *)h->pmap.ensure((size_t)nq * np_max * 4);
         hipLaunchKernelGGL(map_probes_kernel, dim3((unsigned)ceil_div(nq * np_max, (int64_t)256)), dim3(256), 0, st,
                            probes, counts, nq, np_max, h->clist.as<int32_t>(), h->nc, pm);
-        ivfl2_lists_enqueue(h, Q, re, nq, pm, np_max, true, k, precision == 1 ? 3 : 1, out_idx, out_dist, out_count,
+        ivfl2_lists_enqueue(h, Q, re, nq, pm, np_max, true, k, prec_of(precision), out_idx, out_dist, out_count,
                             st);
     });
 }

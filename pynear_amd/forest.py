@@ -27,7 +27,7 @@ class _IVFFlatIndex:
         self._cluster_norms = []     # list of (n_c,) float32 squared norms
         self._orig_indices = []      # list of (n_c,) global row ids
         self._list_centroids = None  # centroid id of each non-empty cluster
-        self.precision = "bf16"      # candidate distances: "bf16" or "bf16x3"; rerank is exact
+        self.precision = "bf16"      # candidate distances: "bf16", "bf16x3" or "fp8"; rerank is fp32
         self._dev = None
 
     # ── Build ──────────────────────────────────────────────────────────────

@@ -52,7 +52,7 @@ def _gmm(n, d, comps, seed, sigma=0.3):
     return (centres[lab] + sigma * rng.standard_normal((n, d))).astype(np.float32)
 
 
-@pytest.mark.parametrize("precision", ["bf16", "bf16x3"])
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3", "fp8"])
 @pytest.mark.parametrize("d", [8, 32, 100])
 def test_full_probe_is_exact(precision, d):
     X = _gmm(4000, d, 30, seed=d)
