@@ -58,7 +58,7 @@ struct pn_ivfl2 {
     pn::DevBuf lcent;        // list centroids, fp32 nl x d (list order)
     float sx8 = 1.f;
     pn::DevBuf qpad, qsplit_h, qsplit_l, qn, probes, pdist, lcount, cursor, pair_q, pair_p;
-    pn::DevBuf items, ntot /* item ranges */, itoff, woff, cand, oidx, odist, ocnt;
+    pn::DevBuf items, ntot /* item ranges */, itoff, woff, cand, qthr, qres8, qresn, oidx, odist, ocnt;
 };
 
 namespace pn {
@@ -69,29 +69,40 @@ namespace pn {
 constexpr int kPairChunk = 1024;
 constexpr int kLdsBins = 8192;
 
-__global__ __launch_bounds__(256) void probe_count_kernel(const int32_t *__restrict__ probes, int64_t total, int nl,
-                                                          int32_t *__restrict__ lcount) {
+// The pairs of a pass are the probe slots p_lo .. p_lo + npw - 1 of every query (rows of np
+// slots): pass pair t is slot p_lo + t % npw of query t / npw.
+__device__ __forceinline__ int64_t pass_slot(int64_t t, int np, int p_lo, int npw) {
+    return npw == np ? t : (t / npw) * np + p_lo + t % npw;
+}
+
+__global__ __launch_bounds__(256) void probe_count_kernel(const int32_t *__restrict__ probes, int64_t total, int np,
+                                                          int p_lo, int npw, int nl, int32_t *__restrict__ lcount) {
     __shared__ int32_t hist[kLdsBins];
     const int64_t c0 = (int64_t)blockIdx.x * kPairChunk, c1 = min(total, c0 + kPairChunk);
     if (nl > kLdsBins) {
-        for (int64_t t = c0 + threadIdx.x; t < c1; t += 256)
-            if (probes[t] >= 0) atomicAdd(&lcount[probes[t]], 1);
+        for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) {
+            const int pc = probes[pass_slot(t, np, p_lo, npw)];
+            if (pc >= 0) atomicAdd(&lcount[pc], 1);
+        }
         return;
     }
     for (int b = threadIdx.x; b < nl; b += 256) hist[b] = 0;
     __syncthreads();
-    for (int64_t t = c0 + threadIdx.x; t < c1; t += 256)
-        if (probes[t] >= 0) atomicAdd(&hist[probes[t]], 1); // -1: no probe in this slot
+    for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) {
+        const int pc = probes[pass_slot(t, np, p_lo, npw)];
+        if (pc >= 0) atomicAdd(&hist[pc], 1); // -1: no probe in this slot
+    }
     __syncthreads();
     for (int b = threadIdx.x; b < nl; b += 256)
         if (hist[b]) atomicAdd(&lcount[b], hist[b]);
 }
 
-constexpr int kIvfIQ = 64;  // queries (pairs of one cluster) per work item
+constexpr int kIvfIQ = 64;   // queries (pairs of one cluster) per work item of the bf16 scan
+constexpr int kIvf8IQ = 128; // ... of the fp8 scan
 constexpr int kIvfKC = 64;  // dims per K chunk
 constexpr int kIvfQS = 72;  // LDS row stride (bf16) of a staged query chunk: 16 B of pad spreads the banks
 
-// Work items are (cluster, first pair, count <= 64), cluster by cluster in the order corder
+// Work items are (cluster, first pair, count <= iq), cluster by cluster in the order corder
 // (lists by decreasing length, so the longest items come first). build_items_kernel (one
 // workgroup) scans the per-cluster pair, item and work counts (an item's work is its list length
 // + 64) and sets each cluster's first pair slot (pair cursor); expand_items_kernel writes the items
@@ -108,7 +119,7 @@ __global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restr
                                                           const int64_t *__restrict__ loff, int nl,
                                                           int32_t *__restrict__ cursor, int32_t *__restrict__ itoff,
                                                           int64_t *__restrict__ woff, int32_t *__restrict__ bins,
-                                                          int nranges) {
+                                                          int nranges, int iq) {
     __shared__ int32_t s_pairs[256], s_items[256];
     __shared__ int64_t s_work[256];
     const int tid = threadIdx.x;
@@ -118,7 +129,7 @@ __global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restr
     int64_t work = 0;
     for (int o = o0; o < o1; ++o) {
         const int c = corder[o], m = lcount[c];
-        const int it = (m + kIvfIQ - 1) / kIvfIQ;
+        const int it = (m + iq - 1) / iq;
         pairs += m;
         nit += it;
         work += (int64_t)it * (loff[c + 1] - loff[c] + 64);
@@ -140,7 +151,7 @@ __global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restr
     int64_t wa = s_work[tid] - work;
     for (int o = o0; o < o1; ++o) {
         const int c = corder[o], m = lcount[c];
-        const int it = (m + kIvfIQ - 1) / kIvfIQ;
+        const int it = (m + iq - 1) / iq;
         cursor[c] = pa;
         itoff[o] = ia;
         woff[o] = wa;
@@ -165,7 +176,7 @@ __global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restr
 __global__ void expand_items_kernel(const int32_t *__restrict__ lcount, const int32_t *__restrict__ corder,
                                     const int64_t *__restrict__ loff, int nl, const int32_t *__restrict__ cursor,
                                     const int32_t *__restrict__ itoff, const int64_t *__restrict__ woff,
-                                    int64_t max_items, int nranges, int4 *__restrict__ items,
+                                    int64_t max_items, int nranges, int iq, int4 *__restrict__ items,
                                     int32_t *__restrict__ bins) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= max_items || g >= bins[3 * kXcds]) return;
@@ -176,7 +187,7 @@ __global__ void expand_items_kernel(const int32_t *__restrict__ lcount, const in
     }
     const int c = corder[lo], m = lcount[c];
     const int s = (int)(g - itoff[lo]);
-    items[g] = make_int4(c, cursor[c] + kIvfIQ * s, min(kIvfIQ, m - kIvfIQ * s), 0);
+    items[g] = make_int4(c, cursor[c] + iq * s, min(iq, m - iq * s), 0);
     if (nranges > 1) { // (with one range, build_items_kernel has set it to all items)
         const int64_t W = max<int64_t>(*(const int64_t *)(bins + 3 * kXcds + 2), 1);
         const int64_t wi = loff[c + 1] - loff[c] + 64;
@@ -189,33 +200,89 @@ __global__ void expand_items_kernel(const int32_t *__restrict__ lcount, const in
 // Writes each pair (query, probe rank) to its cluster's next free slot. Per chunk, one global
 // atomic per cluster reserves the chunk's slots (LDS path); pairs then take ranks inside them.
 __global__ __launch_bounds__(256) void probe_scatter_kernel(const int32_t *__restrict__ probes, int64_t total, int np,
-                                                            int nl, int32_t *__restrict__ cursor,
+                                                            int p_lo, int npw, int nl, int32_t *__restrict__ cursor,
                                                             int32_t *__restrict__ pair_q, int32_t *__restrict__ pair_p) {
     __shared__ int32_t hist[kLdsBins];
     const int64_t c0 = (int64_t)blockIdx.x * kPairChunk, c1 = min(total, c0 + kPairChunk);
     if (nl > kLdsBins) {
         for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) {
-            if (probes[t] < 0) continue;
-            const int pos = atomicAdd(&cursor[probes[t]], 1);
-            pair_q[pos] = (int32_t)(t / np);
-            pair_p[pos] = (int32_t)(t % np);
+            const int64_t sl = pass_slot(t, np, p_lo, npw);
+            if (probes[sl] < 0) continue;
+            const int pos = atomicAdd(&cursor[probes[sl]], 1);
+            pair_q[pos] = (int32_t)(sl / np);
+            pair_p[pos] = (int32_t)(sl % np);
         }
         return;
     }
     for (int b = threadIdx.x; b < nl; b += 256) hist[b] = 0;
     __syncthreads();
-    for (int64_t t = c0 + threadIdx.x; t < c1; t += 256)
-        if (probes[t] >= 0) atomicAdd(&hist[probes[t]], 1);
+    for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) {
+        const int pc = probes[pass_slot(t, np, p_lo, npw)];
+        if (pc >= 0) atomicAdd(&hist[pc], 1);
+    }
     __syncthreads();
     for (int b = threadIdx.x; b < nl; b += 256) // count -> first global slot of this chunk's run
         if (hist[b]) hist[b] = atomicAdd(&cursor[b], hist[b]);
     __syncthreads();
     for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) {
-        if (probes[t] < 0) continue;
-        const int pos = atomicAdd(&hist[probes[t]], 1);
-        pair_q[pos] = (int32_t)(t / np);
-        pair_p[pos] = (int32_t)(t % np);
+        const int64_t sl = pass_slot(t, np, p_lo, npw);
+        if (probes[sl] < 0) continue;
+        const int pos = atomicAdd(&hist[probes[sl]], 1);
+        pair_q[pos] = (int32_t)(sl / np);
+        pair_p[pos] = (int32_t)(sl % np);
     }
+}
+
+// Bound for the second pass: per query, the k-th best approximation among the cap candidate
+// slots of its first probe (k distinct rows are at least that close) times a margin for the
+// approximation's error; a row of another list whose approximation is above it cannot be among
+// the k nearest. kKey32None when that probe has fewer than k candidates. Keys below the
+// returned value are kept. One wave per query; cap <= 256.
+__global__ __launch_bounds__(256) void pass_bound_kernel(const uint64_t *__restrict__ cand, int64_t nq, int np, int cap,
+                                                         int k, float margin, uint32_t *__restrict__ qthr) {
+    const int lane = threadIdx.x & 63;
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= nq) return;
+    const uint64_t *src = cand + q * np * cap;
+    uint64_t L[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) L[j] = kKey64None;
+    for (int e = lane; e < cap; e += 64) list_insert_u64<4>(L, src[e]);
+    uint64_t m = kKey64None;
+    for (int r = 0; r < k; ++r) {
+        m = L[0];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint64_t o = __shfl_xor(m, off, 64);
+            m = o < m ? o : m;
+        }
+        if (m == kKey64None) break;
+        if (L[0] == m) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) L[j] = L[j + 1];
+            L[3] = kKey64None;
+        }
+    }
+    if (lane != 0) return;
+    if (m == kKey64None) {
+        qthr[q] = kKey32None;
+        return;
+    }
+    const float v = key_f32((uint32_t)(m >> 32) | 31u) * margin; // (the key's largest value)
+    qthr[q] = isfinite(v) ? (f32_key(v) | 31u) + 1u : kKey32None;
+}
+
+// The next work item of a workgroup of range xb: from its own range while that has items, then
+// from the other ranges in turn (a range's counter may run past its count). -1: none left.
+__device__ __forceinline__ int take_item(int32_t *__restrict__ bins, int nranges, int xb) {
+    for (int r = 0; r < nranges; ++r) {
+        const int b = xb + r < nranges ? xb + r : xb + r - nranges;
+        const int count = bins[kXcds + b];
+        if (count == 0) continue;
+        const int slot = atomicAdd(&bins[2 * kXcds + b], 1);
+        if (slot < count) return bins[b] + slot;
+    }
+    return -1;
 }
 
 // Candidate slots of (query, probe) pairs without a probe stay empty.
@@ -284,41 +351,67 @@ __device__ __forceinline__ void list_insert16(uint32_t (&key)[16], uint32_t (&Lk
 // Row block done: keys |q|^2 + |x|^2 - 2 q.x (q.x = acc * dscale of the query) of the wave's
 // 64 rows (from row rbase; rows at or past r1 are masked) for its two queries per lane, into the
 // lane's lists. Thresholds are shared by the 8 lists of a query: the other half-wave's at once,
-// the other waves' as posted after their previous row block.
+// the other waves' as posted after their previous row block (and a query may start with a bound
+// from elsewhere). A (tile, query) whose smallest approximation is at or above every lane's
+// threshold is done after that test: most tiles are, once the lists have filled. TWO = false:
+// the item has at most 32 queries and the second query tile is not computed.
+// s_thr[l * tstride + q]: the threshold of list l (of nlist per query) of query q; the lane's
+// lists are number `list` of queries qcol and qcol + 32. norm_lds: the row norms are read from
+// snorm (LDS, the list's rows from r0) instead of lxn (global memory).
 template <int KL>
 __device__ __forceinline__ void rowblock_keys(const f32x16 (&acc)[2][2], float dscale0, float dscale1,
                                               const float *__restrict__ lxn, int64_t rbase, int64_t r1, bool first,
                                               float qn0, float qn1, uint32_t (&Lk)[2][KL], uint32_t (&Li)[2][KL],
-                                              uint32_t (*s_thr)[kIvfIQ], int w, int half, int col) {
+                                              uint32_t *s_thr, int tstride, int nlist, int list, int qcol, int half,
+                                              bool two, const float *snorm = nullptr, bool norm_lds = false,
+                                              int64_t r0 = 0) {
     const int rlim = (int)min<int64_t>(r1 - rbase, 64); // rows of this wave's block inside the list
     uint32_t thr[2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-        uint32_t m = min(Lk[j][KL - 1], s_thr[0][32 * j + col]);
-#pragma unroll
-        for (int l = 1; l < 8; ++l) m = min(m, s_thr[l][32 * j + col]);
+        uint32_t m = Lk[j][KL - 1];
+        for (int l = 0; l < nlist; ++l) m = min(m, s_thr[l * tstride + qcol + 32 * j]);
         thr[j] = m;
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         // 32-bit offsets from the tile's first row; rows past the list end are clamped / masked
         const int lim = rlim - 32 * i; // rows of this tile inside the list (may be <= 0)
-        const float *xb = lim > 0 ? lxn + rbase + 32 * i : lxn + r1 - 1; // (no read past the list)
+        if (lim <= 0) break;           // (wave-uniform; so is every later tile of the block)
         float xr[16];
+        if (norm_lds) { // the list's norms staged in LDS from row r0 (LDS loads, not flat ones)
+            typedef const __attribute__((address_space(3))) float lds_float;
+            lds_float *xs = (lds_float *)snorm + (int)(rbase - r0) + 32 * i;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) xr[r] = xb[max(0, min(8 * (r >> 2) + 4 * half + (r & 3), lim - 1))];
+            for (int r = 0; r < 16; ++r) xr[r] = xs[min(8 * (r >> 2) + 4 * half + (r & 3), lim - 1)];
+        } else {
+            const float *xb = lxn + rbase + 32 * i;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) xr[r] = xb[min(8 * (r >> 2) + 4 * half + (r & 3), lim - 1)];
+        }
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
+            if (j == 1 && !two) break;
             const float qv = j == 0 ? qn0 : qn1;
             const float m2 = -2.f * (j == 0 ? dscale0 : dscale1);
-            uint32_t key[16];
+            float v[16];
+            float vmin = INFINITY;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int off = 8 * (r >> 2) + 4 * half + (r & 3);
-                const float v = fmaxf(0.f, fmaf(m2, acc[i][j][r], qv + xr[r]));
-                key[r] = off < lim ? ((f32_key(v) & ~31u) | (uint32_t)off) : kKey32None;
+                v[r] = fmaxf(0.f, fmaf(m2, acc[i][j][r], qv + xr[r]));
+                vmin = fminf(vmin, v[r]);
             }
             thr[j] = min(thr[j], (uint32_t)__shfl_xor((int)thr[j], 32, 64));
+            // every key of the tile is >= bits(vmin) with the offset bits cleared (masked rows
+            // only lower vmin)
+            if (!__any((f32_key(vmin) & ~31u) < thr[j])) continue;
+            uint32_t key[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) key[r] = (f32_key(v[r]) & ~31u) | (uint32_t)(8 * (r >> 2) + 4 * half + (r & 3));
+            if (lim < 32)
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    key[r] = 8 * (r >> 2) + 4 * half + (r & 3) < lim ? key[r] : kKey32None;
             const uint32_t base = (uint32_t)(rbase + 32 * i);
             if (first && i == 0)
                 list_insert16<KL, true>(key, Lk[j], Li[j], base, thr[j]);
@@ -327,50 +420,52 @@ __device__ __forceinline__ void rowblock_keys(const f32x16 (&acc)[2][2], float d
         }
     }
 #pragma unroll
-    for (int j = 0; j < 2; ++j) s_thr[w * 2 + half][32 * j + col] = thr[j];
+    for (int j = 0; j < 2; ++j) s_thr[list * tstride + qcol + 32 * j] = thr[j];
 }
 
-// Item done: the 8 lists of each query (4 waves x 2 halves) go to LDS (smem; the query buffers
-// are no longer read), then one lane per query merges its KP best into cand[query][probe].
-template <int KL>
+// Item done: the lists of each query (nlist per query: waves x 2 halves) go to LDS (smem; the
+// query buffers are no longer read), at sl[(l * tstride + q) * KL], then one lane per query
+// merges its KP best into cand[query][probe].
+template <int KL, int NLMAX>
 __device__ __forceinline__ void merge_item_lists(unsigned char *smem, const uint32_t (&Lk)[2][KL],
                                                  const uint32_t (&Li)[2][KL], const int32_t *s_q,
                                                  const int32_t *__restrict__ pair_p, int cnt, int np,
-                                                 uint64_t *__restrict__ cand, int w, int half, int col) {
-    constexpr int IQ = kIvfIQ, KP = kIvfKP;
+                                                 uint64_t *__restrict__ cand, int tstride, int nlist, int list,
+                                                 int qcol) {
+    constexpr int KP = kIvfKP;
     const int tid = threadIdx.x;
-    uint64_t *sl = (uint64_t *)smem; // [8 lists][64 queries][KL]
+    uint64_t *sl = (uint64_t *)smem;
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
         for (int e = 0; e < KL; ++e)
-            sl[((w * 2 + half) * IQ + 32 * j + col) * KL + e] =
+            sl[(list * tstride + qcol + 32 * j) * KL + e] =
                 Lk[j][e] == kKey32None ? kKey64None : (((uint64_t)Lk[j][e] << 32) | Li[j][e]);
     __syncthreads();
     if (tid < cnt) {
         uint64_t *out = cand + ((int64_t)s_q[tid] * np + pair_p[tid]) * KP;
-        int p[8];
-        uint64_t head[8];
+        int p[NLMAX];
+        uint64_t head[NLMAX];
 #pragma unroll
-        for (int l = 0; l < 8; ++l) {
+        for (int l = 0; l < NLMAX; ++l) {
             p[l] = 0;
-            head[l] = sl[(l * IQ + tid) * KL];
+            head[l] = l < nlist ? sl[(l * tstride + tid) * KL] : kKey64None;
         }
         for (int e = 0; e < KP; ++e) {
             int best = 0;
             uint64_t bv = head[0];
 #pragma unroll
-            for (int l = 1; l < 8; ++l)
+            for (int l = 1; l < NLMAX; ++l)
                 if (head[l] < bv) {
                     bv = head[l];
                     best = l;
                 }
             out[e] = bv;
 #pragma unroll
-            for (int l = 0; l < 8; ++l)
+            for (int l = 0; l < NLMAX; ++l)
                 if (l == best && bv != kKey64None) {
                     p[l] += 1;
-                    head[l] = p[l] < KL ? sl[(l * IQ + tid) * KL + p[l]] : kKey64None;
+                    head[l] = p[l] < KL ? sl[(l * tstride + tid) * KL + p[l]] : kKey64None;
                 }
         }
     }
@@ -382,10 +477,16 @@ union V8 {
     uint4 u[2];
 };
 
+// fp8 rows are stored by 32-row groups and 64-dim chunks: group g, chunk ch is one 2 KB block of
+// its 32 rows' 64 bytes, so a wave's fragment loads of one chunk read whole blocks.
+__device__ __forceinline__ int64_t fp8_row_chunk(int64_t row, int ch, int nch) {
+    return (((row >> 5) * nch + ch) << 11) + ((row & 31) << 6);
+}
+
 // Rows as residuals to their list's centroid, r = x - c (cluster order; the row's list is found in
 // loff). MODE 0: the largest |r| over all rows into *amax (float bits). MODE 1: |r|^2 into rn[row]
 // and r * scale as fp8 e4m3 (OCP, round to nearest even, clamped to +-448) into L8 (dp bytes per
-// row, zero padding). One wave per row.
+// row with zero padding, laid out by fp8_row_chunk). One wave per row.
 template <int MODE>
 __global__ __launch_bounds__(256) void residual_kernel(const float *__restrict__ X, int64_t n, int d, int dp,
                                                        const int64_t *__restrict__ loff, int nl,
@@ -421,85 +522,130 @@ __global__ __launch_bounds__(256) void residual_kernel(const float *__restrict__
         int w = 0;
         w = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], w, false);
         w = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], w, true);
-        *(int *)(L8 + row * dp + e) = w;
+        *(int *)(L8 + fp8_row_chunk(row, e / kIvfKC, dp / kIvfKC) + e % kIvfKC) = w;
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
     if (lane == 0) rn[row] = ss;
 }
 
+// Query residuals of every pair of an item, (q - c) * scale as fp8 e4m3 (clamped to +-448) into
+// R8[pair slot] (dp bytes, zero padding) and |q - c|^2 into Rn[pair slot]. One workgroup per
+// item (ids past the item count exit), one wave per query at a time; lane l converts dims
+// 4 l + 256 t .. +3.
+__global__ __launch_bounds__(256) void query_residual_kernel(const float *__restrict__ Q, int64_t qstride, int d,
+                                                             int dp, const float *__restrict__ lcent, float scale,
+                                                             const int4 *__restrict__ items,
+                                                             const int32_t *__restrict__ bins,
+                                                             const int32_t *__restrict__ pair_q,
+                                                             uint8_t *__restrict__ R8, float *__restrict__ Rn) {
+    if ((int)blockIdx.x >= bins[3 * kXcds]) return;
+    const int4 it = items[blockIdx.x];
+    const int c = it.x, start = it.y, cnt = it.z;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const float *cc = lcent + (int64_t)c * d;
+    for (int t = w; t < cnt; t += 4) {
+        const float *qrow = Q + (int64_t)pair_q[start + t] * qstride;
+        uint8_t *out = R8 + (int64_t)(start + t) * dp;
+        float ss = 0.f;
+        for (int e0 = 4 * lane; e0 < dp; e0 += 256) {
+            float v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float r = e0 + u < d ? qrow[e0 + u] - cc[e0 + u] : 0.f;
+                ss = fmaf(r, r, ss);
+                v[u] = fminf(fmaxf(r * scale, -448.f), 448.f);
+            }
+            int wq = 0;
+            wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], wq, false);
+            wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], wq, true);
+            *(int *)(out + e0) = wq;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+        if (lane == 0) Rn[start + t] = ss;
+    }
+}
+
 // The list scan in fp8 e4m3 on residuals to the list's centroid c: |q - x|^2 =
 // |q - c|^2 + |x - c|^2 - 2 (q - c).(x - c), so the quantised vectors are the short residuals.
-// Per item the workgroup builds its 64 query residuals (q - c) * scale in LDS once (and their
-// |q - c|^2 in fp32); the K loop then needs no staging: per 64-dim chunk one
-// v_mfma_scale_f32_32x32x64_f8f6f4 (unit scales) per (row tile, query tile), lane l holding bytes
-// 32 (l/32) .. +31 of row / query l % 32 of the chunk; the products are scaled back by
-// 1 / scale^2. Row fragments are loaded one chunk ahead as in the bf16 scan.
+// One workgroup of 8 waves per item of up to 128 queries. With more than 64 queries the waves
+// form two query groups (64 queries each) of four row groups, so each fetched row serves 128
+// queries (the second group's fetch of the same fragment comes from L2); with at most 64, all
+// eight waves are row groups of one query group. The workgroup copies the item's query
+// residuals (q - c) * scale (fp8, made for every pair by query_residual_kernel, with
+// |q - c|^2 in fp32) into LDS once; the K loop then needs no staging: per 64-dim chunk one
+// v_mfma_scale_f32_32x32x64_f8f6f4 (unit scales) per (row tile, query tile), lane l holding
+// bytes 32 (l/32) .. +31 of row / query l % 32 of the chunk; the products are scaled back by
+// 1 / scale^2. Row fragments (fp8_row_chunk layout) are loaded two chunks ahead, across row
+// blocks. Keys, lists and the final merge are the bf16 scan's (8 lists per query in two groups,
+// 16 in one).
 template <int KL>
-__global__ __launch_bounds__(256, 2) void ivfl2_scan_fp8_kernel(
-    const uint8_t *__restrict__ L8, const float *__restrict__ rn, const int64_t *__restrict__ loff, int d, int dp,
-    const float *__restrict__ lcent, float scale, const float *__restrict__ Q, int64_t qstride,
+__global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
+    const uint8_t *__restrict__ L8, const float *__restrict__ rn, const int64_t *__restrict__ loff, int dp,
+    float scale, const uint8_t *__restrict__ R8, const float *__restrict__ Rn,
     const int4 *__restrict__ items, int32_t *__restrict__ bins, int nranges, const int32_t *__restrict__ pair_q,
-    const int32_t *__restrict__ pair_p, int np, uint64_t *__restrict__ cand) {
-    constexpr int IQ = kIvfIQ;
-    extern __shared__ __align__(16) unsigned char dsm[]; // max(IQ x (dp + 16), lists) bytes
+    const int32_t *__restrict__ pair_p, int np, const uint32_t *__restrict__ qthr, uint64_t *__restrict__ cand,
+    int nmax) {
+    constexpr int IQ = kIvf8IQ, NL = 16;
+    extern __shared__ __align__(16) unsigned char dsm[]; // IQ x (dp + 16) bytes, then nmax floats
+    float *s_norm = (float *)(dsm + IQ * (dp + 16)); // the list's row norms when it has <= nmax rows
     __shared__ int32_t s_q[IQ];
-    __shared__ uint32_t s_thr[8][IQ];
+    __shared__ uint32_t s_thr[NL * IQ / 2]; // wide: [8][128]; narrow: [16][64]
     __shared__ float s_qr[IQ];
     __shared__ int s_slot;
     const int QS = dp + 16; // LDS row stride of the residual block (16 B of pad spreads the banks)
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5, col = lane & 31;
     const float dscale = 1.f / (scale * scale);
     const int xb = nranges == 1 ? 0 : blockIdx.x % kXcds;
-    const int first = bins[xb], count = bins[kXcds + xb], total_items = bins[3 * kXcds];
     for (;;) {
-        if (tid == 0) s_slot = atomicAdd(&bins[2 * kXcds + xb], 1);
+        if (tid == 0) s_slot = take_item(bins, nranges, xb);
         __syncthreads();
-        const int slot = s_slot;
-        if (slot >= count || (int64_t)first + slot >= total_items) break;
-        const int4 it = items[first + slot];
+        if (s_slot < 0) break;
+        const int4 it = items[s_slot];
         const int c = it.x, start = it.y, cnt = it.z;
+        const bool wide = cnt > 64;
+        const int qg = wide ? w >> 2 : 0, wr = wide ? w & 3 : w; // query group, row group
+        const int nrg = wide ? 4 : 8, nlist = 2 * nrg, tstride = wide ? IQ : 64;
+        const int q0 = 64 * qg;                 // the wave's first query
+        const bool two = cnt - q0 > 32;         // (else its second query tile is padding)
+        const int nrows = wide ? IQ : 64;       // query rows in LDS
         if (tid < IQ) s_q[tid] = pair_q[start + min(tid, cnt - 1)];
-        for (int e = tid; e < 8 * IQ; e += 256) s_thr[e / IQ][e % IQ] = kKey32None;
-        __syncthreads();
-        { // query residuals, one query per wave at a time: lane l converts dims 4 l + 256 t .. +3
-            const float *cc = lcent + (int64_t)c * d;
-            for (int qq = w; qq < IQ; qq += 4) {
-                const float *qrow = Q + (int64_t)s_q[qq] * qstride;
-                float ss = 0.f;
-                for (int e0 = 4 * lane; e0 < dp; e0 += 256) {
-                    float v[4];
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        const float r = e0 + t < d ? qrow[e0 + t] - cc[e0 + t] : 0.f;
-                        ss = fmaf(r, r, ss);
-                        v[t] = fminf(fmaxf(r * scale, -448.f), 448.f);
-                    }
-                    int wq = 0;
-                    wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], wq, false);
-                    wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], wq, true);
-                    *(int *)(dsm + qq * QS + e0) = wq;
-                }
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
-                if (lane == 0) s_qr[qq] = ss;
+        for (int e = tid; e < NL * IQ / 2; e += 512)
+            s_thr[e] = qthr ? qthr[pair_q[start + min(e % tstride, cnt - 1)]] : kKey32None;
+        { // the item's query residuals (pair slots start ..) into LDS; padding rows repeat the last
+            const int upr = dp / 16;
+            for (int u = tid; u < nrows * upr; u += 512) {
+                const int t = u / upr, e = u - t * upr;
+                *(uint4 *)(dsm + t * QS + 16 * e) =
+                    *(const uint4 *)(R8 + (int64_t)(start + min(t, cnt - 1)) * dp + 16 * e);
             }
+            if (tid < nrows) s_qr[tid] = Rn[start + min(tid, cnt - 1)];
         }
-        __syncthreads();
         const int64_t r0 = loff[c], r1 = loff[c + 1];
+        const bool norm_lds = r1 - r0 <= nmax;
+        if (norm_lds)
+            for (int e = tid; e < (int)(r1 - r0); e += 512) s_norm[e] = rn[r0 + e];
+        __syncthreads();
         const int nch = dp / kIvfKC;
-        const int nrb = (int)((r1 - r0 + 255) / 256);
-        const float qr0 = s_qr[col], qr1 = s_qr[col + 32];
-        V8 acur[2], anext[2];
-        auto load_a = [&](int rb, int ch, V8 (&a)[2]) {
+        const int RB = 64 * nrg; // rows per block
+        const int nrb = (int)((r1 - r0 + RB - 1) / RB);
+        const float qr0 = s_qr[q0 + col], qr1 = s_qr[q0 + 32 + col];
+        // row fragments of step g = (row block g / nch, chunk g % nch), loaded two steps ahead into
+        // a ring of three register sets that rotate by name (three steps per loop trip): a copy
+        // between sets would wait for the loads in flight
+        V8 fa[2], fb[2], fc[2];
+        auto load_a = [&](int g, V8 (&a)[2]) {
+            const int rb = g / nch, ch = g - rb * nch;
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
-                const int64_t row = min(r0 + 256 * rb + 64 * w + 32 * i + col, r1 - 1);
-                const uint4 *src = (const uint4 *)(L8 + row * dp + ch * kIvfKC + 32 * half);
+                const int64_t row = min(r0 + (int64_t)RB * rb + 64 * wr + 32 * i + col, r1 - 1);
+                const uint4 *src = (const uint4 *)(L8 + fp8_row_chunk(row, ch, nch) + 32 * half);
                 a[i].u[0] = src[0];
                 a[i].u[1] = src[1];
             }
         };
+        const int nsteps = nrb * nch;
         uint32_t Lk[2][KL], Li[2][KL];
 #pragma unroll
         for (int j = 0; j < 2; ++j)
@@ -508,40 +654,63 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_fp8_kernel(
                 Lk[j][e] = kKey32None;
                 Li[j][e] = 0u;
             }
-        load_a(0, 0, acur);
-        for (int rb = 0; rb < nrb; ++rb) {
-            f32x16 acc[2][2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = f32x16{};
-            for (int ch = 0; ch < nch; ++ch) {
-                if (ch + 1 < nch) load_a(rb, ch + 1, anext);
-                V8 b[2];
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    const uint4 *sb = (const uint4 *)(dsm + (32 * j + col) * QS + ch * kIvfKC + 32 * half);
-                    b[j].u[0] = sb[0];
-                    b[j].u[1] = sb[1];
-                }
+        f32x16 acc[2][2];
+        int ch = 0, rb = 0;
+        // one step: the load two steps ahead, the chunk's products, and after a row block's last
+        // chunk its keys
+        auto step = [&](int g, const V8 (&cur)[2], V8 (&ahead)[2]) {
+            load_a(min(g + 2, nsteps - 1), ahead); // (unconditional, so the waits stay exact)
+            if (ch == 0)
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
 #pragma unroll
-                    for (int j = 0; j < 2; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(acur[i].v, b[j].v, acc[i][j], 0, 0,
-                                                                                   0, 127, 0, 127);
-                if (ch + 1 < nch) {
-                    acur[0] = anext[0];
-                    acur[1] = anext[1];
-                }
+                    for (int j = 0; j < 2; ++j) acc[i][j] = f32x16{};
+            V8 b[2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const uint4 *sb = (const uint4 *)(dsm + (q0 + 32 * j + col) * QS + ch * kIvfKC + 32 * half);
+                b[j].u[0] = sb[0];
+                b[j].u[1] = sb[1];
             }
-            rowblock_keys<KL>(acc, dscale, dscale, rn, r0 + 256 * rb + 64 * w, r1, rb == 0, qr0, qr1, Lk, Li, s_thr, w,
-                              half, col);
-            if (rb + 1 < nrb) load_a(rb + 1, 0, acur);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    if (j == 0 || two)
+                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cur[i].v, b[j].v, acc[i][j], 0, 0,
+                                                                               0, 127, 0, 127);
+            if (++ch == nch) {
+                rowblock_keys<KL>(acc, dscale, dscale, rn, r0 + (int64_t)RB * rb + 64 * wr, r1, rb == 0, qr0, qr1, Lk,
+                                  Li, s_thr, tstride, nlist, wr * 2 + half, q0 + col, half, two, s_norm, norm_lds, r0);
+                ch = 0;
+                ++rb;
+            }
+        };
+        // Whole trips: up to two steps past the end run on clamped rows of the last chunk and end
+        // in a row block past r1, whose keys are all masked.
+        load_a(0, fa);
+        load_a(min(1, nsteps - 1), fb);
+        for (int g = 0; g < nsteps; g += 3) {
+            step(g, fa, fc);
+            step(g + 1, fb, fa);
+            step(g + 2, fc, fb);
         }
-        __syncthreads(); // the residual block is read by every wave until here
-        merge_item_lists<KL>(dsm, Lk, Li, s_q, pair_p + start, cnt, np, cand, w, half, col);
-        __syncthreads(); // LDS is reused by the next item
+        // the lane's lists of its queries go to their (query, probe) slots as they are: slot list
+        // 2 wr + half of 16, unsorted across lists; with two query groups, lists 8 .. 15 are empty
+        const int cap = 16 * KL;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int t = q0 + 32 * j + col;
+            if (t >= cnt) continue;
+            uint64_t *out = cand + ((int64_t)s_q[t] * np + pair_p[start + t]) * cap + (wr * 2 + half) * KL;
+#pragma unroll
+            for (int e = 0; e < KL; ++e) {
+                out[e] = Lk[j][e] == kKey32None ? kKey64None : (((uint64_t)Lk[j][e] << 32) | Li[j][e]);
+                if (wide) out[8 * KL + e] = kKey64None;
+            }
+        }
+        // (the next item's LDS writes follow the loop's first barrier, which every wave reaches
+        // only after this item's products)
     }
 }
 
@@ -561,8 +730,9 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
     const __bf16 *__restrict__ Lh, const __bf16 *__restrict__ Ll, const float *__restrict__ lxn,
     const int64_t *__restrict__ loff, int dp, const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
     const float *__restrict__ qn, const int4 *__restrict__ items, int32_t *__restrict__ bins, int nranges,
-    const int32_t *__restrict__ pair_q, const int32_t *__restrict__ pair_p, int np, uint64_t *__restrict__ cand) {
-    constexpr int KP = kIvfKP, QS = kIvfQS, IQ = kIvfIQ;
+    const int32_t *__restrict__ pair_q, const int32_t *__restrict__ pair_p, int np, const uint32_t *__restrict__ qthr,
+    uint64_t *__restrict__ cand) {
+    constexpr int QS = kIvfQS, IQ = kIvfIQ;
     constexpr int NPL = PREC == 3 ? 2 : 1;      // staged planes: hi (and lo)
     constexpr int PLANE = IQ * QS;              // bf16 elements of one staged plane
     constexpr int BUF = PLANE * NPL;            // one query-chunk buffer
@@ -575,19 +745,19 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
 
     __shared__ int s_slot;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5, col = lane & 31;
-    // workgroups take the items of their range (per XCD: id % kXcds; else the one range) in order
-    // until it is empty
+    // workgroups take the items of their range (per XCD: id % kXcds; else the one range) in order,
+    // then those left in the other ranges
     const int xb = nranges == 1 ? 0 : blockIdx.x % kXcds;
-    const int first = bins[xb], count = bins[kXcds + xb], total_items = bins[3 * kXcds];
     for (;;) {
-    if (tid == 0) s_slot = atomicAdd(&bins[2 * kXcds + xb], 1);
+    if (tid == 0) s_slot = take_item(bins, nranges, xb);
     __syncthreads();
-    const int slot = s_slot;
-    if (slot >= count || (int64_t)first + slot >= total_items) break; // (the second test never fires)
-    const int4 it = items[first + slot];
+    if (s_slot < 0) break;
+    const int4 it = items[s_slot];
     const int c = it.x, start = it.y, cnt = it.z;
+    const bool two = cnt > 32; // (else the second query tile is padding)
     if (tid < IQ) s_q[tid] = pair_q[start + min(tid, cnt - 1)];
-    for (int e = tid; e < 8 * IQ; e += 256) s_thr[e / IQ][e % IQ] = kKey32None;
+    for (int e = tid; e < 8 * IQ; e += 256)
+        s_thr[e / IQ][e % IQ] = qthr ? qthr[pair_q[start + min(e % IQ, cnt - 1)]] : kKey32None;
     __syncthreads();
     const int64_t r0 = loff[c], r1 = loff[c + 1];
     const int nch = dp / kIvfKC;
@@ -668,6 +838,7 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
                 for (int i = 0; i < 2; ++i)
 #pragma unroll
                     for (int j = 0; j < 2; ++j) {
+                        if (j == 1 && !two) continue;
                         acc[i][j] =
                             __builtin_amdgcn_mfma_f32_32x32x16_bf16(acur[i][s][0], b[j][0], acc[i][j], 0, 0, 0);
                         if (PREC == 3) {
@@ -682,11 +853,11 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
             if (ch + 1 < nch) load_a(rb, ch + 1, acur); // the next row block's first chunk: after the keys
             __syncthreads();
         }
-        rowblock_keys<KL>(acc, dscale0, dscale1, lxn, r0 + 256 * rb + 64 * w, r1, rb == 0, qn0, qn1, Lk, Li, s_thr,
-                          w, half, col);
+        rowblock_keys<KL>(acc, dscale0, dscale1, lxn, r0 + 256 * rb + 64 * w, r1, rb == 0, qn0, qn1, Lk, Li,
+                          &s_thr[0][0], IQ, 8, w * 2 + half, col, half, two);
         if (rb + 1 < nrb) load_a(rb + 1, 0, acur);
     }
-    merge_item_lists<KL>(smem, Lk, Li, s_q, pair_p + start, cnt, np, cand, w, half, col);
+    merge_item_lists<KL, 8>(smem, Lk, Li, s_q, pair_p + start, cnt, np, cand, IQ, 8, w * 2 + half, col);
     __syncthreads(); // LDS (lists, query chunks, s_q, s_thr, s_slot) is reused by the next item
     }
 }
@@ -717,9 +888,19 @@ __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restri
 #pragma unroll
     for (int j = 0; j < PER; ++j) L[j] = kKey64None;
     const uint64_t *src = cand + q * ncand;
+    // bound: when every lane holds a candidate, the largest of the 64 lane minima has at least
+    // K2 = 64 candidates at or below it, so nothing above it is among the K2 best
+    uint64_t lmin = kKey64None;
+    for (int e = lane; e < ncand; e += 64) lmin = min(lmin, src[e]);
+    uint64_t bound = lmin;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint64_t o = __shfl_xor(bound, off, 64);
+        bound = o > bound ? o : bound; // (kKey64None when a lane has none: no bound)
+    }
     for (int e = lane; e < ncand; e += 64) {
         const uint64_t v = src[e];
-        if (v < L[PER - 1]) list_insert_u64<PER>(L, v);
+        if (v <= bound && v < L[PER - 1]) list_insert_u64<PER>(L, v);
     }
     uint64_t mine = kKey64None; // after the loop lane r holds the r-th best candidate
     for (int r = 0; r < kIvfK2; ++r) {
@@ -867,7 +1048,7 @@ static void ivfl2_set_device(pn_ivfl2 *h, const float *dX, int64_t n, int64_t d,
         std::memcpy(&m, &bits, 4);
         h->sx8 = m > 0.f ? std::exp2(std::floor(std::log2(224.f / m))) : 1.f;
         const int dp = h->lrows.dp;
-        uint8_t *L8 = (uint8_t *)h->l8.ensure((size_t)n * dp);
+        uint8_t *L8 = (uint8_t *)h->l8.ensure((size_t)ceil_div(n, (int64_t)32) * 32 * dp); // whole 32-row groups
         float *rn = (float *)h->rn8.ensure((size_t)n * 4);
         hipLaunchKernelGGL(residual_kernel<1>, g, dim3(256), 0, st, Xo, n, (int)d, dp, dof, nl, lc, h->sx8, dm, L8, rn);
         PN_HIP(hipGetLastError());
@@ -935,25 +1116,36 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     int32_t *itoff = (int32_t *)h->itoff.ensure((size_t)h->nl * 4);
     int64_t *woff = (int64_t *)h->woff.ensure((size_t)h->nl * 8);
     const int nranges = tune_get("ivfl2.xcd_ranges") ? kXcds : 1;
-    uint64_t *cand = (uint64_t *)h->cand.ensure((size_t)total * kIvfKP * 8);
+    const int iq = prec == 8 ? kIvf8IQ : kIvfIQ; // queries per item
+    // candidate slots per (query, probe): the bf16 scan merges its lists into the KP best; the
+    // fp8 scan stores its 16 lists of KL as they are
+    const int kl = k <= 16 ? 8 : 16;
+    const int cap = prec == 8 ? 16 * kl : kIvfKP;
+    uint64_t *cand = (uint64_t *)h->cand.ensure((size_t)total * cap * 8);
     {
         StageScope sc(st, STAGE_OTHER);
         if (sparse)
-            hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)ceil_div(total * kIvfKP, (int64_t)256)), dim3(256), 0, st,
-                               cand, total * kIvfKP, kKey64None);
+            hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)ceil_div(total * cap, (int64_t)256)), dim3(256), 0, st,
+                               cand, total * cap, kKey64None);
         if (prec != 8) l2_split_rows(Q, nq, d, re, dp, Qh, Ql, qn, st); // (the fp8 scan converts in LDS)
-
+    }
+    // pairs of probe slots p_lo .. p_lo + npw - 1 -> items of at most 64 pairs of one cluster
+    auto bucket = [&](int p_lo, int npw) {
+        StageScope sc(st, STAGE_OTHER);
+        const int64_t tw = nq * npw;
         PN_HIP(hipMemsetAsync(lc, 0, (size_t)h->nl * 4, st));
-        const unsigned pchunks = (unsigned)ceil_div(total, (int64_t)kPairChunk);
-        hipLaunchKernelGGL(probe_count_kernel, dim3(pchunks), dim3(256), 0, st, pr, total, h->nl, lc);
+        const unsigned pchunks = (unsigned)ceil_div(tw, (int64_t)kPairChunk);
+        hipLaunchKernelGGL(probe_count_kernel, dim3(pchunks), dim3(256), 0, st, pr, tw, np, p_lo, npw, h->nl, lc);
         hipLaunchKernelGGL(build_items_kernel, dim3(1), dim3(256), 0, st, lc, h->corder.as<int32_t>(),
-                           h->loff.as<int64_t>(), h->nl, cur, itoff, woff, bins, nranges);
+                           h->loff.as<int64_t>(), h->nl, cur, itoff, woff, bins, nranges, iq);
         hipLaunchKernelGGL(expand_items_kernel, dim3((unsigned)ceil_div(max_items, (int64_t)256)), dim3(256), 0, st, lc,
                            h->corder.as<int32_t>(), h->loff.as<int64_t>(), h->nl, cur, itoff, woff, max_items, nranges,
-                           items, bins);
-        hipLaunchKernelGGL(probe_scatter_kernel, dim3(pchunks), dim3(256), 0, st, pr, total, np, h->nl, cur, pq, pp);
-    }
-    {
+                           iq, items, bins);
+        hipLaunchKernelGGL(probe_scatter_kernel, dim3(pchunks), dim3(256), 0, st, pr, tw, np, p_lo, npw, h->nl, cur, pq,
+                           pp);
+        PN_HIP(hipGetLastError());
+    };
+    auto scan = [&](const uint32_t *qthr) {
         StageScope sc(st, STAGE_SCAN);
         // a persistent grid of wg_per_cu workgroups per CU (a multiple of kXcds) drains the item ranges
         const int64_t per_xcd = std::max<int64_t>(1, ceil_div((int64_t)h->cus, (int64_t)kXcds) *
@@ -962,7 +1154,7 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         auto launch = [&](auto pc, auto kc) {
             hipLaunchKernelGGL((ivfl2_scan_kernel<decltype(pc)::value, decltype(kc)::value>), grid, dim3(256), 0, st,
                                h->lrows.hi.as<__bf16>(), h->lrows.lo.as<__bf16>(), h->lrows.norm.as<float>(),
-                               h->loff.as<int64_t>(), dp, Qh, Ql, qn, items, bins, nranges, pq, pp, np, cand);
+                               h->loff.as<int64_t>(), dp, Qh, Ql, qn, items, bins, nranges, pq, pp, np, qthr, cand);
         };
         using P1 = std::integral_constant<int, 1>;
         using P3 = std::integral_constant<int, 3>;
@@ -970,26 +1162,55 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         using K16 = std::integral_constant<int, 16>;
         auto launch8 = [&](auto kc) {
             constexpr int KLV = decltype(kc)::value;
-            const size_t lds = std::max<size_t>((size_t)kIvfIQ * (dp + 16), (size_t)8 * kIvfIQ * KLV * 8);
-            PN_REQUIRE(lds <= 96 * 1024, PN_EINVAL, "fp8 precision serves d up to 1472");
+            // LDS: the residual block, then the row norms of lists of up to nmax rows
+            constexpr size_t kLdsMax = 148 * 1024;
+            const size_t rblk = (size_t)kIvf8IQ * (dp + 16);
+            PN_REQUIRE(rblk <= kLdsMax, PN_EINVAL, "fp8 precision serves d up to 1152");
+            const int nmax = (int)std::min<size_t>(8192, (kLdsMax - rblk) / 4 / 64 * 64);
+            const size_t lds = rblk + (size_t)nmax * 4;
             if (lds > 64 * 1024)
                 PN_HIP(hipFuncSetAttribute((const void *)ivfl2_scan_fp8_kernel<KLV>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL((ivfl2_scan_fp8_kernel<KLV>), grid, dim3(256), lds, st, h->l8.as<uint8_t>(),
-                               h->rn8.as<float>(), h->loff.as<int64_t>(), (int)d, dp, h->lcent.as<float>(), h->sx8, Q,
-                               re, items, bins, nranges, pq, pp, np, cand);
+            uint8_t *R8 = (uint8_t *)h->qres8.ensure((size_t)total * dp);
+            float *Rn = (float *)h->qresn.ensure((size_t)total * 4);
+            hipLaunchKernelGGL(query_residual_kernel, dim3((unsigned)max_items), dim3(256), 0, st, Q, re, (int)d, dp,
+                               h->lcent.as<float>(), h->sx8, items, bins, pq, R8, Rn);
+            const dim3 grid8((unsigned)std::min<int64_t>(ceil_div((int64_t)h->cus, (int64_t)kXcds) *
+                                                             std::max<int64_t>(1, tune_get("ivfl2.fp8_wg_per_cu")),
+                                                         max_items) * kXcds);
+            hipLaunchKernelGGL((ivfl2_scan_fp8_kernel<KLV>), grid8, dim3(512), lds, st, h->l8.as<uint8_t>(),
+                               h->rn8.as<float>(), h->loff.as<int64_t>(), dp, h->sx8, R8, Rn, items, bins, nranges,
+                               pq, pp, np, qthr, cand, nmax);
         };
         if (prec == 8)
-            k <= 16 ? launch8(K8{}) : launch8(K16{});
+            kl == 8 ? launch8(K8{}) : launch8(K16{});
         else if (prec == 3)
             k <= 16 ? launch(P3{}, K8{}) : launch(P3{}, K16{});
         else
             k <= 16 ? launch(P1{}, K8{}) : launch(P1{}, K16{});
         PN_HIP(hipGetLastError());
+    };
+    // Two passes when a bound helps: the first probe of every query, then the others with the
+    // bound from the first (pass_bound_kernel); otherwise all probes in one pass.
+    const int64_t margin_pct = tune_get("ivfl2.pass_margin_pct");
+    if (np > 1 && k <= kIvfKP && margin_pct > 0) {
+        uint32_t *qthr = (uint32_t *)h->qthr.ensure((size_t)nq * 4);
+        bucket(0, 1);
+        scan(nullptr);
+        {
+            StageScope sc(st, STAGE_OTHER);
+            hipLaunchKernelGGL(pass_bound_kernel, dim3((unsigned)ceil_div(nq, (int64_t)4)), dim3(256), 0, st, cand, nq,
+                               np, cap, k, 1.f + (float)margin_pct / 100.f, qthr);
+        }
+        bucket(1, np - 1);
+        scan(qthr);
+    } else {
+        bucket(0, np);
+        scan(nullptr);
     }
     {
         StageScope sc(st, STAGE_RERANK);
-        const int ncand = np * kIvfKP;
+        const int ncand = np * cap;
         const bool vec = d % 4 == 0 && re % 4 == 0;
         dim3 g((unsigned)ceil_div(nq, 4));
         auto launch = [&](auto per, auto v) {

@@ -42,6 +42,9 @@ static std::map<std::string, int64_t> &tune_table() {
         t["l2.matrix"] = 1;            // exact L2 over <= 4096 rows of d > 64: key matrix + selection
         t["ivfl2.wg_per_cu"] = 2;      // IVF-L2 list scan: persistent workgroups per CU
         t["ivfl2.xcd_ranges"] = 0;     // 1: one item range per XCD; 0: one queue for all workgroups
+        t["ivfl2.fp8_wg_per_cu"] = 1;  // fp8 scan: workgroups of 8 waves per CU
+        t["ivfl2.pass_margin_pct"] = 0; // > 0: scan the first probe, then the others bounded by its
+                                         // k-th approximation x (1 + pct / 100); 0: one pass
         t["batch.slice_rows"] = 512;   // batched exact search: target rows per slice
         t["batch.wg_per_cu"] = 2;      // batched exact search: persistent scan workgroups per CU
         t["batch.order"] = 1;          // batched exact search: 1 = tiles by decreasing cost

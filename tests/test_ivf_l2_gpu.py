@@ -83,6 +83,29 @@ def test_partial_probe_matches_numpy_restatement(n_probe):
             assert da[a.index(c)] == pytest.approx(db[b.index(c)], rel=1e-4, abs=1e-3)
 
 
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3", "fp8"])
+def test_bounded_second_pass_keeps_results(precision):
+    """The list scan runs the first probe, then the other probes with a per-query bound from the
+    first (ivfl2.pass_margin_pct); switching the bound off (one pass over all probes) must give
+    the same neighbours and distances. Small clusters (fewer rows than k) leave a query unbounded."""
+    from pynear_amd import _native as N
+    X = _gmm(12000, 40, 60, seed=11, sigma=0.4)
+    Q = _gmm(300, 40, 60, seed=12, sigma=0.4)
+    h = N.IvfL2Handle()
+    h.build(X, 80, 20, 5)
+    old = N.tune_get("ivfl2.pass_margin_pct")
+    try:
+        for k, n_probe in ((1, 4), (10, 8), (32, 6), (10, 80)):
+            N.tune_set("ivfl2.pass_margin_pct", 50)
+            a = h.search(Q, n_probe, k, precision)
+            N.tune_set("ivfl2.pass_margin_pct", 0)
+            b = h.search(Q, n_probe, k, precision)
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y), (k, n_probe)
+    finally:
+        N.tune_set("ivfl2.pass_margin_pct", old)
+
+
 def test_reference_ivfflat_behaviour():
     """pynear/tests/test_ivfflat.py:44-161 through the same calls."""
     rng = np.random.default_rng(0)
