@@ -57,7 +57,7 @@ struct pn_ivfl2 {
     pn::DevBuf rn8;          // |x - c|^2 per row (cluster order)
     pn::DevBuf lcent;        // list centroids, fp32 nl x d (list order)
     float sx8 = 1.f;
-    pn::DevBuf qpad, qsplit_h, qsplit_l, qn, probes, pdist, lcount, cursor, pair_q, pair_p;
+    pn::DevBuf qpad, qsplit_h, qsplit_l, qn, probes, pdist, lcount, cursor, pair_q, pair_p, pair_c;
     pn::DevBuf items, ntot /* item ranges */, itoff, woff, cand, qthr, qres8, qresn, oidx, odist, ocnt;
 };
 
@@ -111,8 +111,8 @@ constexpr int kIvfQS = 72;  // LDS row stride (bf16) of a staged query chunk: 16
 // dispatch places on one XCD, so a long list that many queries probe is read into one or two L2s
 // instead of all eight; with nranges = 1 (the default: measured faster, the per-XCD ranges
 // balance worse than one queue) every workgroup takes from one range. bins holds per range its
-// first item, its item count and a zeroed work counter; bins[3 kXcds] the item total and
-// bins[3 kXcds + 1 ..] (as int64) the total work.
+// first item, its item count and a zeroed work counter; bins[3 kXcds] the item total,
+// bins[3 kXcds + 1] the pair total and bins[3 kXcds + 2 ..] (as int64) the total work.
 constexpr int kXcds = 8;
 __global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restrict__ lcount,
                                                           const int32_t *__restrict__ corder,
@@ -167,6 +167,7 @@ __global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restr
     }
     if (tid == 255) {
         bins[3 * kXcds] = s_items[255];
+        bins[3 * kXcds + 1] = s_pairs[255];
         *(int64_t *)(bins + 3 * kXcds + 2) = s_work[255];
     }
 }
@@ -197,11 +198,12 @@ __global__ void expand_items_kernel(const int32_t *__restrict__ lcount, const in
     }
 }
 
-// Writes each pair (query, probe rank) to its cluster's next free slot. Per chunk, one global
+// Writes each pair (query, probe rank, and its list) to its cluster's next free slot. Per chunk, one global
 // atomic per cluster reserves the chunk's slots (LDS path); pairs then take ranks inside them.
 __global__ __launch_bounds__(256) void probe_scatter_kernel(const int32_t *__restrict__ probes, int64_t total, int np,
                                                             int p_lo, int npw, int nl, int32_t *__restrict__ cursor,
-                                                            int32_t *__restrict__ pair_q, int32_t *__restrict__ pair_p) {
+                                                            int32_t *__restrict__ pair_q, int32_t *__restrict__ pair_p,
+                                                            int32_t *__restrict__ pair_c) {
     __shared__ int32_t hist[kLdsBins];
     const int64_t c0 = (int64_t)blockIdx.x * kPairChunk, c1 = min(total, c0 + kPairChunk);
     if (nl > kLdsBins) {
@@ -211,6 +213,7 @@ __global__ __launch_bounds__(256) void probe_scatter_kernel(const int32_t *__res
             const int pos = atomicAdd(&cursor[probes[sl]], 1);
             pair_q[pos] = (int32_t)(sl / np);
             pair_p[pos] = (int32_t)(sl % np);
+            pair_c[pos] = probes[sl];
         }
         return;
     }
@@ -230,6 +233,7 @@ __global__ __launch_bounds__(256) void probe_scatter_kernel(const int32_t *__res
         const int pos = atomicAdd(&hist[probes[sl]], 1);
         pair_q[pos] = (int32_t)(sl / np);
         pair_p[pos] = (int32_t)(sl % np);
+        pair_c[pos] = probes[sl];
     }
 }
 
@@ -529,42 +533,38 @@ __global__ __launch_bounds__(256) void residual_kernel(const float *__restrict__
     if (lane == 0) rn[row] = ss;
 }
 
-// Query residuals of every pair of an item, (q - c) * scale as fp8 e4m3 (clamped to +-448) into
-// R8[pair slot] (dp bytes, zero padding) and |q - c|^2 into Rn[pair slot]. One workgroup per
-// item (ids past the item count exit), one wave per query at a time; lane l converts dims
-// 4 l + 256 t .. +3.
+// Query residuals of every pair, (q - c) * scale as fp8 e4m3 (clamped to +-448) into R8[pair
+// slot] (dp bytes, zero padding) and |q - c|^2 into Rn[pair slot]. One wave per pair slot (slots
+// past the pair total in bins exit); lane l converts dims 4 l + 256 t .. +3.
 __global__ __launch_bounds__(256) void query_residual_kernel(const float *__restrict__ Q, int64_t qstride, int d,
                                                              int dp, const float *__restrict__ lcent, float scale,
-                                                             const int4 *__restrict__ items,
                                                              const int32_t *__restrict__ bins,
                                                              const int32_t *__restrict__ pair_q,
+                                                             const int32_t *__restrict__ pair_c,
                                                              uint8_t *__restrict__ R8, float *__restrict__ Rn) {
-    if ((int)blockIdx.x >= bins[3 * kXcds]) return;
-    const int4 it = items[blockIdx.x];
-    const int c = it.x, start = it.y, cnt = it.z;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const float *cc = lcent + (int64_t)c * d;
-    for (int t = w; t < cnt; t += 4) {
-        const float *qrow = Q + (int64_t)pair_q[start + t] * qstride;
-        uint8_t *out = R8 + (int64_t)(start + t) * dp;
-        float ss = 0.f;
-        for (int e0 = 4 * lane; e0 < dp; e0 += 256) {
-            float v[4];
+    const int64_t sl = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (sl >= bins[3 * kXcds + 1]) return;
+    const float *cc = lcent + (int64_t)pair_c[sl] * d;
+    const float *qrow = Q + (int64_t)pair_q[sl] * qstride;
+    uint8_t *out = R8 + sl * dp;
+    float ss = 0.f;
+    for (int e0 = 4 * lane; e0 < dp; e0 += 256) {
+        float v[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const float r = e0 + u < d ? qrow[e0 + u] - cc[e0 + u] : 0.f;
-                ss = fmaf(r, r, ss);
-                v[u] = fminf(fmaxf(r * scale, -448.f), 448.f);
-            }
-            int wq = 0;
-            wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], wq, false);
-            wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], wq, true);
-            *(int *)(out + e0) = wq;
+        for (int u = 0; u < 4; ++u) {
+            const float r = e0 + u < d ? qrow[e0 + u] - cc[e0 + u] : 0.f;
+            ss = fmaf(r, r, ss);
+            v[u] = fminf(fmaxf(r * scale, -448.f), 448.f);
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
-        if (lane == 0) Rn[start + t] = ss;
+        int wq = 0;
+        wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], wq, false);
+        wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], wq, true);
+        *(int *)(out + e0) = wq;
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+    if (lane == 0) Rn[sl] = ss;
 }
 
 // The list scan in fp8 e4m3 on residuals to the list's centroid c: |q - x|^2 =
@@ -874,7 +874,8 @@ __device__ __forceinline__ uint64_t wmin64(uint64_t v) {
 // One wave per query: the K2 best candidates by the approximation are recomputed in fp32 as
 // sum (q - x)^2 by 16 lanes each (four candidates per round; with VEC, 16-byte loads, so a wave
 // reads four rows per step in 256-byte pieces), then the k smallest (distance, id) are written
-// ascending.
+// ascending. X holds the rows in cluster order (a candidate's position indexes it; lids maps a
+// position to its id), so the rows that queries of one cluster share are near each other.
 template <int PER, bool VEC>
 __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restrict__ X, int64_t d,
                                                            const int32_t *__restrict__ lids, const float *__restrict__ Q,
@@ -913,19 +914,20 @@ __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restri
         }
         if (lane == r) mine = m;
     }
-    const int32_t myid = mine == kKey64None ? -1 : lids[(uint32_t)(mine & 0xFFFFFFFFull)];
+    const int32_t mypos = mine == kKey64None ? -1 : (int32_t)(uint32_t)(mine & 0xFFFFFFFFull);
+    const int32_t myid = mypos < 0 ? -1 : lids[mypos];
     const float *qrow = Q + q * qstride;
     uint64_t key = kKey64None;
     const int g = lane >> 4, c = lane & 15;
 #pragma unroll 1
     for (int round = 0; round < kIvfK2 / 4; ++round) {
-        const int32_t id = __shfl(myid, 4 * round + g, 64); // group g: candidate 4 round + g
+        const int32_t pos = __shfl(mypos, 4 * round + g, 64); // group g: candidate 4 round + g
         float part = 0.f;
-        if (id >= 0) {
-            const float *xr = X + (int64_t)id * d;
+        if (pos >= 0) {
+            const float *xr = X + (int64_t)pos * d; // (rows in cluster order)
             if (VEC) {
                 float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
-#pragma unroll 4
+#pragma unroll 6
                 for (int64_t i = 4 * c; i < d; i += 64) {
                     const float4 a = *(const float4 *)(qrow + i), b = *(const float4 *)(xr + i);
                     const float t0 = a.x - b.x, t1 = a.y - b.y, t2 = a.z - b.z, t3 = a.w - b.w;
@@ -1001,8 +1003,6 @@ static void ivfl2_set_device(pn_ivfl2 *h, const float *dX, int64_t n, int64_t d,
     h->n = n;
     h->d = d;
     h->nl = nl;
-    float *X = (float *)h->X.ensure((size_t)n * d * 4);
-    PN_HIP(hipMemcpyAsync(X, dX, (size_t)n * d * 4, hipMemcpyDeviceToDevice, st));
     int32_t *dl = (int32_t *)h->lids.ensure((size_t)n * 4);
     int64_t *dof = (int64_t *)h->loff.ensure((size_t)(nl + 1) * 8);
     PN_HIP(hipMemcpyAsync(dl, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
@@ -1017,10 +1017,10 @@ static void ivfl2_set_device(pn_ivfl2 *h, const float *dX, int64_t n, int64_t d,
     for (int32_t j = 0; j < nl; ++j) clist[keep[j]] = j;
     h->nc = (int32_t)nc;
     PN_HIP(hipMemcpyAsync(h->clist.ensure((size_t)nc * 4), clist.data(), (size_t)nc * 4, hipMemcpyHostToDevice, st));
-    // rows in cluster order -> bf16 split (+ zero rows past the end, see the scan kernel)
-    DevBuf ordered;
-    float *Xo = (float *)ordered.ensure((size_t)n * d * 4);
-    hipLaunchKernelGGL(gather_rows_f32_kernel, dim3((unsigned)ceil_div(n * d, 256)), dim3(256), 0, st, X, d, dl, n, Xo);
+    // rows in cluster order (kept for the exact rerank) -> bf16 split (+ zero rows past the end,
+    // see the scan kernel)
+    float *Xo = (float *)h->X.ensure((size_t)n * d * 4);
+    hipLaunchKernelGGL(gather_rows_f32_kernel, dim3((unsigned)ceil_div(n * d, 256)), dim3(256), 0, st, dX, d, dl, n, Xo);
     l2_prepare(h->lrows, Xo, n, d, d, st, kIvfKC); // whole K chunks: the scan has no dim tail
     // coarse index over the kept centroids
     bool scan_ok = false;
@@ -1109,6 +1109,7 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     int32_t *cur = (int32_t *)h->cursor.ensure((size_t)h->nl * 4);
     int32_t *pq = (int32_t *)h->pair_q.ensure((size_t)total * 4);
     int32_t *pp = (int32_t *)h->pair_p.ensure((size_t)total * 4);
+    int32_t *pc = (int32_t *)h->pair_c.ensure((size_t)total * 4);
     // each cluster has at most one partly filled item
     const int64_t max_items = ceil_div(total, kIvfIQ) + h->nl;
     int4 *items = (int4 *)h->items.ensure((size_t)max_items * 16);
@@ -1142,7 +1143,7 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
                            h->corder.as<int32_t>(), h->loff.as<int64_t>(), h->nl, cur, itoff, woff, max_items, nranges,
                            iq, items, bins);
         hipLaunchKernelGGL(probe_scatter_kernel, dim3(pchunks), dim3(256), 0, st, pr, tw, np, p_lo, npw, h->nl, cur, pq,
-                           pp);
+                           pp, pc);
         PN_HIP(hipGetLastError());
     };
     auto scan = [&](const uint32_t *qthr) {
@@ -1173,8 +1174,8 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             uint8_t *R8 = (uint8_t *)h->qres8.ensure((size_t)total * dp);
             float *Rn = (float *)h->qresn.ensure((size_t)total * 4);
-            hipLaunchKernelGGL(query_residual_kernel, dim3((unsigned)max_items), dim3(256), 0, st, Q, re, (int)d, dp,
-                               h->lcent.as<float>(), h->sx8, items, bins, pq, R8, Rn);
+            hipLaunchKernelGGL(query_residual_kernel, dim3((unsigned)ceil_div(total, (int64_t)4)), dim3(256), 0, st, Q,
+                               re, (int)d, dp, h->lcent.as<float>(), h->sx8, bins, pq, pc, R8, Rn);
             const dim3 grid8((unsigned)std::min<int64_t>(ceil_div((int64_t)h->cus, (int64_t)kXcds) *
                                                              std::max<int64_t>(1, tune_get("ivfl2.fp8_wg_per_cu")),
                                                          max_items) * kXcds);

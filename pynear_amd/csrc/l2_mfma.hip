@@ -269,7 +269,7 @@ void l2_split_rows(const float *X, int64_t n, int64_t d, int64_t stride, int dp,
     PN_HIP(hipGetLastError());
 }
 
-static int kl_for(int kk) { return kk <= 6 ? 8 : (kk <= 12 ? 16 : 32); }
+static int kl_for(int kk) { return kk <= 8 ? 8 : (kk <= 12 ? 16 : 32); }
 
 void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const float *Q, int64_t qstride, int64_t nq,
                     int kk, int32_t *out_idx, float *out_dist, L2Work &wk, int cus, hipStream_t st) {
