@@ -472,33 +472,39 @@ def run_learned(steps, warmup, cpu, target=0.95):
 
     base = sweep_fixed("bf16x3")
     fast = sweep_fixed("bf16")
-    _log(f"learned: fixed n_probe bf16x3 {base}, bf16 {fast}")
+    fast8 = sweep_fixed("fp8")
+    _log(f"learned: fixed n_probe bf16x3 {base}, bf16 {fast}, fp8 {fast8}")
     np_max = 256
 
-    def learned(mass):
+    def learned(mass, prec):
         def step():
             probes, counts = probe.select(dq.ptr, nq, mass, 1, np_max, stream)
-            h.search_probes_device(dq.ptr, nq, probes.ptr, counts.ptr, np_max, k, oi.ptr, od.ptr, oc.ptr, "bf16",
+            h.search_probes_device(dq.ptr, nq, probes.ptr, counts.ptr, np_max, k, oi.ptr, od.ptr, oc.ptr, prec,
                                    stream.handle)
         return step
 
-    lrn = None
-    for mass in (0.8, 0.85, 0.9, 0.92, 0.94, 0.95, 0.96, 0.97, 0.98, 0.99, 0.995, 0.998, 0.999, 0.9995, 0.9999):
-        learned(mass)()
-        stream.synchronize()
-        r = recall()
-        if r >= target:
-            _, counts = probe.select(dq.ptr, nq, mass, 1, np_max)
-            ms = _time_device(N, hipdev, learned(mass), steps, warmup, stream)[0] / steps
-            lrn = {"mass": mass, "mean_n_probe": round(float(counts.numpy().mean()), 2), "recall_at_10": round(r, 4),
-                   "ms_per_step": round(ms, 4), "qps": round(nq / (ms / 1e3), 1)}
-            break
-    _log(f"learned: adaptive {lrn}")
+    by_prec = {}
+    for prec in ("fp8", "bf16"):
+        for mass in (0.8, 0.85, 0.9, 0.92, 0.94, 0.95, 0.96, 0.97, 0.98, 0.99, 0.995, 0.998, 0.999, 0.9995, 0.9999):
+            learned(mass, prec)()
+            stream.synchronize()
+            r = recall()
+            if r >= target:
+                _, counts = probe.select(dq.ptr, nq, mass, 1, np_max)
+                ms = _time_device(N, hipdev, learned(mass, prec), steps, warmup, stream)[0] / steps
+                by_prec[prec] = {"precision": prec, "mass": mass, "mean_n_probe": round(float(counts.numpy().mean()), 2),
+                                 "recall_at_10": round(r, 4), "ms_per_step": round(ms, 4),
+                                 "qps": round(nq / (ms / 1e3), 1)}
+                break
+        _log(f"learned: adaptive {prec} {by_prec.get(prec)}")
+    lrn = max(by_prec.values(), key=lambda v: v["qps"]) if by_prec else None
     out = {"workload": (f"learned-probe IVF-L2, Zipf(1.1) GMM N={len(X):,} x d={X.shape[1]}, nlist={nlist}, "
-                        f"MLP {X.shape[1]}->256->{nlist} bf16 trained on {len(QT):,} queries, {nq:,} held-out, k={k}"),
+                        f"MLP {X.shape[1]}->256->{nlist} bf16 trained on {len(QT):,} queries, {nq:,} held-out, k={k}; "
+                        f"list scan in {lrn['precision'] if lrn else '-'}, fp32 rerank of 64"),
            "value": lrn["qps"] if lrn else None, "unit": "queries/s",
            "ms_per_step": lrn["ms_per_step"] if lrn else None,
-           "learned": lrn, "fixed_nprobe_fp32": base, "fixed_nprobe_bf16": fast,
+           "learned": lrn, "learned_by_precision": by_prec, "fixed_nprobe_fp32": base, "fixed_nprobe_bf16": fast,
+           "fixed_nprobe_fp8": fast8,
            "speedup_vs_fixed_fp32": round(lrn["qps"] / base["qps"], 3) if lrn and base else None,
            "speedup_vs_fixed_bf16": round(lrn["qps"] / fast["qps"], 3) if lrn and fast else None,
            "train": {"epochs": len(hist), "loss": [round(v, 4) for v in hist], "seconds": round(t_train, 2)},
@@ -606,7 +612,7 @@ def main():
     for key in ("cpu_baseline", "precision", "recall_at_10", "n_probe", "fp64_rel_err", "stages_ms_per_step", "sweep", "setup_s",
                 "cpu_sample_recall_vs_port", "cpu_sample_distances_equal", "schedule", "scan_tiles",
                 "equal_to_plain_searches", "plain_searches_ms", "sweep_fp8", "learned", "fixed_nprobe_fp32", "fixed_nprobe_bf16",
-                "speedup_vs_fixed_fp32", "speedup_vs_fixed_bf16", "train"):
+                "speedup_vs_fixed_fp32", "speedup_vs_fixed_bf16", "train", "learned_by_precision", "fixed_nprobe_fp8"):
         if key in main_r:
             result[key] = main_r[key]
     if args.workload == "ivf" and not args.no_secondary:

@@ -58,7 +58,8 @@ struct pn_ivfl2 {
     pn::DevBuf lcent;        // list centroids, fp32 nl x d (list order)
     float sx8 = 1.f;
     pn::DevBuf qpad, qsplit_h, qsplit_l, qn, probes, pdist, lcount, cursor, pair_q, pair_p, pair_c;
-    pn::DevBuf items, ntot /* item ranges */, itoff, woff, cand, qthr, qres8, qresn, oidx, odist, ocnt;
+    pn::DevBuf items, ntot /* item ranges */, itoff, woff, cand, qoff, qthr, qres8, qresn, qkey, qorder, ohist, oidx,
+        odist, ocnt;
 };
 
 namespace pn {
@@ -99,6 +100,7 @@ __global__ __launch_bounds__(256) void probe_count_kernel(const int32_t *__restr
 
 constexpr int kIvfIQ = 64;   // queries (pairs of one cluster) per work item of the bf16 scan
 constexpr int kIvf8IQ = 128; // ... of the fp8 scan
+constexpr int kIvf8Cap = 64; // candidates kept per (query, probe) by the fp8 scan
 constexpr int kIvfKC = 64;  // dims per K chunk
 constexpr int kIvfQS = 72;  // LDS row stride (bf16) of a staged query chunk: 16 B of pad spreads the banks
 
@@ -237,17 +239,47 @@ __global__ __launch_bounds__(256) void probe_scatter_kernel(const int32_t *__res
     }
 }
 
+// Candidate slot of (query q, probe p): q np + p for np probes per query, or qoff[q] + p when
+// queries have their own counts (qoff: exclusive prefix of the per-query probe spans).
+__device__ __forceinline__ int64_t cand_slot(const int32_t *__restrict__ qoff, int64_t q, int np, int p) {
+    return qoff ? (int64_t)qoff[q] + p : q * np + p;
+}
+
+// Per-query probe span (last used slot + 1; 0 for none) into span[q], span[nq] = 0; an exclusive
+// scan of span then gives qoff.
+__global__ void probe_span_kernel(const int32_t *__restrict__ probes, int64_t nq, int np, int32_t *__restrict__ span) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > nq) return;
+    int n = 0;
+    if (q < nq)
+        for (int j = np - 1; j >= 0; --j)
+            if (probes[q * np + j] >= 0) {
+                n = j + 1;
+                break;
+            }
+    span[q] = n;
+}
+
+// Fills the candidate slots of every pair (qoff[nq] of them, cap entries each) with v.
+__global__ void fill_slots_kernel(uint64_t *__restrict__ p, const int32_t *__restrict__ qoff, int64_t nq, int cap,
+                                  uint64_t v) {
+    const int64_t n = (int64_t)qoff[nq] * cap;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 // Bound for the second pass: per query, the k-th best approximation among the cap candidate
 // slots of its first probe (k distinct rows are at least that close) times a margin for the
 // approximation's error; a row of another list whose approximation is above it cannot be among
 // the k nearest. kKey32None when that probe has fewer than k candidates. Keys below the
 // returned value are kept. One wave per query; cap <= 256.
 __global__ __launch_bounds__(256) void pass_bound_kernel(const uint64_t *__restrict__ cand, int64_t nq, int np, int cap,
-                                                         int k, float margin, uint32_t *__restrict__ qthr) {
+                                                         const int32_t *__restrict__ qoff, int k, float margin,
+                                                         uint32_t *__restrict__ qthr) {
     const int lane = threadIdx.x & 63;
     const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (q >= nq) return;
-    const uint64_t *src = cand + q * np * cap;
+    const uint64_t *src = cand + cand_slot(qoff, q, np, 0) * cap;
     uint64_t L[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) L[j] = kKey64None;
@@ -434,8 +466,8 @@ template <int KL, int NLMAX>
 __device__ __forceinline__ void merge_item_lists(unsigned char *smem, const uint32_t (&Lk)[2][KL],
                                                  const uint32_t (&Li)[2][KL], const int32_t *s_q,
                                                  const int32_t *__restrict__ pair_p, int cnt, int np,
-                                                 uint64_t *__restrict__ cand, int tstride, int nlist, int list,
-                                                 int qcol) {
+                                                 const int32_t *__restrict__ qoff, uint64_t *__restrict__ cand,
+                                                 int tstride, int nlist, int list, int qcol) {
     constexpr int KP = kIvfKP;
     const int tid = threadIdx.x;
     uint64_t *sl = (uint64_t *)smem;
@@ -447,7 +479,7 @@ __device__ __forceinline__ void merge_item_lists(unsigned char *smem, const uint
                 Lk[j][e] == kKey32None ? kKey64None : (((uint64_t)Lk[j][e] << 32) | Li[j][e]);
     __syncthreads();
     if (tid < cnt) {
-        uint64_t *out = cand + ((int64_t)s_q[tid] * np + pair_p[tid]) * KP;
+        uint64_t *out = cand + cand_slot(qoff, s_q[tid], np, pair_p[tid]) * KP;
         int p[NLMAX];
         uint64_t head[NLMAX];
 #pragma unroll
@@ -567,6 +599,41 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
     if (lane == 0) Rn[sl] = ss;
 }
 
+// Sorts 64 E keys held E per lane (element lane + 64 s in v[s]) ascending across the wave
+// (bitonic network; partners within a lane for strides >= 64, else one lane shuffle away). The
+// input is already made of sorted runs of K0 / 2 that alternate ascending / descending, so the
+// network starts at blocks of K0.
+template <int E, int K0>
+__device__ __forceinline__ void wave_sort_u64(uint64_t (&v)[E], int lane) {
+    constexpr int N = 64 * E;
+#pragma unroll
+    for (int k = K0; k <= N; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+#pragma unroll
+                for (int s = 0; s < E; ++s) {
+                    const int t = s ^ (j / 64);
+                    if (t > s) {
+                        const bool asc = ((lane + 64 * s) & k) == 0;
+                        const uint64_t a = v[s], b = v[t];
+                        const bool sw = asc ? a > b : a < b;
+                        v[s] = sw ? b : a;
+                        v[t] = sw ? a : b;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < E; ++s) {
+                    const uint64_t o = __shfl_xor(v[s], j, 64);
+                    const bool asc = ((lane + 64 * s) & k) == 0, lower = (lane & j) == 0;
+                    const uint64_t mn = o < v[s] ? o : v[s], mx = o < v[s] ? v[s] : o;
+                    v[s] = lower == asc ? mn : mx;
+                }
+            }
+        }
+}
+
 // The list scan in fp8 e4m3 on residuals to the list's centroid c: |q - x|^2 =
 // |q - c|^2 + |x - c|^2 - 2 (q - c).(x - c), so the quantised vectors are the short residuals.
 // One workgroup of 8 waves per item of up to 128 queries. With more than 64 queries the waves
@@ -585,8 +652,8 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
     const uint8_t *__restrict__ L8, const float *__restrict__ rn, const int64_t *__restrict__ loff, int dp,
     float scale, const uint8_t *__restrict__ R8, const float *__restrict__ Rn,
     const int4 *__restrict__ items, int32_t *__restrict__ bins, int nranges, const int32_t *__restrict__ pair_q,
-    const int32_t *__restrict__ pair_p, int np, const uint32_t *__restrict__ qthr, uint64_t *__restrict__ cand,
-    int nmax) {
+    const int32_t *__restrict__ pair_p, int np, const int32_t *__restrict__ qoff, const uint32_t *__restrict__ qthr,
+    uint64_t *__restrict__ cand, int nmax) {
     constexpr int IQ = kIvf8IQ, NL = 16;
     extern __shared__ __align__(16) unsigned char dsm[]; // IQ x (dp + 16) bytes, then nmax floats
     float *s_norm = (float *)(dsm + IQ * (dp + 16)); // the list's row norms when it has <= nmax rows
@@ -695,22 +762,43 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
             step(g + 1, fb, fa);
             step(g + 2, fc, fb);
         }
-        // the lane's lists of its queries go to their (query, probe) slots as they are: slot list
-        // 2 wr + half of 16, unsorted across lists; with two query groups, lists 8 .. 15 are empty
-        const int cap = 16 * KL;
+        // the lists go to LDS (the residual block is no longer read), then each query's nlist x KL
+        // entries are sorted across one wave and its kIvf8Cap = 64 best go to its (query, probe)
+        // slot
+        __syncthreads();
+        uint64_t *sl = (uint64_t *)dsm; // [list][query (tstride)][KL]
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int t = q0 + 32 * j + col;
-            if (t >= cnt) continue;
-            uint64_t *out = cand + ((int64_t)s_q[t] * np + pair_p[start + t]) * cap + (wr * 2 + half) * KL;
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int e = 0; e < KL; ++e) {
-                out[e] = Lk[j][e] == kKey32None ? kKey64None : (((uint64_t)Lk[j][e] << 32) | Li[j][e]);
-                if (wide) out[8 * KL + e] = kKey64None;
+            for (int e = 0; e < KL; ++e)
+                sl[((wr * 2 + half) * tstride + q0 + 32 * j + col) * KL + e] =
+                    Lk[j][e] == kKey32None ? kKey64None : (((uint64_t)Lk[j][e] << 32) | Li[j][e]);
+        __syncthreads();
+        // (each lane list is sorted: odd lists are read backwards, so the runs alternate)
+        auto entry = [&](int m, int t) {
+            const int l = m / KL, e = m % KL;
+            return sl[(l * tstride + t) * KL + ((l & 1) ? KL - 1 - e : e)];
+        };
+        for (int t = w; t < cnt; t += 8) {
+            uint64_t *out = cand + cand_slot(qoff, s_q[t], np, pair_p[start + t]) * kIvf8Cap;
+            if (nlist * KL <= 64) { // two query groups (8 lists), KL = 8: one entry per lane
+                uint64_t v[1] = {entry(lane, t)};
+                wave_sort_u64<1, 2 * KL>(v, lane);
+                out[lane] = v[0];
+            } else {
+                constexpr int E = KL / 4; // 16 lists x KL entries, E per lane
+                uint64_t v[E];
+#pragma unroll
+                for (int s2 = 0; s2 < E; ++s2) {
+                    const int m = lane + 64 * s2;
+                    v[s2] = m < nlist * KL ? entry(m, t) : kKey64None;
+                }
+                wave_sort_u64<E, 2 * KL>(v, lane);
+                out[lane] = v[0]; // the 64 best
             }
         }
         // (the next item's LDS writes follow the loop's first barrier, which every wave reaches
-        // only after this item's products)
+        // only after this item's sorts)
     }
 }
 
@@ -730,8 +818,8 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
     const __bf16 *__restrict__ Lh, const __bf16 *__restrict__ Ll, const float *__restrict__ lxn,
     const int64_t *__restrict__ loff, int dp, const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
     const float *__restrict__ qn, const int4 *__restrict__ items, int32_t *__restrict__ bins, int nranges,
-    const int32_t *__restrict__ pair_q, const int32_t *__restrict__ pair_p, int np, const uint32_t *__restrict__ qthr,
-    uint64_t *__restrict__ cand) {
+    const int32_t *__restrict__ pair_q, const int32_t *__restrict__ pair_p, int np, const int32_t *__restrict__ qoff,
+    const uint32_t *__restrict__ qthr, uint64_t *__restrict__ cand) {
     constexpr int QS = kIvfQS, IQ = kIvfIQ;
     constexpr int NPL = PREC == 3 ? 2 : 1;      // staged planes: hi (and lo)
     constexpr int PLANE = IQ * QS;              // bf16 elements of one staged plane
@@ -857,7 +945,7 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
                           &s_thr[0][0], IQ, 8, w * 2 + half, col, half, two);
         if (rb + 1 < nrb) load_a(rb + 1, 0, acur);
     }
-    merge_item_lists<KL, 8>(smem, Lk, Li, s_q, pair_p + start, cnt, np, cand, IQ, 8, w * 2 + half, col);
+    merge_item_lists<KL, 8>(smem, Lk, Li, s_q, pair_p + start, cnt, np, qoff, cand, IQ, 8, w * 2 + half, col);
     __syncthreads(); // LDS (lists, query chunks, s_q, s_thr, s_slot) is reused by the next item
     }
 }
@@ -871,6 +959,55 @@ __device__ __forceinline__ uint64_t wmin64(uint64_t v) {
     return v;
 }
 
+// Rerank order: queries grouped by their first probed list (a counting sort), so that queries
+// sharing candidate rows are reranked together. key = the list of the first valid probe slot
+// (nl: none).
+__global__ void order_count_kernel(const int32_t *__restrict__ probes, int64_t nq, int np, int nl,
+                                   int32_t *__restrict__ key, int32_t *__restrict__ hist) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    int c = nl;
+    for (int j = 0; j < np; ++j) {
+        const int v = probes[q * np + j];
+        if (v >= 0) {
+            c = v;
+            break;
+        }
+    }
+    key[q] = c;
+    atomicAdd(&hist[c], 1);
+}
+
+// Exclusive scan of hist[0 .. m) in place (one workgroup of 256).
+__global__ __launch_bounds__(256) void order_scan_kernel(int32_t *__restrict__ hist, int m) {
+    __shared__ int32_t part[256];
+    const int tid = threadIdx.x, per = (m + 255) / 256;
+    const int o0 = min(m, tid * per), o1 = min(m, o0 + per);
+    int sum = 0;
+    for (int o = o0; o < o1; ++o) sum += hist[o];
+    part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int a = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += a;
+        __syncthreads();
+    }
+    int run = part[tid] - sum;
+    for (int o = o0; o < o1; ++o) {
+        const int v = hist[o];
+        hist[o] = run;
+        run += v;
+    }
+}
+
+__global__ void order_scatter_kernel(const int32_t *__restrict__ key, int64_t nq, int32_t *__restrict__ cursor,
+                                     int32_t *__restrict__ qorder) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    qorder[atomicAdd(&cursor[key[q]], 1)] = (int32_t)q;
+}
+
 // One wave per query: the K2 best candidates by the approximation are recomputed in fp32 as
 // sum (q - x)^2 by 16 lanes each (four candidates per round; with VEC, 16-byte loads, so a wave
 // reads four rows per step in 256-byte pieces), then the k smallest (distance, id) are written
@@ -879,16 +1016,21 @@ __device__ __forceinline__ uint64_t wmin64(uint64_t v) {
 template <int PER, bool VEC>
 __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restrict__ X, int64_t d,
                                                            const int32_t *__restrict__ lids, const float *__restrict__ Q,
-                                                           int64_t qstride, const uint64_t *__restrict__ cand, int ncand,
+                                                           int64_t qstride, const uint64_t *__restrict__ cand, int np,
+                                                           int cap, const int32_t *__restrict__ qoff,
                                                            int64_t nq, int k, int32_t *__restrict__ out_idx,
-                                                           float *__restrict__ out_dist, int32_t *__restrict__ out_cnt) {
+                                                           float *__restrict__ out_dist, int32_t *__restrict__ out_cnt,
+                                                           const int32_t *__restrict__ qorder, int64_t per_xcd) {
     const int lane = threadIdx.x & 63;
-    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (q >= nq) return;
+    // workgroup b runs on XCD b % 8: each XCD takes a contiguous run of qorder
+    const int64_t pos = ((int64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds) * 4 + (threadIdx.x >> 6);
+    if (pos >= nq) return;
+    const int64_t q = qorder[pos];
     uint64_t L[PER]; // this lane's candidates, sorted
 #pragma unroll
     for (int j = 0; j < PER; ++j) L[j] = kKey64None;
-    const uint64_t *src = cand + q * ncand;
+    const uint64_t *src = cand + cand_slot(qoff, q, np, 0) * cap;
+    const int ncand = (qoff ? qoff[q + 1] - qoff[q] : np) * cap;
     // bound: when every lane holds a candidate, the largest of the 64 lane minima has at least
     // K2 = 64 candidates at or below it, so nothing above it is among the K2 best
     uint64_t lmin = kKey64None;
@@ -1118,16 +1260,24 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     int64_t *woff = (int64_t *)h->woff.ensure((size_t)h->nl * 8);
     const int nranges = tune_get("ivfl2.xcd_ranges") ? kXcds : 1;
     const int iq = prec == 8 ? kIvf8IQ : kIvfIQ; // queries per item
-    // candidate slots per (query, probe): the bf16 scan merges its lists into the KP best; the
-    // fp8 scan stores its 16 lists of KL as they are
+    // candidate slots per (query, probe): the bf16 scan merges its lists into the KP = 32 best,
+    // the fp8 scan (more lists, coarser keys) into the 64 best
     const int kl = k <= 16 ? 8 : 16;
-    const int cap = prec == 8 ? 16 * kl : kIvfKP;
+    const int cap = prec == 8 ? kIvf8Cap : kIvfKP;
     uint64_t *cand = (uint64_t *)h->cand.ensure((size_t)total * cap * 8);
+    // with per-query probe counts, each query's slots are packed after the previous query's
+    // (qoff); slots of unused probes inside a query's span stay empty
+    const int32_t *qoff = nullptr;
     {
         StageScope sc(st, STAGE_OTHER);
-        if (sparse)
-            hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)ceil_div(total * cap, (int64_t)256)), dim3(256), 0, st,
-                               cand, total * cap, kKey64None);
+        if (sparse) {
+            int32_t *span = (int32_t *)h->qoff.ensure((size_t)(nq + 1) * 4);
+            hipLaunchKernelGGL(probe_span_kernel, dim3((unsigned)ceil_div(nq + 1, (int64_t)256)), dim3(256), 0, st, pr,
+                               nq, np, span);
+            hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(256), 0, st, span, (int)(nq + 1));
+            hipLaunchKernelGGL(fill_slots_kernel, dim3(2048), dim3(256), 0, st, cand, span, nq, cap, kKey64None);
+            qoff = span;
+        }
         if (prec != 8) l2_split_rows(Q, nq, d, re, dp, Qh, Ql, qn, st); // (the fp8 scan converts in LDS)
     }
     // pairs of probe slots p_lo .. p_lo + npw - 1 -> items of at most 64 pairs of one cluster
@@ -1155,7 +1305,8 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         auto launch = [&](auto pc, auto kc) {
             hipLaunchKernelGGL((ivfl2_scan_kernel<decltype(pc)::value, decltype(kc)::value>), grid, dim3(256), 0, st,
                                h->lrows.hi.as<__bf16>(), h->lrows.lo.as<__bf16>(), h->lrows.norm.as<float>(),
-                               h->loff.as<int64_t>(), dp, Qh, Ql, qn, items, bins, nranges, pq, pp, np, qthr, cand);
+                               h->loff.as<int64_t>(), dp, Qh, Ql, qn, items, bins, nranges, pq, pp, np, qoff, qthr,
+                               cand);
         };
         using P1 = std::integral_constant<int, 1>;
         using P3 = std::integral_constant<int, 3>;
@@ -1168,7 +1319,7 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
             const size_t rblk = (size_t)kIvf8IQ * (dp + 16);
             PN_REQUIRE(rblk <= kLdsMax, PN_EINVAL, "fp8 precision serves d up to 1152");
             const int nmax = (int)std::min<size_t>(8192, (kLdsMax - rblk) / 4 / 64 * 64);
-            const size_t lds = rblk + (size_t)nmax * 4;
+            const size_t lds = std::max(rblk + (size_t)nmax * 4, (size_t)16 * 64 * KLV * 8); // (and the lists)
             if (lds > 64 * 1024)
                 PN_HIP(hipFuncSetAttribute((const void *)ivfl2_scan_fp8_kernel<KLV>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1181,7 +1332,7 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
                                                          max_items) * kXcds);
             hipLaunchKernelGGL((ivfl2_scan_fp8_kernel<KLV>), grid8, dim3(512), lds, st, h->l8.as<uint8_t>(),
                                h->rn8.as<float>(), h->loff.as<int64_t>(), dp, h->sx8, R8, Rn, items, bins, nranges,
-                               pq, pp, np, qthr, cand, nmax);
+                               pq, pp, np, qoff, qthr, cand, nmax);
         };
         if (prec == 8)
             kl == 8 ? launch8(K8{}) : launch8(K16{});
@@ -1201,7 +1352,7 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         {
             StageScope sc(st, STAGE_OTHER);
             hipLaunchKernelGGL(pass_bound_kernel, dim3((unsigned)ceil_div(nq, (int64_t)4)), dim3(256), 0, st, cand, nq,
-                               np, cap, k, 1.f + (float)margin_pct / 100.f, qthr);
+                               np, cap, qoff, k, 1.f + (float)margin_pct / 100.f, qthr);
         }
         bucket(1, np - 1);
         scan(qthr);
@@ -1213,10 +1364,21 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         StageScope sc(st, STAGE_RERANK);
         const int ncand = np * cap;
         const bool vec = d % 4 == 0 && re % 4 == 0;
-        dim3 g((unsigned)ceil_div(nq, 4));
+        // queries by first probed list
+        int32_t *qkey = (int32_t *)h->qkey.ensure((size_t)nq * 4);
+        int32_t *qorder = (int32_t *)h->qorder.ensure((size_t)nq * 4);
+        int32_t *ohist = (int32_t *)h->ohist.ensure((size_t)(h->nl + 1) * 4);
+        PN_HIP(hipMemsetAsync(ohist, 0, (size_t)(h->nl + 1) * 4, st));
+        const unsigned gq = (unsigned)ceil_div(nq, (int64_t)256);
+        hipLaunchKernelGGL(order_count_kernel, dim3(gq), dim3(256), 0, st, pr, nq, np, h->nl, qkey, ohist);
+        hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(256), 0, st, ohist, h->nl + 1);
+        hipLaunchKernelGGL(order_scatter_kernel, dim3(gq), dim3(256), 0, st, qkey, nq, ohist, qorder);
+        const int64_t per_xcd = ceil_div(ceil_div(nq, (int64_t)4), (int64_t)kXcds);
+        dim3 g((unsigned)(per_xcd * kXcds));
         auto launch = [&](auto per, auto v) {
             hipLaunchKernelGGL((ivfl2_rerank_kernel<decltype(per)::value, decltype(v)::value>), g, dim3(256), 0, st,
-                               h->X.as<float>(), d, h->lids.as<int32_t>(), Q, re, cand, ncand, nq, k, oi, od, oc);
+                               h->X.as<float>(), d, h->lids.as<int32_t>(), Q, re, cand, np, cap, qoff, nq, k, oi, od,
+                               oc, qorder, per_xcd);
         };
         using T = std::true_type;
         using F = std::false_type;
