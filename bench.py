@@ -226,7 +226,7 @@ def run_exact(workload, steps, warmup, cpu):
     return out
 
 
-def run_ivf(steps, warmup, cpu, probes=(8, 16, 32)):
+def run_ivf(steps, warmup, cpu, probes=(8, 16, 32), precisions=("bf16", "fp8")):
     from pynear_amd import _native as N
     from pynear_amd import hipdev
     t0 = time.perf_counter()
@@ -255,7 +255,7 @@ def run_ivf(steps, warmup, cpu, probes=(8, 16, 32)):
     stream = hipdev.Stream()
     sizes = np.bincount(labels, minlength=nlist)
     sweeps, results = {}, {}
-    for prec in ("bf16", "fp8"):  # candidate distances: bf16 or fp8 e4m3 products, fp32 rerank either way
+    for prec in precisions:  # candidate distances: bf16 or fp8 e4m3 products, fp32 rerank either way
         sweep = sweeps[prec] = {}
         for npb in probes:
             hi, hd, hc = h.search(Q, npb, k, prec)
@@ -276,7 +276,7 @@ def run_ivf(steps, warmup, cpu, probes=(8, 16, 32)):
         if ok:
             best[prec] = ok[0]
     if not best:
-        best = {"bf16": probes[-1]}
+        best = {precisions[0]: probes[-1]}
     prec = max(best, key=lambda pr_: sweeps[pr_][best[pr_]]["qps"])
     chosen = best[prec]
     S = sweeps[prec][chosen]
@@ -566,6 +566,8 @@ def main():
     ap.add_argument("--workload", default="ivf", choices=["ivf", "hamming", "l2", "batch", "learned"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="ivf only: skip the exact workloads")
+    ap.add_argument("--ivf-probes", default="8,16,32", help="ivf only: n_probe values swept")
+    ap.add_argument("--ivf-precisions", default="bf16,fp8", help="ivf only: candidate precisions swept")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device-to-device copies of 1 GiB (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -578,7 +580,8 @@ def main():
         sys.exit(2)
     cpu = not args.no_cpu_baseline
     if args.workload == "ivf":
-        main_r = run_ivf(args.steps, args.warmup, cpu)
+        main_r = run_ivf(args.steps, args.warmup, cpu, tuple(int(v) for v in args.ivf_probes.split(",")),
+                         tuple(args.ivf_precisions.split(",")))
         dtype = main_r["dtype"]
     elif args.workload == "batch":
         main_r = run_batch(args.steps, args.warmup, cpu)

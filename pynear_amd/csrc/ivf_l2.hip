@@ -1294,6 +1294,10 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
                            iq, items, bins);
         hipLaunchKernelGGL(probe_scatter_kernel, dim3(pchunks), dim3(256), 0, st, pr, tw, np, p_lo, npw, h->nl, cur, pq,
                            pp, pc);
+        if (prec == 8) // fp8 query residuals of every pair
+            hipLaunchKernelGGL(query_residual_kernel, dim3((unsigned)ceil_div(tw, (int64_t)4)), dim3(256), 0, st, Q, re,
+                               (int)d, dp, h->lcent.as<float>(), h->sx8, bins, pq, pc,
+                               (uint8_t *)h->qres8.ensure((size_t)total * dp), (float *)h->qresn.ensure((size_t)total * 4));
         PN_HIP(hipGetLastError());
     };
     auto scan = [&](const uint32_t *qthr) {
@@ -1323,15 +1327,12 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
             if (lds > 64 * 1024)
                 PN_HIP(hipFuncSetAttribute((const void *)ivfl2_scan_fp8_kernel<KLV>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            uint8_t *R8 = (uint8_t *)h->qres8.ensure((size_t)total * dp);
-            float *Rn = (float *)h->qresn.ensure((size_t)total * 4);
-            hipLaunchKernelGGL(query_residual_kernel, dim3((unsigned)ceil_div(total, (int64_t)4)), dim3(256), 0, st, Q,
-                               re, (int)d, dp, h->lcent.as<float>(), h->sx8, bins, pq, pc, R8, Rn);
             const dim3 grid8((unsigned)std::min<int64_t>(ceil_div((int64_t)h->cus, (int64_t)kXcds) *
                                                              std::max<int64_t>(1, tune_get("ivfl2.fp8_wg_per_cu")),
                                                          max_items) * kXcds);
             hipLaunchKernelGGL((ivfl2_scan_fp8_kernel<KLV>), grid8, dim3(512), lds, st, h->l8.as<uint8_t>(),
-                               h->rn8.as<float>(), h->loff.as<int64_t>(), dp, h->sx8, R8, Rn, items, bins, nranges,
+                               h->rn8.as<float>(), h->loff.as<int64_t>(), dp, h->sx8, h->qres8.as<uint8_t>(),
+                               h->qresn.as<float>(), items, bins, nranges,
                                pq, pp, np, qoff, qthr, cand, nmax);
         };
         if (prec == 8)
