@@ -52,24 +52,34 @@ __global__ void max_f32_kernel(const float *__restrict__ v, int64_t n, unsigned 
     if (threadIdx.x == 0) atomicMax(out, __float_as_uint(red[0])); // non-negative floats order as uints
 }
 
-// A workgroup owns 128 queries (block blockIdx.x) and one slice of rows (blockIdx.y).
+// A workgroup owns 128 queries (one query block) and one slice of rows: with slice_major the
+// slice is blockIdx.x and the block blockIdx.y, else the other way round.
 template <int KL, int PREC>
 __global__ __launch_bounds__(256) void l2_mfma_scan_kernel(const __bf16 *__restrict__ Xh, const __bf16 *__restrict__ Xl,
                                                            const float *__restrict__ xn, int64_t n, int dp,
                                                            const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
                                                            const float *__restrict__ qn, int64_t rows_per_slice,
-                                                           uint64_t *__restrict__ part, int64_t nq_pad) {
+                                                           uint64_t *__restrict__ part, int64_t nq_pad, bool slice_major) {
     __shared__ uint64_t lds[4][32][KL];
-    scan_generic<KL, PREC>(lds, Xh, Xl, xn, n, dp, Qh, Ql, qn, rows_per_slice, part, nq_pad, blockIdx.x, blockIdx.y);
+    scan_generic<KL, PREC>(lds, Xh, Xl, xn, n, dp, Qh, Ql, qn, rows_per_slice, part, nq_pad,
+                           slice_major ? blockIdx.y : blockIdx.x, slice_major ? blockIdx.x : blockIdx.y);
+}
+
+__global__ void fill_u32_kernel(uint32_t *__restrict__ p, int64_t n, uint32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 template <int KL, int PREC, int NS>
 __global__ __launch_bounds__(256) void l2_mfma_scan_small_kernel(
     const __bf16 *__restrict__ Xh, const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
     const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql, const float *__restrict__ qn,
-    int64_t rows_per_slice, uint64_t *__restrict__ part, int64_t nq_pad) {
+    int64_t rows_per_slice, uint64_t *__restrict__ part, int64_t nq_pad, bool slice_major, uint32_t *__restrict__ gthr) {
     __shared__ uint64_t lds[4][32][KL];
-    scan_small<KL, PREC, NS>(lds, Xh, Xl, xn, n, Qh, Ql, qn, rows_per_slice, part, nq_pad, blockIdx.x, blockIdx.y);
+    // slice_major: blockIdx.x is the slice, so workgroup b (on XCD b % 8) of every query block
+    // scans the same eighth of the slices, and those rows stay in that XCD's L2
+    scan_small<KL, PREC, NS>(lds, Xh, Xl, xn, n, Qh, Ql, qn, rows_per_slice, part, nq_pad,
+                             slice_major ? blockIdx.y : blockIdx.x, slice_major ? blockIdx.x : blockIdx.y, gthr);
 }
 
 __global__ __launch_bounds__(256) void l2_rerank_kernel(const float *__restrict__ X, int64_t xstride, int64_t d,
@@ -275,9 +285,9 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
                     int kk, int32_t *out_idx, float *out_dist, L2Work &wk, int cus, hipStream_t st) {
     const int64_t n = xs.n, d = xs.d;
     const int dp = xs.dp;
-    const int KL = kl_for(kk);
-    const int K2 = 2 * KL;
-    const int K2c = (int)std::min<int64_t>(K2, n);
+    int KL = kl_for(kk);
+    int K2 = 2 * KL;
+    int K2c = (int)std::min<int64_t>(K2, n);
     const int64_t gx = ceil_div(nq, 128);
     const int64_t nq_pad = gx * 128;
     // queries: split into hi/lo (padded to nq_pad rows of dp; padding rows are zero)
@@ -349,11 +359,27 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
     const int64_t rps = ceil_div(ceil_div(n, S), 32) * 32; // slices start on 32-row boundaries
     S = ceil_div(n, rps);
     const int64_t nlists = S;
+    // with at least 4 slices, lists of 4 give K2c = 8 candidates, enough for k <= 6 (fewer keys per
+    // lane to keep sorted)
+    if (kk <= 6 && S >= 4 && tune_get("l2.kl4") != 0) {
+        KL = 4;
+        K2 = 8;
+        K2c = (int)std::min<int64_t>(K2, n);
+    }
     uint64_t *part = (uint64_t *)wk.part.ensure((size_t)nlists * KL * nq_pad * 8);
     const int prec = (int)tune_get("l2.mfma_terms");
     {
         StageScope sc(st, STAGE_SCAN);
-        dim3 grid((unsigned)gx, (unsigned)S);
+        // slice-major when the query blocks fit the grid's y dimension (see the kernels)
+        const bool sm = gx <= 65535 && tune_get("l2.slice_major") != 0;
+        // thresholds shared across slices (small scan): one key per query, cleared by a kernel
+        uint32_t *gthr = nullptr;
+        if (dp <= 64 && S > 1 && tune_get("l2.shared_thr") != 0) {
+            gthr = (uint32_t *)wk.gthr.ensure((size_t)nq_pad * 4);
+            hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)ceil_div(nq_pad, (int64_t)256)), dim3(256), 0, st, gthr,
+                               nq_pad, kKey32None);
+        }
+        dim3 grid(sm ? (unsigned)S : (unsigned)gx, sm ? (unsigned)gx : (unsigned)S);
         auto launch = [&](auto klc, auto pc) {
             constexpr int KLV = decltype(klc)::value, PV = decltype(pc)::value;
             const __bf16 *xh = xs.hi.as<__bf16>(), *xl = xs.lo.as<__bf16>();
@@ -361,34 +387,37 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
             switch (dp) {
             case 16:
                 hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 1>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
-                                   Ql, qn, rps, part, nq_pad);
+                                   Ql, qn, rps, part, nq_pad, sm, gthr);
                 break;
             case 32:
                 hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 2>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
-                                   Ql, qn, rps, part, nq_pad);
+                                   Ql, qn, rps, part, nq_pad, sm, gthr);
                 break;
             case 48:
                 hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 3>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
-                                   Ql, qn, rps, part, nq_pad);
+                                   Ql, qn, rps, part, nq_pad, sm, gthr);
                 break;
             case 64:
                 hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 4>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
-                                   Ql, qn, rps, part, nq_pad);
+                                   Ql, qn, rps, part, nq_pad, sm, gthr);
                 break;
             default:
                 hipLaunchKernelGGL((l2_mfma_scan_kernel<KLV, PV>), grid, dim3(256), 0, st, xh, xl, xnr, n, dp, Qh, Ql, qn,
-                                   rps, part, nq_pad);
+                                   rps, part, nq_pad, sm);
             }
         };
+        using I4 = std::integral_constant<int, 4>;
         using I8 = std::integral_constant<int, 8>;
         using I16 = std::integral_constant<int, 16>;
         using I32 = std::integral_constant<int, 32>;
         using P1 = std::integral_constant<int, 1>;
         using P3 = std::integral_constant<int, 3>;
         if (prec == 1) {
-            if (KL == 8) launch(I8{}, P1{}); else if (KL == 16) launch(I16{}, P1{}); else launch(I32{}, P1{});
+            if (KL == 4) launch(I4{}, P1{}); else if (KL == 8) launch(I8{}, P1{});
+            else if (KL == 16) launch(I16{}, P1{}); else launch(I32{}, P1{});
         } else {
-            if (KL == 8) launch(I8{}, P3{}); else if (KL == 16) launch(I16{}, P3{}); else launch(I32{}, P3{});
+            if (KL == 4) launch(I4{}, P3{}); else if (KL == 8) launch(I8{}, P3{});
+            else if (KL == 16) launch(I16{}, P3{}); else launch(I32{}, P3{});
         }
         PN_HIP(hipGetLastError());
     }

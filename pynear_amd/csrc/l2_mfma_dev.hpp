@@ -40,14 +40,23 @@ template <int KL, bool MASK, bool FIRST>
 __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4 (&xv)[4], float qnv, int64_t t,
                                                  int64_t r1, int half, uint32_t (&Lk)[KL], uint32_t (&Li)[KL],
                                                  uint32_t &thr) {
+    float v[16];
+    float vmin = INFINITY;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const float xr = (r & 3) == 0 ? xv[r >> 2].x : (r & 3) == 1 ? xv[r >> 2].y : (r & 3) == 2 ? xv[r >> 2].z : xv[r >> 2].w;
+        v[r] = fmaxf(0.f, (qnv + xr) - 2.f * acc[r]);
+        vmin = fminf(vmin, v[r]);
+    }
+    // every key of the tile is >= bits(vmin) with the offset bits cleared (masked rows only lower
+    // vmin): a tile no lane can take from is done
+    if (!__any((f32_key(vmin) & ~31u) < thr)) return;
     uint32_t key[16];
     uint32_t mn = kKey32None;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int off = 8 * (r >> 2) + 4 * half + (r & 3);
-        const float xr = (r & 3) == 0 ? xv[r >> 2].x : (r & 3) == 1 ? xv[r >> 2].y : (r & 3) == 2 ? xv[r >> 2].z : xv[r >> 2].w;
-        const float v = fmaxf(0.f, (qnv + xr) - 2.f * acc[r]);
-        key[r] = !MASK || t + off < r1 ? ((f32_key(v) & ~31u) | (uint32_t)off) : kKey32None;
+        key[r] = !MASK || t + off < r1 ? ((f32_key(v[r]) & ~31u) | (uint32_t)off) : kKey32None;
         mn = min(mn, key[r]);
     }
     if (FIRST) { // empty list: the tile's KL smallest keys, sorted, by a network instead of KL rounds
@@ -159,14 +168,22 @@ __device__ __forceinline__ void scan_generic(uint64_t (*lds)[32][KL], const __bf
 }
 
 // The same scan for dp = 16 NS (NS <= 4): the lane's query fragments stay in registers for the
-// whole slice, and the next tile's row fragments and norms are loaded before the current tile's
-// products and keys, so the memory latency of one tile hides behind the work of the previous one.
+// whole slice, and row fragments and norms are loaded two tiles ahead into a ring of three
+// register sets that rotate by name (three tiles per loop trip; a copy between sets would wait
+// for the loads in flight), so the memory latency hides behind the work of the two tiles before.
+//
+// gthr (optional, one key per query, all kKey32None before the launch) shares thresholds across
+// the slices of a query: a lane whose list is full publishes its last key with atomicMin and
+// every tile takes the published minimum (read with the tile's rows) as its own threshold too.
+// Keys at or above it are dropped; the smallest last key of all lists still bounds every dropped
+// key from below, as the completeness proof needs.
 template <int KL, int PREC, int NS>
 __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16 *__restrict__ Xh,
                                            const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
                                            const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
                                            const float *__restrict__ qn, int64_t rows_per_slice,
-                                           uint64_t *__restrict__ part, int64_t nq_pad, int64_t qb, int64_t s) {
+                                           uint64_t *__restrict__ part, int64_t nq_pad, int64_t qb, int64_t s,
+                                           uint32_t *__restrict__ gthr = nullptr) {
     constexpr int dp = 16 * NS;
     constexpr int NPL = PREC == 3 ? 2 : 1;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
@@ -186,42 +203,59 @@ __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16
         Li[j] = 0u;
     }
     uint32_t thr = kKey32None;
-    bf16x8 a[NS][NPL], an[NS][NPL];
-    float4 xv[4], xvn[4];
-    auto load_tile = [&](int64_t t, bf16x8 (&fa)[NS][NPL], float4 (&fx)[4]) {
+    // tiles i = 0 .. nt - 1 at rows r0 + 32 i; loads past the last tile re-read it (padded arrays:
+    // in bounds), and up to two tiles past the end are computed on those rows and masked whole
+    struct Tile {
+        bf16x8 a[NS][NPL];
+        float4 xv[4];
+        uint32_t g; // the shared threshold as read with this tile
+    };
+    uint32_t published = kKey32None;
+    auto load_tile = [&](int64_t t, Tile &f) __attribute__((always_inline)) {
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            fa[k][0] = *(const bf16x8 *)(Xh + (t + col) * dp + 16 * k + 8 * half);
-            if (PREC == 3) fa[k][NPL - 1] = *(const bf16x8 *)(Xl + (t + col) * dp + 16 * k + 8 * half);
+            f.a[k][0] = *(const bf16x8 *)(Xh + (t + col) * dp + 16 * k + 8 * half);
+            if (PREC == 3) f.a[k][NPL - 1] = *(const bf16x8 *)(Xl + (t + col) * dp + 16 * k + 8 * half);
         }
 #pragma unroll
-        for (int g = 0; g < 4; ++g) fx[g] = *(const float4 *)(xn + t + 8 * g + 4 * half);
+        for (int g = 0; g < 4; ++g) f.xv[g] = *(const float4 *)(xn + t + 8 * g + 4 * half);
+        f.g = gthr ? __hip_atomic_load(gthr + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kKey32None;
     };
-    if (r0 < r1) load_tile(r0, a, xv);
-    for (int64_t t = r0; t < r1; t += 32) {
-        if (t + 32 < r1) load_tile(t + 32, an, xvn); // padded arrays: in bounds
+    const int64_t nt = r0 < r1 ? (r1 - r0 + 31) / 32 : 0;
+    auto step = [&](int64_t i, const Tile &cur, Tile &ahead) __attribute__((always_inline)) {
+        load_tile(r0 + 32 * min(i + 2, nt - 1), ahead);
+        const int64_t t = r0 + 32 * i;
         f32x16 acc = {};
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][0], b[k][0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur.a[k][0], b[k][0], acc, 0, 0, 0);
             if (PREC == 3) {
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][0], b[k][NPL - 1], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][NPL - 1], b[k][0], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur.a[k][0], b[k][NPL - 1], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur.a[k][NPL - 1], b[k][0], acc, 0, 0, 0);
             }
         }
         thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64)); // the other half-wave's list of this query
+        thr = min(thr, cur.g);                                     // ... and the other slices' lists
         if (t == r0) // only a slice's last tile can hold rows past its end
-            t + 32 <= r1 ? tile_keys_insert<KL, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
-                         : tile_keys_insert<KL, true, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+            t + 32 <= r1 ? tile_keys_insert<KL, false, true>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr)
+                         : tile_keys_insert<KL, true, true>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr);
         else
-            t + 32 <= r1 ? tile_keys_insert<KL, false, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
-                         : tile_keys_insert<KL, true, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
-#pragma unroll
-        for (int k = 0; k < NS; ++k)
-#pragma unroll
-            for (int pl = 0; pl < NPL; ++pl) a[k][pl] = an[k][pl];
-#pragma unroll
-        for (int g = 0; g < 4; ++g) xv[g] = xvn[g];
+            t + 32 <= r1 ? tile_keys_insert<KL, false, false>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr)
+                         : tile_keys_insert<KL, true, false>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr);
+        if (gthr && Lk[KL - 1] < published) { // a full list publishes its last key
+            published = Lk[KL - 1];
+            atomicMin(gthr + q, published);
+        }
+    };
+    if (nt > 0) {
+        Tile f0, f1, f2;
+        load_tile(r0, f0);
+        load_tile(r0 + 32 * min((int64_t)1, nt - 1), f1);
+        for (int64_t i = 0; i < nt; i += 3) {
+            step(i, f0, f2);
+            step(i + 1, f1, f0);
+            step(i + 2, f2, f1);
+        }
     }
     half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + (s * KL) * nq_pad + q, nq_pad);
 }

@@ -41,6 +41,10 @@ static std::map<std::string, int64_t> &tune_table() {
         t["l2.mfma_terms"] = 3;        // 3: hi.hi + hi.lo + lo.hi products; 1: hi.hi only
         t["l2.matrix"] = 1;            // exact L2 over <= 4096 rows of d > 64: key matrix + selection
         t["ivfl2.wg_per_cu"] = 2;      // IVF-L2 list scan: persistent workgroups per CU
+        t["l2.shared_thr"] = 0;       // exact L2 small scan: thresholds shared across slices (atomics;
+                                      // measured 5x slower from contention, off)
+        t["l2.kl4"] = 1;              // exact L2 scan: per-lane lists of 4 for k <= 6 (with >= 4 slices)
+        t["l2.slice_major"] = 1;      // exact L2 scan grid: slices along x (rows of a slice stay in one XCD's L2)
         t["ivfl2.xcd_ranges"] = 0;     // 1: one item range per XCD; 0: one queue for all workgroups
         t["ivfl2.fp8_wg_per_cu"] = 1;  // fp8 scan: workgroups of 8 waves per CU
         t["ivfl2.pass_margin_pct"] = 0; // > 0: scan the first probe, then the others bounded by its
