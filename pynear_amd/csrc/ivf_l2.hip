@@ -57,7 +57,7 @@ struct pn_ivfl2 {
     pn::DevBuf rn8;          // |x - c|^2 per row (cluster order)
     pn::DevBuf lcent;        // list centroids, fp32 nl x d (list order)
     float sx8 = 1.f;
-    pn::DevBuf qpad, qsplit_h, qsplit_l, qn, probes, pdist, lcount, cursor, pair_q, pair_p, pair_c;
+    pn::DevBuf qpad, qsplit_h, qsplit_l, qn, probes, pdist, lcount, cursor, pair_q, pair_p, pair_c, pslot;
     pn::DevBuf items, ntot /* item ranges */, itoff, woff, cand, qoff, qthr, qres8, qresn, qkey, qorder, ohist, oidx,
         odist, ocnt;
 };
@@ -200,12 +200,13 @@ __global__ void expand_items_kernel(const int32_t *__restrict__ lcount, const in
     }
 }
 
-// Writes each pair (query, probe rank, and its list) to its cluster's next free slot. Per chunk, one global
+// Writes each pair (query, probe rank, and its list) to its cluster's next free slot, and the
+// slot into pslot[query np + probe rank]. Per chunk, one global
 // atomic per cluster reserves the chunk's slots (LDS path); pairs then take ranks inside them.
 __global__ __launch_bounds__(256) void probe_scatter_kernel(const int32_t *__restrict__ probes, int64_t total, int np,
                                                             int p_lo, int npw, int nl, int32_t *__restrict__ cursor,
                                                             int32_t *__restrict__ pair_q, int32_t *__restrict__ pair_p,
-                                                            int32_t *__restrict__ pair_c) {
+                                                            int32_t *__restrict__ pair_c, int32_t *__restrict__ pslot) {
     __shared__ int32_t hist[kLdsBins];
     const int64_t c0 = (int64_t)blockIdx.x * kPairChunk, c1 = min(total, c0 + kPairChunk);
     if (nl > kLdsBins) {
@@ -216,6 +217,7 @@ __global__ __launch_bounds__(256) void probe_scatter_kernel(const int32_t *__res
             pair_q[pos] = (int32_t)(sl / np);
             pair_p[pos] = (int32_t)(sl % np);
             pair_c[pos] = probes[sl];
+            pslot[sl] = pos;
         }
         return;
     }
@@ -236,6 +238,7 @@ __global__ __launch_bounds__(256) void probe_scatter_kernel(const int32_t *__res
         pair_q[pos] = (int32_t)(sl / np);
         pair_p[pos] = (int32_t)(sl % np);
         pair_c[pos] = probes[sl];
+        pslot[sl] = pos;
     }
 }
 
@@ -566,37 +569,68 @@ __global__ __launch_bounds__(256) void residual_kernel(const float *__restrict__
 }
 
 // Query residuals of every pair, (q - c) * scale as fp8 e4m3 (clamped to +-448) into R8[pair
-// slot] (dp bytes, zero padding) and |q - c|^2 into Rn[pair slot]. One wave per pair slot (slots
-// past the pair total in bins exit); lane l converts dims 4 l + 256 t .. +3.
+// slot] (dp bytes, zero padding) and |q - c|^2 into Rn[pair slot]. One wave per query: its row is
+// read once (dims 4 l + 256 t .. +3 in lane l), then each of its probes p_lo .. p_lo + npw - 1
+// (list probes[q np + p], slot pslot[q np + p]) is converted; slots of absent probes are skipped.
+template <int NV>
 __global__ __launch_bounds__(256) void query_residual_kernel(const float *__restrict__ Q, int64_t qstride, int d,
                                                              int dp, const float *__restrict__ lcent, float scale,
-                                                             const int32_t *__restrict__ bins,
-                                                             const int32_t *__restrict__ pair_q,
-                                                             const int32_t *__restrict__ pair_c,
+                                                             const int32_t *__restrict__ probes, int64_t nq, int np,
+                                                             int p_lo, int npw, const int32_t *__restrict__ pslot,
                                                              uint8_t *__restrict__ R8, float *__restrict__ Rn) {
-    const int64_t sl = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (sl >= bins[3 * kXcds + 1]) return;
-    const float *cc = lcent + (int64_t)pair_c[sl] * d;
-    const float *qrow = Q + (int64_t)pair_q[sl] * qstride;
-    uint8_t *out = R8 + sl * dp;
-    float ss = 0.f;
-    for (int e0 = 4 * lane; e0 < dp; e0 += 256) {
-        float v[4];
+    if (q >= nq) return;
+    const float *qrow = Q + q * qstride;
+    const bool vec = (d & 3) == 0 && (qstride & 3) == 0; // 16-byte loads (rows of whole float4)
+    float qv[NV][4]; // NV = dp / 256 rounded up
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const float r = e0 + u < d ? qrow[e0 + u] - cc[e0 + u] : 0.f;
-            ss = fmaf(r, r, ss);
-            v[u] = fminf(fmaxf(r * scale, -448.f), 448.f);
+    for (int t = 0; t < NV; ++t) {
+        const int e0 = 4 * lane + 256 * t;
+        if (vec) {
+            const float4 v = e0 < d ? *(const float4 *)(qrow + e0) : make_float4(0.f, 0.f, 0.f, 0.f);
+            qv[t][0] = v.x, qv[t][1] = v.y, qv[t][2] = v.z, qv[t][3] = v.w;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) qv[t][u] = e0 + u < d ? qrow[e0 + u] : 0.f;
         }
-        int wq = 0;
-        wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], wq, false);
-        wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], wq, true);
-        *(int *)(out + e0) = wq;
     }
+    for (int p = p_lo; p < p_lo + npw; ++p) {
+        const int c = probes[q * np + p];
+        if (c < 0) continue; // (wave-uniform)
+        const int64_t sl = pslot[q * np + p];
+        const float *cc = lcent + (int64_t)c * d;
+        uint8_t *out = R8 + sl * dp;
+        float ss = 0.f;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
-    if (lane == 0) Rn[sl] = ss;
+        for (int t = 0; t < NV; ++t) {
+            const int e0 = 4 * lane + 256 * t;
+            if (e0 < dp) {
+                float cv[4];
+                if (vec) {
+                    const float4 c4 = e0 < d ? *(const float4 *)(cc + e0) : make_float4(0.f, 0.f, 0.f, 0.f);
+                    cv[0] = c4.x, cv[1] = c4.y, cv[2] = c4.z, cv[3] = c4.w;
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) cv[u] = e0 + u < d ? cc[e0 + u] : 0.f;
+                }
+                float v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const float r = e0 + u < d ? qv[t][u] - cv[u] : 0.f;
+                    ss = fmaf(r, r, ss);
+                    v[u] = fminf(fmaxf(r * scale, -448.f), 448.f);
+                }
+                int wq = 0;
+                wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], wq, false);
+                wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], wq, true);
+                *(int *)(out + e0) = wq;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+        if (lane == 0) Rn[sl] = ss;
+    }
 }
 
 // Sorts 64 E keys held E per lane (element lane + 64 s in v[s]) ascending across the wave
@@ -1252,6 +1286,7 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     int32_t *pq = (int32_t *)h->pair_q.ensure((size_t)total * 4);
     int32_t *pp = (int32_t *)h->pair_p.ensure((size_t)total * 4);
     int32_t *pc = (int32_t *)h->pair_c.ensure((size_t)total * 4);
+    int32_t *psl = (int32_t *)h->pslot.ensure((size_t)total * 4);
     // each cluster has at most one partly filled item
     const int64_t max_items = ceil_div(total, kIvfIQ) + h->nl;
     int4 *items = (int4 *)h->items.ensure((size_t)max_items * 16);
@@ -1293,11 +1328,26 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
                            h->corder.as<int32_t>(), h->loff.as<int64_t>(), h->nl, cur, itoff, woff, max_items, nranges,
                            iq, items, bins);
         hipLaunchKernelGGL(probe_scatter_kernel, dim3(pchunks), dim3(256), 0, st, pr, tw, np, p_lo, npw, h->nl, cur, pq,
-                           pp, pc);
-        if (prec == 8) // fp8 query residuals of every pair
-            hipLaunchKernelGGL(query_residual_kernel, dim3((unsigned)ceil_div(tw, (int64_t)4)), dim3(256), 0, st, Q, re,
-                               (int)d, dp, h->lcent.as<float>(), h->sx8, bins, pq, pc,
-                               (uint8_t *)h->qres8.ensure((size_t)total * dp), (float *)h->qresn.ensure((size_t)total * 4));
+                           pp, pc, psl);
+        if (prec == 8) { // fp8 query residuals of every pair
+            uint8_t *R8 = (uint8_t *)h->qres8.ensure((size_t)total * dp);
+            float *Rn = (float *)h->qresn.ensure((size_t)total * 4);
+            const dim3 gq((unsigned)ceil_div(nq, (int64_t)4));
+            const float *lcp = h->lcent.as<float>();
+            PN_REQUIRE(dp <= 1152, PN_EINVAL, "fp8 precision serves d up to 1152");
+            if (dp <= 256)
+                hipLaunchKernelGGL(query_residual_kernel<1>, gq, dim3(256), 0, st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq,
+                                   np, p_lo, npw, psl, R8, Rn);
+            else if (dp <= 512)
+                hipLaunchKernelGGL(query_residual_kernel<2>, gq, dim3(256), 0, st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq,
+                                   np, p_lo, npw, psl, R8, Rn);
+            else if (dp <= 768)
+                hipLaunchKernelGGL(query_residual_kernel<3>, gq, dim3(256), 0, st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq,
+                                   np, p_lo, npw, psl, R8, Rn);
+            else
+                hipLaunchKernelGGL(query_residual_kernel<5>, gq, dim3(256), 0, st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq,
+                                   np, p_lo, npw, psl, R8, Rn);
+        }
         PN_HIP(hipGetLastError());
     };
     auto scan = [&](const uint32_t *qthr) {
