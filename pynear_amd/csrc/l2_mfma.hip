@@ -298,7 +298,7 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
     float *qn = (float *)qs.norm.ensure((size_t)nq_pad * 4);
     hipLaunchKernelGGL(l2_split_kernel, dim3((unsigned)ceil_div(nq_pad, 4)), dim3(256), 0, st, Q, nq, nq_pad, d,
                        qstride, dp, Qh, Ql, qn);
-    const int prec_m = (int)tune_get("l2.mfma_terms");
+    const int prec_m = (int)tune_get("l2.matrix_terms");
     if (n <= 4096 && dp > 64 && tune_get("l2.matrix") != 0) {
         // small database of wide rows: key matrix per query chunk (<= 256 MB), then selection
         int32_t *cidx = (int32_t *)wk.cidx.ensure((size_t)nq * K2c * 4);
