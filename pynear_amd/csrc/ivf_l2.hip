@@ -464,14 +464,13 @@ __device__ __forceinline__ void rowblock_keys(const f32x16 (&acc)[2][2], float d
 
 // Item done: the lists of each query (nlist per query: waves x 2 halves) go to LDS (smem; the
 // query buffers are no longer read), at sl[(l * tstride + q) * KL], then one lane per query
-// merges its KP best into cand[query][probe].
+// merges its kp best into cand[query][probe] (kp slots per pair).
 template <int KL, int NLMAX>
 __device__ __forceinline__ void merge_item_lists(unsigned char *smem, const uint32_t (&Lk)[2][KL],
                                                  const uint32_t (&Li)[2][KL], const int32_t *s_q,
                                                  const int32_t *__restrict__ pair_p, int cnt, int np,
                                                  const int32_t *__restrict__ qoff, uint64_t *__restrict__ cand,
-                                                 int tstride, int nlist, int list, int qcol) {
-    constexpr int KP = kIvfKP;
+                                                 int tstride, int nlist, int list, int qcol, int kp) {
     const int tid = threadIdx.x;
     uint64_t *sl = (uint64_t *)smem;
 #pragma unroll
@@ -482,7 +481,7 @@ __device__ __forceinline__ void merge_item_lists(unsigned char *smem, const uint
                 Lk[j][e] == kKey32None ? kKey64None : (((uint64_t)Lk[j][e] << 32) | Li[j][e]);
     __syncthreads();
     if (tid < cnt) {
-        uint64_t *out = cand + cand_slot(qoff, s_q[tid], np, pair_p[tid]) * KP;
+        uint64_t *out = cand + cand_slot(qoff, s_q[tid], np, pair_p[tid]) * kp;
         int p[NLMAX];
         uint64_t head[NLMAX];
 #pragma unroll
@@ -490,7 +489,7 @@ __device__ __forceinline__ void merge_item_lists(unsigned char *smem, const uint
             p[l] = 0;
             head[l] = l < nlist ? sl[(l * tstride + tid) * KL] : kKey64None;
         }
-        for (int e = 0; e < KP; ++e) {
+        for (int e = 0; e < kp; ++e) {
             int best = 0;
             uint64_t bv = head[0];
 #pragma unroll
@@ -853,7 +852,7 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
     const int64_t *__restrict__ loff, int dp, const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
     const float *__restrict__ qn, const int4 *__restrict__ items, int32_t *__restrict__ bins, int nranges,
     const int32_t *__restrict__ pair_q, const int32_t *__restrict__ pair_p, int np, const int32_t *__restrict__ qoff,
-    const uint32_t *__restrict__ qthr, uint64_t *__restrict__ cand) {
+    const uint32_t *__restrict__ qthr, uint64_t *__restrict__ cand, int kp) {
     constexpr int QS = kIvfQS, IQ = kIvfIQ;
     constexpr int NPL = PREC == 3 ? 2 : 1;      // staged planes: hi (and lo)
     constexpr int PLANE = IQ * QS;              // bf16 elements of one staged plane
@@ -979,7 +978,7 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
                           &s_thr[0][0], IQ, 8, w * 2 + half, col, half, two);
         if (rb + 1 < nrb) load_a(rb + 1, 0, acur);
     }
-    merge_item_lists<KL, 8>(smem, Lk, Li, s_q, pair_p + start, cnt, np, qoff, cand, IQ, 8, w * 2 + half, col);
+    merge_item_lists<KL, 8>(smem, Lk, Li, s_q, pair_p + start, cnt, np, qoff, cand, IQ, 8, w * 2 + half, col, kp);
     __syncthreads(); // LDS (lists, query chunks, s_q, s_thr, s_slot) is reused by the next item
     }
 }
@@ -1298,7 +1297,7 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     // candidate slots per (query, probe): the bf16 scan merges its lists into the KP = 32 best,
     // the fp8 scan (more lists, coarser keys) into the 64 best
     const int kl = k <= 16 ? 8 : 16;
-    const int cap = prec == 8 ? kIvf8Cap : kIvfKP;
+    const int cap = prec == 8 || k > kIvfKP ? kIvf8Cap : kIvfKP; // (k > 32: 64 per probe)
     uint64_t *cand = (uint64_t *)h->cand.ensure((size_t)total * cap * 8);
     // with per-query probe counts, each query's slots are packed after the previous query's
     // (qoff); slots of unused probes inside a query's span stay empty
@@ -1360,7 +1359,7 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
             hipLaunchKernelGGL((ivfl2_scan_kernel<decltype(pc)::value, decltype(kc)::value>), grid, dim3(256), 0, st,
                                h->lrows.hi.as<__bf16>(), h->lrows.lo.as<__bf16>(), h->lrows.norm.as<float>(),
                                h->loff.as<int64_t>(), dp, Qh, Ql, qn, items, bins, nranges, pq, pp, np, qoff, qthr,
-                               cand);
+                               cand, cap);
         };
         using P1 = std::integral_constant<int, 1>;
         using P3 = std::integral_constant<int, 3>;

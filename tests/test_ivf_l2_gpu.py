@@ -106,6 +106,48 @@ def test_bounded_second_pass_keeps_results(precision):
         N.tune_set("ivfl2.pass_margin_pct", old)
 
 
+@pytest.mark.parametrize("precision", ["bf16", "fp8"])
+@pytest.mark.parametrize("k", [5, 20, 40])
+def test_given_probes_with_holes_equal_fixed_probes(precision, k):
+    """Probes given per query with varying counts and unusable slots (ids of empty clusters, -1)
+    give the same neighbours as the fixed-nprobe search over the same lists; k above 16 takes the
+    wider per-lane lists."""
+    from pynear_amd import _native as N, hipdev
+    X = _gmm(15000, 48, 50, seed=21)
+    Q = _gmm(200, 48, 50, seed=22)
+    h = N.IvfL2Handle()
+    labels, cent = h.build(X, 60, 15, 3)
+    keep = np.flatnonzero(np.bincount(labels, minlength=60))
+    d2 = ((Q[:, None, :].astype(np.float64) - cent[keep][None].astype(np.float64)) ** 2).sum(-1)
+    order = keep[np.argsort(d2, axis=1, kind="stable")]
+    rng = np.random.default_rng(23)
+    counts = rng.integers(1, 9, len(Q)).astype(np.int32)
+    np_max = 12
+    probes = np.full((len(Q), np_max), -1, np.int32)
+    empty = np.setdiff1d(np.arange(60), keep)
+    for i in range(len(Q)):
+        pr = order[i, :counts[i]].tolist()
+        if empty.size and i % 3 == 0:  # an empty cluster's id inside the span is skipped
+            pr.insert(1, int(empty[0]))
+        probes[i, :len(pr)] = pr
+        counts[i] = len(pr)
+    dq, dpr, dc = hipdev.DeviceArray.from_numpy(Q), hipdev.DeviceArray.from_numpy(probes), hipdev.DeviceArray.from_numpy(counts)
+    oi = hipdev.DeviceArray((len(Q), k), np.int32)
+    od = hipdev.DeviceArray((len(Q), k), np.float32)
+    oc = hipdev.DeviceArray((len(Q),), np.int32)
+    h.search_probes_device(dq.ptr, len(Q), dpr.ptr, dc.ptr, np_max, k, oi.ptr, od.ptr, oc.ptr, precision)
+    gi, gd, gc = oi.numpy(), od.numpy(), oc.numpy()
+    # reference: exact top-k over the same lists per query (fp64), then agreement up to near ties
+    for i in range(len(Q)):
+        lists = [c for c in probes[i, :counts[i]] if c >= 0 and c in set(keep.tolist())]
+        rows = np.flatnonzero(np.isin(labels, lists))
+        dd = ((X[rows].astype(np.float64) - Q[i].astype(np.float64)) ** 2).sum(1)
+        o = np.lexsort((rows, dd))[:k]
+        assert gc[i] == min(k, len(rows))
+        assert len(set(gi[i, :gc[i]].tolist()) & set(rows[o].tolist())) >= gc[i] - 1, i
+        np.testing.assert_allclose(gd[i, :gc[i]], dd[o][:gc[i]], rtol=2e-5, atol=1e-5)
+
+
 def test_reference_ivfflat_behaviour():
     """pynear/tests/test_ivfflat.py:44-161 through the same calls."""
     rng = np.random.default_rng(0)
