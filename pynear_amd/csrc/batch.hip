@@ -4,8 +4,9 @@
 // min(n, k) neighbours, ascending (distance, index), distances bit-identical to the reference's
 // dist_l2_f_avx2). A plan lays the sub-searches out as flat work lists and runs them as
 //   1. one split launch (bf16 hi/lo + norms of every query of every index),
-//   2. one persistent scan launch: all (index, 128-query block, row slice) tiles of all indices in
-//      one queue, longest tiles first; each tile runs the same device body as the single search,
+//   2. persistent scan launches: all (index, 128-query block, row slice) tiles of all indices in
+//      two queues (rows of 128, 192 or 256 padded columns, whose scan holds more registers, and
+//      the rest), longest tiles first; each tile runs the same device body as the single search,
 //   3. one merge, one rerank (with the completeness proof) and one fallback launch over all queries,
 // captured once in a HIP graph that every run replays. Slice length, persistent grid size and
 // tile order are the plan's schedule; pn_batch_tune measures a set of schedules on the device and
@@ -67,7 +68,7 @@ __device__ __forceinline__ int find_entry(const int64_t *base, int n, int64_t x)
 __global__ __launch_bounds__(256) void batch_split_kernel(const BatchEntry *__restrict__ E, const int64_t *__restrict__ row_base,
                                                           int ne, int64_t rows, int32_t *__restrict__ counter) {
     const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0; // the scan queue of this run
+    if (blockIdx.x == 0 && threadIdx.x < 2) counter[threadIdx.x] = 0; // the scan queues of this run
     if (r >= rows) return;
     const BatchEntry &e = E[find_entry(row_base, ne, r)];
     split_row(e.Q, e.nq, e.d, e.qstride, e.dp, e.Qh, e.Ql, e.qn, r - e.row_base);
@@ -86,7 +87,21 @@ __device__ __forceinline__ void batch_tile(uint64_t *lds_raw, const BatchEntry &
     }
 }
 
+// Tiles of rows padded to 128, 192 or 256 columns: the scan with register-resident queries.
+template <int KL>
+__device__ __forceinline__ void batch_tile_wide(uint64_t *lds_raw, const BatchEntry &e, int64_t qb, int64_t s) {
+    auto lds = reinterpret_cast<uint64_t (*)[32][KL]>(lds_raw);
+    if (e.dp == 128) scan_wide<KL, 3, 2>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
+    else if (e.dp == 192) scan_wide<KL, 3, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
+    else scan_wide<KL, 3, 4>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
+}
+
+static bool wide_dp(int dp) { return dp == 128 || dp == 192 || dp == 256; }
+
 // Persistent: workgroups take tiles (entry, query block, slice) from one queue until it is empty.
+// WIDE: the queue of wide-row tiles (their scan holds more registers, so it runs as a launch of
+// its own at its own occupancy).
+template <bool WIDE>
 __global__ __launch_bounds__(256) void batch_scan_kernel(const BatchEntry *__restrict__ E,
                                                          const int2 *__restrict__ tiles, int64_t ntiles,
                                                          int32_t *__restrict__ counter) {
@@ -100,9 +115,15 @@ __global__ __launch_bounds__(256) void batch_scan_kernel(const BatchEntry *__res
         const int2 t = tiles[slot];
         const BatchEntry &e = E[t.x];
         const int64_t qb = t.y / e.S, s = t.y % e.S;
-        if (e.KL == 8) batch_tile<8>(lds_raw, e, qb, s);
-        else if (e.KL == 16) batch_tile<16>(lds_raw, e, qb, s);
-        else batch_tile<32>(lds_raw, e, qb, s);
+        if (WIDE) {
+            if (e.KL == 8) batch_tile_wide<8>(lds_raw, e, qb, s);
+            else if (e.KL == 16) batch_tile_wide<16>(lds_raw, e, qb, s);
+            else batch_tile_wide<32>(lds_raw, e, qb, s);
+        } else {
+            if (e.KL == 8) batch_tile<8>(lds_raw, e, qb, s);
+            else if (e.KL == 16) batch_tile<16>(lds_raw, e, qb, s);
+            else batch_tile<32>(lds_raw, e, qb, s);
+        }
         __syncthreads(); // LDS and s_slot are reused by the next tile
     }
 }
@@ -151,7 +172,7 @@ struct pn_batch {
     std::vector<int32_t *> oi;
     std::vector<float *> od;
     pn::BatchCfg cfg;
-    int64_t ntiles = 0, rows = 0, queries = 0;
+    int64_t ntiles = 0, ntiles_wide = 0, rows = 0, queries = 0;
     pn::DevBuf entries, row_base, q_base, tiles, counter, scratch;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -192,7 +213,8 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     struct Off { size_t qh, ql, qn, part, cidx, capx, flags; };
     std::vector<Off> O(ne);
     int64_t rows = 0, queries = 0;
-    std::vector<std::pair<double, int2>> tl;
+    std::vector<std::pair<double, int2>> tl, tw; // narrow and wide-row tiles
+    const bool wide_on = tune_get("l2.wide") != 0;
     for (int i = 0; i < ne; ++i) {
         pn_exact *h = b->idx[i];
         BatchEntry &e = E[i];
@@ -226,10 +248,15 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
         O[i].capx = take((size_t)e.nq * e.K2c * 4);
         O[i].flags = take((size_t)e.nq * 4);
         const double cost = (double)e.rps * (e.dp / 16 + 4); // a tile: rows x (products + keys)
-        for (int64_t t = 0; t < e.gx * e.S; ++t) tl.push_back({cost, make_int2(i, (int)t)});
+        auto &dst = wide_on && wide_dp(e.dp) ? tw : tl;
+        for (int64_t t = 0; t < e.gx * e.S; ++t) dst.push_back({cost, make_int2(i, (int)t)});
     }
-    if (cfg.order == 1)
+    if (cfg.order == 1) {
         std::stable_sort(tl.begin(), tl.end(), [](const auto &a, const auto &c) { return a.first > c.first; });
+        std::stable_sort(tw.begin(), tw.end(), [](const auto &a, const auto &c) { return a.first > c.first; });
+    }
+    const int64_t nnarrow = (int64_t)tl.size();
+    tl.insert(tl.end(), tw.begin(), tw.end());
     char *base = (char *)b->scratch.ensure(bytes);
     for (int i = 0; i < ne; ++i) {
         pn_exact *h = b->idx[i];
@@ -253,6 +280,7 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     std::vector<int2> tiles(tl.size());
     for (size_t t = 0; t < tl.size(); ++t) tiles[t] = tl[t].second;
     b->ntiles = (int64_t)tiles.size();
+    b->ntiles_wide = b->ntiles - nnarrow;
     b->rows = rows;
     b->queries = queries;
     hipStream_t st = b->cap_stream;
@@ -262,7 +290,7 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     PN_HIP(hipMemcpyAsync(b->q_base.ensure(ne * 8), qbase.data(), ne * 8, hipMemcpyHostToDevice, st));
     PN_HIP(hipMemcpyAsync(b->tiles.ensure(std::max<size_t>(1, tiles.size()) * 8), tiles.data(), tiles.size() * 8,
                           hipMemcpyHostToDevice, st));
-    b->counter.ensure(8);
+    b->counter.ensure(8); // two queue counters
     PN_HIP(hipStreamSynchronize(st));
     b->cfg = cfg;
 
@@ -278,12 +306,20 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     const BatchEntry *dE = b->entries.as<BatchEntry>();
     const int64_t *dR = b->row_base.as<int64_t>(), *dQ = b->q_base.as<int64_t>();
     int32_t *ctr = b->counter.as<int32_t>();
+    const int64_t nnar = b->ntiles - b->ntiles_wide;
     const unsigned grid_scan =
-        (unsigned)std::max<int64_t>(1, std::min<int64_t>(b->ntiles, (int64_t)b->cus * std::max(1, cfg.wg_per_cu)));
+        (unsigned)std::max<int64_t>(1, std::min<int64_t>(nnar, (int64_t)b->cus * std::max(1, cfg.wg_per_cu)));
+    // the wide scan holds one wave per SIMD: one workgroup per CU
+    const unsigned grid_wide = (unsigned)std::max<int64_t>(1, std::min<int64_t>(b->ntiles_wide, (int64_t)b->cus));
     PN_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(batch_split_kernel, dim3((unsigned)ceil_div(rows, (int64_t)4)), dim3(256), 0, st, dE, dR, ne,
                        rows, ctr);
-    hipLaunchKernelGGL(batch_scan_kernel, dim3(grid_scan), dim3(256), 0, st, dE, b->tiles.as<int2>(), b->ntiles, ctr);
+    if (nnar > 0)
+        hipLaunchKernelGGL(batch_scan_kernel<false>, dim3(grid_scan), dim3(256), 0, st, dE, b->tiles.as<int2>(), nnar,
+                           ctr);
+    if (b->ntiles_wide > 0)
+        hipLaunchKernelGGL(batch_scan_kernel<true>, dim3(grid_wide), dim3(256), 0, st, dE, b->tiles.as<int2>() + nnar,
+                           b->ntiles_wide, ctr + 1);
     hipLaunchKernelGGL(batch_merge_kernel, dim3((unsigned)ceil_div(queries, (int64_t)4)), dim3(256), 0, st, dE, dQ, ne,
                        queries);
     hipLaunchKernelGGL(batch_rerank_kernel, dim3((unsigned)ceil_div(queries, (int64_t)4)), dim3(256), 0, st, dE, dQ,

@@ -82,6 +82,16 @@ __global__ __launch_bounds__(256) void l2_mfma_scan_small_kernel(
                              slice_major ? blockIdx.y : blockIdx.x, slice_major ? blockIdx.x : blockIdx.y, gthr);
 }
 
+template <int KL, int PREC, int NC>
+__global__ __launch_bounds__(256) void l2_mfma_scan_wide_kernel(
+    const __bf16 *__restrict__ Xh, const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
+    const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql, const float *__restrict__ qn,
+    int64_t rows_per_slice, uint64_t *__restrict__ part, int64_t nq_pad, bool slice_major) {
+    __shared__ uint64_t lds[4][32][KL];
+    scan_wide<KL, PREC, NC>(lds, Xh, Xl, xn, n, Qh, Ql, qn, rows_per_slice, part, nq_pad,
+                            slice_major ? blockIdx.y : blockIdx.x, slice_major ? blockIdx.x : blockIdx.y);
+}
+
 __global__ __launch_bounds__(256) void l2_rerank_kernel(const float *__restrict__ X, int64_t xstride, int64_t d,
                                                         const float *__restrict__ Q, int64_t qstride,
                                                         const float *__restrict__ qn, const unsigned int *__restrict__ xn_max_bits,
@@ -380,6 +390,7 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
                                nq_pad, kKey32None);
         }
         dim3 grid(sm ? (unsigned)S : (unsigned)gx, sm ? (unsigned)gx : (unsigned)S);
+        const bool wide = tune_get("l2.wide") != 0; // register-resident queries for dp = 128, 192, 256
         auto launch = [&](auto klc, auto pc) {
             constexpr int KLV = decltype(klc)::value, PV = decltype(pc)::value;
             const __bf16 *xh = xs.hi.as<__bf16>(), *xl = xs.lo.as<__bf16>();
@@ -401,6 +412,22 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
                 hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 4>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
                                    Ql, qn, rps, part, nq_pad, sm, gthr);
                 break;
+            case 128:
+            case 192:
+            case 256:
+                if (wide) {
+                    if (dp == 128)
+                        hipLaunchKernelGGL((l2_mfma_scan_wide_kernel<KLV, PV, 2>), grid, dim3(256), 0, st, xh, xl, xnr,
+                                           n, Qh, Ql, qn, rps, part, nq_pad, sm);
+                    else if (dp == 192)
+                        hipLaunchKernelGGL((l2_mfma_scan_wide_kernel<KLV, PV, 3>), grid, dim3(256), 0, st, xh, xl, xnr,
+                                           n, Qh, Ql, qn, rps, part, nq_pad, sm);
+                    else
+                        hipLaunchKernelGGL((l2_mfma_scan_wide_kernel<KLV, PV, 4>), grid, dim3(256), 0, st, xh, xl, xnr,
+                                           n, Qh, Ql, qn, rps, part, nq_pad, sm);
+                    break;
+                }
+                [[fallthrough]];
             default:
                 hipLaunchKernelGGL((l2_mfma_scan_kernel<KLV, PV>), grid, dim3(256), 0, st, xh, xl, xnr, n, dp, Qh, Ql, qn,
                                    rps, part, nq_pad, sm);

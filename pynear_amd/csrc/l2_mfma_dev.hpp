@@ -260,6 +260,82 @@ __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16
     half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + (s * KL) * nq_pad + q, nq_pad);
 }
 
+// The same scan for dp = 64 NC (NC = 2..4): the lane's query fragments stay in registers for the
+// whole slice (the generic scan reads them again for every tile, doubling the row-operand
+// traffic through L1), and the row fragments stream through NC register sets, one per 64-column
+// chunk of a tile: set c is refilled with the next tile's chunk c right after its products, so a
+// load has the other NC - 1 chunks' products to arrive. The next tile's norms are loaded after
+// the keys of this one and wait behind a whole tile of products.
+template <int KL, int PREC, int NC>
+__device__ __forceinline__ void scan_wide(uint64_t (*lds)[32][KL], const __bf16 *__restrict__ Xh,
+                                          const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
+                                          const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
+                                          const float *__restrict__ qn, int64_t rows_per_slice,
+                                          uint64_t *__restrict__ part, int64_t nq_pad, int64_t qb, int64_t s) {
+    constexpr int dp = 64 * NC;
+    constexpr int NPL = PREC == 3 ? 2 : 1;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
+    const int64_t q = qb * 128 + 32 * w + col;
+    const int64_t r0 = s * rows_per_slice, r1 = min(n, r0 + rows_per_slice);
+    const float qnv = qn[q];
+    bf16x8 b[4 * NC][NPL];
+#pragma unroll
+    for (int k = 0; k < 4 * NC; ++k) {
+        b[k][0] = *(const bf16x8 *)(Qh + q * dp + 16 * k + 8 * half);
+        if (PREC == 3) b[k][NPL - 1] = *(const bf16x8 *)(Ql + q * dp + 16 * k + 8 * half);
+    }
+    uint32_t Lk[KL], Li[KL];
+#pragma unroll
+    for (int j = 0; j < KL; ++j) {
+        Lk[j] = kKey32None;
+        Li[j] = 0u;
+    }
+    uint32_t thr = kKey32None;
+    const int64_t nt = r0 < r1 ? (r1 - r0 + 31) / 32 : 0;
+    if (nt > 0) {
+        bf16x8 a[4 * NC][NPL]; // chunk c: a[4c .. 4c + 3]
+#pragma unroll
+        for (int k = 0; k < 4 * NC; ++k) {
+            a[k][0] = *(const bf16x8 *)(Xh + (r0 + col) * dp + 16 * k + 8 * half);
+            if (PREC == 3) a[k][NPL - 1] = *(const bf16x8 *)(Xl + (r0 + col) * dp + 16 * k + 8 * half);
+        }
+        float4 xv[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) xv[g] = *(const float4 *)(xn + r0 + 8 * g + 4 * half);
+        for (int64_t i = 0; i < nt; ++i) {
+            const int64_t t = r0 + 32 * i;
+            const int64_t tn = r0 + 32 * min(i + 1, nt - 1); // past the last tile: that tile again
+            f32x16 acc = {};
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+#pragma unroll
+                for (int k = 4 * c; k < 4 * c + 4; ++k) {
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][0], b[k][0], acc, 0, 0, 0);
+                    if (PREC == 3) {
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][0], b[k][NPL - 1], acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][NPL - 1], b[k][0], acc, 0, 0, 0);
+                    }
+                }
+#pragma unroll
+                for (int k = 4 * c; k < 4 * c + 4; ++k) {
+                    a[k][0] = *(const bf16x8 *)(Xh + (tn + col) * dp + 16 * k + 8 * half);
+                    if (PREC == 3) a[k][NPL - 1] = *(const bf16x8 *)(Xl + (tn + col) * dp + 16 * k + 8 * half);
+                }
+            }
+            thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64)); // the other half-wave's list of this query
+            if (t == r0)
+                t + 32 <= r1 ? tile_keys_insert<KL, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
+                             : tile_keys_insert<KL, true, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+            else
+                t + 32 <= r1 ? tile_keys_insert<KL, false, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
+                             : tile_keys_insert<KL, true, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+#pragma unroll
+            for (int g = 0; g < 4; ++g) xv[g] = *(const float4 *)(xn + tn + 8 * g + 4 * half);
+        }
+    }
+    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + (s * KL) * nq_pad + q, nq_pad);
+}
+
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {

@@ -21,15 +21,30 @@ __device__ __forceinline__ void merge_wave_query(const uint64_t *__restrict__ pa
     for (int j = 0; j < K; ++j) L[j] = kKey64None;
     for (int64_t s = lane; s < slices; s += 64) {
         const uint64_t *src = part + (s * klist) * nq_pad + q;
-        for (int j = 0; j < klist; ++j) {
-            uint64_t key = src[(int64_t)j * nq_pad];
-            if (key == kKey64None) break;
-            if (kind == OUT_L2_F32) {
-                float dd = sqrtf(key_f32((uint32_t)(key >> 32)));
-                key = ((uint64_t)f32_key(dd) << 32) | (key & 0xFFFFFFFFull);
+        // a slice's list is sorted: keys are read 8 at a time (independent loads), and the list
+        // is done at its first empty key or the first key the private list cannot take
+        bool done = false;
+        for (int j0 = 0; j0 < klist && !done; j0 += 8) {
+            uint64_t kb[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) kb[u] = j0 + u < klist ? src[(int64_t)(j0 + u) * nq_pad] : kKey64None;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                uint64_t key = kb[u];
+                if (done || key == kKey64None) {
+                    done = true;
+                    continue;
+                }
+                if (kind == OUT_L2_F32) {
+                    float dd = sqrtf(key_f32((uint32_t)(key >> 32)));
+                    key = ((uint64_t)f32_key(dd) << 32) | (key & 0xFFFFFFFFull);
+                }
+                if (key >= L[K - 1]) {
+                    done = true;
+                    continue;
+                }
+                list_insert_u64<K>(L, key);
             }
-            if (key >= L[K - 1]) break;
-            list_insert_u64<K>(L, key);
         }
     }
     for (int r = 0; r < kk; ++r) {

@@ -216,7 +216,7 @@ class _Tune:
             N.tune_set(k, v)
 
 
-@pytest.mark.parametrize("d", [1, 3, 16, 32, 48, 100, 256])
+@pytest.mark.parametrize("d", [1, 3, 16, 32, 48, 100, 128, 192, 256])
 @pytest.mark.parametrize("k", [1, 5, 10, 32])
 def test_l2_mfma_path_bit_exact(d, k):
     """The MFMA candidate scan + exact rerank returns the oracle's result bit for bit."""
@@ -232,7 +232,7 @@ def test_l2_mfma_path_bit_exact(d, k):
     np.testing.assert_array_equal(gi, oi)
 
 
-@pytest.mark.parametrize("d", [8, 32, 64])
+@pytest.mark.parametrize("d", [8, 32, 64, 128])
 def test_l2_fallback_path_exact(d):
     """With single-product bf16 the error bound is wide, queries fall back to the exact scan,
     and the result is still the oracle's (and identical to the VALU scan's)."""
@@ -251,6 +251,26 @@ def test_l2_fallback_path_exact(d):
     np.testing.assert_array_equal(gi, oi)
     np.testing.assert_array_equal(vd, od)
     np.testing.assert_array_equal(vi, oi)
+
+
+@pytest.mark.parametrize("d", [128, 192, 256])
+def test_l2_wide_scan_equals_generic_scan(d):
+    """Rows of 128, 192 and 256 columns: the scan with register-resident queries and the generic
+    scan give the same result (several slices per query block, a ragged last slice, k = 7)."""
+    rng = np.random.default_rng(d + 7)
+    X = rng.standard_normal((9001, d)).astype(np.float32)
+    Q = rng.standard_normal((260, d)).astype(np.float32)
+    h = N.ExactHandle(N.METRIC_L2)
+    h.set(X)
+    with _Tune(l2__wide=1):
+        wi, wd = h.search(Q, 7)
+    with _Tune(l2__wide=0):
+        gi, gd = h.search(Q, 7)
+    np.testing.assert_array_equal(wi, gi)
+    np.testing.assert_array_equal(wd, gd)
+    oi, od = oracle.knn_float(N.METRIC_L2, X, Q[::13], 7)
+    np.testing.assert_array_equal(wi[::13], oi)
+    np.testing.assert_array_equal(wd[::13], od)
 
 
 def test_l2_config1_shape_sample():
