@@ -130,16 +130,22 @@ __global__ __launch_bounds__(256) void batch_scan_kernel(const BatchEntry *__res
 
 __global__ __launch_bounds__(256) void batch_merge_kernel(const BatchEntry *__restrict__ E,
                                                           const int64_t *__restrict__ q_base, int ne, int64_t nq) {
-    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ uint64_t wbuf_all[4][256];
+    uint64_t *wbuf = wbuf_all[threadIdx.x >> 6];
+    const int64_t g = xcd_contiguous_block() * 4 + (threadIdx.x >> 6);
     if (g >= nq) return;
     const BatchEntry &e = E[find_entry(q_base, ne, g)];
     const int64_t q = g - e.q_base;
-    if (e.K2 <= 16)
-        merge_wave_query<16>(e.part, e.S, e.KL, e.nq_pad, q, e.K2c, OUT_F32, e.cidx, e.capx);
-    else if (e.K2 <= 32)
-        merge_wave_query<32>(e.part, e.S, e.KL, e.nq_pad, q, e.K2c, OUT_F32, e.cidx, e.capx);
+    // a lane that merges one slice holds at most KL keys; with more slices per lane it needs K2c
+    const int cap = e.S <= 64 ? e.KL : e.K2;
+    if (cap <= 8)
+        merge_wave_query<8>(e.part, e.S, e.KL, e.nq_pad, q, e.K2c, OUT_F32, e.cidx, e.capx, wbuf);
+    else if (cap <= 16)
+        merge_wave_query<16>(e.part, e.S, e.KL, e.nq_pad, q, e.K2c, OUT_F32, e.cidx, e.capx, wbuf);
+    else if (cap <= 32)
+        merge_wave_query<32>(e.part, e.S, e.KL, e.nq_pad, q, e.K2c, OUT_F32, e.cidx, e.capx, wbuf);
     else
-        merge_wave_query<64>(e.part, e.S, e.KL, e.nq_pad, q, e.K2c, OUT_F32, e.cidx, e.capx);
+        merge_wave_query<64>(e.part, e.S, e.KL, e.nq_pad, q, e.K2c, OUT_F32, e.cidx, e.capx, wbuf);
 }
 
 __global__ __launch_bounds__(256) void batch_rerank_kernel(const BatchEntry *__restrict__ E,
@@ -320,7 +326,7 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     if (b->ntiles_wide > 0)
         hipLaunchKernelGGL(batch_scan_kernel<true>, dim3(grid_wide), dim3(256), 0, st, dE, b->tiles.as<int2>() + nnar,
                            b->ntiles_wide, ctr + 1);
-    hipLaunchKernelGGL(batch_merge_kernel, dim3((unsigned)ceil_div(queries, (int64_t)4)), dim3(256), 0, st, dE, dQ, ne,
+    hipLaunchKernelGGL(batch_merge_kernel, dim3(xcd_grid(ceil_div(queries, (int64_t)4))), dim3(256), 0, st, dE, dQ, ne,
                        queries);
     hipLaunchKernelGGL(batch_rerank_kernel, dim3((unsigned)ceil_div(queries, (int64_t)4)), dim3(256), 0, st, dE, dQ,
                        ne, queries);

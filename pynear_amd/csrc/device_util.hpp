@@ -19,6 +19,18 @@ constexpr int64_t kMaxRowsPerSlice = int64_t(1) << kHamIdxBits;
 constexpr uint32_t kKey32None = 0xFFFFFFFFu;
 constexpr uint64_t kKey64None = ~uint64_t(0);
 
+// Workgroups are dealt to the eight XCDs round robin (workgroup b runs on XCD b % 8). For a grid
+// of a multiple of 8 workgroups this numbering gives each XCD one contiguous eighth of the block
+// indices, so blocks that read neighbouring columns of an [entry][query] array (16 queries of 8
+// bytes per 128-byte line) share one L2 instead of fetching each line into four.
+__device__ __forceinline__ int64_t xcd_contiguous_block() {
+    const int64_t per = gridDim.x >> 3;
+    return (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+// Grid size for xcd_contiguous_block: blocks rounded up to a multiple of 8.
+inline unsigned xcd_grid(int64_t blocks) { return (unsigned)((blocks + 7) / 8 * 8); }
+
 // Insert `key` into the ascending list L[0..K-1] (L[K-1] is dropped). With L sorted,
 // max(L[j-1], min(key, L[j])) == med3(L[j-1], key, L[j]), one v_med3_u32 per slot.
 template <int K> __device__ __forceinline__ void list_insert_u32(uint32_t (&L)[K], uint32_t key) {
@@ -34,6 +46,41 @@ template <int K> __device__ __forceinline__ void list_insert_u64(uint64_t (&L)[K
         L[j] = L[j - 1] > lo ? L[j - 1] : lo;
     }
     L[0] = key < L[0] ? key : L[0];
+}
+
+// Sorts 64 E keys held E per lane (element lane + 64 s in v[s]) ascending across the wave
+// (bitonic network; partners within a lane for strides >= 64, else one lane shuffle away). The
+// input is already made of sorted runs of K0 / 2 that alternate ascending / descending, so the
+// network starts at blocks of K0.
+template <int E, int K0>
+__device__ __forceinline__ void wave_sort_u64(uint64_t (&v)[E], int lane) {
+    constexpr int N = 64 * E;
+#pragma unroll
+    for (int k = K0; k <= N; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+#pragma unroll
+                for (int s = 0; s < E; ++s) {
+                    const int t = s ^ (j / 64);
+                    if (t > s) {
+                        const bool asc = ((lane + 64 * s) & k) == 0;
+                        const uint64_t a = v[s], b = v[t];
+                        const bool sw = asc ? a > b : a < b;
+                        v[s] = sw ? b : a;
+                        v[t] = sw ? a : b;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < E; ++s) {
+                    const uint64_t o = __shfl_xor(v[s], j, 64);
+                    const bool asc = ((lane + 64 * s) & k) == 0, lower = (lane & j) == 0;
+                    const uint64_t mn = o < v[s] ? o : v[s], mx = o < v[s] ? v[s] : o;
+                    v[s] = lower == asc ? mn : mx;
+                }
+            }
+        }
 }
 
 // popcount(x) + acc as one v_bcnt_u32_b32. Written as an (ALU-only) asm statement because the

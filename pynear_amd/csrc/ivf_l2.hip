@@ -632,41 +632,6 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
     }
 }
 
-// Sorts 64 E keys held E per lane (element lane + 64 s in v[s]) ascending across the wave
-// (bitonic network; partners within a lane for strides >= 64, else one lane shuffle away). The
-// input is already made of sorted runs of K0 / 2 that alternate ascending / descending, so the
-// network starts at blocks of K0.
-template <int E, int K0>
-__device__ __forceinline__ void wave_sort_u64(uint64_t (&v)[E], int lane) {
-    constexpr int N = 64 * E;
-#pragma unroll
-    for (int k = K0; k <= N; k <<= 1)
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= 64) {
-#pragma unroll
-                for (int s = 0; s < E; ++s) {
-                    const int t = s ^ (j / 64);
-                    if (t > s) {
-                        const bool asc = ((lane + 64 * s) & k) == 0;
-                        const uint64_t a = v[s], b = v[t];
-                        const bool sw = asc ? a > b : a < b;
-                        v[s] = sw ? b : a;
-                        v[t] = sw ? a : b;
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int s = 0; s < E; ++s) {
-                    const uint64_t o = __shfl_xor(v[s], j, 64);
-                    const bool asc = ((lane + 64 * s) & k) == 0, lower = (lane & j) == 0;
-                    const uint64_t mn = o < v[s] ? o : v[s], mx = o < v[s] ? v[s] : o;
-                    v[s] = lower == asc ? mn : mx;
-                }
-            }
-        }
-}
-
 // The list scan in fp8 e4m3 on residuals to the list's centroid c: |q - x|^2 =
 // |q - c|^2 + |x - c|^2 - 2 (q - c).(x - c), so the quantised vectors are the short residuals.
 // One workgroup of 8 waves per item of up to 128 queries. With more than 64 queries the waves

@@ -459,7 +459,7 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
     {
         StageScope sc(st, STAGE_RERANK);
         const float eps_coef = (prec == 1 ? 0.0078125f : 6.103515625e-05f) + (float)dp * 4.76837158203125e-07f;
-        hipLaunchKernelGGL(l2_rerank_kernel, dim3((unsigned)ceil_div(nq, 4)), dim3(256), 0, st, X, xstride, d, Q,
+        hipLaunchKernelGGL(l2_rerank_kernel, dim3((unsigned)ceil_div(nq, (int64_t)4)), dim3(256), 0, st, X, xstride, d, Q,
                            qstride, qn, xs.norm_max.as<unsigned int>(), cidx, capx, K2c, part, nlists, KL, nq_pad, nq,
                            kk, n, eps_coef, out_idx, out_dist, flags);
         PN_HIP(hipGetLastError());

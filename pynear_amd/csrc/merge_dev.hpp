@@ -8,50 +8,129 @@
 namespace pn {
 
 // One wave per query q: lane l merges the per-slice lists (klist keys each) of slices l, l+64, ...
-// into a private sorted list of capacity K, then the wave extracts the kk smallest keys by kk
-// rounds of a wave-wide minimum (keys are unique since indices are). The result is the exact kk
-// smallest of all lists when K >= min(kk, klist * slices per lane).
+// into a private sorted list of capacity K, then the wave orders what the lanes hold and writes
+// the kk smallest keys (keys are unique since indices are). The result is the exact kk smallest
+// of all lists when K >= min(kk, klist * slices per lane).
+//
+// A sampled bound first limits what the lanes insert: lane l takes the minimum of up to four
+// entries at grid positions l + 64 u (slice g % slices, entry g / slices: the first entries of
+// every slice), and the wave maximum of these minima has at least 64 distinct entries at or below
+// it (one per lane), so the kk <= 64 smallest are all at or below it. A lane without a sample
+// leaves the bound open (the empty key is the largest).
+//
+// Ordering: when the lanes hold at most 256 keys together, they are packed into the wave's 256
+// entries of LDS (wbuf) and sorted by one bitonic network across the wave; otherwise kk rounds of
+// a wave-wide minimum pop them one by one.
+//
+// kind OUT_L2_F32 orders by sqrt of the stored value: a slice stays sorted in the high word, but
+// entries whose square roots tie may follow in any index order, so a slice is left only at a key
+// whose high word exceeds the limit's.
 template <int K>
 __device__ __forceinline__ void merge_wave_query(const uint64_t *__restrict__ part, int64_t slices, int klist,
                                                  int64_t nq_pad, int64_t q, int kk, int kind,
-                                                 int32_t *__restrict__ out_idx, void *__restrict__ out_dist) {
+                                                 int32_t *__restrict__ out_idx, void *__restrict__ out_dist,
+                                                 uint64_t *wbuf) {
     const int lane = threadIdx.x & 63;
+    const bool tr = kind == OUT_L2_F32;
+    auto transform = [&](uint64_t key) {
+        if (tr && key != kKey64None) {
+            const float dd = sqrtf(key_f32((uint32_t)(key >> 32)));
+            key = ((uint64_t)f32_key(dd) << 32) | (key & 0xFFFFFFFFull);
+        }
+        return key;
+    };
+    auto write_out = [&](int r, uint64_t m) {
+        out_idx[q * kk + r] = m == kKey64None ? -1 : (int32_t)(uint32_t)(m & 0xFFFFFFFFull);
+        const uint32_t hi = (uint32_t)(m >> 32);
+        if (kind == OUT_HAMMING_I32)
+            ((int32_t *)out_dist)[q * kk + r] = (int32_t)hi;
+        else
+            ((float *)out_dist)[q * kk + r] = key_f32(hi);
+    };
+    uint64_t bnd;
+    {
+        const int G = (int)min<int64_t>(slices * klist, 256), ns = (int)min<int64_t>(slices, 256);
+        uint64_t smp[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int g = lane + 64 * u;
+            const int gs = g < G ? g : 0;
+            const int64_t sl = gs % ns, p = gs / ns;
+            const uint64_t v = part[(sl * klist + p) * nq_pad + q];
+            smp[u] = g < G ? v : kKey64None;
+        }
+        uint64_t mn = kKey64None;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t v = transform(smp[u]);
+            mn = v < mn ? v : mn;
+        }
+        bnd = mn;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint64_t o = __shfl_xor(bnd, off, 64);
+            bnd = o > bnd ? o : bnd;
+        }
+    }
     uint64_t L[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) L[j] = kKey64None;
     for (int64_t s = lane; s < slices; s += 64) {
         const uint64_t *src = part + (s * klist) * nq_pad + q;
-        // a slice's list is sorted: keys are read 8 at a time (independent loads), and the list
-        // is done at its first empty key or the first key the private list cannot take
+        // a slice's list is sorted: keys are read 8 at a time (independent loads, clamped to the
+        // list), and the list is done at its first empty key or the first key past the limit
+        // (the bound or the private list's last key)
         bool done = false;
         for (int j0 = 0; j0 < klist && !done; j0 += 8) {
             uint64_t kb[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) kb[u] = j0 + u < klist ? src[(int64_t)(j0 + u) * nq_pad] : kKey64None;
+            for (int u = 0; u < 8; ++u) kb[u] = src[(int64_t)min(j0 + u, klist - 1) * nq_pad];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                uint64_t key = kb[u];
-                if (done || key == kKey64None) {
+                if (done || j0 + u >= klist || kb[u] == kKey64None) {
                     done = true;
                     continue;
                 }
-                if (kind == OUT_L2_F32) {
-                    float dd = sqrtf(key_f32((uint32_t)(key >> 32)));
-                    key = ((uint64_t)f32_key(dd) << 32) | (key & 0xFFFFFFFFull);
-                }
-                if (key >= L[K - 1]) {
-                    done = true;
+                const uint64_t key = transform(kb[u]);
+                const uint64_t lim = bnd < L[K - 1] ? bnd : L[K - 1];
+                if (key > bnd || key >= L[K - 1]) {
+                    if (!tr || (key >> 32) > (lim >> 32)) done = true;
                     continue;
                 }
                 list_insert_u64<K>(L, key);
             }
         }
     }
+    int c = 0; // keys this lane holds
+#pragma unroll
+    for (int j = 0; j < K; ++j) c += L[j] != kKey64None ? 1 : 0;
+    int off = c; // inclusive prefix over lanes
+#pragma unroll
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const int t = __shfl_up(off, dd, 64);
+        if (lane >= dd) off += t;
+    }
+    const int total = __shfl(off, 63, 64);
+    off -= c;
+    if (total <= 256) {
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (j < c) wbuf[off + j] = L[j];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint64_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = lane + 64 * u < total ? wbuf[lane + 64 * u] : kKey64None;
+        wave_sort_u64<4, 2>(v, lane);
+        if (lane < kk) write_out(lane, v[0]); // (kk <= 64: the kk smallest are v[0] of lanes 0 .. kk-1)
+        return;
+    }
     for (int r = 0; r < kk; ++r) {
         uint64_t m = L[0];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            uint64_t o = __shfl_xor(m, off, 64);
+        for (int sh = 32; sh > 0; sh >>= 1) {
+            uint64_t o = __shfl_xor(m, sh, 64);
             m = o < m ? o : m;
         }
         if (L[0] == m && m != kKey64None) { // the unique owner pops its head
@@ -59,14 +138,7 @@ __device__ __forceinline__ void merge_wave_query(const uint64_t *__restrict__ pa
             for (int j = 0; j < K - 1; ++j) L[j] = L[j + 1];
             L[K - 1] = kKey64None;
         }
-        if (lane == 0) {
-            out_idx[q * kk + r] = m == kKey64None ? -1 : (int32_t)(uint32_t)(m & 0xFFFFFFFFull);
-            uint32_t hi = (uint32_t)(m >> 32);
-            if (kind == OUT_HAMMING_I32)
-                ((int32_t *)out_dist)[q * kk + r] = (int32_t)hi;
-            else
-                ((float *)out_dist)[q * kk + r] = key_f32(hi);
-        }
+        if (lane == 0) write_out(r, m);
     }
 }
 

@@ -32,9 +32,15 @@ __global__ __launch_bounds__(256) void merge_lists_kernel(const uint64_t *__rest
             uint64_t key = src[(int64_t)j * nq_pad];
             if (key == kKey64None) break;
             if (kind == OUT_L2_F32) {
-                // order by the final (sqrt) distance; sqrt is monotone so the slice stays sorted
+                // order by the final (sqrt) distance; sqrt is monotone, so the slice stays sorted
+                // in the high word, but keys whose roots tie may follow in any index order: the
+                // slice is left only past the last key's high word
                 float dd = sqrtf(key_f32((uint32_t)(key >> 32)));
                 key = ((uint64_t)f32_key(dd) << 32) | (key & 0xFFFFFFFFull);
+                if (key >= L[K - 1]) {
+                    if ((key >> 32) > (L[K - 1] >> 32)) break;
+                    continue;
+                }
             }
             if (key >= L[K - 1]) break;
             list_insert_u64<K>(L, key);
@@ -58,15 +64,16 @@ __global__ __launch_bounds__(256) void merge_lists_wave_kernel(const uint64_t *_
                                                                int klist, int64_t nq, int64_t nq_pad, int kk,
                                                                int kind, int32_t *__restrict__ out_idx,
                                                                void *__restrict__ out_dist) {
-    const int64_t q = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    __shared__ uint64_t wbuf[4][256];
+    const int64_t q = xcd_contiguous_block() * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (q >= nq) return; // whole wave exits together
-    merge_wave_query<K>(part, slices, klist, nq_pad, q, kk, kind, out_idx, out_dist);
+    merge_wave_query<K>(part, slices, klist, nq_pad, q, kk, kind, out_idx, out_dist, wbuf[threadIdx.x >> 6]);
 }
 
 void merge_lists(const uint64_t *part, int64_t slices, int K, int64_t nq, int64_t nq_pad, int kk, int kind,
                  int32_t *out_idx, void *out_dist, hipStream_t st) {
     const bool wave = tune_get("merge.wave_per_query") != 0;
-    dim3 grid((unsigned)(wave ? ceil_div(nq, 4) : ceil_div(nq, 256)));
+    dim3 grid(wave ? xcd_grid(ceil_div(nq, 4)) : (unsigned)ceil_div(nq, 256));
 #define PN_MERGE(KV)                                                                                   \
     do {                                                                                               \
         if (wave)                                                                                      \
@@ -91,7 +98,7 @@ void merge_lists(const uint64_t *part, int64_t slices, int K, int64_t nq, int64_
 
 void merge_lists_wave(const uint64_t *part, int64_t slices, int klist, int cap, int64_t nq, int64_t nq_pad, int kk,
                       int kind, int32_t *out_idx, void *out_dist, hipStream_t st) {
-    dim3 grid((unsigned)ceil_div(nq, 4));
+    dim3 grid(xcd_grid(ceil_div(nq, 4)));
 #define PN_MERGEW(KV)                                                                                        \
     hipLaunchKernelGGL((merge_lists_wave_kernel<KV>), grid, dim3(256), 0, st, part, slices, klist, nq, nq_pad, kk, \
                        kind, out_idx, out_dist)

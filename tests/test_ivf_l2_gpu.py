@@ -106,6 +106,24 @@ def test_bounded_second_pass_keeps_results(precision):
         N.tune_set("ivfl2.pass_margin_pct", old)
 
 
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3", "fp8"])
+def test_repeated_searches_identical(precision):
+    """Lists probed by a few hundred queries each split into several work items that workgroups
+    take in whatever order they reach the queue: repeated searches still return identical
+    neighbours and distances."""
+    from pynear_amd import _native as N
+    X = _gmm(30000, 64, 24, seed=31, sigma=0.3)
+    Q = _gmm(3000, 64, 24, seed=32, sigma=0.3)
+    h = N.IvfL2Handle()
+    h.build(X, 24, 10, 7)
+    for k, n_probe in ((10, 4), (32, 3)):
+        first = h.search(Q, n_probe, k, precision)
+        for _ in range(2):
+            again = h.search(Q, n_probe, k, precision)
+            for x, y in zip(first, again):
+                assert np.array_equal(x, y), (k, n_probe)
+
+
 @pytest.mark.parametrize("precision", ["bf16", "fp8"])
 @pytest.mark.parametrize("k", [5, 20, 40])
 def test_given_probes_with_holes_equal_fixed_probes(precision, k):
