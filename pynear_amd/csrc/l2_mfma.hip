@@ -121,76 +121,102 @@ __global__ __launch_bounds__(256) void l2_fallback_kernel(const float *__restric
 // distances of a query chunk to all rows are one product (bf16, three terms per K step) written
 // as a key matrix; a wave per query then takes the K2c smallest (key, row) pairs, and the exact
 // rerank, proof and fallback are the scan path's. A workgroup computes a 128 x 128 block (wave:
-// 64 x 64 as 2 x 2 MFMA tiles); rows are clamped on load and masked on store.
+// 64 x 64 as 2 x 2 MFMA tiles) from operands staged in LDS; rows are clamped on load and masked
+// on store.
 template <int PREC>
 __global__ __launch_bounds__(256) void l2_matrix_keys_kernel(const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
                                                              const float *__restrict__ qn, int M,
                                                              const __bf16 *__restrict__ Xh, const __bf16 *__restrict__ Xl,
                                                              const float *__restrict__ xn, int N, int dp,
                                                              uint32_t *__restrict__ keys) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
-    const int m0 = blockIdx.y * 128 + (w >> 1) * 64, n0 = blockIdx.x * 128 + (w & 1) * 64;
-    int64_t ao[2], bo[2];
+    constexpr int NPL = PREC == 3 ? 2 : 1; // planes: hi (and lo)
+    constexpr int KC = 32;                  // columns per staged chunk
+    constexpr int RS = KC + 8;              // LDS row stride (bf16): 80 bytes, conflict-free b128 reads
+    __shared__ __align__(16) __bf16 sq[2][NPL][128 * RS], sx[2][NPL][128 * RS]; // two chunk buffers
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5, col = lane & 31;
+    // a 1-D grid of 8 x ncb x ceil(nqb / 8) workgroups: workgroup b runs on XCD b % 8, and every
+    // column block of query block qb goes to XCD qb % 8, so a query block is fetched into one L2
+    const int ncb = (N + 127) / 128, nqb = (M + 127) / 128;
+    const int jb = (int)(blockIdx.x >> 3), cb = jb % ncb, qb = (jb / ncb) * 8 + (int)(blockIdx.x & 7);
+    if (qb >= nqb) return; // (uniform over the workgroup, before any barrier)
+    const int m0 = qb * 128 + (w >> 1) * 64, n0 = cb * 128 + (w & 1) * 64;
+    // Chunks of KC columns of the workgroup's 128 query rows and 128 rows are staged in LDS by
+    // coalesced loads (four lanes read a row's 64 contiguous bytes: whole L1 accesses, and each
+    // row is fetched once per workgroup, not once per wave that multiplies it); the next chunk's
+    // loads are in registers while the waves multiply the current one from LDS, and go to the
+    // other buffer after the products (one barrier per chunk). Staging role of
+    // thread t: rows (t >> 2) and (t >> 2) + 64, 16-byte column t & 3 of the chunk; columns at or
+    // past dp (the last chunk when dp % 32 == 16) are staged as zeros.
+    const int sr = tid >> 2, sc = tid & 3;
+    int64_t gq[2], gx[2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        ao[i] = (int64_t)min(m0 + 32 * i + col, M - 1) * dp + 8 * half;
-        bo[i] = (int64_t)min(n0 + 32 * i + col, N - 1) * dp + 8 * half;
+    for (int u = 0; u < 2; ++u) {
+        gq[u] = (int64_t)min(qb * 128 + sr + 64 * u, M - 1) * dp + 8 * sc;
+        gx[u] = (int64_t)min(cb * 128 + sr + 64 * u, N - 1) * dp + 8 * sc;
     }
+    bf16x8 rq[NPL][2], rx[NPL][2];
+    auto gload = [&](int k0) __attribute__((always_inline)) {
+        const bool in = k0 + 8 * sc < dp; // (dp is a multiple of 16: a piece is all in or all out)
+        const int kk0 = in ? k0 : 0;
+#pragma unroll
+        for (int pl = 0; pl < NPL; ++pl)
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const bf16x8 vq = *(const bf16x8 *)((pl == 0 ? Qh : Ql) + gq[u] + kk0);
+                const bf16x8 vx = *(const bf16x8 *)((pl == 0 ? Xh : Xl) + gx[u] + kk0);
+                rq[pl][u] = in ? vq : bf16x8{};
+                rx[pl][u] = in ? vx : bf16x8{};
+            }
+    };
+    auto sstore = [&](int bf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int pl = 0; pl < NPL; ++pl)
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                *(bf16x8 *)(&sq[bf][pl][(sr + 64 * u) * RS + 8 * sc]) = rq[pl][u];
+                *(bf16x8 *)(&sx[bf][pl][(sr + 64 * u) * RS + 8 * sc]) = rx[pl][u];
+            }
+    };
     f32x16 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x16{};
-    // K steps in groups of CH: the group's loads are issued before its products
-    constexpr int CH = PREC == 3 ? 2 : 4;
-    int k0 = 0;
-    for (; k0 + 16 * CH <= dp; k0 += 16 * CH) {
-        bf16x8 a[CH][2], b[CH][2], al[CH][2], bl[CH][2];
+    const int qr = (w >> 1) * 64 + col, xr = (w & 1) * 64 + col; // the wave's rows in the staged block
+    const int nch = (dp + KC - 1) / KC;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+        const int bf = c & 1;
+        if (c + 1 < nch) gload(KC * (c + 1)); // (in flight during the products)
 #pragma unroll
-        for (int u = 0; u < CH; ++u)
+        for (int ks = 0; ks < KC / 16; ++ks) {
+            bf16x8 a[NPL][2], b[NPL][2];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                a[u][i] = *(const bf16x8 *)(Qh + ao[i] + k0 + 16 * u);
-                b[u][i] = *(const bf16x8 *)(Xh + bo[i] + k0 + 16 * u);
-                if (PREC == 3) {
-                    al[u][i] = *(const bf16x8 *)(Ql + ao[i] + k0 + 16 * u);
-                    bl[u][i] = *(const bf16x8 *)(Xl + bo[i] + k0 + 16 * u);
+            for (int pl = 0; pl < NPL; ++pl)
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    a[pl][i] = *(const bf16x8 *)(&sq[bf][pl][(qr + 32 * i) * RS + 16 * ks + 8 * half]);
+                    b[pl][i] = *(const bf16x8 *)(&sx[bf][pl][(xr + 32 * i) * RS + 16 * ks + 8 * half]);
                 }
-            }
-#pragma unroll
-        for (int u = 0; u < CH; ++u)
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u][i], b[u][j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
                     if (PREC == 3) {
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u][i], bl[u][j], acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[u][i], b[u][j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[NPL - 1][j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[NPL - 1][i], b[0][j], acc[i][j], 0, 0, 0);
                     }
                 }
-    }
-    for (; k0 < dp; k0 += 16) {
-        bf16x8 a[2], b[2], al[2], bl[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            a[i] = *(const bf16x8 *)(Qh + ao[i] + k0);
-            b[i] = *(const bf16x8 *)(Xh + bo[i] + k0);
-            if (PREC == 3) {
-                al[i] = *(const bf16x8 *)(Ql + ao[i] + k0);
-                bl[i] = *(const bf16x8 *)(Xl + bo[i] + k0);
-            }
         }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-                if (PREC == 3) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bl[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], b[j], acc[i][j], 0, 0, 0);
-                }
-            }
+        if (c + 1 < nch) {
+            // the other buffer was last read for chunk c - 1, which every wave finished before
+            // the previous barrier
+            sstore(bf ^ 1);
+            __syncthreads();
+        }
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -320,7 +346,7 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
             const int64_t m = std::min<int64_t>(chunk, nq - c0);
             {
                 StageScope sc(st, STAGE_SCAN);
-                dim3 grid((unsigned)ceil_div(n, (int64_t)128), (unsigned)ceil_div(m, (int64_t)128));
+                dim3 grid((unsigned)(8 * ceil_div(n, (int64_t)128) * ceil_div(ceil_div(m, (int64_t)128), (int64_t)8)));
                 if (prec_m == 1)
                     hipLaunchKernelGGL((l2_matrix_keys_kernel<1>), grid, dim3(256), 0, st, Qh + c0 * dp, Ql + c0 * dp,
                                        qn + c0, (int)m, xs.hi.as<__bf16>(), xs.lo.as<__bf16>(), xs.norm.as<float>(),

@@ -253,6 +253,22 @@ def test_l2_fallback_path_exact(d):
     np.testing.assert_array_equal(vi, oi)
 
 
+@pytest.mark.parametrize("d,n", [(100, 3000), (256, 4096), (768, 1500), (80, 37)])
+@pytest.mark.parametrize("k", [1, 10, 32])
+def test_l2_small_database_of_wide_rows(d, n, k):
+    """At most 4096 rows wider than 64 columns take the key-matrix path (a product per query chunk
+    with a four-step load pipeline, then selection, rerank and fallback): bit-exact to the oracle."""
+    rng = np.random.default_rng(d + n + k)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    Q = rng.standard_normal((150, d)).astype(np.float32)
+    h = N.ExactHandle(N.METRIC_L2)
+    h.set(X)
+    gi, gd = h.search(Q, k)
+    oi, od = oracle.knn_float(N.METRIC_L2, X, Q, k)
+    np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(gi, oi)
+
+
 @pytest.mark.parametrize("d", [128, 192, 256])
 def test_l2_wide_scan_equals_generic_scan(d):
     """Rows of 128, 192 and 256 columns: the scan with register-resident queries and the generic
