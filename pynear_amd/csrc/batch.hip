@@ -89,11 +89,15 @@ __device__ __forceinline__ void batch_tile(uint64_t *lds_raw, const BatchEntry &
 
 // Tiles of rows padded to 128, 192 or 256 columns: the scan with register-resident queries.
 template <int KL>
-__device__ __forceinline__ void batch_tile_wide(uint64_t *lds_raw, const BatchEntry &e, int64_t qb, int64_t s) {
+__device__ __forceinline__ void batch_tile_wide(uint64_t *lds_raw, __bf16 *stage, const BatchEntry &e, int64_t qb,
+                                                int64_t s) {
     auto lds = reinterpret_cast<uint64_t (*)[32][KL]>(lds_raw);
-    if (e.dp == 128) scan_wide<KL, 3, 2>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
-    else if (e.dp == 192) scan_wide<KL, 3, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
-    else scan_wide<KL, 3, 4>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
+    if (e.dp == 128)
+        scan_wide<KL, 3, 2>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
+    else if (e.dp == 192)
+        scan_wide<KL, 3, 3>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
+    else
+        scan_wide<KL, 3, 4>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
 }
 
 static bool wide_dp(int dp) { return dp == 128 || dp == 192 || dp == 256; }
@@ -106,6 +110,7 @@ __global__ __launch_bounds__(256) void batch_scan_kernel(const BatchEntry *__res
                                                          const int2 *__restrict__ tiles, int64_t ntiles,
                                                          int32_t *__restrict__ counter) {
     __shared__ uint64_t lds_raw[4 * 32 * 32];
+    __shared__ __align__(16) __bf16 stage[WIDE ? wide_stage_elems<4, 2>() : 8];
     __shared__ int64_t s_slot;
     for (;;) {
         if (threadIdx.x == 0) s_slot = atomicAdd(counter, 1);
@@ -116,9 +121,9 @@ __global__ __launch_bounds__(256) void batch_scan_kernel(const BatchEntry *__res
         const BatchEntry &e = E[t.x];
         const int64_t qb = t.y / e.S, s = t.y % e.S;
         if (WIDE) {
-            if (e.KL == 8) batch_tile_wide<8>(lds_raw, e, qb, s);
-            else if (e.KL == 16) batch_tile_wide<16>(lds_raw, e, qb, s);
-            else batch_tile_wide<32>(lds_raw, e, qb, s);
+            if (e.KL == 8) batch_tile_wide<8>(lds_raw, stage, e, qb, s);
+            else if (e.KL == 16) batch_tile_wide<16>(lds_raw, stage, e, qb, s);
+            else batch_tile_wide<32>(lds_raw, stage, e, qb, s);
         } else {
             if (e.KL == 8) batch_tile<8>(lds_raw, e, qb, s);
             else if (e.KL == 16) batch_tile<16>(lds_raw, e, qb, s);

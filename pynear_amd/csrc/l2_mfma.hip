@@ -88,7 +88,8 @@ __global__ __launch_bounds__(256) void l2_mfma_scan_wide_kernel(
     const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql, const float *__restrict__ qn,
     int64_t rows_per_slice, uint64_t *__restrict__ part, int64_t nq_pad, bool slice_major) {
     __shared__ uint64_t lds[4][32][KL];
-    scan_wide<KL, PREC, NC>(lds, Xh, Xl, xn, n, Qh, Ql, qn, rows_per_slice, part, nq_pad,
+    __shared__ __align__(16) __bf16 stage[wide_stage_elems<NC, PREC == 3 ? 2 : 1>()];
+    scan_wide<KL, PREC, NC>(lds, stage, Xh, Xl, xn, n, Qh, Ql, qn, rows_per_slice, part, nq_pad,
                             slice_major ? blockIdx.y : blockIdx.x, slice_major ? blockIdx.x : blockIdx.y);
 }
 

@@ -260,21 +260,30 @@ __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16
     half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + (s * KL) * nq_pad + q, nq_pad);
 }
 
-// The same scan for dp = 64 NC (NC = 2..4): the lane's query fragments stay in registers for the
-// whole slice (the generic scan reads them again for every tile, doubling the row-operand
-// traffic through L1), and the row fragments stream through NC register sets, one per 64-column
-// chunk of a tile: set c is refilled with the next tile's chunk c right after its products, so a
-// load has the other NC - 1 chunks' products to arrive. The next tile's norms are loaded after
-// the keys of this one and wait behind a whole tile of products.
+// LDS staging area of scan_wide (bf16 elements): two buffers of NPL planes of 32 rows of
+// 64 NC + 8 columns (16 bytes of pad per row).
+template <int NC, int NPL> constexpr int wide_stage_elems() { return 2 * NPL * 32 * (64 * NC + 8); }
+
+// The same scan for dp = 64 NC (NC = 2..4). The lane's query fragments stay in registers for the
+// whole slice. A tile's rows are staged once per workgroup into LDS by coalesced loads (16-byte
+// pieces, consecutive threads along a row: whole 64-byte L1 accesses) and every wave reads its
+// row fragments from there: direct fragment loads would touch 32 half-used segments per load and
+// fetch each tile once per wave, and L1 accesses, not products, would bound the scan. The next
+// tile's pieces are loaded into registers during this tile's products and stored into the other
+// buffer after them (one barrier per tile; rows are padded: the loads need no guard).
 template <int KL, int PREC, int NC>
-__device__ __forceinline__ void scan_wide(uint64_t (*lds)[32][KL], const __bf16 *__restrict__ Xh,
+__device__ __forceinline__ void scan_wide(uint64_t (*lds)[32][KL], __bf16 *stage, const __bf16 *__restrict__ Xh,
                                           const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
                                           const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
                                           const float *__restrict__ qn, int64_t rows_per_slice,
                                           uint64_t *__restrict__ part, int64_t nq_pad, int64_t qb, int64_t s) {
     constexpr int dp = 64 * NC;
     constexpr int NPL = PREC == 3 ? 2 : 1;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
+    constexpr int RS = dp + 8;          // LDS row stride (bf16)
+    constexpr int PLANE = 32 * RS, BUF = NPL * PLANE;
+    constexpr int PPR = dp / 8;         // 16-byte pieces per row
+    constexpr int NPC = NPL * NC;       // pieces per thread per tile (NPL x 32 x PPR / 256)
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5, col = lane & 31;
     const int64_t q = qb * 128 + 32 * w + col;
     const int64_t r0 = s * rows_per_slice, r1 = min(n, r0 + rows_per_slice);
     const float qnv = qn[q];
@@ -293,33 +302,45 @@ __device__ __forceinline__ void scan_wide(uint64_t (*lds)[32][KL], const __bf16 
     uint32_t thr = kKey32None;
     const int64_t nt = r0 < r1 ? (r1 - r0 + 31) / 32 : 0;
     if (nt > 0) {
-        bf16x8 a[4 * NC][NPL]; // chunk c: a[4c .. 4c + 3]
+        // staging role: piece tid + 256 u is plane u / NC, then row and piece of that plane
+        int goff[NPC], soff[NPC];
 #pragma unroll
-        for (int k = 0; k < 4 * NC; ++k) {
-            a[k][0] = *(const bf16x8 *)(Xh + (r0 + col) * dp + 16 * k + 8 * half);
-            if (PREC == 3) a[k][NPL - 1] = *(const bf16x8 *)(Xl + (r0 + col) * dp + 16 * k + 8 * half);
+        for (int u = 0; u < NPC; ++u) {
+            const int rem = tid + 256 * (u % NC), row = rem / PPR, pc = rem % PPR;
+            goff[u] = row * dp + 8 * pc;
+            soff[u] = (u / NC) * PLANE + row * RS + 8 * pc;
         }
+        bf16x8 sreg[NPC];
+        auto gload = [&](int64_t t) __attribute__((always_inline)) {
+#pragma unroll
+            for (int u = 0; u < NPC; ++u)
+                sreg[u] = *(const bf16x8 *)((u / NC == 0 ? Xh : Xl) + t * dp + goff[u]);
+        };
+        auto sstore = [&](int bf) __attribute__((always_inline)) {
+#pragma unroll
+            for (int u = 0; u < NPC; ++u) *(bf16x8 *)(stage + bf * BUF + soff[u]) = sreg[u];
+        };
+        gload(r0);
+        sstore(0);
         float4 xv[4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) xv[g] = *(const float4 *)(xn + r0 + 8 * g + 4 * half);
+        __syncthreads();
         for (int64_t i = 0; i < nt; ++i) {
+            const int bf = (int)(i & 1);
             const int64_t t = r0 + 32 * i;
-            const int64_t tn = r0 + 32 * min(i + 1, nt - 1); // past the last tile: that tile again
+            const int64_t tn = r0 + 32 * min(i + 1, nt - 1);
+            if (i + 1 < nt) gload(tn); // (in flight during the products)
+            const __bf16 *sa = stage + bf * BUF + col * RS + 8 * half;
             f32x16 acc = {};
 #pragma unroll
-            for (int c = 0; c < NC; ++c) {
-#pragma unroll
-                for (int k = 4 * c; k < 4 * c + 4; ++k) {
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][0], b[k][0], acc, 0, 0, 0);
-                    if (PREC == 3) {
-                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][0], b[k][NPL - 1], acc, 0, 0, 0);
-                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][NPL - 1], b[k][0], acc, 0, 0, 0);
-                    }
-                }
-#pragma unroll
-                for (int k = 4 * c; k < 4 * c + 4; ++k) {
-                    a[k][0] = *(const bf16x8 *)(Xh + (tn + col) * dp + 16 * k + 8 * half);
-                    if (PREC == 3) a[k][NPL - 1] = *(const bf16x8 *)(Xl + (tn + col) * dp + 16 * k + 8 * half);
+            for (int k = 0; k < 4 * NC; ++k) {
+                const bf16x8 ah = *(const bf16x8 *)(sa + 16 * k);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b[k][0], acc, 0, 0, 0);
+                if (PREC == 3) {
+                    const bf16x8 al = *(const bf16x8 *)(sa + PLANE + 16 * k);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b[k][NPL - 1], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b[k][0], acc, 0, 0, 0);
                 }
             }
             thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64)); // the other half-wave's list of this query
@@ -331,6 +352,10 @@ __device__ __forceinline__ void scan_wide(uint64_t (*lds)[32][KL], const __bf16 
                              : tile_keys_insert<KL, true, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
 #pragma unroll
             for (int g = 0; g < 4; ++g) xv[g] = *(const float4 *)(xn + tn + 8 * g + 4 * half);
+            // the other buffer was last read for tile i - 1, which every wave finished before the
+            // previous barrier
+            if (i + 1 < nt) sstore(bf ^ 1);
+            __syncthreads();
         }
     }
     half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + (s * KL) * nq_pad + q, nq_pad);
