@@ -65,21 +65,16 @@ __global__ __launch_bounds__(256) void l2_mfma_scan_kernel(const __bf16 *__restr
                            slice_major ? blockIdx.y : blockIdx.x, slice_major ? blockIdx.x : blockIdx.y);
 }
 
-__global__ void fill_u32_kernel(uint32_t *__restrict__ p, int64_t n, uint32_t v) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 template <int KL, int PREC, int NS>
 __global__ __launch_bounds__(256) void l2_mfma_scan_small_kernel(
     const __bf16 *__restrict__ Xh, const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
     const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql, const float *__restrict__ qn,
-    int64_t rows_per_slice, uint64_t *__restrict__ part, int64_t nq_pad, bool slice_major, uint32_t *__restrict__ gthr) {
+    int64_t rows_per_slice, uint64_t *__restrict__ part, int64_t nq_pad, bool slice_major) {
     __shared__ uint64_t lds[4][32][KL];
     // slice_major: blockIdx.x is the slice, so workgroup b (on XCD b % 8) of every query block
     // scans the same eighth of the slices, and those rows stay in that XCD's L2
     scan_small<KL, PREC, NS>(lds, Xh, Xl, xn, n, Qh, Ql, qn, rows_per_slice, part, nq_pad,
-                             slice_major ? blockIdx.y : blockIdx.x, slice_major ? blockIdx.x : blockIdx.y, gthr);
+                             slice_major ? blockIdx.y : blockIdx.x, slice_major ? blockIdx.x : blockIdx.y);
 }
 
 template <int KL, int PREC, int NC>
@@ -409,13 +404,6 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
         StageScope sc(st, STAGE_SCAN);
         // slice-major when the query blocks fit the grid's y dimension (see the kernels)
         const bool sm = gx <= 65535 && tune_get("l2.slice_major") != 0;
-        // thresholds shared across slices (small scan): one key per query, cleared by a kernel
-        uint32_t *gthr = nullptr;
-        if (dp <= 64 && S > 1 && tune_get("l2.shared_thr") != 0) {
-            gthr = (uint32_t *)wk.gthr.ensure((size_t)nq_pad * 4);
-            hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)ceil_div(nq_pad, (int64_t)256)), dim3(256), 0, st, gthr,
-                               nq_pad, kKey32None);
-        }
         dim3 grid(sm ? (unsigned)S : (unsigned)gx, sm ? (unsigned)gx : (unsigned)S);
         const bool wide = tune_get("l2.wide") != 0; // register-resident queries for dp = 128, 192, 256
         auto launch = [&](auto klc, auto pc) {
@@ -425,19 +413,19 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
             switch (dp) {
             case 16:
                 hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 1>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
-                                   Ql, qn, rps, part, nq_pad, sm, gthr);
+                                   Ql, qn, rps, part, nq_pad, sm);
                 break;
             case 32:
                 hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 2>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
-                                   Ql, qn, rps, part, nq_pad, sm, gthr);
+                                   Ql, qn, rps, part, nq_pad, sm);
                 break;
             case 48:
                 hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 3>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
-                                   Ql, qn, rps, part, nq_pad, sm, gthr);
+                                   Ql, qn, rps, part, nq_pad, sm);
                 break;
             case 64:
                 hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 4>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
-                                   Ql, qn, rps, part, nq_pad, sm, gthr);
+                                   Ql, qn, rps, part, nq_pad, sm);
                 break;
             case 128:
             case 192:

@@ -55,7 +55,7 @@ struct L2Set {
 // Per-index scratch of a search.
 struct L2Work {
     L2Set qsplit;
-    DevBuf part, cidx, capx, flags, gthr;
+    DevBuf part, cidx, capx, flags;
 };
 
 // X: device rows (fp32, row stride `stride` floats).

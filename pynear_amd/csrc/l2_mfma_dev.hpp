@@ -33,7 +33,7 @@ __device__ __forceinline__ void split_row(const float *__restrict__ X, int64_t n
 // the lane's sorted list (Lk, Li) of KL. A key is the approximation's float bits with the low 5
 // bits replaced by the row's offset in the tile: keys stay unique inside a tile, the minimum names
 // its row, and (key & ~31) is a lower bound of the approximation. Passing keys (below thr) are
-// inserted one per lane per round; rounds run while any lane of the wave still has one. thr may
+// inserted in ascending order, the lane's two smallest first; thr may
 // come from the other half-wave's list of the same query: rows at or above it cannot reach the
 // KL best of the two lists together.
 template <int KL, bool MASK, bool FIRST>
@@ -52,12 +52,13 @@ __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4
     // vmin): a tile no lane can take from is done
     if (!__any((f32_key(vmin) & ~31u) < thr)) return;
     uint32_t key[16];
-    uint32_t mn = kKey32None;
+    uint32_t m1 = kKey32None, m2 = kKey32None; // the lane's two smallest keys
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int off = 8 * (r >> 2) + 4 * half + (r & 3);
         key[r] = !MASK || t + off < r1 ? ((f32_key(v[r]) & ~31u) | (uint32_t)off) : kKey32None;
-        mn = min(mn, key[r]);
+        m2 = min(m2, max(m1, key[r]));
+        m1 = min(m1, key[r]);
     }
     if (FIRST) { // empty list: the tile's KL smallest keys, sorted, by a network instead of KL rounds
         sort16(key);
@@ -69,27 +70,36 @@ __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4
         thr = min(thr, Lk[KL - 1]);
         return;
     }
-    while (__any(mn < thr)) {
-        if (mn < thr) {
+    // keys below thr go in ascending order (thr falls as the list fills): the two smallest are at
+    // hand; a lane that takes both may take more, found one round each among the keys above the
+    // last taken (keys are unique)
+    auto insert = [&](uint32_t mk) __attribute__((always_inline)) {
+        const uint32_t ir = (uint32_t)t + (mk & 31u);
 #pragma unroll
-            for (int r = 0; r < 16; ++r) key[r] = key[r] == mn ? kKey32None : key[r];
-            const uint32_t ir = (uint32_t)t + (mn & 31u);
-#pragma unroll
-            for (int j = KL - 1; j > 0; --j) {
-                const bool up = mn < Lk[j - 1];
-                const bool here = mn < Lk[j];
-                Li[j] = up ? Li[j - 1] : (here ? ir : Li[j]);
-                Lk[j] = max(Lk[j - 1], min(mn, Lk[j]));
-            }
-            if (mn < Lk[0]) {
-                Li[0] = ir;
-                Lk[0] = mn;
-            }
-            thr = min(thr, Lk[KL - 1]);
+        for (int j = KL - 1; j > 0; --j) {
+            const bool up = mk < Lk[j - 1];
+            const bool here = mk < Lk[j];
+            Li[j] = up ? Li[j - 1] : (here ? ir : Li[j]);
+            Lk[j] = max(Lk[j - 1], min(mk, Lk[j]));
         }
+        if (mk < Lk[0]) {
+            Li[0] = ir;
+            Lk[0] = mk;
+        }
+        thr = min(thr, Lk[KL - 1]);
+    };
+    if (m1 < thr) insert(m1);
+    if (!__any(m2 < thr)) return;
+    if (m2 < thr) insert(m2);
+    uint32_t last = m2, mn = kKey32None;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) mn = min(mn, key[r] > last ? key[r] : kKey32None);
+    while (__any(mn < thr)) {
+        if (mn < thr) insert(mn);
+        last = mn;
         mn = kKey32None;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) mn = min(mn, key[r]);
+        for (int r = 0; r < 16; ++r) mn = min(mn, key[r] > last ? key[r] : kKey32None);
     }
 }
 
@@ -171,19 +181,12 @@ __device__ __forceinline__ void scan_generic(uint64_t (*lds)[32][KL], const __bf
 // whole slice, and row fragments and norms are loaded two tiles ahead into a ring of three
 // register sets that rotate by name (three tiles per loop trip; a copy between sets would wait
 // for the loads in flight), so the memory latency hides behind the work of the two tiles before.
-//
-// gthr (optional, one key per query, all kKey32None before the launch) shares thresholds across
-// the slices of a query: a lane whose list is full publishes its last key with atomicMin and
-// every tile takes the published minimum (read with the tile's rows) as its own threshold too.
-// Keys at or above it are dropped; the smallest last key of all lists still bounds every dropped
-// key from below, as the completeness proof needs.
 template <int KL, int PREC, int NS>
 __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16 *__restrict__ Xh,
                                            const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
                                            const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
                                            const float *__restrict__ qn, int64_t rows_per_slice,
-                                           uint64_t *__restrict__ part, int64_t nq_pad, int64_t qb, int64_t s,
-                                           uint32_t *__restrict__ gthr = nullptr) {
+                                           uint64_t *__restrict__ part, int64_t nq_pad, int64_t qb, int64_t s) {
     constexpr int dp = 16 * NS;
     constexpr int NPL = PREC == 3 ? 2 : 1;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
@@ -208,18 +211,20 @@ __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16
     struct Tile {
         bf16x8 a[NS][NPL];
         float4 xv[4];
-        uint32_t g; // the shared threshold as read with this tile
     };
-    uint32_t published = kKey32None;
+    // addresses: a wave-uniform tile base plus a loop-invariant 32-bit lane offset (scalar base,
+    // vector offset and immediate loads, no 64-bit arithmetic per load)
+    const int aoff = col * dp + 8 * half, noff = 4 * half;
     auto load_tile = [&](int64_t t, Tile &f) __attribute__((always_inline)) {
+        const __bf16 *th = Xh + t * dp, *tl = Xl + t * dp;
+        const float *tn = xn + t;
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            f.a[k][0] = *(const bf16x8 *)(Xh + (t + col) * dp + 16 * k + 8 * half);
-            if (PREC == 3) f.a[k][NPL - 1] = *(const bf16x8 *)(Xl + (t + col) * dp + 16 * k + 8 * half);
+            f.a[k][0] = *(const bf16x8 *)(th + aoff + 16 * k);
+            if (PREC == 3) f.a[k][NPL - 1] = *(const bf16x8 *)(tl + aoff + 16 * k);
         }
 #pragma unroll
-        for (int g = 0; g < 4; ++g) f.xv[g] = *(const float4 *)(xn + t + 8 * g + 4 * half);
-        f.g = gthr ? __hip_atomic_load(gthr + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kKey32None;
+        for (int g = 0; g < 4; ++g) f.xv[g] = *(const float4 *)(tn + noff + 8 * g);
     };
     const int64_t nt = r0 < r1 ? (r1 - r0 + 31) / 32 : 0;
     auto step = [&](int64_t i, const Tile &cur, Tile &ahead) __attribute__((always_inline)) {
@@ -235,17 +240,12 @@ __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16
             }
         }
         thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64)); // the other half-wave's list of this query
-        thr = min(thr, cur.g);                                     // ... and the other slices' lists
         if (t == r0) // only a slice's last tile can hold rows past its end
             t + 32 <= r1 ? tile_keys_insert<KL, false, true>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr)
                          : tile_keys_insert<KL, true, true>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr);
         else
             t + 32 <= r1 ? tile_keys_insert<KL, false, false>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr)
                          : tile_keys_insert<KL, true, false>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr);
-        if (gthr && Lk[KL - 1] < published) { // a full list publishes its last key
-            published = Lk[KL - 1];
-            atomicMin(gthr + q, published);
-        }
     };
     if (nt > 0) {
         Tile f0, f1, f2;

@@ -41,8 +41,6 @@ static std::map<std::string, int64_t> &tune_table() {
         t["l2.mfma_terms"] = 3;        // 3: hi.hi + hi.lo + lo.hi products; 1: hi.hi only
         t["l2.matrix"] = 1;            // exact L2 over <= 4096 rows of d > 64: key matrix + selection
         t["ivfl2.wg_per_cu"] = 2;      // IVF-L2 list scan: persistent workgroups per CU
-        t["l2.shared_thr"] = 0;       // exact L2 small scan: thresholds shared across slices (atomics;
-                                      // measured 5x slower from contention, off)
         t["l2.wide"] = 1;             // exact L2 scan for dp = 128, 192, 256: queries held in registers
         t["l2.matrix_terms"] = 3;     // key-matrix path (<= 4096 wide rows): bf16 products per term (1 or 3)
         t["l2.kl4"] = 1;              // exact L2 scan: per-lane lists of 4 for k <= 6 (with >= 4 slices)
