@@ -4,9 +4,11 @@
 // min(n, k) neighbours, ascending (distance, index), distances bit-identical to the reference's
 // dist_l2_f_avx2). A plan lays the sub-searches out as flat work lists and runs them as
 //   1. one split launch (bf16 hi/lo + norms of every query of every index),
-//   2. persistent scan launches: all (index, 128-query block, row slice) tiles of all indices in
-//      two queues (rows of 128, 192 or 256 padded columns, whose scan holds more registers, and
-//      the rest), longest tiles first; each tile runs the same device body as the single search,
+//   2. persistent scan launches: all (index, query block, row slice) tiles of all indices in
+//      three queues (rows of 128, 192 or 256 padded columns, whose scan holds more registers and
+//      stages tiles in LDS; the same for indices of at most 64 queries, whose waves take
+//      different slices instead of padding queries; the rest), longest tiles first; each tile
+//      runs the same device body as the single search,
 //   3. one merge, one rerank (with the completeness proof) and one fallback launch over all queries,
 // captured once in a HIP graph that every run replays. Slice length, persistent grid size and
 // tile order are the plan's schedule; pn_batch_tune measures a set of schedules on the device and
@@ -37,6 +39,7 @@ struct BatchEntry {
     const unsigned int *xn_max;
     int64_t xstride, n, d;
     int dp, kk, KL, K2, K2c;
+    int G; // 32-query groups per scan workgroup: 4, or 2 / 1 for at most 64 / 32 queries
     float eps_coef;
     const float *Q;
     int64_t qstride, nq, nq_pad, gx, S, rps;
@@ -68,7 +71,7 @@ __device__ __forceinline__ int find_entry(const int64_t *base, int n, int64_t x)
 __global__ __launch_bounds__(256) void batch_split_kernel(const BatchEntry *__restrict__ E, const int64_t *__restrict__ row_base,
                                                           int ne, int64_t rows, int32_t *__restrict__ counter) {
     const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (blockIdx.x == 0 && threadIdx.x < 2) counter[threadIdx.x] = 0; // the scan queues of this run
+    if (blockIdx.x == 0 && threadIdx.x < 3) counter[threadIdx.x] = 0; // the scan queues of this run
     if (r >= rows) return;
     const BatchEntry &e = E[find_entry(row_base, ne, r)];
     split_row(e.Q, e.nq, e.d, e.qstride, e.dp, e.Qh, e.Ql, e.qn, r - e.row_base);
@@ -78,10 +81,10 @@ template <int KL>
 __device__ __forceinline__ void batch_tile(uint64_t *lds_raw, const BatchEntry &e, int64_t qb, int64_t s) {
     auto lds = reinterpret_cast<uint64_t (*)[32][KL]>(lds_raw);
     switch (e.dp) {
-    case 16: scan_small<KL, 3, 1>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s); break;
-    case 32: scan_small<KL, 3, 2>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s); break;
-    case 48: scan_small<KL, 3, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s); break;
-    case 64: scan_small<KL, 3, 4>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s); break;
+    case 16: scan_small<KL, 3, 1>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s, e.G); break;
+    case 32: scan_small<KL, 3, 2>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s, e.G); break;
+    case 48: scan_small<KL, 3, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s, e.G); break;
+    case 64: scan_small<KL, 3, 4>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s, e.G); break;
     default:
         scan_generic<KL, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.dp, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
     }
@@ -100,17 +103,30 @@ __device__ __forceinline__ void batch_tile_wide(uint64_t *lds_raw, __bf16 *stage
         scan_wide<KL, 3, 4>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
 }
 
+// Wide-row tiles of indices with at most 64 queries: the waves scan different slices.
+template <int KL>
+__device__ __forceinline__ void batch_tile_wide_direct(uint64_t *lds_raw, const BatchEntry &e, int64_t qb, int64_t s) {
+    auto lds = reinterpret_cast<uint64_t (*)[32][KL]>(lds_raw);
+    if (e.dp == 128)
+        scan_wide_direct<KL, 3, 2>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s, e.G);
+    else if (e.dp == 192)
+        scan_wide_direct<KL, 3, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s, e.G);
+    else
+        scan_wide_direct<KL, 3, 4>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s, e.G);
+}
+
 static bool wide_dp(int dp) { return dp == 128 || dp == 192 || dp == 256; }
 
-// Persistent: workgroups take tiles (entry, query block, slice) from one queue until it is empty.
-// WIDE: the queue of wide-row tiles (their scan holds more registers, so it runs as a launch of
-// its own at its own occupancy).
-template <bool WIDE>
+// Persistent: workgroups take tiles (entry, query block, slice group) from one queue until it is
+// empty. One launch per tile class, each at its own occupancy: CLS 0 rows of at most 64 padded
+// columns (or other widths), CLS 1 rows of 128, 192 or 256 columns (LDS-staged tiles), CLS 2
+// those of indices with at most 64 queries (waves on different slices, direct loads).
+template <int CLS>
 __global__ __launch_bounds__(256) void batch_scan_kernel(const BatchEntry *__restrict__ E,
                                                          const int2 *__restrict__ tiles, int64_t ntiles,
                                                          int32_t *__restrict__ counter) {
     __shared__ uint64_t lds_raw[4 * 32 * 32];
-    __shared__ __align__(16) __bf16 stage[WIDE ? wide_stage_elems<4, 2>() : 8];
+    __shared__ __align__(16) __bf16 stage[CLS == 1 ? wide_stage_elems<4, 2>() : 8];
     __shared__ int64_t s_slot;
     for (;;) {
         if (threadIdx.x == 0) s_slot = atomicAdd(counter, 1);
@@ -120,10 +136,14 @@ __global__ __launch_bounds__(256) void batch_scan_kernel(const BatchEntry *__res
         const int2 t = tiles[slot];
         const BatchEntry &e = E[t.x];
         const int64_t qb = t.y / e.S, s = t.y % e.S;
-        if (WIDE) {
+        if (CLS == 1) {
             if (e.KL == 8) batch_tile_wide<8>(lds_raw, stage, e, qb, s);
             else if (e.KL == 16) batch_tile_wide<16>(lds_raw, stage, e, qb, s);
             else batch_tile_wide<32>(lds_raw, stage, e, qb, s);
+        } else if (CLS == 2) {
+            if (e.KL == 8) batch_tile_wide_direct<8>(lds_raw, e, qb, s);
+            else if (e.KL == 16) batch_tile_wide_direct<16>(lds_raw, e, qb, s);
+            else batch_tile_wide_direct<32>(lds_raw, e, qb, s);
         } else {
             if (e.KL == 8) batch_tile<8>(lds_raw, e, qb, s);
             else if (e.KL == 16) batch_tile<16>(lds_raw, e, qb, s);
@@ -183,7 +203,7 @@ struct pn_batch {
     std::vector<int32_t *> oi;
     std::vector<float *> od;
     pn::BatchCfg cfg;
-    int64_t ntiles = 0, ntiles_wide = 0, rows = 0, queries = 0;
+    int64_t ntiles = 0, ncls[3] = {0, 0, 0}, rows = 0, queries = 0;
     pn::DevBuf entries, row_base, q_base, tiles, counter, scratch;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -224,7 +244,7 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     struct Off { size_t qh, ql, qn, part, cidx, capx, flags; };
     std::vector<Off> O(ne);
     int64_t rows = 0, queries = 0;
-    std::vector<std::pair<double, int2>> tl, tw; // narrow and wide-row tiles
+    std::vector<std::pair<double, int2>> tc[3]; // tiles of each scan class
     const bool wide_on = tune_get("l2.wide") != 0;
     for (int i = 0; i < ne; ++i) {
         pn_exact *h = b->idx[i];
@@ -243,8 +263,14 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
         e.gx = ceil_div(e.nq, (int64_t)128);
         e.nq_pad = e.gx * 128;
         int64_t S = std::max<int64_t>(1, ceil_div(e.n, (int64_t)std::max(32, cfg.slice_rows)));
+        // few queries: fewer 32-query groups per workgroup and its waves on consecutive slices, so
+        // no wave multiplies padding queries (S rounded up to whole groups of 4 / G slices; the
+        // trailing ones may hold no rows)
+        e.G = e.nq <= 32 ? 1 : (e.nq <= 64 ? 2 : 4);
+        const int64_t m = 4 / e.G;
+        S = ceil_div(S, m) * m;
         e.rps = ceil_div(ceil_div(e.n, S), (int64_t)32) * 32;
-        e.S = ceil_div(e.n, e.rps);
+        e.S = e.G == 4 ? ceil_div(e.n, e.rps) : S;
         e.row_base = rows;
         e.q_base = queries;
         rbase[i] = rows;
@@ -259,15 +285,17 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
         O[i].capx = take((size_t)e.nq * e.K2c * 4);
         O[i].flags = take((size_t)e.nq * 4);
         const double cost = (double)e.rps * (e.dp / 16 + 4); // a tile: rows x (products + keys)
-        auto &dst = wide_on && wide_dp(e.dp) ? tw : tl;
-        for (int64_t t = 0; t < e.gx * e.S; ++t) dst.push_back({cost, make_int2(i, (int)t)});
+        const int cls = wide_on && wide_dp(e.dp) ? (e.G == 4 ? 1 : 2) : 0;
+        // tiles: (query block, slice) for G = 4; (the one query block, group of 4 / G slices) else
+        const int64_t ntile = e.G == 4 ? e.gx * e.S : e.S / m;
+        for (int64_t t = 0; t < ntile; ++t) tc[cls].push_back({cost, make_int2(i, (int)t)});
     }
-    if (cfg.order == 1) {
-        std::stable_sort(tl.begin(), tl.end(), [](const auto &a, const auto &c) { return a.first > c.first; });
-        std::stable_sort(tw.begin(), tw.end(), [](const auto &a, const auto &c) { return a.first > c.first; });
+    std::vector<std::pair<double, int2>> tl;
+    for (int c = 0; c < 3; ++c) {
+        if (cfg.order == 1)
+            std::stable_sort(tc[c].begin(), tc[c].end(), [](const auto &a, const auto &x) { return a.first > x.first; });
+        tl.insert(tl.end(), tc[c].begin(), tc[c].end());
     }
-    const int64_t nnarrow = (int64_t)tl.size();
-    tl.insert(tl.end(), tw.begin(), tw.end());
     char *base = (char *)b->scratch.ensure(bytes);
     for (int i = 0; i < ne; ++i) {
         pn_exact *h = b->idx[i];
@@ -291,7 +319,7 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     std::vector<int2> tiles(tl.size());
     for (size_t t = 0; t < tl.size(); ++t) tiles[t] = tl[t].second;
     b->ntiles = (int64_t)tiles.size();
-    b->ntiles_wide = b->ntiles - nnarrow;
+    for (int c = 0; c < 3; ++c) b->ncls[c] = (int64_t)tc[c].size();
     b->rows = rows;
     b->queries = queries;
     hipStream_t st = b->cap_stream;
@@ -301,7 +329,7 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     PN_HIP(hipMemcpyAsync(b->q_base.ensure(ne * 8), qbase.data(), ne * 8, hipMemcpyHostToDevice, st));
     PN_HIP(hipMemcpyAsync(b->tiles.ensure(std::max<size_t>(1, tiles.size()) * 8), tiles.data(), tiles.size() * 8,
                           hipMemcpyHostToDevice, st));
-    b->counter.ensure(8); // two queue counters
+    b->counter.ensure(16); // one queue counter per scan class
     PN_HIP(hipStreamSynchronize(st));
     b->cfg = cfg;
 
@@ -317,20 +345,23 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     const BatchEntry *dE = b->entries.as<BatchEntry>();
     const int64_t *dR = b->row_base.as<int64_t>(), *dQ = b->q_base.as<int64_t>();
     int32_t *ctr = b->counter.as<int32_t>();
-    const int64_t nnar = b->ntiles - b->ntiles_wide;
+    const int64_t *nc = b->ncls;
     const unsigned grid_scan =
-        (unsigned)std::max<int64_t>(1, std::min<int64_t>(nnar, (int64_t)b->cus * std::max(1, cfg.wg_per_cu)));
-    // the wide scan holds one wave per SIMD: one workgroup per CU
-    const unsigned grid_wide = (unsigned)std::max<int64_t>(1, std::min<int64_t>(b->ntiles_wide, (int64_t)b->cus));
+        (unsigned)std::max<int64_t>(1, std::min<int64_t>(nc[0], (int64_t)b->cus * std::max(1, cfg.wg_per_cu)));
+    // the wide scans hold one wave per SIMD: one workgroup per CU
+    const unsigned grid_wide = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nc[1], (int64_t)b->cus));
+    const unsigned grid_wd = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nc[2], (int64_t)b->cus));
     PN_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(batch_split_kernel, dim3((unsigned)ceil_div(rows, (int64_t)4)), dim3(256), 0, st, dE, dR, ne,
                        rows, ctr);
-    if (nnar > 0)
-        hipLaunchKernelGGL(batch_scan_kernel<false>, dim3(grid_scan), dim3(256), 0, st, dE, b->tiles.as<int2>(), nnar,
-                           ctr);
-    if (b->ntiles_wide > 0)
-        hipLaunchKernelGGL(batch_scan_kernel<true>, dim3(grid_wide), dim3(256), 0, st, dE, b->tiles.as<int2>() + nnar,
-                           b->ntiles_wide, ctr + 1);
+    const int2 *dt = b->tiles.as<int2>();
+    if (nc[0] > 0)
+        hipLaunchKernelGGL(batch_scan_kernel<0>, dim3(grid_scan), dim3(256), 0, st, dE, dt, nc[0], ctr);
+    if (nc[1] > 0)
+        hipLaunchKernelGGL(batch_scan_kernel<1>, dim3(grid_wide), dim3(256), 0, st, dE, dt + nc[0], nc[1], ctr + 1);
+    if (nc[2] > 0)
+        hipLaunchKernelGGL(batch_scan_kernel<2>, dim3(grid_wd), dim3(256), 0, st, dE, dt + nc[0] + nc[1], nc[2],
+                           ctr + 2);
     hipLaunchKernelGGL(batch_merge_kernel, dim3(xcd_grid(ceil_div(queries, (int64_t)4))), dim3(256), 0, st, dE, dQ, ne,
                        queries);
     hipLaunchKernelGGL(batch_rerank_kernel, dim3((unsigned)ceil_div(queries, (int64_t)4)), dim3(256), 0, st, dE, dQ,

@@ -33,9 +33,8 @@ __device__ __forceinline__ void split_row(const float *__restrict__ X, int64_t n
 // the lane's sorted list (Lk, Li) of KL. A key is the approximation's float bits with the low 5
 // bits replaced by the row's offset in the tile: keys stay unique inside a tile, the minimum names
 // its row, and (key & ~31) is a lower bound of the approximation. Passing keys (below thr) are
-// inserted in ascending order, the lane's two smallest first; thr may
-// come from the other half-wave's list of the same query: rows at or above it cannot reach the
-// KL best of the two lists together.
+// inserted in ascending order; thr may come from the other half-wave's list of the same query:
+// rows at or above it cannot reach the KL best of the two lists together.
 template <int KL, bool MASK, bool FIRST>
 __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4 (&xv)[4], float qnv, int64_t t,
                                                  int64_t r1, int half, uint32_t (&Lk)[KL], uint32_t (&Li)[KL],
@@ -52,7 +51,7 @@ __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4
     // vmin): a tile no lane can take from is done
     if (!__any((f32_key(vmin) & ~31u) < thr)) return;
     uint32_t key[16];
-    uint32_t m1 = kKey32None, m2 = kKey32None; // the lane's two smallest keys
+    uint32_t m1 = kKey32None, m2 = kKey32None; // the lane's two smallest keys (FAST2)
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int off = 8 * (r >> 2) + 4 * half + (r & 3);
@@ -60,6 +59,10 @@ __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4
         m2 = min(m2, max(m1, key[r]));
         m1 = min(m1, key[r]);
     }
+    // FAST2: for short lists the two smallest keys are found in the pass above and go in without a
+    // rescan of the tile's keys; longer lists (whose insertion dominates and whose registers are
+    // scarce) take every key by rounds
+    constexpr bool FAST2 = KL <= 8;
     if (FIRST) { // empty list: the tile's KL smallest keys, sorted, by a network instead of KL rounds
         sort16(key);
 #pragma unroll
@@ -70,9 +73,8 @@ __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4
         thr = min(thr, Lk[KL - 1]);
         return;
     }
-    // keys below thr go in ascending order (thr falls as the list fills): the two smallest are at
-    // hand; a lane that takes both may take more, found one round each among the keys above the
-    // last taken (keys are unique)
+    // keys below thr go in ascending order (thr falls as the list fills), one round each among the
+    // keys above the last taken (keys are unique); with FAST2 the two smallest go in first
     auto insert = [&](uint32_t mk) __attribute__((always_inline)) {
         const uint32_t ir = (uint32_t)t + (mk & 31u);
 #pragma unroll
@@ -88,15 +90,18 @@ __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4
         }
         thr = min(thr, Lk[KL - 1]);
     };
-    if (m1 < thr) insert(m1);
-    if (!__any(m2 < thr)) return;
-    if (m2 < thr) insert(m2);
-    uint32_t last = m2, mn = kKey32None;
+    uint32_t mn = m1;
+    if (FAST2) {
+        if (m1 < thr) insert(m1);
+        if (!__any(m2 < thr)) return;
+        if (m2 < thr) insert(m2);
+        mn = kKey32None;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) mn = min(mn, key[r] > last ? key[r] : kKey32None);
+        for (int r = 0; r < 16; ++r) mn = min(mn, key[r] > m2 ? key[r] : kKey32None);
+    }
     while (__any(mn < thr)) {
         if (mn < thr) insert(mn);
-        last = mn;
+        const uint32_t last = mn;
         mn = kKey32None;
 #pragma unroll
         for (int r = 0; r < 16; ++r) mn = min(mn, key[r] > last ? key[r] : kKey32None);
@@ -186,11 +191,15 @@ __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16
                                            const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
                                            const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
                                            const float *__restrict__ qn, int64_t rows_per_slice,
-                                           uint64_t *__restrict__ part, int64_t nq_pad, int64_t qb, int64_t s) {
+                                           uint64_t *__restrict__ part, int64_t nq_pad, int64_t qb, int64_t s,
+                                           int g = 4) {
     constexpr int dp = 16 * NS;
     constexpr int NPL = PREC == 3 ? 2 : 1;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
-    const int64_t q = qb * 128 + 32 * w + col;
+    // g query groups of 32 per workgroup (4; 2 or 1 for a block of at most 64 or 32 queries): wave w
+    // takes group w % g of query block qb (of 32 g queries) and slice s (4 / g) + w / g
+    const int64_t q = qb * 32 * g + 32 * (w % g) + col;
+    s = s * (4 / g) + w / g;
     const int64_t r0 = s * rows_per_slice, r1 = min(n, r0 + rows_per_slice);
     const float qnv = qn[q];
     bf16x8 b[NS][NPL];
@@ -255,6 +264,84 @@ __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16
             step(i, f0, f2);
             step(i + 1, f1, f0);
             step(i + 2, f2, f1);
+        }
+    }
+    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + (s * KL) * nq_pad + q, nq_pad);
+}
+
+// The wide scan without LDS staging, for workgroups whose waves scan different slices (blocks of
+// at most 64 queries: g = 1 or 2 query groups, as in scan_small): the row fragments stream
+// through NC register sets, one per 64-column chunk of a tile, set c refilled with the next
+// tile's chunk c right after its products.
+template <int KL, int PREC, int NC>
+__device__ __forceinline__ void scan_wide_direct(uint64_t (*lds)[32][KL], const __bf16 *__restrict__ Xh,
+                                                 const __bf16 *__restrict__ Xl, const float *__restrict__ xn,
+                                                 int64_t n, const __bf16 *__restrict__ Qh,
+                                                 const __bf16 *__restrict__ Ql, const float *__restrict__ qn,
+                                                 int64_t rows_per_slice, uint64_t *__restrict__ part, int64_t nq_pad,
+                                                 int64_t qb, int64_t s, int g) {
+    constexpr int dp = 64 * NC;
+    constexpr int NPL = PREC == 3 ? 2 : 1;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
+    const int64_t q = qb * 32 * g + 32 * (w % g) + col;
+    s = s * (4 / g) + w / g;
+    const int64_t r0 = s * rows_per_slice, r1 = min(n, r0 + rows_per_slice);
+    const float qnv = qn[q];
+    bf16x8 b[4 * NC][NPL];
+#pragma unroll
+    for (int k = 0; k < 4 * NC; ++k) {
+        b[k][0] = *(const bf16x8 *)(Qh + q * dp + 16 * k + 8 * half);
+        if (PREC == 3) b[k][NPL - 1] = *(const bf16x8 *)(Ql + q * dp + 16 * k + 8 * half);
+    }
+    uint32_t Lk[KL], Li[KL];
+#pragma unroll
+    for (int j = 0; j < KL; ++j) {
+        Lk[j] = kKey32None;
+        Li[j] = 0u;
+    }
+    uint32_t thr = kKey32None;
+    const int aoff = col * dp + 8 * half, noff = 4 * half;
+    const int64_t nt = r0 < r1 ? (r1 - r0 + 31) / 32 : 0;
+    if (nt > 0) {
+        bf16x8 a[4 * NC][NPL]; // chunk c: a[4c .. 4c + 3]
+#pragma unroll
+        for (int k = 0; k < 4 * NC; ++k) {
+            a[k][0] = *(const bf16x8 *)(Xh + r0 * dp + aoff + 16 * k);
+            if (PREC == 3) a[k][NPL - 1] = *(const bf16x8 *)(Xl + r0 * dp + aoff + 16 * k);
+        }
+        float4 xv[4];
+#pragma unroll
+        for (int gi = 0; gi < 4; ++gi) xv[gi] = *(const float4 *)(xn + r0 + noff + 8 * gi);
+        for (int64_t i = 0; i < nt; ++i) {
+            const int64_t t = r0 + 32 * i;
+            const int64_t tn = r0 + 32 * min(i + 1, nt - 1); // past the last tile: that tile again
+            const __bf16 *th = Xh + tn * dp, *tl = Xl + tn * dp;
+            f32x16 acc = {};
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+#pragma unroll
+                for (int k = 4 * c; k < 4 * c + 4; ++k) {
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][0], b[k][0], acc, 0, 0, 0);
+                    if (PREC == 3) {
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][0], b[k][NPL - 1], acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k][NPL - 1], b[k][0], acc, 0, 0, 0);
+                    }
+                }
+#pragma unroll
+                for (int k = 4 * c; k < 4 * c + 4; ++k) {
+                    a[k][0] = *(const bf16x8 *)(th + aoff + 16 * k);
+                    if (PREC == 3) a[k][NPL - 1] = *(const bf16x8 *)(tl + aoff + 16 * k);
+                }
+            }
+            thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64)); // the other half-wave's list of this query
+            if (t == r0)
+                t + 32 <= r1 ? tile_keys_insert<KL, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
+                             : tile_keys_insert<KL, true, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+            else
+                t + 32 <= r1 ? tile_keys_insert<KL, false, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
+                             : tile_keys_insert<KL, true, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+#pragma unroll
+            for (int gi = 0; gi < 4; ++gi) xv[gi] = *(const float4 *)(xn + tn + noff + 8 * gi);
         }
     }
     half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + (s * KL) * nq_pad + q, nq_pad);

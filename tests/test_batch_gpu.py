@@ -21,8 +21,12 @@ def _make(seed, specs):
     return out
 
 
+# (n, d, queries, k); indices of at most 32 / 64 queries take one / two query groups per scan
+# workgroup (its waves on different slices): wide rows (d 128 .. 256) and narrow ones, one or many
+# slices, a slice group with empty trailing slices (300 rows)
 SPECS = [(1000, 64, 7, 1), (5000, 128, 300, 10), (20000, 256, 40, 32), (3000, 64, 1, 10), (777, 100, 129, 5),
-         (64, 16, 20, 32), (40, 32, 5, 32), (12000, 128, 257, 1), (2500, 8, 33, 12)]
+         (64, 16, 20, 32), (40, 32, 5, 32), (12000, 128, 257, 1), (2500, 8, 33, 12), (30000, 256, 3, 1),
+         (9000, 192, 64, 10), (7000, 64, 33, 5), (300, 256, 1, 10)]
 
 
 def _plain(idx, Q, k):
