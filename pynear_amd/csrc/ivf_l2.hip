@@ -1019,6 +1019,7 @@ __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restri
                                                            int64_t nq, int k, int32_t *__restrict__ out_idx,
                                                            float *__restrict__ out_dist, int32_t *__restrict__ out_cnt,
                                                            const int32_t *__restrict__ qorder, int64_t per_xcd) {
+    __shared__ uint64_t s_sort[4][256];
     const int lane = threadIdx.x & 63;
     // workgroup b runs on XCD b % 8: each XCD takes a contiguous run of qorder
     const int64_t pos = ((int64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds) * 4 + (threadIdx.x >> 6);
@@ -1043,16 +1044,44 @@ __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restri
         const uint64_t v = src[e];
         if (v <= bound && v < L[PER - 1]) list_insert_u64<PER>(L, v);
     }
-    uint64_t mine = kKey64None; // after the loop lane r holds the r-th best candidate
-    for (int r = 0; r < kIvfK2; ++r) {
-        const uint64_t m = wmin64(L[0]);
-        if (m == kKey64None) break;
-        if (L[0] == m) {
+    uint64_t mine = kKey64None; // lane r takes the r-th best candidate
+    // at most 256 held in all: packed into the wave's LDS and sorted by one bitonic network; else
+    // K2 rounds of a wave-wide minimum
+    int held = 0;
 #pragma unroll
-            for (int j = 0; j < PER - 1; ++j) L[j] = L[j + 1];
-            L[PER - 1] = kKey64None;
+    for (int j = 0; j < PER; ++j) held += L[j] != kKey64None ? 1 : 0;
+    int off = held;
+#pragma unroll
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const int t = __shfl_up(off, dd, 64);
+        if (lane >= dd) off += t;
+    }
+    const int total = __shfl(off, 63, 64);
+    off -= held;
+    if (total <= 256) {
+        uint64_t *wb = s_sort[threadIdx.x >> 6];
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            if (j < held) wb[off + j] = L[j];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint64_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = lane + 64 * u < total ? wb[lane + 64 * u] : kKey64None;
+        wave_sort_u64<4, 2>(v, lane);
+        mine = v[0];
+    } else {
+        for (int r = 0; r < kIvfK2; ++r) {
+            const uint64_t m = wmin64(L[0]);
+            if (m == kKey64None) break;
+            if (L[0] == m) {
+#pragma unroll
+                for (int j = 0; j < PER - 1; ++j) L[j] = L[j + 1];
+                L[PER - 1] = kKey64None;
+            }
+            if (lane == r) mine = m;
         }
-        if (lane == r) mine = m;
     }
     const int32_t mypos = mine == kKey64None ? -1 : (int32_t)(uint32_t)(mine & 0xFFFFFFFFull);
     const int32_t myid = mypos < 0 ? -1 : lids[mypos];
