@@ -253,6 +253,27 @@ def test_l2_fallback_path_exact(d):
     np.testing.assert_array_equal(vi, oi)
 
 
+@pytest.mark.parametrize("mfma", [0, 1])
+def test_l2_sqrt_tie_orders_by_index(mfma):
+    """Two rows whose squared distances differ (1 and 1 + 2^-23) but whose float square roots are
+    equal (1.0) tie on the returned distance, so the lower index comes first, although its squared
+    distance is the larger one; the other rows are far. Both selection paths (VALU scan with the
+    sqrt-ordered merge, and the MFMA scan with its exact rerank) keep that order."""
+    rng = np.random.default_rng(5)
+    X = (rng.standard_normal((3000, 2)) * 0.1 + 10.0).astype(np.float32)
+    X[2] = [1.0, 0.00028]  # squared distance to the origin 1.0000001, root 1.0
+    X[5] = [1.0, 0.0]      # squared distance 1.0, root 1.0
+    Q = np.zeros((3, 2), np.float32)
+    h = N.ExactHandle(N.METRIC_L2)
+    h.set(X)
+    with _Tune(l2__mfma=mfma):
+        gi, gd = h.search(Q, 2)
+    oi, od = oracle.knn_float(N.METRIC_L2, X, Q, 2)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+    assert gi[0].tolist() == [2, 5] and gd[0].tolist() == [1.0, 1.0]
+
+
 @pytest.mark.parametrize("d,n", [(100, 3000), (256, 4096), (768, 1500), (80, 37)])
 @pytest.mark.parametrize("k", [1, 10, 32])
 def test_l2_small_database_of_wide_rows(d, n, k):
