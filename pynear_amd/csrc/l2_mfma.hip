@@ -236,10 +236,55 @@ __global__ __launch_bounds__(256) void l2_matrix_keys_kernel(const __bf16 *__res
 __global__ __launch_bounds__(256) void l2_matrix_select_kernel(const uint32_t *__restrict__ keys, int N, int64_t M,
                                                                int K2c, int32_t *__restrict__ cidx,
                                                                float *__restrict__ capx) {
+    __shared__ uint64_t s_sel[4][256];
     const int lane = threadIdx.x & 63;
     const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (q >= M) return;
     const uint32_t *row = keys + q * N;
+    // Bounded selection: the K2c-th smallest of the 64 lane minima (one wave sort) bounds the K2c
+    // smallest keys from above (the K2c smallest minima are at or below it); the keys at or below
+    // it are packed into LDS and sorted, and the first K2c are the result. capx[K2c - 1] is then
+    // the K2c-th smallest key, a lower bound of every key left out. More than 256 keys at or below
+    // the bound: the chunked sort below.
+    {
+        uint64_t mn = kKey64None;
+        for (int c = lane; c < N; c += 64) {
+            const uint64_t v = ((uint64_t)row[c] << 32) | (uint32_t)c;
+            mn = v < mn ? v : mn;
+        }
+        uint64_t sm[1] = {mn};
+        wave_sort_u64<1, 2>(sm, lane);
+        const uint64_t bnd = __shfl(sm[0], K2c - 1, 64);
+        int held = 0;
+        for (int c = lane; c < N; c += 64) held += (((uint64_t)row[c] << 32) | (uint32_t)c) <= bnd ? 1 : 0;
+        int off = held;
+#pragma unroll
+        for (int dd = 1; dd < 64; dd <<= 1) {
+            const int t = __shfl_up(off, dd, 64);
+            if (lane >= dd) off += t;
+        }
+        const int total = __shfl(off, 63, 64);
+        off -= held;
+        if (total <= 256) {
+            uint64_t *wb = s_sel[threadIdx.x >> 6];
+            for (int c = lane; c < N; c += 64) {
+                const uint64_t v = ((uint64_t)row[c] << 32) | (uint32_t)c;
+                if (v <= bnd) wb[off++] = v;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            uint64_t v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = lane + 64 * u < total ? wb[lane + 64 * u] : kKey64None;
+            wave_sort_u64<4, 2>(v, lane);
+            if (lane < K2c) { // (K2c <= 64: the K2c smallest are v[0] of lanes 0 .. K2c - 1)
+                cidx[q * K2c + lane] = v[0] == kKey64None ? -1 : (int32_t)(uint32_t)(v[0] & 0xFFFFFFFFull);
+                capx[q * K2c + lane] = key_f32((uint32_t)(v[0] >> 32));
+            }
+            return;
+        }
+    }
     uint64_t L[16], C[16];
     uint64_t dropped = kKey64None;
     for (int c0 = 0; c0 < N; c0 += 16 * 64) {
