@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void l2_fallback_kernel(const float *__restric
 // 64 x 64 as 2 x 2 MFMA tiles) from operands staged in LDS; rows are clamped on load and masked
 // on store.
 template <int PREC>
-__global__ __launch_bounds__(256) void l2_matrix_keys_kernel(const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
+__global__ __launch_bounds__(256, 3) void l2_matrix_keys_kernel(const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
                                                              const float *__restrict__ qn, int M,
                                                              const __bf16 *__restrict__ Xh, const __bf16 *__restrict__ Xl,
                                                              const float *__restrict__ xn, int N, int dp,
@@ -128,7 +128,8 @@ __global__ __launch_bounds__(256) void l2_matrix_keys_kernel(const __bf16 *__res
     constexpr int NPL = PREC == 3 ? 2 : 1; // planes: hi (and lo)
     constexpr int KC = 32;                  // columns per staged chunk
     constexpr int RS = KC + 8;              // LDS row stride (bf16): 80 bytes, conflict-free b128 reads
-    __shared__ __align__(16) __bf16 sq[2][NPL][128 * RS], sx[2][NPL][128 * RS]; // two chunk buffers
+    constexpr int NB = 1;                   // chunk buffers (1: 40 KB of LDS, three workgroups per CU)
+    __shared__ __align__(16) __bf16 sq[NB][NPL][128 * RS], sx[NB][NPL][128 * RS];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5, col = lane & 31;
     // a 1-D grid of 8 x ncb x ceil(nqb / 8) workgroups: workgroup b runs on XCD b % 8, and every
     // column block of query block qb goes to XCD qb % 8, so a query block is fetched into one L2
@@ -139,8 +140,9 @@ __global__ __launch_bounds__(256) void l2_matrix_keys_kernel(const __bf16 *__res
     // Chunks of KC columns of the workgroup's 128 query rows and 128 rows are staged in LDS by
     // coalesced loads (four lanes read a row's 64 contiguous bytes: whole L1 accesses, and each
     // row is fetched once per workgroup, not once per wave that multiplies it); the next chunk's
-    // loads are in registers while the waves multiply the current one from LDS, and go to the
-    // other buffer after the products (one barrier per chunk). Staging role of
+    // loads are in registers while the waves multiply the current one from LDS and are stored
+    // after the products. One buffer (40 KB) and at most 168 registers let three workgroups share
+    // a CU, so a 10,000 x 1,024 product's 632 workgroups run in one round on 256 CUs. Staging role of
     // thread t: rows (t >> 2) and (t >> 2) + 64, 16-byte column t & 3 of the chunk; columns at or
     // past dp (the last chunk when dp % 32 == 16) are staged as zeros.
     const int sr = tid >> 2, sc = tid & 3;
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(256) void l2_matrix_keys_kernel(const __bf16 *__res
     sstore(0);
     __syncthreads();
     for (int c = 0; c < nch; ++c) {
-        const int bf = c & 1;
+        const int bf = NB == 1 ? 0 : c & 1;
         if (c + 1 < nch) gload(KC * (c + 1)); // (in flight during the products)
 #pragma unroll
         for (int ks = 0; ks < KC / 16; ++ks) {
@@ -208,9 +210,10 @@ __global__ __launch_bounds__(256) void l2_matrix_keys_kernel(const __bf16 *__res
                 }
         }
         if (c + 1 < nch) {
-            // the other buffer was last read for chunk c - 1, which every wave finished before
-            // the previous barrier
-            sstore(bf ^ 1);
+            // two buffers: the other one was last read for chunk c - 1, which every wave finished
+            // before the previous barrier; one buffer: wait until every wave is done with chunk c
+            if (NB == 1) __syncthreads();
+            sstore(NB == 1 ? 0 : bf ^ 1);
             __syncthreads();
         }
     }
