@@ -9,7 +9,8 @@
 //      stages tiles in LDS; the same for indices of at most 64 queries, whose waves take
 //      different slices instead of padding queries; the rest), longest tiles first; each tile
 //      runs the same device body as the single search,
-//   3. one merge, one rerank (with the completeness proof) and one fallback launch over all queries,
+//   3. merge launches (one per private-list size), one rerank (with the completeness proof) and one
+//      fallback launch over all queries,
 // captured once in a HIP graph that every run replays. Slice length, persistent grid size and
 // tile order are the plan's schedule; pn_batch_tune measures a set of schedules on the device and
 // keeps the fastest (optionally persisted in a text table keyed by the batch's shape).
@@ -21,6 +22,7 @@
 #include <numeric>
 #include <sstream>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -153,25 +155,23 @@ __global__ __launch_bounds__(256) void batch_scan_kernel(const BatchEntry *__res
     }
 }
 
+// One launch per private-list size K (the kernel's registers grow with K, and with them its
+// occupancy falls): gids lists the batch's global query numbers whose merge takes lists of K.
+template <int K>
 __global__ __launch_bounds__(256) void batch_merge_kernel(const BatchEntry *__restrict__ E,
-                                                          const int64_t *__restrict__ q_base, int ne, int64_t nq) {
+                                                          const int64_t *__restrict__ q_base, int ne,
+                                                          const int32_t *__restrict__ gids, int64_t count) {
     __shared__ uint64_t wbuf_all[4][256];
     uint64_t *wbuf = wbuf_all[threadIdx.x >> 6];
-    const int64_t g = xcd_contiguous_block() * 4 + (threadIdx.x >> 6);
-    if (g >= nq) return;
+    const int64_t i = xcd_contiguous_block() * 4 + (threadIdx.x >> 6);
+    if (i >= count) return;
+    const int64_t g = gids[i];
     const BatchEntry &e = E[find_entry(q_base, ne, g)];
-    const int64_t q = g - e.q_base;
-    // a lane that merges one slice holds at most KL keys; with more slices per lane it needs K2c
-    const int cap = e.S <= 64 ? e.KL : e.K2;
-    if (cap <= 8)
-        merge_wave_query<8>(e.part, e.S, e.KL, e.nq_pad, q, e.K2c, OUT_F32, e.cidx, e.capx, wbuf);
-    else if (cap <= 16)
-        merge_wave_query<16>(e.part, e.S, e.KL, e.nq_pad, q, e.K2c, OUT_F32, e.cidx, e.capx, wbuf);
-    else if (cap <= 32)
-        merge_wave_query<32>(e.part, e.S, e.KL, e.nq_pad, q, e.K2c, OUT_F32, e.cidx, e.capx, wbuf);
-    else
-        merge_wave_query<64>(e.part, e.S, e.KL, e.nq_pad, q, e.K2c, OUT_F32, e.cidx, e.capx, wbuf);
+    merge_wave_query<K>(e.part, e.S, e.KL, e.nq_pad, g - e.q_base, e.K2c, OUT_F32, e.cidx, e.capx, wbuf);
 }
+
+// a lane that merges one slice holds at most KL keys; with more slices per lane it needs K2c
+static int merge_cap(const BatchEntry &e) { return e.S <= 64 ? e.KL : e.K2; }
 
 __global__ __launch_bounds__(256) void batch_rerank_kernel(const BatchEntry *__restrict__ E,
                                                            const int64_t *__restrict__ q_base, int ne, int64_t nq) {
@@ -204,7 +204,8 @@ struct pn_batch {
     std::vector<float *> od;
     pn::BatchCfg cfg;
     int64_t ntiles = 0, ncls[3] = {0, 0, 0}, rows = 0, queries = 0;
-    pn::DevBuf entries, row_base, q_base, tiles, counter, scratch;
+    pn::DevBuf entries, row_base, q_base, tiles, counter, scratch, mgids;
+    int64_t mcount[4] = {0, 0, 0, 0}; // merge queries per list size 8, 16, 32, 64
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -322,7 +323,21 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     for (int c = 0; c < 3; ++c) b->ncls[c] = (int64_t)tc[c].size();
     b->rows = rows;
     b->queries = queries;
+    // merge query lists by private-list size
+    std::vector<int32_t> mg[4];
+    for (int i = 0; i < ne; ++i) {
+        const int cap = merge_cap(E[i]);
+        const int c = cap <= 8 ? 0 : (cap <= 16 ? 1 : (cap <= 32 ? 2 : 3));
+        for (int64_t q = 0; q < E[i].nq; ++q) mg[c].push_back((int32_t)(E[i].q_base + q));
+    }
+    std::vector<int32_t> mall;
+    for (int c = 0; c < 4; ++c) {
+        b->mcount[c] = (int64_t)mg[c].size();
+        mall.insert(mall.end(), mg[c].begin(), mg[c].end());
+    }
     hipStream_t st = b->cap_stream;
+    PN_HIP(hipMemcpyAsync(b->mgids.ensure(std::max<size_t>(1, mall.size()) * 4), mall.data(), mall.size() * 4,
+                          hipMemcpyHostToDevice, st));
     PN_HIP(hipMemcpyAsync(b->entries.ensure(ne * sizeof(BatchEntry)), E.data(), ne * sizeof(BatchEntry),
                           hipMemcpyHostToDevice, st));
     PN_HIP(hipMemcpyAsync(b->row_base.ensure(ne * 8), rbase.data(), ne * 8, hipMemcpyHostToDevice, st));
@@ -362,8 +377,21 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     if (nc[2] > 0)
         hipLaunchKernelGGL(batch_scan_kernel<2>, dim3(grid_wd), dim3(256), 0, st, dE, dt + nc[0] + nc[1], nc[2],
                            ctr + 2);
-    hipLaunchKernelGGL(batch_merge_kernel, dim3(xcd_grid(ceil_div(queries, (int64_t)4))), dim3(256), 0, st, dE, dQ, ne,
-                       queries);
+    {
+        const int32_t *mg = b->mgids.as<int32_t>();
+        int64_t o = 0;
+        auto merge = [&](auto kc, int c) {
+            constexpr int KV = decltype(kc)::value;
+            if (b->mcount[c] > 0)
+                hipLaunchKernelGGL((batch_merge_kernel<KV>), dim3(xcd_grid(ceil_div(b->mcount[c], (int64_t)4))),
+                                   dim3(256), 0, st, dE, dQ, ne, mg + o, b->mcount[c]);
+            o += b->mcount[c];
+        };
+        merge(std::integral_constant<int, 8>{}, 0);
+        merge(std::integral_constant<int, 16>{}, 1);
+        merge(std::integral_constant<int, 32>{}, 2);
+        merge(std::integral_constant<int, 64>{}, 3);
+    }
     hipLaunchKernelGGL(batch_rerank_kernel, dim3((unsigned)ceil_div(queries, (int64_t)4)), dim3(256), 0, st, dE, dQ,
                        ne, queries);
     hipLaunchKernelGGL(batch_fallback_kernel, dim3((unsigned)queries), dim3(256), 0, st, dE, dQ, ne);
