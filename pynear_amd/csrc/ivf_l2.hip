@@ -594,29 +594,36 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
             for (int u = 0; u < 4; ++u) qv[t][u] = e0 + u < d ? qrow[e0 + u] : 0.f;
         }
     }
-    for (int p = p_lo; p < p_lo + npw; ++p) {
-        const int c = probes[q * np + p];
-        if (c < 0) continue; // (wave-uniform)
-        const int64_t sl = pslot[q * np + p];
-        const float *cc = lcent + (int64_t)c * d;
+    // probes in chunks of 64: lane j reads probe j's cluster and slot once (no dependent loads per
+    // probe), and the next probe's centroid row is loaded while this one's residual is made (two
+    // register sets rotated by name)
+    typedef float Row[NV][4];
+    auto load_c = [&](int c, Row &cv) __attribute__((always_inline)) {
+        const float *cc = lcent + (int64_t)max(c, 0) * d;
+#pragma unroll
+        for (int t = 0; t < NV; ++t) {
+            const int e0 = 4 * lane + 256 * t;
+            if (vec) {
+                const float4 c4 = c >= 0 && e0 < d ? *(const float4 *)(cc + e0) : make_float4(0.f, 0.f, 0.f, 0.f);
+                cv[t][0] = c4.x, cv[t][1] = c4.y, cv[t][2] = c4.z, cv[t][3] = c4.w;
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) cv[t][u] = c >= 0 && e0 + u < d ? cc[e0 + u] : 0.f;
+            }
+        }
+    };
+    auto emit = [&](int c, int64_t sl, const Row &cv) __attribute__((always_inline)) {
+        if (c < 0) return; // (wave-uniform)
         uint8_t *out = R8 + sl * dp;
         float ss = 0.f;
 #pragma unroll
         for (int t = 0; t < NV; ++t) {
             const int e0 = 4 * lane + 256 * t;
             if (e0 < dp) {
-                float cv[4];
-                if (vec) {
-                    const float4 c4 = e0 < d ? *(const float4 *)(cc + e0) : make_float4(0.f, 0.f, 0.f, 0.f);
-                    cv[0] = c4.x, cv[1] = c4.y, cv[2] = c4.z, cv[3] = c4.w;
-                } else {
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) cv[u] = e0 + u < d ? cc[e0 + u] : 0.f;
-                }
                 float v[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const float r = e0 + u < d ? qv[t][u] - cv[u] : 0.f;
+                    const float r = e0 + u < d ? qv[t][u] - cv[t][u] : 0.f;
                     ss = fmaf(r, r, ss);
                     v[u] = fminf(fmaxf(r * scale, -448.f), 448.f);
                 }
@@ -629,6 +636,21 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
         if (lane == 0) Rn[sl] = ss;
+    };
+    for (int pb = p_lo; pb < p_lo + npw; pb += 64) {
+        const int nj = min(64, p_lo + npw - pb);
+        const int myc = lane < nj ? probes[q * np + pb + lane] : -1;
+        const int64_t mysl = lane < nj ? (int64_t)pslot[q * np + pb + lane] : 0;
+        Row ca, cb;
+        load_c(__shfl(myc, 0, 64), ca);
+        for (int j = 0; j < nj; j += 2) {
+            load_c(__shfl(myc, min(j + 1, nj - 1), 64), cb);
+            emit(__shfl(myc, j, 64), __shfl(mysl, j, 64), ca);
+            if (j + 1 < nj) {
+                load_c(__shfl(myc, min(j + 2, nj - 1), 64), ca);
+                emit(__shfl(myc, j + 1, 64), __shfl(mysl, j + 1, 64), cb);
+            }
+        }
     }
 }
 
