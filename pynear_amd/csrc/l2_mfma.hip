@@ -249,7 +249,47 @@ __global__ __launch_bounds__(256) void l2_matrix_select_kernel(const uint32_t *_
     // it are packed into LDS and sorted, and the first K2c are the result. capx[K2c - 1] is then
     // the K2c-th smallest key, a lower bound of every key left out. More than 256 keys at or below
     // the bound: the chunked sort below.
-    {
+    if (N <= 16 * 64) { // the row's keys stay in registers (16 per lane) for the three passes
+        uint32_t kr[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) kr[j] = lane + 64 * j < N ? row[lane + 64 * j] : kKey32None;
+        auto key = [&](int j) { return ((uint64_t)kr[j] << 32) | (uint32_t)(lane + 64 * j); };
+        uint64_t mn = kKey64None;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) mn = lane + 64 * j < N && key(j) < mn ? key(j) : mn;
+        uint64_t sm[1] = {mn};
+        wave_sort_u64<1, 2>(sm, lane);
+        const uint64_t bnd = __shfl(sm[0], K2c - 1, 64);
+        int held = 0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) held += lane + 64 * j < N && key(j) <= bnd ? 1 : 0;
+        int off = held;
+#pragma unroll
+        for (int dd = 1; dd < 64; dd <<= 1) {
+            const int t = __shfl_up(off, dd, 64);
+            if (lane >= dd) off += t;
+        }
+        const int total = __shfl(off, 63, 64);
+        off -= held;
+        if (total <= 256) {
+            uint64_t *wb = s_sel[threadIdx.x >> 6];
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (lane + 64 * j < N && key(j) <= bnd) wb[off++] = key(j);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            uint64_t v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = lane + 64 * u < total ? wb[lane + 64 * u] : kKey64None;
+            wave_sort_u64<4, 2>(v, lane);
+            if (lane < K2c) {
+                cidx[q * K2c + lane] = v[0] == kKey64None ? -1 : (int32_t)(uint32_t)(v[0] & 0xFFFFFFFFull);
+                capx[q * K2c + lane] = key_f32((uint32_t)(v[0] >> 32));
+            }
+            return;
+        }
+    } else {
         uint64_t mn = kKey64None;
         for (int c = lane; c < N; c += 64) {
             const uint64_t v = ((uint64_t)row[c] << 32) | (uint32_t)c;

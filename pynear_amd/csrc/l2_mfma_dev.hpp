@@ -15,13 +15,31 @@ __device__ __forceinline__ void split_row(const float *__restrict__ X, int64_t n
                                           int64_t row) {
     const int lane = threadIdx.x & 63;
     double acc = 0.0;
-    for (int e = lane; e < dp; e += 64) {
-        const float x = e < d && row < n ? X[row * stride + e] : 0.f;
-        const __bf16 h = (__bf16)x;
-        const __bf16 l = (__bf16)(x - (float)h);
-        H[row * dp + e] = h;
-        Lo[row * dp + e] = l;
-        acc += (double)x * (double)x;
+    if ((d & 3) == 0 && (stride & 3) == 0 && ((uintptr_t)X & 15) == 0) {
+        // four consecutive columns per lane: 16-byte row loads, 8-byte stores of each plane (dp is a
+        // multiple of 16; the norm's sum runs in another order, which fp64 makes immaterial)
+        for (int e = 4 * lane; e < dp; e += 256) {
+            const float4 x4 = e < d && row < n ? *(const float4 *)(X + row * stride + e) : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float xs[4] = {x4.x, x4.y, x4.z, x4.w};
+            union { __bf16 b[4]; uint2 u; } hh, ll;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                hh.b[u] = (__bf16)xs[u];
+                ll.b[u] = (__bf16)(xs[u] - (float)hh.b[u]);
+                acc += (double)xs[u] * (double)xs[u];
+            }
+            *(uint2 *)(H + row * dp + e) = hh.u;
+            *(uint2 *)(Lo + row * dp + e) = ll.u;
+        }
+    } else {
+        for (int e = lane; e < dp; e += 64) {
+            const float x = e < d && row < n ? X[row * stride + e] : 0.f;
+            const __bf16 h = (__bf16)x;
+            const __bf16 l = (__bf16)(x - (float)h);
+            H[row * dp + e] = h;
+            Lo[row * dp + e] = l;
+            acc += (double)x * (double)x;
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
