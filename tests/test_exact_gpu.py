@@ -188,6 +188,34 @@ def test_merge_and_slicing_variants(wave, wpc):
     np.testing.assert_array_equal(gi, oi)
 
 
+@pytest.mark.parametrize("case", ["hamming", "l2_mfma", "l2_valu"])
+def test_all_rows_equal_ties_by_index(case):
+    """Every row equal: every distance ties, so the k results are rows 0 .. k - 1. With tens of
+    thousands of tied keys the wave merge's sampled bound admits thousands of them, which takes
+    its round-by-round path (more than 256 held keys) instead of the single sort."""
+    rng = np.random.default_rng(17)
+    k = 32 if case != "hamming" else 16
+    if case == "hamming":
+        X = np.tile(rng.integers(0, 256, (1, 32), dtype=np.uint8), (50_000, 1))
+        Q = rng.integers(0, 256, (70, 32), dtype=np.uint8)
+        h = N.ExactHandle(N.METRIC_HAMMING)
+    else:
+        X = np.tile(rng.standard_normal((1, 32)).astype(np.float32), (30_000, 1))
+        Q = rng.standard_normal((70, 32)).astype(np.float32)
+        h = N.ExactHandle(N.METRIC_L2)
+    h.set(X)
+    with _Tune(l2__mfma=0 if case == "l2_valu" else 1):
+        gi, gd = h.search(Q, k)
+    assert (gi == np.arange(k)[None, :]).all()
+    assert (gd == gd[:, :1]).all()
+    if case == "hamming":
+        oi, od = oracle.knn_hamming(X, Q, k)
+    else:
+        oi, od = oracle.knn_float(N.METRIC_L2, X, Q, k)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+
+
 def test_pairwise_distances_bit_exact():
     rng = np.random.default_rng(9)
     for d in (1, 3, 8, 13, 32, 77):
