@@ -83,12 +83,12 @@ template <int KL>
 __device__ __forceinline__ void batch_tile(uint64_t *lds_raw, const BatchEntry &e, int64_t qb, int64_t s) {
     auto lds = reinterpret_cast<uint64_t (*)[32][KL]>(lds_raw);
     switch (e.dp) {
-    case 16: scan_small<KL, 3, 1>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s, e.G); break;
-    case 32: scan_small<KL, 3, 2>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s, e.G); break;
-    case 48: scan_small<KL, 3, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s, e.G); break;
-    case 64: scan_small<KL, 3, 4>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s, e.G); break;
+    case 16: scan_small<KL, 3, 1>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G); break;
+    case 32: scan_small<KL, 3, 2>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G); break;
+    case 48: scan_small<KL, 3, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G); break;
+    case 64: scan_small<KL, 3, 4>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G); break;
     default:
-        scan_generic<KL, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.dp, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
+        scan_generic<KL, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.dp, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s);
     }
 }
 
@@ -98,11 +98,11 @@ __device__ __forceinline__ void batch_tile_wide(uint64_t *lds_raw, __bf16 *stage
                                                 int64_t s) {
     auto lds = reinterpret_cast<uint64_t (*)[32][KL]>(lds_raw);
     if (e.dp == 128)
-        scan_wide<KL, 3, 2>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
+        scan_wide<KL, 3, 2>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s);
     else if (e.dp == 192)
-        scan_wide<KL, 3, 3>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
+        scan_wide<KL, 3, 3>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s);
     else
-        scan_wide<KL, 3, 4>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s);
+        scan_wide<KL, 3, 4>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s);
 }
 
 // Wide-row tiles of indices with at most 64 queries: the waves scan different slices.
@@ -110,11 +110,11 @@ template <int KL>
 __device__ __forceinline__ void batch_tile_wide_direct(uint64_t *lds_raw, const BatchEntry &e, int64_t qb, int64_t s) {
     auto lds = reinterpret_cast<uint64_t (*)[32][KL]>(lds_raw);
     if (e.dp == 128)
-        scan_wide_direct<KL, 3, 2>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s, e.G);
+        scan_wide_direct<KL, 3, 2>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G);
     else if (e.dp == 192)
-        scan_wide_direct<KL, 3, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s, e.G);
+        scan_wide_direct<KL, 3, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G);
     else
-        scan_wide_direct<KL, 3, 4>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.nq_pad, qb, s, e.G);
+        scan_wide_direct<KL, 3, 4>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G);
 }
 
 static bool wide_dp(int dp) { return dp == 128 || dp == 192 || dp == 256; }
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void batch_merge_kernel(const BatchEntry *__re
     if (i >= count) return;
     const int64_t g = gids[i];
     const BatchEntry &e = E[find_entry(q_base, ne, g)];
-    merge_wave_query<K>(e.part, e.S, e.KL, e.nq_pad, g - e.q_base, e.K2c, OUT_F32, e.cidx, e.capx, wbuf);
+    merge_wave_query<K>(e.part, e.S, e.KL, e.S * e.KL, 1, g - e.q_base, e.K2c, OUT_F32, e.cidx, e.capx, wbuf);
 }
 
 // a lane that merges one slice holds at most KL keys; with more slices per lane it needs K2c
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void batch_rerank_kernel(const BatchEntry *__r
     if (g >= nq) return;
     const BatchEntry &e = E[find_entry(q_base, ne, g)];
     rerank_query(e.X, e.xstride, e.d, e.Q, e.qstride, e.qn, e.xn_max, e.cidx, e.capx, e.K2c, e.part, e.S, e.KL,
-                 e.nq_pad, e.kk, e.n, e.eps_coef, e.out_idx, e.out_dist, e.flags, g - e.q_base);
+                 e.S * e.KL, e.kk, e.n, e.eps_coef, e.out_idx, e.out_dist, e.flags, g - e.q_base);
 }
 
 __global__ __launch_bounds__(256) void batch_fallback_kernel(const BatchEntry *__restrict__ E,

@@ -43,8 +43,9 @@ void merge_lists(const uint64_t *part, int64_t slices, int K, int64_t nq, int64_
 
 // Wave-per-query merge regardless of the tuning table: per-slice lists of klist keys, per-lane
 // capacity cap (1, 2, 4, ..., 64). Exact when cap >= kk (or cap >= klist with one slice per lane).
-void merge_lists_wave(const uint64_t *part, int64_t slices, int klist, int cap, int64_t nq, int64_t nq_pad, int kk,
-                      int kind, int32_t *out_idx, void *out_dist, hipStream_t st);
+// Entry j of slice s's list of query q is part[q qs + (s klist + j) es].
+void merge_lists_wave(const uint64_t *part, int64_t slices, int klist, int cap, int64_t nq, int64_t qs, int64_t es,
+                      int kk, int kind, int32_t *out_idx, void *out_dist, hipStream_t st);
 
 // Exact k-NN through a full distance matrix and a segmented radix sort; any k, any width.
 // metric: PN_METRIC_*; rows/queries padded to `row_elems` (words for Hamming, floats otherwise).

@@ -59,9 +59,9 @@ __global__ __launch_bounds__(256) void l2_mfma_scan_kernel(const __bf16 *__restr
                                                            const float *__restrict__ xn, int64_t n, int dp,
                                                            const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
                                                            const float *__restrict__ qn, int64_t rows_per_slice,
-                                                           uint64_t *__restrict__ part, int64_t nq_pad, bool slice_major) {
+                                                           uint64_t *__restrict__ part, int64_t qsl, bool slice_major) {
     __shared__ uint64_t lds[4][32][KL];
-    scan_generic<KL, PREC>(lds, Xh, Xl, xn, n, dp, Qh, Ql, qn, rows_per_slice, part, nq_pad,
+    scan_generic<KL, PREC>(lds, Xh, Xl, xn, n, dp, Qh, Ql, qn, rows_per_slice, part, qsl,
                            slice_major ? blockIdx.y : blockIdx.x, slice_major ? blockIdx.x : blockIdx.y);
 }
 
@@ -69,11 +69,11 @@ template <int KL, int PREC, int NS>
 __global__ __launch_bounds__(256) void l2_mfma_scan_small_kernel(
     const __bf16 *__restrict__ Xh, const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
     const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql, const float *__restrict__ qn,
-    int64_t rows_per_slice, uint64_t *__restrict__ part, int64_t nq_pad, bool slice_major) {
+    int64_t rows_per_slice, uint64_t *__restrict__ part, int64_t qsl, bool slice_major) {
     __shared__ uint64_t lds[4][32][KL];
     // slice_major: blockIdx.x is the slice, so workgroup b (on XCD b % 8) of every query block
     // scans the same eighth of the slices, and those rows stay in that XCD's L2
-    scan_small<KL, PREC, NS>(lds, Xh, Xl, xn, n, Qh, Ql, qn, rows_per_slice, part, nq_pad,
+    scan_small<KL, PREC, NS>(lds, Xh, Xl, xn, n, Qh, Ql, qn, rows_per_slice, part, qsl,
                              slice_major ? blockIdx.y : blockIdx.x, slice_major ? blockIdx.x : blockIdx.y);
 }
 
@@ -81,10 +81,10 @@ template <int KL, int PREC, int NC>
 __global__ __launch_bounds__(256) void l2_mfma_scan_wide_kernel(
     const __bf16 *__restrict__ Xh, const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
     const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql, const float *__restrict__ qn,
-    int64_t rows_per_slice, uint64_t *__restrict__ part, int64_t nq_pad, bool slice_major) {
+    int64_t rows_per_slice, uint64_t *__restrict__ part, int64_t qsl, bool slice_major) {
     __shared__ uint64_t lds[4][32][KL];
     __shared__ __align__(16) __bf16 stage[wide_stage_elems<NC, PREC == 3 ? 2 : 1>()];
-    scan_wide<KL, PREC, NC>(lds, stage, Xh, Xl, xn, n, Qh, Ql, qn, rows_per_slice, part, nq_pad,
+    scan_wide<KL, PREC, NC>(lds, stage, Xh, Xl, xn, n, Qh, Ql, qn, rows_per_slice, part, qsl,
                             slice_major ? blockIdx.y : blockIdx.x, slice_major ? blockIdx.x : blockIdx.y);
 }
 
@@ -93,12 +93,12 @@ __global__ __launch_bounds__(256) void l2_rerank_kernel(const float *__restrict_
                                                         const float *__restrict__ qn, const unsigned int *__restrict__ xn_max_bits,
                                                         const int32_t *__restrict__ cidx, const float *__restrict__ capx,
                                                         int K2c, const uint64_t *__restrict__ part, int64_t nlists,
-                                                        int KL, int64_t nq_pad, int64_t nq, int kk, int64_t n,
+                                                        int KL, int64_t qsl, int64_t nq, int kk, int64_t n,
                                                         float eps_coef, int32_t *__restrict__ out_idx,
                                                         float *__restrict__ out_dist, int32_t *__restrict__ flags) {
     const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (q < nq)
-        rerank_query(X, xstride, d, Q, qstride, qn, xn_max_bits, cidx, capx, K2c, part, nlists, KL, nq_pad, kk, n,
+        rerank_query(X, xstride, d, Q, qstride, qn, xn_max_bits, cidx, capx, K2c, part, nlists, KL, qsl, kk, n,
                      eps_coef, out_idx, out_dist, flags, q);
 }
 
@@ -486,7 +486,9 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
         K2 = 8;
         K2c = (int)std::min<int64_t>(K2, n);
     }
+    // per-slice lists, query-major: entry j of slice s's list of query q at (q S + s) KL + j
     uint64_t *part = (uint64_t *)wk.part.ensure((size_t)nlists * KL * nq_pad * 8);
+    const int64_t qsl = nlists * KL;
     const int prec = (int)tune_get("l2.mfma_terms");
     {
         StageScope sc(st, STAGE_SCAN);
@@ -501,19 +503,19 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
             switch (dp) {
             case 16:
                 hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 1>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
-                                   Ql, qn, rps, part, nq_pad, sm);
+                                   Ql, qn, rps, part, qsl, sm);
                 break;
             case 32:
                 hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 2>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
-                                   Ql, qn, rps, part, nq_pad, sm);
+                                   Ql, qn, rps, part, qsl, sm);
                 break;
             case 48:
                 hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 3>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
-                                   Ql, qn, rps, part, nq_pad, sm);
+                                   Ql, qn, rps, part, qsl, sm);
                 break;
             case 64:
                 hipLaunchKernelGGL((l2_mfma_scan_small_kernel<KLV, PV, 4>), grid, dim3(256), 0, st, xh, xl, xnr, n, Qh,
-                                   Ql, qn, rps, part, nq_pad, sm);
+                                   Ql, qn, rps, part, qsl, sm);
                 break;
             case 128:
             case 192:
@@ -521,19 +523,19 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
                 if (wide) {
                     if (dp == 128)
                         hipLaunchKernelGGL((l2_mfma_scan_wide_kernel<KLV, PV, 2>), grid, dim3(256), 0, st, xh, xl, xnr,
-                                           n, Qh, Ql, qn, rps, part, nq_pad, sm);
+                                           n, Qh, Ql, qn, rps, part, qsl, sm);
                     else if (dp == 192)
                         hipLaunchKernelGGL((l2_mfma_scan_wide_kernel<KLV, PV, 3>), grid, dim3(256), 0, st, xh, xl, xnr,
-                                           n, Qh, Ql, qn, rps, part, nq_pad, sm);
+                                           n, Qh, Ql, qn, rps, part, qsl, sm);
                     else
                         hipLaunchKernelGGL((l2_mfma_scan_wide_kernel<KLV, PV, 4>), grid, dim3(256), 0, st, xh, xl, xnr,
-                                           n, Qh, Ql, qn, rps, part, nq_pad, sm);
+                                           n, Qh, Ql, qn, rps, part, qsl, sm);
                     break;
                 }
                 [[fallthrough]];
             default:
                 hipLaunchKernelGGL((l2_mfma_scan_kernel<KLV, PV>), grid, dim3(256), 0, st, xh, xl, xnr, n, dp, Qh, Ql, qn,
-                                   rps, part, nq_pad, sm);
+                                   rps, part, qsl, sm);
             }
         };
         using I4 = std::integral_constant<int, 4>;
@@ -557,13 +559,13 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
     {
         StageScope sc(st, STAGE_SELECT);
         // a lane that merges one slice holds at most KL keys; with more slices per lane it needs K2c
-        merge_lists_wave(part, nlists, KL, nlists <= 64 ? KL : K2, nq, nq_pad, K2c, OUT_F32, cidx, capx, st);
+        merge_lists_wave(part, nlists, KL, nlists <= 64 ? KL : K2, nq, qsl, 1, K2c, OUT_F32, cidx, capx, st);
     }
     {
         StageScope sc(st, STAGE_RERANK);
         const float eps_coef = (prec == 1 ? 0.0078125f : 6.103515625e-05f) + (float)dp * 4.76837158203125e-07f;
         hipLaunchKernelGGL(l2_rerank_kernel, dim3((unsigned)ceil_div(nq, (int64_t)4)), dim3(256), 0, st, X, xstride, d, Q,
-                           qstride, qn, xs.norm_max.as<unsigned int>(), cidx, capx, K2c, part, nlists, KL, nq_pad, nq,
+                           qstride, qn, xs.norm_max.as<unsigned int>(), cidx, capx, K2c, part, nlists, KL, qsl, nq,
                            kk, n, eps_coef, out_idx, out_dist, flags);
         PN_HIP(hipGetLastError());
     }

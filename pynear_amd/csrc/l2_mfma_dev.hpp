@@ -127,11 +127,12 @@ __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4
 }
 
 // Half-wave merge: lanes l and l+32 hold lists of the same query; lane l writes the KL smallest
-// of both to out[j * nq_pad]. Keys a list dropped are >= its last key >= the merged last key, so
+// of both to out[j] (the query's list for this slice, contiguous). Keys a list dropped are >= its
+// last key >= the merged last key, so
 // the merged last key still bounds every dropped candidate from below.
 template <int KL>
 __device__ __forceinline__ void half_merge_out(uint64_t (*lds)[KL], int half, const uint32_t (&Lk)[KL],
-                                               const uint32_t (&Li)[KL], uint64_t *out, int64_t nq_pad) {
+                                               const uint32_t (&Li)[KL], uint64_t *out) {
     if (half == 1) {
 #pragma unroll
         for (int j = 0; j < KL; ++j) lds[0][j] = Lk[j] == kKey32None ? kKey64None : (((uint64_t)Lk[j] << 32) | Li[j]);
@@ -147,7 +148,7 @@ __device__ __forceinline__ void half_merge_out(uint64_t (*lds)[KL], int half, co
                 if (e == pa) va = Lk[e] == kKey32None ? kKey64None : (((uint64_t)Lk[e] << 32) | Li[e]);
             const uint64_t vb = pb < KL ? lds[0][pb] : kKey64None;
             const bool takea = va <= vb;
-            out[(int64_t)j * nq_pad] = takea ? va : vb;
+            out[j] = takea ? va : vb;
             pa += takea ? 1 : 0;
             pb += takea ? 0 : 1;
         }
@@ -159,14 +160,14 @@ __device__ __forceinline__ void scan_generic(uint64_t (*lds)[32][KL], const __bf
                                              const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
                                              int dp, const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
                                              const float *__restrict__ qn, int64_t rows_per_slice,
-                                             uint64_t *__restrict__ part, int64_t nq_pad, int64_t qb, int64_t s) {
+                                             uint64_t *__restrict__ part, int64_t qsl, int64_t qb, int64_t s) {
     // A workgroup owns 128 queries (wave w: queries 32w..32w+31 of the block) and one slice of
     // rows; the four waves walk the same 32-row tiles, so each tile is fetched from L2 once per
     // workgroup and served to the other waves from L1. Row arrays are padded with zero rows to a
     // multiple of 32 and slices start on 32-row boundaries: tile loads need no guard, rows >= r1
     // are masked.
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
-    const int64_t q = qb * 128 + 32 * w + col; // query buffers are padded to nq_pad rows
+    const int64_t q = qb * 128 + 32 * w + col; // query buffers are padded to whole 128-query blocks
     const int64_t r0 = s * rows_per_slice, r1 = min(n, r0 + rows_per_slice);
     const float qnv = qn[q];
     const __bf16 *qh = Qh + q * dp + 8 * half;
@@ -197,7 +198,7 @@ __device__ __forceinline__ void scan_generic(uint64_t (*lds)[32][KL], const __bf
             t + 32 <= r1 ? tile_keys_insert<KL, false, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
                          : tile_keys_insert<KL, true, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
     }
-    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + (s * KL) * nq_pad + q, nq_pad);
+    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL);
 }
 
 // The same scan for dp = 16 NS (NS <= 4): the lane's query fragments stay in registers for the
@@ -209,7 +210,7 @@ __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16
                                            const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
                                            const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
                                            const float *__restrict__ qn, int64_t rows_per_slice,
-                                           uint64_t *__restrict__ part, int64_t nq_pad, int64_t qb, int64_t s,
+                                           uint64_t *__restrict__ part, int64_t qsl, int64_t qb, int64_t s,
                                            int g = 4) {
     constexpr int dp = 16 * NS;
     constexpr int NPL = PREC == 3 ? 2 : 1;
@@ -284,7 +285,7 @@ __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16
             step(i + 2, f2, f1);
         }
     }
-    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + (s * KL) * nq_pad + q, nq_pad);
+    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL);
 }
 
 // The wide scan without LDS staging, for workgroups whose waves scan different slices (blocks of
@@ -296,7 +297,7 @@ __device__ __forceinline__ void scan_wide_direct(uint64_t (*lds)[32][KL], const 
                                                  const __bf16 *__restrict__ Xl, const float *__restrict__ xn,
                                                  int64_t n, const __bf16 *__restrict__ Qh,
                                                  const __bf16 *__restrict__ Ql, const float *__restrict__ qn,
-                                                 int64_t rows_per_slice, uint64_t *__restrict__ part, int64_t nq_pad,
+                                                 int64_t rows_per_slice, uint64_t *__restrict__ part, int64_t qsl,
                                                  int64_t qb, int64_t s, int g) {
     constexpr int dp = 64 * NC;
     constexpr int NPL = PREC == 3 ? 2 : 1;
@@ -362,7 +363,7 @@ __device__ __forceinline__ void scan_wide_direct(uint64_t (*lds)[32][KL], const 
             for (int gi = 0; gi < 4; ++gi) xv[gi] = *(const float4 *)(xn + tn + noff + 8 * gi);
         }
     }
-    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + (s * KL) * nq_pad + q, nq_pad);
+    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL);
 }
 
 // LDS staging area of scan_wide (bf16 elements): two buffers of NPL planes of 32 rows of
@@ -381,7 +382,7 @@ __device__ __forceinline__ void scan_wide(uint64_t (*lds)[32][KL], __bf16 *stage
                                           const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
                                           const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
                                           const float *__restrict__ qn, int64_t rows_per_slice,
-                                          uint64_t *__restrict__ part, int64_t nq_pad, int64_t qb, int64_t s) {
+                                          uint64_t *__restrict__ part, int64_t qsl, int64_t qb, int64_t s) {
     constexpr int dp = 64 * NC;
     constexpr int NPL = PREC == 3 ? 2 : 1;
     constexpr int RS = dp + 8;          // LDS row stride (bf16)
@@ -463,7 +464,7 @@ __device__ __forceinline__ void scan_wide(uint64_t (*lds)[32][KL], __bf16 *stage
             __syncthreads();
         }
     }
-    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + (s * KL) * nq_pad + q, nq_pad);
+    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL);
 }
 
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
@@ -480,14 +481,14 @@ __device__ __forceinline__ void rerank_query(const float *__restrict__ X, int64_
                                              const float *__restrict__ Q, int64_t qstride,
                                              const float *__restrict__ qn, const unsigned int *__restrict__ xn_max_bits,
                                              const int32_t *__restrict__ cidx, const float *__restrict__ capx, int K2c,
-                                             const uint64_t *__restrict__ part, int64_t nlists, int KL, int64_t nq_pad,
+                                             const uint64_t *__restrict__ part, int64_t nlists, int KL, int64_t qsl,
                                              int kk, int64_t n, float eps_coef, int32_t *__restrict__ out_idx,
                                              float *__restrict__ out_dist, int32_t *__restrict__ flags, int64_t q) {
     const int lane = threadIdx.x & 63;
     // the smallest approximation any full list dropped: its last key with the row bits cleared
     uint32_t bmin = kKey32None;
     for (int64_t l = lane; l < nlists; l += 64) {
-        const uint32_t last = (uint32_t)(part[(l * KL + KL - 1) * nq_pad + q] >> 32);
+        const uint32_t last = (uint32_t)(part[q * qsl + l * KL + KL - 1] >> 32);
         bmin = min(bmin, last == kKey32None ? kKey32None : (last & ~31u));
     }
 #pragma unroll

@@ -8,15 +8,19 @@
 namespace pn {
 
 // One wave per query q: lane l merges the per-slice lists (klist keys each) of slices l, l+64, ...
+// Entry j of slice s's list of query q is part[q qs + (s klist + j) es] (query-major: qs =
+// slices klist, es = 1; list-major: qs = 1, es = the padded query count).
 // into a private sorted list of capacity K, then the wave orders what the lanes hold and writes
 // the kk smallest keys (keys are unique since indices are). The result is the exact kk smallest
 // of all lists when K >= min(kk, klist * slices per lane).
 //
-// A sampled bound first limits what the lanes insert: lane l takes the minimum of up to four
-// entries at grid positions l + 64 u (slice g % slices, entry g / slices: the first entries of
-// every slice), and the wave maximum of these minima has at least 64 distinct entries at or below
-// it (one per lane), so the kk <= 64 smallest are all at or below it. A lane without a sample
-// leaves the bound open (the empty key is the largest).
+// A sampled bound first limits what the lanes insert: lane l takes up to four entries at grid
+// positions l + 64 u (slice g % slices, entry g / slices: the first entries of every slice). For
+// kk <= 16 the bound is the wave maximum of the lanes' minima, which has at least 64 distinct
+// entries at or below it (one per lane); for larger kk it is the kk-th smallest sample (the 256
+// sorted across the wave), with the kk smallest samples at or below it. Either way the kk <= 64
+// smallest entries are all at or below it. Missing samples are empty keys (the largest): a wave
+// with too few samples leaves the bound open.
 //
 // Ordering: when the lanes hold at most 256 keys together, they are packed into the wave's 256
 // entries of LDS (wbuf) and sorted by one bitonic network across the wave; otherwise kk rounds of
@@ -27,7 +31,7 @@ namespace pn {
 // whose high word exceeds the limit's.
 template <int K>
 __device__ __forceinline__ void merge_wave_query(const uint64_t *__restrict__ part, int64_t slices, int klist,
-                                                 int64_t nq_pad, int64_t q, int kk, int kind,
+                                                 int64_t qs, int64_t es, int64_t q, int kk, int kind,
                                                  int32_t *__restrict__ out_idx, void *__restrict__ out_dist,
                                                  uint64_t *wbuf) {
     const int lane = threadIdx.x & 63;
@@ -56,27 +60,34 @@ __device__ __forceinline__ void merge_wave_query(const uint64_t *__restrict__ pa
             const int g = lane + 64 * u;
             const int gs = g < G ? g : 0;
             const int64_t sl = gs % ns, p = gs / ns;
-            const uint64_t v = part[(sl * klist + p) * nq_pad + q];
+            const uint64_t v = part[q * qs + (sl * klist + p) * es];
             smp[u] = g < G ? v : kKey64None;
         }
-        uint64_t mn = kKey64None;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint64_t v = transform(smp[u]);
-            mn = v < mn ? v : mn;
-        }
-        bnd = mn;
+        for (int u = 0; u < 4; ++u) smp[u] = transform(smp[u]);
+        if (kk > 16) {
+            // the kk-th smallest of the 256 samples (one wave sort): at least kk entries lie at or
+            // below it (the kk smallest samples), and it is much tighter than the largest lane
+            // minimum when kk is large
+            wave_sort_u64<4, 2>(smp, lane);
+            bnd = __shfl(smp[0], kk - 1, 64); // (kk <= 64: rank kk - 1 is v[0] of lane kk - 1)
+        } else {
+            uint64_t mn = kKey64None;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const uint64_t o = __shfl_xor(bnd, off, 64);
-            bnd = o > bnd ? o : bnd;
+            for (int u = 0; u < 4; ++u) mn = smp[u] < mn ? smp[u] : mn;
+            bnd = mn;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const uint64_t o = __shfl_xor(bnd, off, 64);
+                bnd = o > bnd ? o : bnd;
+            }
         }
     }
     uint64_t L[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) L[j] = kKey64None;
     for (int64_t s = lane; s < slices; s += 64) {
-        const uint64_t *src = part + (s * klist) * nq_pad + q;
+        const uint64_t *src = part + q * qs + (s * klist) * es;
         // a slice's list is sorted: keys are read 8 at a time (independent loads, clamped to the
         // list), and the list is done at its first empty key or the first key past the limit
         // (the bound or the private list's last key)
@@ -84,7 +95,7 @@ __device__ __forceinline__ void merge_wave_query(const uint64_t *__restrict__ pa
         for (int j0 = 0; j0 < klist && !done; j0 += 8) {
             uint64_t kb[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) kb[u] = src[(int64_t)min(j0 + u, klist - 1) * nq_pad];
+            for (int u = 0; u < 8; ++u) kb[u] = src[(int64_t)min(j0 + u, klist - 1) * es];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 if (done || j0 + u >= klist || kb[u] == kKey64None) {

@@ -61,13 +61,13 @@ __global__ __launch_bounds__(256) void merge_lists_kernel(const uint64_t *__rest
 
 template <int K>
 __global__ __launch_bounds__(256) void merge_lists_wave_kernel(const uint64_t *__restrict__ part, int64_t slices,
-                                                               int klist, int64_t nq, int64_t nq_pad, int kk,
+                                                               int klist, int64_t nq, int64_t qs, int64_t es, int kk,
                                                                int kind, int32_t *__restrict__ out_idx,
                                                                void *__restrict__ out_dist) {
     __shared__ uint64_t wbuf[4][256];
     const int64_t q = xcd_contiguous_block() * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (q >= nq) return; // whole wave exits together
-    merge_wave_query<K>(part, slices, klist, nq_pad, q, kk, kind, out_idx, out_dist, wbuf[threadIdx.x >> 6]);
+    merge_wave_query<K>(part, slices, klist, qs, es, q, kk, kind, out_idx, out_dist, wbuf[threadIdx.x >> 6]);
 }
 
 void merge_lists(const uint64_t *part, int64_t slices, int K, int64_t nq, int64_t nq_pad, int kk, int kind,
@@ -78,7 +78,7 @@ void merge_lists(const uint64_t *part, int64_t slices, int K, int64_t nq, int64_
     do {                                                                                               \
         if (wave)                                                                                      \
             hipLaunchKernelGGL((merge_lists_wave_kernel<KV>), grid, dim3(256), 0, st, part, slices, KV, nq,    \
-                               nq_pad, kk, kind, out_idx, out_dist);                                   \
+                               (int64_t)1, nq_pad, kk, kind, out_idx, out_dist);                       \
         else                                                                                           \
             hipLaunchKernelGGL((merge_lists_kernel<KV>), grid, dim3(256), 0, st, part, slices, nq, nq_pad, kk, \
                                kind, out_idx, out_dist);                                               \
@@ -96,11 +96,11 @@ void merge_lists(const uint64_t *part, int64_t slices, int K, int64_t nq, int64_
     PN_HIP(hipGetLastError());
 }
 
-void merge_lists_wave(const uint64_t *part, int64_t slices, int klist, int cap, int64_t nq, int64_t nq_pad, int kk,
-                      int kind, int32_t *out_idx, void *out_dist, hipStream_t st) {
+void merge_lists_wave(const uint64_t *part, int64_t slices, int klist, int cap, int64_t nq, int64_t qs, int64_t es,
+                      int kk, int kind, int32_t *out_idx, void *out_dist, hipStream_t st) {
     dim3 grid(xcd_grid(ceil_div(nq, 4)));
 #define PN_MERGEW(KV)                                                                                        \
-    hipLaunchKernelGGL((merge_lists_wave_kernel<KV>), grid, dim3(256), 0, st, part, slices, klist, nq, nq_pad, kk, \
+    hipLaunchKernelGGL((merge_lists_wave_kernel<KV>), grid, dim3(256), 0, st, part, slices, klist, nq, qs, es, kk, \
                        kind, out_idx, out_dist)
     switch (cap) {
     case 1: PN_MERGEW(1); break;
