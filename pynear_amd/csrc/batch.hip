@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void batch_rerank_kernel(const BatchEntry *__r
     const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (g >= nq) return;
     const BatchEntry &e = E[find_entry(q_base, ne, g)];
-    rerank_query(e.X, e.xstride, e.d, e.Q, e.qstride, e.qn, e.xn_max, e.cidx, e.capx, e.K2c, e.part, e.S, e.KL,
+    rerank_query<false>(e.X, e.xstride, e.d, e.Q, e.qstride, e.qn, e.xn_max, e.cidx, e.capx, e.K2c, e.part, e.S, e.KL,
                  e.S * e.KL, e.kk, e.n, e.eps_coef, e.out_idx, e.out_dist, e.flags, g - e.q_base);
 }
 

@@ -88,6 +88,7 @@ __global__ __launch_bounds__(256) void l2_mfma_scan_wide_kernel(
                             slice_major ? blockIdx.y : blockIdx.x, slice_major ? blockIdx.x : blockIdx.y);
 }
 
+template <bool REG>
 __global__ __launch_bounds__(256) void l2_rerank_kernel(const float *__restrict__ X, int64_t xstride, int64_t d,
                                                         const float *__restrict__ Q, int64_t qstride,
                                                         const float *__restrict__ qn, const unsigned int *__restrict__ xn_max_bits,
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(256) void l2_rerank_kernel(const float *__restrict_
                                                         float *__restrict__ out_dist, int32_t *__restrict__ flags) {
     const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (q < nq)
-        rerank_query(X, xstride, d, Q, qstride, qn, xn_max_bits, cidx, capx, K2c, part, nlists, KL, qsl, kk, n,
+        rerank_query<REG>(X, xstride, d, Q, qstride, qn, xn_max_bits, cidx, capx, K2c, part, nlists, KL, qsl, kk, n,
                      eps_coef, out_idx, out_dist, flags, q);
 }
 
@@ -399,6 +400,13 @@ void l2_split_rows(const float *X, int64_t n, int64_t d, int64_t stride, int dp,
     PN_HIP(hipGetLastError());
 }
 
+// Register-resident query chains in the rerank pay off when few queries leave the machine idle
+// and each wave's latency decides (measured: 4,096 queries are faster without).
+static bool rerank_reg(int64_t nq) {
+    const int64_t t = tune_get("l2.rerank_reg");
+    return t == 1 || (t == 2 && nq <= 1024);
+}
+
 static int kl_for(int kk) { return kk <= 8 ? 8 : (kk <= 12 ? 16 : 32); }
 
 void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const float *Q, int64_t qstride, int64_t nq,
@@ -451,7 +459,7 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
         {
             StageScope sc(st, STAGE_RERANK);
             const float eps_coef = (prec_m == 1 ? 0.0078125f : 6.103515625e-05f) + (float)dp * 4.76837158203125e-07f;
-            hipLaunchKernelGGL(l2_rerank_kernel, dim3((unsigned)ceil_div(nq, (int64_t)4)), dim3(256), 0, st, X, xstride,
+            hipLaunchKernelGGL(rerank_reg(nq) ? l2_rerank_kernel<true> : l2_rerank_kernel<false>, dim3((unsigned)ceil_div(nq, (int64_t)4)), dim3(256), 0, st, X, xstride,
                                d, Q, qstride, qn, xs.norm_max.as<unsigned int>(), cidx, capx, K2c,
                                (const uint64_t *)nullptr, (int64_t)0, KL, nq_pad, nq, kk, n, eps_coef, out_idx,
                                out_dist, flags);
@@ -564,7 +572,7 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
     {
         StageScope sc(st, STAGE_RERANK);
         const float eps_coef = (prec == 1 ? 0.0078125f : 6.103515625e-05f) + (float)dp * 4.76837158203125e-07f;
-        hipLaunchKernelGGL(l2_rerank_kernel, dim3((unsigned)ceil_div(nq, (int64_t)4)), dim3(256), 0, st, X, xstride, d, Q,
+        hipLaunchKernelGGL(rerank_reg(nq) ? l2_rerank_kernel<true> : l2_rerank_kernel<false>, dim3((unsigned)ceil_div(nq, (int64_t)4)), dim3(256), 0, st, X, xstride, d, Q,
                            qstride, qn, xs.norm_max.as<unsigned int>(), cidx, capx, K2c, part, nlists, KL, qsl, nq,
                            kk, n, eps_coef, out_idx, out_dist, flags);
         PN_HIP(hipGetLastError());

@@ -476,7 +476,51 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
     return v;
 }
 
+// Exact squared sums (the reference's FMA chains, as l2_core_group8) of the candidates in lanes
+// 0 .. nkeep-1 for rows of d in [8 ST, 8 ST + 7]: the query's chain elements stay in registers and
+// each 8-lane group runs two candidates a round (16 a round for the wave) with all row loads
+// issued before either chain. A missing candidate reads row 0 (n >= 1) and its sum is dropped.
+// Candidate j's sum ends in lane j.
+template <int ST>
+__device__ __forceinline__ float rerank_cores_st(const float *__restrict__ qrow, const float *__restrict__ X,
+                                                 int64_t xstride, int64_t d, int64_t n, int32_t myidx, int nkeep,
+                                                 int lane) {
+    const int grp = lane >> 3, c8 = lane & 7;
+    float qa[ST];
+#pragma unroll
+    for (int u = 0; u < ST; ++u) qa[u] = qrow[c8 + 8 * u];
+    float core = 0.f;
+    for (int base = 0; base < nkeep; base += 16) {
+        const int ca = base + grp, cb = base + 8 + grp;
+        const int32_t ia = __shfl(myidx, ca & 63, 64), ib = __shfl(myidx, cb & 63, 64);
+        const float *xa = X + (ca < nkeep && ia >= 0 && ia < n ? (int64_t)ia : 0) * xstride;
+        const float *xb = X + (cb < nkeep && ib >= 0 && ib < n ? (int64_t)ib : 0) * xstride;
+        float ta[ST], tb[ST];
+#pragma unroll
+        for (int u = 0; u < ST; ++u) {
+            ta[u] = xa[c8 + 8 * u];
+            tb[u] = xb[c8 + 8 * u];
+        }
+        float aa = 0.f, ab = 0.f;
+#pragma unroll
+        for (int u = 0; u < ST; ++u) {
+            const float t = qa[u] - ta[u], s = qa[u] - tb[u];
+            aa = fmaf(t, t, aa);
+            ab = fmaf(s, s, ab);
+        }
+        const float va = l2_group8_finish(aa, qrow, xa, d, c8);
+        const float vb = l2_group8_finish(ab, qrow, xb, d, c8);
+        const int j = lane - base; // candidate ca's owner for j in 0..7, cb's for 8..15
+        const float ga = __shfl(va, (j & 7) * 8, 64), gb = __shfl(vb, (j & 7) * 8, 64);
+        if (j >= 0 && j < 8) core = ga;
+        else if (j >= 8 && j < 16) core = gb;
+    }
+    return core;
+}
+
 // One wave per query q: exact rerank of K2c candidates, (distance, index) order, completeness check.
+// REG: rows of 8 u + (0..7) floats for u in {4, 8, 16, 24, 32} go through rerank_cores_st.
+template <bool REG>
 __device__ __forceinline__ void rerank_query(const float *__restrict__ X, int64_t xstride, int64_t d,
                                              const float *__restrict__ Q, int64_t qstride,
                                              const float *__restrict__ qn, const unsigned int *__restrict__ xn_max_bits,
@@ -494,36 +538,76 @@ __device__ __forceinline__ void rerank_query(const float *__restrict__ X, int64_
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) bmin = min(bmin, (uint32_t)__shfl_xor((int)bmin, off, 64));
 
-    // exact squared sums of the K2c candidates: 8-lane groups (lane c of a group runs the
-    // reference's FMA chain c, reading 8 consecutive floats with its group), 8 candidates a round;
-    // candidate j's sum ends in lane j
-    const int32_t myidx = lane < K2c ? cidx[q * K2c + lane] : -1;
-    const int grp = lane >> 3, c8 = lane & 7;
+    // Candidates come sorted by approximation (a key's high word holds it with the low 5 bits
+    // replaced by row bits, so the true approximation lies between the word with those bits clear
+    // and with them set). Every approximation is within eps of the exact value, so a candidate
+    // whose approximation exceeds the kk-th candidate's by more than 2 eps has an exact value
+    // above each of the first kk candidates' and cannot be among the kk nearest: its row is not
+    // read. The kept candidates are a prefix.
+    const float sq = sqrtf(qn[q]) + sqrtf(__uint_as_float(*xn_max_bits));
+    const float eps = eps_coef * sq * sq;
+    int32_t myidx = lane < K2c ? cidx[q * K2c + lane] : -1;
+    if (K2c > kk) {
+        const uint32_t hk = f32_key(capx[q * K2c + kk - 1]);
+        if (hk != kKey32None && lane < K2c) {
+            const float upper = key_f32(hk | 31u);
+            const float lower = key_f32(f32_key(capx[q * K2c + lane]) & ~31u);
+            if (lower - upper > 3.f * eps) myidx = -1;
+        }
+    }
+    const uint64_t kept = __ballot(myidx >= 0);
+    const int nkeep = kept ? 64 - __clzll((long long)kept) : 0;
+    const float *qrow = Q + q * qstride;
     float core = 0.f;
-    for (int base = 0; base < K2c; base += 8) {
-        const int cand = base + grp;
-        const int32_t idx = __shfl(myidx, cand & 63, 64);
-        float v = 0.f;
-        if (cand < K2c && idx >= 0 && idx < n) // uniform over the group
-            v = l2_core_group8(Q + q * qstride, X + (int64_t)idx * xstride, d, c8);
-        const bool owner = lane >= base && lane < base + 8;
-        const float got = __shfl(v, owner ? 8 * (lane - base) : 0, 64);
-        if (owner) core = got;
+    switch (REG ? d >> 3 : 0) {
+    case 4: core = rerank_cores_st<4>(qrow, X, xstride, d, n, myidx, nkeep, lane); break;
+    case 8: core = rerank_cores_st<8>(qrow, X, xstride, d, n, myidx, nkeep, lane); break;
+    case 16: core = rerank_cores_st<16>(qrow, X, xstride, d, n, myidx, nkeep, lane); break;
+    case 24: core = rerank_cores_st<24>(qrow, X, xstride, d, n, myidx, nkeep, lane); break;
+    case 32: core = rerank_cores_st<32>(qrow, X, xstride, d, n, myidx, nkeep, lane); break;
+    default: {
+        // 8-lane groups (lane c of a group runs the reference's FMA chain c, reading 8
+        // consecutive floats with its group), 8 candidates a round; candidate j's sum ends in lane j
+        const int grp = lane >> 3, c8 = lane & 7;
+        for (int base = 0; base < nkeep; base += 8) {
+            const int cand = base + grp;
+            const int32_t idx = __shfl(myidx, cand & 63, 64);
+            float v = 0.f;
+            if (cand < K2c && idx >= 0 && idx < n) // uniform over the group
+                v = l2_core_group8(qrow, X + (int64_t)idx * xstride, d, c8);
+            const bool owner = lane >= base && lane < base + 8;
+            const float got = __shfl(v, owner ? 8 * (lane - base) : 0, 64);
+            if (owner) core = got;
+        }
+    }
     }
     uint64_t key = kKey64None;
     if (lane < K2c && myidx >= 0 && myidx < n) key = ((uint64_t)f32_key(sqrtf(core)) << 32) | (uint32_t)myidx;
     float dk_core = 0.f;
-    for (int r = 0; r < kk; ++r) {
-        const uint64_t m = wave_min_u64(key);
-        const unsigned long long own = __ballot(key == m);
-        const int owner = __ffsll((unsigned long long)own) - 1;
-        const float cr = __shfl(core, owner, 64);
-        if (key == m) key = kKey64None;
-        if (lane == 0) {
-            out_idx[q * kk + r] = (int32_t)(uint32_t)(m & 0xFFFFFFFFull);
-            out_dist[q * kk + r] = key_f32((uint32_t)(m >> 32));
+    if (kk >= 4) {
+        // one bitonic sort of the wave's 64 keys: lane r holds the r-th smallest
+        uint64_t v[1] = {key};
+        wave_sort_u64<1, 2>(v, lane);
+        if (lane < kk) {
+            out_idx[q * kk + lane] = (int32_t)(uint32_t)(v[0] & 0xFFFFFFFFull);
+            out_dist[q * kk + lane] = key_f32((uint32_t)(v[0] >> 32));
         }
-        dk_core = cr;
+        const uint64_t mk = __shfl(v[0], kk - 1, 64);
+        const unsigned long long own = __ballot(key == mk);
+        dk_core = __shfl(core, __ffsll(own) - 1, 64);
+    } else {
+        for (int r = 0; r < kk; ++r) {
+            const uint64_t m = wave_min_u64(key);
+            const unsigned long long own = __ballot(key == m);
+            const int owner = __ffsll((unsigned long long)own) - 1;
+            const float cr = __shfl(core, owner, 64);
+            if (key == m) key = kKey64None;
+            if (lane == 0) {
+                out_idx[q * kk + r] = (int32_t)(uint32_t)(m & 0xFFFFFFFFull);
+                out_dist[q * kk + r] = key_f32((uint32_t)(m >> 32));
+            }
+            dk_core = cr;
+        }
     }
     if (lane == 0) {
         int flag = 0;
@@ -531,8 +615,6 @@ __device__ __forceinline__ void rerank_query(const float *__restrict__ X, int64_
             const uint32_t al = __float_as_uint(capx[q * K2c + K2c - 1]);
             const uint32_t bnd = min(bmin, al == kKey32None ? kKey32None : (al & ~31u));
             const float bound = bnd == kKey32None ? INFINITY : key_f32(bnd);
-            const float s = sqrtf(qn[q]) + sqrtf(__uint_as_float(*xn_max_bits));
-            const float eps = eps_coef * s * s;
             flag = !(bound - eps > dk_core);
         }
         flags[q] = flag;

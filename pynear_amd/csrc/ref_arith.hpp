@@ -42,6 +42,29 @@ __device__ inline float l2_core(const float *a, const float *b, int64_t d) {
     return (m[0] + m[1]) + (m[2] + m[3]);
 }
 
+// End of the 8-lane squared sum (below): lane c holds FMA chain c over the full blocks of 8; the
+// fold, the fused tails and the horizontal adds follow. The result is valid in lane 0 of the group.
+__device__ __forceinline__ float l2_group8_finish(float acc, const float *a, const float *b, int64_t d, int c) {
+    const int64_t nb = d / 8 * 8;
+    const float hi = __shfl_down(acc, 4, 8);
+    float m = hi + acc; // lanes 0..3: m[c] = acc[c + 4] + acc[c]
+    if (c < 4) {
+        int64_t i = nb, rem = d - nb;
+        if (rem >= 4) {
+            const float t = a[i + c] - b[i + c];
+            m = fmaf(t, t, m);
+            i += 4;
+            rem -= 4;
+        }
+        if (rem > 0) {
+            const float t = c < rem ? a[i + c] - b[i + c] : 0.f;
+            m = fmaf(t, t, m);
+        }
+    }
+    const float p = m + __shfl_down(m, 1, 8); // lane 0: m0 + m1; lane 2: m2 + m3
+    return p + __shfl_down(p, 2, 8);
+}
+
 // Squared-sum core of dist_l2_f_avx2 with 8 lanes per row: lane c runs FMA chain c (elements
 // c, c+8, ...), then the fold, the fused tails and the horizontal adds; the result is valid in
 // lane 0 of the group.
@@ -66,23 +89,7 @@ __device__ __forceinline__ float l2_core_group8(const float *a, const float *b, 
         const float t = a[i] - b[i];
         acc = fmaf(t, t, acc);
     }
-    const float hi = __shfl_down(acc, 4, 8);
-    float m = hi + acc; // lanes 0..3: m[c] = acc[c + 4] + acc[c]
-    if (c < 4) {
-        int64_t i = nb, rem = d - nb;
-        if (rem >= 4) {
-            const float t = a[i + c] - b[i + c];
-            m = fmaf(t, t, m);
-            i += 4;
-            rem -= 4;
-        }
-        if (rem > 0) {
-            const float t = c < rem ? a[i + c] - b[i + c] : 0.f;
-            m = fmaf(t, t, m);
-        }
-    }
-    const float p = m + __shfl_down(m, 1, 8); // lane 0: m0 + m1; lane 2: m2 + m3
-    return p + __shfl_down(p, 2, 8);
+    return l2_group8_finish(acc, a, b, d, c);
 }
 
 __device__ inline float l1_dist(const float *a, const float *b, int64_t d) {

@@ -260,6 +260,26 @@ def test_l2_mfma_path_bit_exact(d, k):
     np.testing.assert_array_equal(gi, oi)
 
 
+@pytest.mark.parametrize("d", [33, 39, 71, 135, 196, 263])
+@pytest.mark.parametrize("k", [1, 10, 32])
+@pytest.mark.parametrize("reg", [0, 1])
+def test_l2_mfma_rerank_row_tails(d, k, reg):
+    """Rows of 8 u + r floats (r = 1..7) through both rerank gathers (query chains in registers or
+    re-read): the fused 4-wide and masked tails follow the reference's order bit for bit; clustered
+    rows put many candidates near the k-th distance, so few are skipped."""
+    rng = np.random.default_rng(d * 7 + k)
+    C = rng.standard_normal((20, d)).astype(np.float32)
+    X = (C[rng.integers(0, 20, 5000)] + 0.05 * rng.standard_normal((5000, d))).astype(np.float32)
+    Q = (C[rng.integers(0, 20, 97)] + 0.05 * rng.standard_normal((97, d))).astype(np.float32)
+    h = N.ExactHandle(N.METRIC_L2)
+    h.set(X)
+    with _Tune(l2__mfma=1, l2__rerank_reg=reg):
+        gi, gd = h.search(Q, k)
+    oi, od = oracle.knn_float(N.METRIC_L2, X, Q, k)
+    np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(gi, oi)
+
+
 @pytest.mark.parametrize("d", [8, 32, 64, 128])
 def test_l2_fallback_path_exact(d):
     """With single-product bf16 the error bound is wide, queries fall back to the exact scan,
