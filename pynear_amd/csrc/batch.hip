@@ -170,6 +170,18 @@ __global__ __launch_bounds__(256) void batch_merge_kernel(const BatchEntry *__re
     merge_wave_query<K>(e.part, e.S, e.KL, e.S * e.KL, 1, g - e.q_base, e.K2c, OUT_F32, e.cidx, e.capx, wbuf);
 }
 
+// The merge without private lists: one launch for all queries of the batch.
+__global__ __launch_bounds__(256) void batch_merge_lean_kernel(const BatchEntry *__restrict__ E,
+                                                               const int64_t *__restrict__ q_base, int ne,
+                                                               int64_t nq, bool rounds_only) {
+    __shared__ uint64_t wbuf_all[4][256];
+    const int64_t g = xcd_contiguous_block() * 4 + (threadIdx.x >> 6);
+    if (g >= nq) return;
+    const BatchEntry &e = E[find_entry(q_base, ne, g)];
+    merge_wave_lean(e.part, e.S, e.KL, e.S * e.KL, 1, g - e.q_base, e.K2c, OUT_F32, e.cidx, e.capx,
+                    wbuf_all[threadIdx.x >> 6], rounds_only);
+}
+
 // a lane that merges one slice holds at most KL keys; with more slices per lane it needs K2c
 static int merge_cap(const BatchEntry &e) { return e.S <= 64 ? e.KL : e.K2; }
 
@@ -377,7 +389,10 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     if (nc[2] > 0)
         hipLaunchKernelGGL(batch_scan_kernel<2>, dim3(grid_wd), dim3(256), 0, st, dE, dt + nc[0] + nc[1], nc[2],
                            ctr + 2);
-    {
+    if (tune_get("merge.lean") != 0) {
+        hipLaunchKernelGGL(batch_merge_lean_kernel, dim3(xcd_grid(ceil_div(queries, (int64_t)4))), dim3(256), 0, st,
+                           dE, dQ, ne, queries, tune_get("merge.lean") == 2);
+    } else {
         const int32_t *mg = b->mgids.as<int32_t>();
         int64_t o = 0;
         auto merge = [&](auto kc, int c) {

@@ -70,9 +70,34 @@ __global__ __launch_bounds__(256) void merge_lists_wave_kernel(const uint64_t *_
     merge_wave_query<K>(part, slices, klist, qs, es, q, kk, kind, out_idx, out_dist, wbuf[threadIdx.x >> 6]);
 }
 
+__global__ __launch_bounds__(256) void merge_lists_lean_kernel(const uint64_t *__restrict__ part, int64_t slices,
+                                                               int klist, int64_t nq, int64_t qs, int64_t es, int kk,
+                                                               int kind, int32_t *__restrict__ out_idx,
+                                                               void *__restrict__ out_dist, bool rounds_only) {
+    __shared__ uint64_t wbuf[4][256];
+    const int64_t q = xcd_contiguous_block() * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (q >= nq) return; // whole wave exits together
+    merge_wave_lean(part, slices, klist, qs, es, q, kk, kind, out_idx, out_dist, wbuf[threadIdx.x >> 6], rounds_only);
+}
+
+// merge.lean: 1 for private-list capacities of 16 or more (below that the private lists cost
+// little and the lean merge's extra pass shows), 3 always, 0 never, 2 always with the rounds path
+// only (a test setting)
+static bool lean_merge(int kk, int cap) {
+    const int64_t t = tune_get("merge.lean");
+    return kk <= 64 && (t >= 2 || (t == 1 && cap >= 16));
+}
+static bool lean_rounds() { return tune_get("merge.lean") == 2; }
+
 void merge_lists(const uint64_t *part, int64_t slices, int K, int64_t nq, int64_t nq_pad, int kk, int kind,
                  int32_t *out_idx, void *out_dist, hipStream_t st) {
     const bool wave = tune_get("merge.wave_per_query") != 0;
+    if (wave && lean_merge(kk, K)) {
+        hipLaunchKernelGGL(merge_lists_lean_kernel, dim3(xcd_grid(ceil_div(nq, 4))), dim3(256), 0, st, part, slices,
+                           K, nq, (int64_t)1, nq_pad, kk, kind, out_idx, out_dist, lean_rounds());
+        PN_HIP(hipGetLastError());
+        return;
+    }
     dim3 grid(wave ? xcd_grid(ceil_div(nq, 4)) : (unsigned)ceil_div(nq, 256));
 #define PN_MERGE(KV)                                                                                   \
     do {                                                                                               \
@@ -99,6 +124,12 @@ void merge_lists(const uint64_t *part, int64_t slices, int K, int64_t nq, int64_
 void merge_lists_wave(const uint64_t *part, int64_t slices, int klist, int cap, int64_t nq, int64_t qs, int64_t es,
                       int kk, int kind, int32_t *out_idx, void *out_dist, hipStream_t st) {
     dim3 grid(xcd_grid(ceil_div(nq, 4)));
+    if (lean_merge(kk, cap)) {
+        hipLaunchKernelGGL(merge_lists_lean_kernel, grid, dim3(256), 0, st, part, slices, klist, nq, qs, es, kk, kind,
+                           out_idx, out_dist, lean_rounds());
+        PN_HIP(hipGetLastError());
+        return;
+    }
 #define PN_MERGEW(KV)                                                                                        \
     hipLaunchKernelGGL((merge_lists_wave_kernel<KV>), grid, dim3(256), 0, st, part, slices, klist, nq, qs, es, kk, \
                        kind, out_idx, out_dist)

@@ -59,6 +59,28 @@ def test_batch_matches_oracle():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("lean", [0, 2])
+def test_batch_merge_variants(lean):
+    """The merge with private lists (one launch per list size) and the lean merge's rounds path
+    give the plain searches' results (the default lean merge is covered by the other tests)."""
+    from pynear_amd import _native as N
+
+    subs = _make(4, SPECS[:8])
+    old = N.tune_get("merge.lean")
+    try:
+        N.tune_set("merge.lean", lean)
+        b = BatchSearch([(idx, Q, k) for idx, X, Q, k in subs])
+        b.run()
+        res = b.results()
+    finally:
+        N.tune_set("merge.lean", old)
+    for (idx, X, Q, k), (bi, bd) in zip(subs, res):
+        pi, pd = _plain(idx, Q, k)
+        assert np.array_equal(bi, pi), (X.shape, Q.shape, k)
+        assert np.array_equal(bd, pd)
+
+
+@pytest.mark.gpu
 def test_batch_tuned_schedule_keeps_results(tmp_path):
     subs = _make(2, SPECS[:5])
     b = BatchSearch([(idx, Q, k) for idx, X, Q, k in subs])

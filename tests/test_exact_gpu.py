@@ -168,31 +168,28 @@ def test_device_entry_point_pads_queries(metric):
     np.testing.assert_array_equal(od.numpy(), rd)
 
 
-@pytest.mark.parametrize("wave", [0, 1])
+@pytest.mark.parametrize("wave,lean", [(0, 0), (1, 0), (1, 1), (1, 2), (1, 3)])
 @pytest.mark.parametrize("wpc", [1, 8])
-def test_merge_and_slicing_variants(wave, wpc):
-    """Both merge kernels and different slice counts give identical results."""
+def test_merge_and_slicing_variants(wave, lean, wpc):
+    """Every merge (lane per query; wave per query with private lists; the lean wave merge and
+    its rounds path) and different slice counts give identical results."""
     X, Q = _hamming_case(40_000, 32, 300, seed=wpc * 10 + wave)
     h = N.ExactHandle(N.METRIC_HAMMING)
     h.set(X)
-    old = (N.tune_get("merge.wave_per_query"), N.tune_get("hamming.wg_per_cu"))
-    try:
-        N.tune_set("merge.wave_per_query", wave)
-        N.tune_set("hamming.wg_per_cu", wpc)
+    with _Tune(merge__wave_per_query=wave, merge__lean=lean, hamming__wg_per_cu=wpc):
         gi, gd = h.search(Q, 16)
-    finally:
-        N.tune_set("merge.wave_per_query", old[0])
-        N.tune_set("hamming.wg_per_cu", old[1])
     oi, od = oracle.knn_hamming(X, Q, 16)
     np.testing.assert_array_equal(gd, od)
     np.testing.assert_array_equal(gi, oi)
 
 
+@pytest.mark.parametrize("lean", [0, 1, 2, 3])
 @pytest.mark.parametrize("case", ["hamming", "l2_mfma", "l2_valu"])
-def test_all_rows_equal_ties_by_index(case):
+def test_all_rows_equal_ties_by_index(case, lean):
     """Every row equal: every distance ties, so the k results are rows 0 .. k - 1. With tens of
-    thousands of tied keys the wave merge's sampled bound admits thousands of them, which takes
-    its round-by-round path (more than 256 held keys) instead of the single sort."""
+    thousands of tied keys the wave merge's sampled bound admits thousands of them: the merge with
+    private lists takes its round-by-round path, the lean merge lowers its bound (or, forced, takes
+    its rounds)."""
     rng = np.random.default_rng(17)
     k = 32 if case != "hamming" else 16
     if case == "hamming":
@@ -204,7 +201,7 @@ def test_all_rows_equal_ties_by_index(case):
         Q = rng.standard_normal((70, 32)).astype(np.float32)
         h = N.ExactHandle(N.METRIC_L2)
     h.set(X)
-    with _Tune(l2__mfma=0 if case == "l2_valu" else 1):
+    with _Tune(l2__mfma=0 if case == "l2_valu" else 1, merge__lean=lean):
         gi, gd = h.search(Q, k)
     assert (gi == np.arange(k)[None, :]).all()
     assert (gd == gd[:, :1]).all()
@@ -214,6 +211,26 @@ def test_all_rows_equal_ties_by_index(case):
         oi, od = oracle.knn_float(N.METRIC_L2, X, Q, k)
     np.testing.assert_array_equal(gi, oi)
     np.testing.assert_array_equal(gd, od)
+
+
+@pytest.mark.parametrize("lean", [0, 1, 2, 3])
+@pytest.mark.parametrize("nq,k", [(1, 32), (3, 10), (200, 32), (200, 5)])
+def test_merge_rows_sorted_by_cluster(lean, nq, k):
+    """Rows stored cluster by cluster: a query's neighbours crowd a few consecutive slices (a few
+    lanes of the merge hold most keys under the sampled bound), with one to a few hundred queries
+    (up to eight slices per lane)."""
+    rng = np.random.default_rng(nq * 100 + k)
+    C = rng.standard_normal((40, 32)).astype(np.float32) * 4
+    lab = np.sort(rng.integers(0, 40, 120_000))
+    X = (C[lab] + 0.3 * rng.standard_normal((lab.size, 32))).astype(np.float32)
+    Q = (C[rng.integers(0, 40, nq)] + 0.3 * rng.standard_normal((nq, 32))).astype(np.float32)
+    h = N.ExactHandle(N.METRIC_L2)
+    h.set(X)
+    with _Tune(l2__mfma=1, merge__lean=lean):
+        gi, gd = h.search(Q, k)
+    oi, od = oracle.knn_float(N.METRIC_L2, X, Q, k)
+    np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(gi, oi)
 
 
 def test_pairwise_distances_bit_exact():
