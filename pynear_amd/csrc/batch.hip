@@ -167,7 +167,8 @@ __global__ __launch_bounds__(256) void batch_merge_kernel(const BatchEntry *__re
     if (i >= count) return;
     const int64_t g = gids[i];
     const BatchEntry &e = E[find_entry(q_base, ne, g)];
-    merge_wave_query<K>(e.part, e.S, e.KL, e.S * e.KL, 1, g - e.q_base, e.K2c, OUT_F32, e.cidx, e.capx, wbuf);
+    uint64_t unused;
+    merge_wave_query<K>(e.part, e.S, e.KL, e.S * e.KL, 1, g - e.q_base, e.K2c, OUT_F32, e.cidx, e.capx, wbuf, unused);
 }
 
 // The merge without private lists: one launch for all queries of the batch.
@@ -178,8 +179,25 @@ __global__ __launch_bounds__(256) void batch_merge_lean_kernel(const BatchEntry 
     const int64_t g = xcd_contiguous_block() * 4 + (threadIdx.x >> 6);
     if (g >= nq) return;
     const BatchEntry &e = E[find_entry(q_base, ne, g)];
+    uint64_t unused;
     merge_wave_lean(e.part, e.S, e.KL, e.S * e.KL, 1, g - e.q_base, e.K2c, OUT_F32, e.cidx, e.capx,
-                    wbuf_all[threadIdx.x >> 6], rounds_only);
+                    wbuf_all[threadIdx.x >> 6], rounds_only, unused);
+}
+
+// The lean merge and the rerank of every query in one launch (the candidates stay in registers).
+__global__ __launch_bounds__(256) void batch_merge_rerank_kernel(const BatchEntry *__restrict__ E,
+                                                                 const int64_t *__restrict__ q_base, int ne,
+                                                                 int64_t nq, bool rounds_only) {
+    __shared__ uint64_t wbuf_all[4][256];
+    const int64_t g = xcd_contiguous_block() * 4 + (threadIdx.x >> 6);
+    if (g >= nq) return;
+    const BatchEntry &e = E[find_entry(q_base, ne, g)];
+    const int64_t q = g - e.q_base;
+    uint64_t mine;
+    merge_wave_lean<true>(e.part, e.S, e.KL, e.S * e.KL, 1, q, e.K2c, OUT_F32, nullptr, nullptr,
+                          wbuf_all[threadIdx.x >> 6], rounds_only, mine);
+    rerank_regs<false>(e.X, e.xstride, e.d, e.Q, e.qstride, e.qn, e.xn_max, mine, e.K2c, e.part, e.S, e.KL,
+                       e.S * e.KL, e.kk, e.n, e.eps_coef, e.out_idx, e.out_dist, e.flags, q);
 }
 
 // a lane that merges one slice holds at most KL keys; with more slices per lane it needs K2c
@@ -389,7 +407,11 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     if (nc[2] > 0)
         hipLaunchKernelGGL(batch_scan_kernel<2>, dim3(grid_wd), dim3(256), 0, st, dE, dt + nc[0] + nc[1], nc[2],
                            ctr + 2);
-    if (tune_get("merge.lean") != 0) {
+    const bool fused = tune_get("merge.lean") != 0 && tune_get("l2.fuse_rerank") != 0;
+    if (fused) {
+        hipLaunchKernelGGL(batch_merge_rerank_kernel, dim3(xcd_grid(ceil_div(queries, (int64_t)4))), dim3(256), 0, st,
+                           dE, dQ, ne, queries, tune_get("merge.lean") == 2);
+    } else if (tune_get("merge.lean") != 0) {
         hipLaunchKernelGGL(batch_merge_lean_kernel, dim3(xcd_grid(ceil_div(queries, (int64_t)4))), dim3(256), 0, st,
                            dE, dQ, ne, queries, tune_get("merge.lean") == 2);
     } else {
@@ -407,8 +429,9 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
         merge(std::integral_constant<int, 32>{}, 2);
         merge(std::integral_constant<int, 64>{}, 3);
     }
-    hipLaunchKernelGGL(batch_rerank_kernel, dim3((unsigned)ceil_div(queries, (int64_t)4)), dim3(256), 0, st, dE, dQ,
-                       ne, queries);
+    if (!fused)
+        hipLaunchKernelGGL(batch_rerank_kernel, dim3((unsigned)ceil_div(queries, (int64_t)4)), dim3(256), 0, st, dE,
+                           dQ, ne, queries);
     hipLaunchKernelGGL(batch_fallback_kernel, dim3((unsigned)queries), dim3(256), 0, st, dE, dQ, ne);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(st, &g);

@@ -26,6 +26,7 @@
 #include "kernels.hpp"
 #include "l2_mfma.hpp"
 #include "l2_mfma_dev.hpp"
+#include "merge_dev.hpp"
 #include "ref_arith.hpp"
 
 namespace pn {
@@ -101,6 +102,29 @@ __global__ __launch_bounds__(256) void l2_rerank_kernel(const float *__restrict_
     if (q < nq)
         rerank_query<REG>(X, xstride, d, Q, qstride, qn, xn_max_bits, cidx, capx, K2c, part, nlists, KL, qsl, kk, n,
                      eps_coef, out_idx, out_dist, flags, q);
+}
+
+// The wave merge of a query's slice lists (query-major, KL keys each) and the exact rerank of its
+// K2c candidates in one launch: the candidates pass from the merge to the rerank in registers.
+// K: the merge's private-list capacity; 0: the merge without private lists.
+template <int K, bool REG>
+__global__ __launch_bounds__(256) void l2_merge_rerank_kernel(
+    const uint64_t *__restrict__ part, int64_t nlists, int KL, int64_t qsl, int K2c, bool rounds_only,
+    const float *__restrict__ X, int64_t xstride, int64_t d, const float *__restrict__ Q, int64_t qstride,
+    const float *__restrict__ qn, const unsigned int *__restrict__ xn_max_bits, int64_t nq, int kk, int64_t n,
+    float eps_coef, int32_t *__restrict__ out_idx, float *__restrict__ out_dist, int32_t *__restrict__ flags) {
+    __shared__ uint64_t wbuf[4][256];
+    const int64_t q = xcd_contiguous_block() * 4 + (threadIdx.x >> 6);
+    if (q >= nq) return; // whole wave exits together
+    uint64_t mine;
+    if constexpr (K == 0)
+        merge_wave_lean<true>(part, nlists, KL, qsl, 1, q, K2c, OUT_F32, nullptr, nullptr, wbuf[threadIdx.x >> 6],
+                              rounds_only, mine);
+    else
+        merge_wave_query<K, true>(part, nlists, KL, qsl, 1, q, K2c, OUT_F32, nullptr, nullptr, wbuf[threadIdx.x >> 6],
+                                  mine);
+    rerank_regs<REG>(X, xstride, d, Q, qstride, qn, xn_max_bits, mine, K2c, part, nlists, KL, qsl, kk, n, eps_coef,
+                     out_idx, out_dist, flags, q);
 }
 
 // One workgroup per query; unflagged queries exit at once.
@@ -564,14 +588,33 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
     int32_t *cidx = (int32_t *)wk.cidx.ensure((size_t)nq * K2c * 4);
     float *capx = (float *)wk.capx.ensure((size_t)nq * K2c * 4);
     int32_t *flags = (int32_t *)wk.flags.ensure((size_t)nq * 4);
-    {
+    const float eps_coef = (prec == 1 ? 0.0078125f : 6.103515625e-05f) + (float)dp * 4.76837158203125e-07f;
+    // a lane that merges one slice holds at most KL keys; with more slices per lane it needs K2c
+    const int cap = nlists <= 64 ? KL : K2;
+    const int64_t lt = tune_get("merge.lean");
+    const bool lean = K2c <= 64 && (lt >= 2 || (lt == 1 && cap >= 16)); // as merge_lists_wave
+    if (tune_get("l2.fuse_rerank") != 0 && (lean || cap == 4 || cap == 8)) {
         StageScope sc(st, STAGE_SELECT);
-        // a lane that merges one slice holds at most KL keys; with more slices per lane it needs K2c
-        merge_lists_wave(part, nlists, KL, nlists <= 64 ? KL : K2, nq, qsl, 1, K2c, OUT_F32, cidx, capx, st);
-    }
-    {
+        const bool reg = rerank_reg(nq);
+        auto go = [&](auto kc, auto rc) {
+            hipLaunchKernelGGL((l2_merge_rerank_kernel<decltype(kc)::value, decltype(rc)::value>),
+                               dim3(xcd_grid(ceil_div(nq, (int64_t)4))), dim3(256), 0, st, part, nlists, KL, qsl, K2c,
+                               lt == 2, X, xstride, d, Q, qstride, qn, xs.norm_max.as<unsigned int>(), nq, kk, n,
+                               eps_coef, out_idx, out_dist, flags);
+        };
+        using R0 = std::false_type;
+        using R1 = std::true_type;
+        const int kc = lean ? 0 : cap;
+        if (kc == 0) reg ? go(std::integral_constant<int, 0>{}, R1{}) : go(std::integral_constant<int, 0>{}, R0{});
+        else if (kc == 4) reg ? go(std::integral_constant<int, 4>{}, R1{}) : go(std::integral_constant<int, 4>{}, R0{});
+        else reg ? go(std::integral_constant<int, 8>{}, R1{}) : go(std::integral_constant<int, 8>{}, R0{});
+        PN_HIP(hipGetLastError());
+    } else {
+        {
+            StageScope sc(st, STAGE_SELECT);
+            merge_lists_wave(part, nlists, KL, cap, nq, qsl, 1, K2c, OUT_F32, cidx, capx, st);
+        }
         StageScope sc(st, STAGE_RERANK);
-        const float eps_coef = (prec == 1 ? 0.0078125f : 6.103515625e-05f) + (float)dp * 4.76837158203125e-07f;
         hipLaunchKernelGGL(rerank_reg(nq) ? l2_rerank_kernel<true> : l2_rerank_kernel<false>, dim3((unsigned)ceil_div(nq, (int64_t)4)), dim3(256), 0, st, X, xstride, d, Q,
                            qstride, qn, xs.norm_max.as<unsigned int>(), cidx, capx, K2c, part, nlists, KL, qsl, nq,
                            kk, n, eps_coef, out_idx, out_dist, flags);

@@ -520,14 +520,16 @@ __device__ __forceinline__ float rerank_cores_st(const float *__restrict__ qrow,
 
 // One wave per query q: exact rerank of K2c candidates, (distance, index) order, completeness check.
 // REG: rows of 8 u + (0..7) floats for u in {4, 8, 16, 24, 32} go through rerank_cores_st.
+// Candidate j (j < K2c) is the key `mine` of lane j: its approximation's bits in the high word, its
+// row in the low word (empty key: no candidate).
 template <bool REG>
-__device__ __forceinline__ void rerank_query(const float *__restrict__ X, int64_t xstride, int64_t d,
-                                             const float *__restrict__ Q, int64_t qstride,
-                                             const float *__restrict__ qn, const unsigned int *__restrict__ xn_max_bits,
-                                             const int32_t *__restrict__ cidx, const float *__restrict__ capx, int K2c,
-                                             const uint64_t *__restrict__ part, int64_t nlists, int KL, int64_t qsl,
-                                             int kk, int64_t n, float eps_coef, int32_t *__restrict__ out_idx,
-                                             float *__restrict__ out_dist, int32_t *__restrict__ flags, int64_t q) {
+__device__ __forceinline__ void rerank_regs(const float *__restrict__ X, int64_t xstride, int64_t d,
+                                            const float *__restrict__ Q, int64_t qstride,
+                                            const float *__restrict__ qn, const unsigned int *__restrict__ xn_max_bits,
+                                            uint64_t mine, int K2c, const uint64_t *__restrict__ part, int64_t nlists,
+                                            int KL, int64_t qsl, int kk, int64_t n, float eps_coef,
+                                            int32_t *__restrict__ out_idx, float *__restrict__ out_dist,
+                                            int32_t *__restrict__ flags, int64_t q) {
     const int lane = threadIdx.x & 63;
     // the smallest approximation any full list dropped: its last key with the row bits cleared
     uint32_t bmin = kKey32None;
@@ -546,15 +548,16 @@ __device__ __forceinline__ void rerank_query(const float *__restrict__ X, int64_
     // read. The kept candidates are a prefix.
     const float sq = sqrtf(qn[q]) + sqrtf(__uint_as_float(*xn_max_bits));
     const float eps = eps_coef * sq * sq;
-    int32_t myidx = lane < K2c ? cidx[q * K2c + lane] : -1;
+    int32_t myidx = lane < K2c && mine != kKey64None ? (int32_t)(uint32_t)(mine & 0xFFFFFFFFull) : -1;
     if (K2c > kk) {
-        const uint32_t hk = f32_key(capx[q * K2c + kk - 1]);
+        const uint32_t hk = (uint32_t)(__shfl(mine, kk - 1, 64) >> 32);
         if (hk != kKey32None && lane < K2c) {
             const float upper = key_f32(hk | 31u);
-            const float lower = key_f32(f32_key(capx[q * K2c + lane]) & ~31u);
+            const float lower = key_f32((uint32_t)(mine >> 32) & ~31u);
             if (lower - upper > 3.f * eps) myidx = -1;
         }
     }
+    const uint32_t al = (uint32_t)(__shfl(mine, K2c - 1, 64) >> 32); // the last candidate's approximation
     const uint64_t kept = __ballot(myidx >= 0);
     const int nkeep = kept ? 64 - __clzll((long long)kept) : 0;
     const float *qrow = Q + q * qstride;
@@ -612,13 +615,32 @@ __device__ __forceinline__ void rerank_query(const float *__restrict__ X, int64_
     if (lane == 0) {
         int flag = 0;
         if (n > K2c) {
-            const uint32_t al = __float_as_uint(capx[q * K2c + K2c - 1]);
             const uint32_t bnd = min(bmin, al == kKey32None ? kKey32None : (al & ~31u));
             const float bound = bnd == kKey32None ? INFINITY : key_f32(bnd);
             flag = !(bound - eps > dk_core);
         }
         flags[q] = flag;
     }
+}
+
+// rerank_regs over candidates stored as rows cidx[q, :K2c] (-1: none) and approximations
+// capx[q, :K2c].
+template <bool REG>
+__device__ __forceinline__ void rerank_query(const float *__restrict__ X, int64_t xstride, int64_t d,
+                                             const float *__restrict__ Q, int64_t qstride,
+                                             const float *__restrict__ qn, const unsigned int *__restrict__ xn_max_bits,
+                                             const int32_t *__restrict__ cidx, const float *__restrict__ capx, int K2c,
+                                             const uint64_t *__restrict__ part, int64_t nlists, int KL, int64_t qsl,
+                                             int kk, int64_t n, float eps_coef, int32_t *__restrict__ out_idx,
+                                             float *__restrict__ out_dist, int32_t *__restrict__ flags, int64_t q) {
+    const int lane = threadIdx.x & 63;
+    uint64_t mine = kKey64None;
+    if (lane < K2c) {
+        const int32_t ci = cidx[q * K2c + lane];
+        if (ci >= 0) mine = ((uint64_t)f32_key(capx[q * K2c + lane]) << 32) | (uint32_t)ci;
+    }
+    rerank_regs<REG>(X, xstride, d, Q, qstride, qn, xn_max_bits, mine, K2c, part, nlists, KL, qsl, kk, n, eps_coef,
+                     out_idx, out_dist, flags, q);
 }
 
 // Exact full scan of query q by one workgroup (called for flagged queries only).

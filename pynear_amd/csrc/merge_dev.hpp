@@ -69,11 +69,12 @@ __device__ __forceinline__ uint64_t merge_sample_bound(const uint64_t *__restric
     return bnd;
 }
 
-template <int K>
+// KEEP: nothing is written; lane r < kk receives the r-th smallest key in `mine` (others: empty).
+template <int K, bool KEEP = false>
 __device__ __forceinline__ void merge_wave_query(const uint64_t *__restrict__ part, int64_t slices, int klist,
                                                  int64_t qs, int64_t es, int64_t q, int kk, int kind,
                                                  int32_t *__restrict__ out_idx, void *__restrict__ out_dist,
-                                                 uint64_t *wbuf) {
+                                                 uint64_t *wbuf, uint64_t &mine) {
     const int lane = threadIdx.x & 63;
     const bool tr = kind == OUT_L2_F32;
     auto transform = [&](uint64_t key) { return merge_transform(key, tr); };
@@ -86,6 +87,7 @@ __device__ __forceinline__ void merge_wave_query(const uint64_t *__restrict__ pa
             ((float *)out_dist)[q * kk + r] = key_f32(hi);
     };
     const uint64_t bnd = merge_sample_bound(part, slices, klist, qs, es, q, kk, tr, lane);
+    if (KEEP) mine = kKey64None;
     uint64_t L[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) L[j] = kKey64None;
@@ -137,7 +139,8 @@ __device__ __forceinline__ void merge_wave_query(const uint64_t *__restrict__ pa
 #pragma unroll
         for (int u = 0; u < 4; ++u) v[u] = lane + 64 * u < total ? wbuf[lane + 64 * u] : kKey64None;
         wave_sort_u64<4, 2>(v, lane);
-        if (lane < kk) write_out(lane, v[0]); // (kk <= 64: the kk smallest are v[0] of lanes 0 .. kk-1)
+        if (KEEP) mine = lane < kk ? v[0] : kKey64None;
+        else if (lane < kk) write_out(lane, v[0]); // (kk <= 64: the kk smallest are v[0] of lanes 0 .. kk-1)
         return;
     }
     for (int r = 0; r < kk; ++r) {
@@ -152,7 +155,11 @@ __device__ __forceinline__ void merge_wave_query(const uint64_t *__restrict__ pa
             for (int j = 0; j < K - 1; ++j) L[j] = L[j + 1];
             L[K - 1] = kKey64None;
         }
-        if (lane == 0) write_out(r, m);
+        if (KEEP) {
+            if (lane == r) mine = m;
+        } else if (lane == 0) {
+            write_out(r, m);
+        }
     }
 }
 
@@ -166,11 +173,12 @@ __device__ __forceinline__ void merge_wave_query(const uint64_t *__restrict__ pa
 // the lanes' four smallest (kk keys lie at or below it) and the count repeats, twice at most; a
 // wave that still holds more than 256 keys pops the kk smallest in rounds of a wave minimum over
 // the lanes' four smallest, a lane refilling them from its lists when they run out. rounds_only
-// takes the rounds at once (a test switch for that path).
+// takes the rounds at once (a test switch for that path). KEEP as merge_wave_query.
+template <bool KEEP = false>
 __device__ __forceinline__ void merge_wave_lean(const uint64_t *__restrict__ part, int64_t slices, int klist,
                                                 int64_t qs, int64_t es, int64_t q, int kk, int kind,
                                                 int32_t *__restrict__ out_idx, void *__restrict__ out_dist,
-                                                uint64_t *wbuf, bool rounds_only) {
+                                                uint64_t *wbuf, bool rounds_only, uint64_t &mine) {
     const int lane = threadIdx.x & 63;
     const bool tr = kind == OUT_L2_F32;
     auto write_out = [&](int r, uint64_t m) {
@@ -251,9 +259,11 @@ __device__ __forceinline__ void merge_wave_lean(const uint64_t *__restrict__ par
 #pragma unroll
         for (int u = 0; u < 4; ++u) v[u] = lane + 64 * u < total ? wbuf[lane + 64 * u] : kKey64None;
         wave_sort_u64<4, 2>(v, lane);
-        if (lane < kk) write_out(lane, v[0]); // (kk <= 64: the kk smallest are v[0] of lanes 0 .. kk-1)
+        if (KEEP) mine = lane < kk ? v[0] : kKey64None;
+        else if (lane < kk) write_out(lane, v[0]); // (kk <= 64: the kk smallest are v[0] of lanes 0 .. kk-1)
         return;
     }
+    if (KEEP) mine = kKey64None;
     // rounds: L4 holds the lane's four smallest keys at or below bnd; `more` marks keys beyond them
     bool more = c > 4;
     for (int r = 0; r < kk; ++r) {
@@ -276,7 +286,11 @@ __device__ __forceinline__ void merge_wave_lean(const uint64_t *__restrict__ par
                 more = cnt > 4;
             }
         }
-        if (lane == 0) write_out(r, m);
+        if (KEEP) {
+            if (lane == r) mine = m;
+        } else if (lane == 0) {
+            write_out(r, m);
+        }
     }
 }
 

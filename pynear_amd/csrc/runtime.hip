@@ -42,6 +42,7 @@ static std::map<std::string, int64_t> &tune_table() {
         t["l2.matrix"] = 1;            // exact L2 over <= 4096 rows of d > 64: key matrix + selection
         t["ivfl2.wg_per_cu"] = 2;      // IVF-L2 list scan: persistent workgroups per CU
         t["l2.wide"] = 1;             // exact L2 scan for dp = 128, 192, 256: queries held in registers
+        t["l2.fuse_rerank"] = 1;      // exact L2: merge and rerank in one launch (candidates in registers)
         t["merge.lean"] = 1;          // wave merges without private lists: 1 for capacities >= 16, 3 always, 2 rounds only
         t["l2.rerank_reg"] = 2;       // exact L2 rerank: query chains in registers (1 on, 0 off, 2 for <= 1024 queries)
         t["l2.matrix_terms"] = 3;     // key-matrix path (<= 4096 wide rows): bf16 products per term (1 or 3)

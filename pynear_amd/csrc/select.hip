@@ -67,7 +67,8 @@ __global__ __launch_bounds__(256) void merge_lists_wave_kernel(const uint64_t *_
     __shared__ uint64_t wbuf[4][256];
     const int64_t q = xcd_contiguous_block() * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (q >= nq) return; // whole wave exits together
-    merge_wave_query<K>(part, slices, klist, qs, es, q, kk, kind, out_idx, out_dist, wbuf[threadIdx.x >> 6]);
+    uint64_t unused;
+    merge_wave_query<K>(part, slices, klist, qs, es, q, kk, kind, out_idx, out_dist, wbuf[threadIdx.x >> 6], unused);
 }
 
 __global__ __launch_bounds__(256) void merge_lists_lean_kernel(const uint64_t *__restrict__ part, int64_t slices,
@@ -77,7 +78,9 @@ __global__ __launch_bounds__(256) void merge_lists_lean_kernel(const uint64_t *_
     __shared__ uint64_t wbuf[4][256];
     const int64_t q = xcd_contiguous_block() * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (q >= nq) return; // whole wave exits together
-    merge_wave_lean(part, slices, klist, qs, es, q, kk, kind, out_idx, out_dist, wbuf[threadIdx.x >> 6], rounds_only);
+    uint64_t unused;
+    merge_wave_lean(part, slices, klist, qs, es, q, kk, kind, out_idx, out_dist, wbuf[threadIdx.x >> 6], rounds_only,
+                    unused);
 }
 
 // merge.lean: 1 for private-list capacities of 16 or more (below that the private lists cost

@@ -59,21 +59,24 @@ def test_batch_matches_oracle():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("lean", [0, 2])
-def test_batch_merge_variants(lean):
-    """The merge with private lists (one launch per list size) and the lean merge's rounds path
-    give the plain searches' results (the default lean merge is covered by the other tests)."""
+@pytest.mark.parametrize("lean,fuse", [(0, 1), (2, 1), (1, 0), (2, 0)])
+def test_batch_merge_variants(lean, fuse):
+    """The merge with private lists (one launch per list size), the lean merge's rounds path, and
+    merge and rerank in separate launches give the plain searches' results (the default, the lean
+    merge fused with the rerank, is covered by the other tests)."""
     from pynear_amd import _native as N
 
     subs = _make(4, SPECS[:8])
-    old = N.tune_get("merge.lean")
+    old = (N.tune_get("merge.lean"), N.tune_get("l2.fuse_rerank"))
     try:
         N.tune_set("merge.lean", lean)
+        N.tune_set("l2.fuse_rerank", fuse)
         b = BatchSearch([(idx, Q, k) for idx, X, Q, k in subs])
         b.run()
         res = b.results()
     finally:
-        N.tune_set("merge.lean", old)
+        N.tune_set("merge.lean", old[0])
+        N.tune_set("l2.fuse_rerank", old[1])
     for (idx, X, Q, k), (bi, bd) in zip(subs, res):
         pi, pd = _plain(idx, Q, k)
         assert np.array_equal(bi, pi), (X.shape, Q.shape, k)

@@ -213,12 +213,13 @@ def test_all_rows_equal_ties_by_index(case, lean):
     np.testing.assert_array_equal(gd, od)
 
 
+@pytest.mark.parametrize("fuse", [0, 1])
 @pytest.mark.parametrize("lean", [0, 1, 2, 3])
 @pytest.mark.parametrize("nq,k", [(1, 32), (3, 10), (200, 32), (200, 5)])
-def test_merge_rows_sorted_by_cluster(lean, nq, k):
+def test_merge_rows_sorted_by_cluster(fuse, lean, nq, k):
     """Rows stored cluster by cluster: a query's neighbours crowd a few consecutive slices (a few
     lanes of the merge hold most keys under the sampled bound), with one to a few hundred queries
-    (up to eight slices per lane)."""
+    (up to eight slices per lane); merge and rerank in one launch or two."""
     rng = np.random.default_rng(nq * 100 + k)
     C = rng.standard_normal((40, 32)).astype(np.float32) * 4
     lab = np.sort(rng.integers(0, 40, 120_000))
@@ -226,7 +227,7 @@ def test_merge_rows_sorted_by_cluster(lean, nq, k):
     Q = (C[rng.integers(0, 40, nq)] + 0.3 * rng.standard_normal((nq, 32))).astype(np.float32)
     h = N.ExactHandle(N.METRIC_L2)
     h.set(X)
-    with _Tune(l2__mfma=1, merge__lean=lean):
+    with _Tune(l2__mfma=1, merge__lean=lean, l2__fuse_rerank=fuse):
         gi, gd = h.search(Q, k)
     oi, od = oracle.knn_float(N.METRIC_L2, X, Q, k)
     np.testing.assert_array_equal(gd, od)
