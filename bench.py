@@ -127,11 +127,13 @@ def learned_workload(n=2_000_000, d=512, comps=4096, sigma=0.25, n_train=200_000
 
 # ---------------------------------------------------------------- helpers
 def _time_device(N, hipdev, step, steps, warmup, stream):
+    """Times `steps` steps between two HIP events with nothing else on the stream, then runs the
+    same number of steps again with per-stage HIP events (the stage breakdown and per-launch kernel
+    times; those events add their own gaps, which stay out of the timed steps)."""
     for _ in range(warmup):
         step()
     stream.synchronize()
-    N.stage_timing(True)
-    N.stage_times()
+    N.stage_timing(False)
     ev0, ev1 = hipdev.Event(), hipdev.Event()
     hipdev.synchronize()
     t0 = time.perf_counter()
@@ -142,6 +144,11 @@ def _time_device(N, hipdev, step, steps, warmup, stream):
     ev1.synchronize()
     hipdev.synchronize()
     wall = time.perf_counter() - t0
+    N.stage_timing(True)
+    N.stage_times()
+    for _ in range(steps):
+        step()
+    stream.synchronize()
     N.stage_timing(False)
     stages = N.stage_times()
     return ev0.elapsed_ms(ev1), wall, stages

@@ -213,11 +213,16 @@ __global__ __launch_bounds__(256) void batch_rerank_kernel(const BatchEntry *__r
 }
 
 __global__ __launch_bounds__(256) void batch_fallback_kernel(const BatchEntry *__restrict__ E,
-                                                             const int64_t *__restrict__ q_base, int ne) {
+                                                             const int64_t *__restrict__ q_base, int ne, int64_t nq) {
     __shared__ uint64_t wres[4][32];
-    const BatchEntry &e = E[find_entry(q_base, ne, blockIdx.x)];
-    const int64_t q = blockIdx.x - e.q_base;
-    if (e.flags[q]) fallback_query<32>(wres, e.X, e.xstride, e.n, e.d, e.Q, e.qstride, e.kk, e.out_idx, e.out_dist, q);
+    for (int64_t g = blockIdx.x; g < nq; g += gridDim.x) { // grid-stride: most queries are not flagged
+        const BatchEntry &e = E[find_entry(q_base, ne, g)];
+        const int64_t q = g - e.q_base;
+        if (e.flags[q]) { // uniform over the workgroup
+            fallback_query<32>(wres, e.X, e.xstride, e.n, e.d, e.Q, e.qstride, e.kk, e.out_idx, e.out_dist, q);
+            __syncthreads(); // wres is reused by the next query
+        }
+    }
 }
 
 static int kl_of(int kk) { return kk <= 6 ? 8 : (kk <= 12 ? 16 : 32); } // as the single search
@@ -432,7 +437,8 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     if (!fused)
         hipLaunchKernelGGL(batch_rerank_kernel, dim3((unsigned)ceil_div(queries, (int64_t)4)), dim3(256), 0, st, dE,
                            dQ, ne, queries);
-    hipLaunchKernelGGL(batch_fallback_kernel, dim3((unsigned)queries), dim3(256), 0, st, dE, dQ, ne);
+    hipLaunchKernelGGL(batch_fallback_kernel, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(queries, 2 * (int64_t)b->cus))),
+                       dim3(256), 0, st, dE, dQ, ne, queries);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(st, &g);
     PN_HIP(ce);

@@ -127,14 +127,19 @@ __global__ __launch_bounds__(256) void l2_merge_rerank_kernel(
                      out_idx, out_dist, flags, q);
 }
 
-// One workgroup per query; unflagged queries exit at once.
+// A workgroup per flagged query (grid-stride over the queries; most are not flagged).
 template <int K>
 __global__ __launch_bounds__(256) void l2_fallback_kernel(const float *__restrict__ X, int64_t xstride, int64_t n,
                                                           int64_t d, const float *__restrict__ Q, int64_t qstride,
                                                           const int32_t *__restrict__ flags, int kk,
-                                                          int32_t *__restrict__ out_idx, float *__restrict__ out_dist) {
+                                                          int32_t *__restrict__ out_idx, float *__restrict__ out_dist,
+                                                          int64_t nq) {
     __shared__ uint64_t wres[4][K];
-    if (flags[blockIdx.x]) fallback_query<K>(wres, X, xstride, n, d, Q, qstride, kk, out_idx, out_dist, blockIdx.x);
+    for (int64_t q = blockIdx.x; q < nq; q += gridDim.x)
+        if (flags[q]) { // uniform over the workgroup
+            fallback_query<K>(wres, X, xstride, n, d, Q, qstride, kk, out_idx, out_dist, q);
+            __syncthreads(); // wres is reused by the next query
+        }
 }
 
 // ---------------------------------------------------------------- small-database path
@@ -441,6 +446,7 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
     int K2 = 2 * KL;
     int K2c = (int)std::min<int64_t>(K2, n);
     const int64_t gx = ceil_div(nq, 128);
+    const unsigned fb_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nq, 2 * (int64_t)cus)); // grid-stride
     const int64_t nq_pad = gx * 128;
     // queries: split into hi/lo (padded to nq_pad rows of dp; padding rows are zero)
     L2Set &qs = wk.qsplit;
@@ -492,14 +498,14 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
         {
             StageScope sc(st, STAGE_OTHER);
             if (kk <= 8)
-                hipLaunchKernelGGL((l2_fallback_kernel<8>), dim3((unsigned)nq), dim3(256), 0, st, X, xstride, n, d, Q,
-                                   qstride, flags, kk, out_idx, out_dist);
+                hipLaunchKernelGGL((l2_fallback_kernel<8>), dim3(fb_grid), dim3(256), 0, st, X, xstride, n, d, Q,
+                                   qstride, flags, kk, out_idx, out_dist, nq);
             else if (kk <= 16)
-                hipLaunchKernelGGL((l2_fallback_kernel<16>), dim3((unsigned)nq), dim3(256), 0, st, X, xstride, n, d, Q,
-                                   qstride, flags, kk, out_idx, out_dist);
+                hipLaunchKernelGGL((l2_fallback_kernel<16>), dim3(fb_grid), dim3(256), 0, st, X, xstride, n, d, Q,
+                                   qstride, flags, kk, out_idx, out_dist, nq);
             else
-                hipLaunchKernelGGL((l2_fallback_kernel<32>), dim3((unsigned)nq), dim3(256), 0, st, X, xstride, n, d, Q,
-                                   qstride, flags, kk, out_idx, out_dist);
+                hipLaunchKernelGGL((l2_fallback_kernel<32>), dim3(fb_grid), dim3(256), 0, st, X, xstride, n, d, Q,
+                                   qstride, flags, kk, out_idx, out_dist, nq);
             PN_HIP(hipGetLastError());
         }
         return;
@@ -623,8 +629,8 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
     {
         StageScope sc(st, STAGE_OTHER);
 #define PN_L2FB(KV)                                                                                              \
-    hipLaunchKernelGGL((l2_fallback_kernel<KV>), dim3((unsigned)nq), dim3(256), 0, st, X, xstride, n, d, Q, qstride, \
-                       flags, kk, out_idx, out_dist)
+    hipLaunchKernelGGL((l2_fallback_kernel<KV>), dim3(fb_grid), dim3(256), 0, st, X, xstride, n, d, Q, qstride, \
+                       flags, kk, out_idx, out_dist, nq)
         if (kk <= 8) PN_L2FB(8); else if (kk <= 16) PN_L2FB(16); else PN_L2FB(32);
 #undef PN_L2FB
         PN_HIP(hipGetLastError());

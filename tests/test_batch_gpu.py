@@ -84,6 +84,25 @@ def test_batch_merge_variants(lean, fuse):
 
 
 @pytest.mark.gpu
+def test_batch_fallback_many_flagged_queries():
+    """An index of identical rows ties every distance, so none of its 1,500 queries can be proved
+    complete from the candidates: all go to the fallback scan (more queries than fallback
+    workgroups, so each takes several in turn). Results are the oracle's: rows 0 .. k-1."""
+    rng = np.random.default_rng(8)
+    Xe = np.tile(rng.standard_normal((1, 64)).astype(np.float32), (3000, 1))
+    Qe = rng.standard_normal((1500, 64)).astype(np.float32)
+    ie = VPTreeL2Index()
+    ie.set(Xe)
+    subs = [(ie, Xe, Qe, 10)] + _make(9, [(2000, 128, 100, 5)])
+    b = BatchSearch([(idx, Q, k) for idx, X, Q, k in subs])
+    b.run()
+    for (idx, X, Q, k), (bi, bd) in zip(subs, b.results()):
+        ri, rd = oracle.knn_float(oracle.L2, X, Q, k)
+        assert np.array_equal(bi, ri)
+        assert np.array_equal(bd, rd)
+
+
+@pytest.mark.gpu
 def test_batch_tuned_schedule_keeps_results(tmp_path):
     subs = _make(2, SPECS[:5])
     b = BatchSearch([(idx, Q, k) for idx, X, Q, k in subs])

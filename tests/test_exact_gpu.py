@@ -319,6 +319,21 @@ def test_l2_fallback_path_exact(d):
     np.testing.assert_array_equal(vi, oi)
 
 
+def test_l2_fallback_many_flagged_queries():
+    """More flagged queries than fallback workgroups (two per CU): each workgroup takes several
+    queries in turn; every result is the oracle's."""
+    rng = np.random.default_rng(77)
+    X = rng.random((6000, 32)).astype(np.float32)
+    Q = rng.random((1500, 32)).astype(np.float32)
+    h = N.ExactHandle(N.METRIC_L2)
+    h.set(X)
+    with _Tune(l2__mfma_terms=1):
+        gi, gd = h.search(Q, 10)
+    oi, od = oracle.knn_float(N.METRIC_L2, X, Q, 10)
+    np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(gi, oi)
+
+
 @pytest.mark.parametrize("mfma", [0, 1])
 def test_l2_sqrt_tie_orders_by_index(mfma):
     """Two rows whose squared distances differ (1 and 1 + 2^-23) but whose float square roots are
