@@ -121,10 +121,11 @@ __global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restr
                                                           const int64_t *__restrict__ loff, int nl,
                                                           int32_t *__restrict__ cursor, int32_t *__restrict__ itoff,
                                                           int64_t *__restrict__ woff, int32_t *__restrict__ bins,
-                                                          int nranges, int iq) {
+                                                          int nranges, int iq, int32_t *__restrict__ zero, int nzero) {
     __shared__ int32_t s_pairs[256], s_items[256];
     __shared__ int64_t s_work[256];
     const int tid = threadIdx.x;
+    for (int i = tid; i < nzero; i += 256) zero[i] = 0; // (the rerank's histogram, cleared here for free)
     const int per = (nl + 255) / 256;
     const int o0 = min(nl, tid * per), o1 = min(nl, o0 + per);
     int pairs = 0, nit = 0;
@@ -206,8 +207,11 @@ __global__ void expand_items_kernel(const int32_t *__restrict__ lcount, const in
 __global__ __launch_bounds__(256) void probe_scatter_kernel(const int32_t *__restrict__ probes, int64_t total, int np,
                                                             int p_lo, int npw, int nl, int32_t *__restrict__ cursor,
                                                             int32_t *__restrict__ pair_q, int32_t *__restrict__ pair_p,
-                                                            int32_t *__restrict__ pair_c, int32_t *__restrict__ pslot) {
+                                                            int32_t *__restrict__ pair_c, int32_t *__restrict__ pslot,
+                                                            int32_t *__restrict__ lcount) {
     __shared__ int32_t hist[kLdsBins];
+    // the list counts' last reader (expand_items_kernel) has run: leave them zero for the next pass
+    for (int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x; b < nl; b += (int64_t)gridDim.x * 256) lcount[b] = 0;
     const int64_t c0 = (int64_t)blockIdx.x * kPairChunk, c1 = min(total, c0 + kPairChunk);
     if (nl > kLdsBins) {
         for (int64_t t = c0 + threadIdx.x; t < c1; t += 256) {
@@ -1296,7 +1300,13 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     __bf16 *Ql = (__bf16 *)h->qsplit_l.ensure((size_t)nq * dp * 2);
     float *qn = (float *)h->qn.ensure((size_t)nq * 4);
     const int64_t total = nq * np;
+    // per-list pair counts: zero between passes (probe_scatter_kernel clears them after use), so a
+    // new allocation is cleared once here
+    const bool lc_new = !h->lcount.p || h->lcount.bytes < (size_t)h->nl * 4;
     int32_t *lc = (int32_t *)h->lcount.ensure((size_t)h->nl * 4);
+    if (lc_new) PN_HIP(hipMemsetAsync(lc, 0, (size_t)h->nl * 4, st));
+    // the rerank's histogram of first probes (cleared by build_items_kernel)
+    int32_t *ohist = (int32_t *)h->ohist.ensure((size_t)(h->nl + 1) * 4);
     int32_t *cur = (int32_t *)h->cursor.ensure((size_t)h->nl * 4);
     int32_t *pq = (int32_t *)h->pair_q.ensure((size_t)total * 4);
     int32_t *pp = (int32_t *)h->pair_p.ensure((size_t)total * 4);
@@ -1334,16 +1344,15 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     auto bucket = [&](int p_lo, int npw) {
         StageScope sc(st, STAGE_OTHER);
         const int64_t tw = nq * npw;
-        PN_HIP(hipMemsetAsync(lc, 0, (size_t)h->nl * 4, st));
         const unsigned pchunks = (unsigned)ceil_div(tw, (int64_t)kPairChunk);
         hipLaunchKernelGGL(probe_count_kernel, dim3(pchunks), dim3(256), 0, st, pr, tw, np, p_lo, npw, h->nl, lc);
         hipLaunchKernelGGL(build_items_kernel, dim3(1), dim3(256), 0, st, lc, h->corder.as<int32_t>(),
-                           h->loff.as<int64_t>(), h->nl, cur, itoff, woff, bins, nranges, iq);
+                           h->loff.as<int64_t>(), h->nl, cur, itoff, woff, bins, nranges, iq, ohist, h->nl + 1);
         hipLaunchKernelGGL(expand_items_kernel, dim3((unsigned)ceil_div(max_items, (int64_t)256)), dim3(256), 0, st, lc,
                            h->corder.as<int32_t>(), h->loff.as<int64_t>(), h->nl, cur, itoff, woff, max_items, nranges,
                            iq, items, bins);
         hipLaunchKernelGGL(probe_scatter_kernel, dim3(pchunks), dim3(256), 0, st, pr, tw, np, p_lo, npw, h->nl, cur, pq,
-                           pp, pc, psl);
+                           pp, pc, psl, lc);
         if (prec == 8) { // fp8 query residuals of every pair
             uint8_t *R8 = (uint8_t *)h->qres8.ensure((size_t)total * dp);
             float *Rn = (float *)h->qresn.ensure((size_t)total * 4);
@@ -1433,8 +1442,6 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         // queries by first probed list
         int32_t *qkey = (int32_t *)h->qkey.ensure((size_t)nq * 4);
         int32_t *qorder = (int32_t *)h->qorder.ensure((size_t)nq * 4);
-        int32_t *ohist = (int32_t *)h->ohist.ensure((size_t)(h->nl + 1) * 4);
-        PN_HIP(hipMemsetAsync(ohist, 0, (size_t)(h->nl + 1) * 4, st));
         const unsigned gq = (unsigned)ceil_div(nq, (int64_t)256);
         hipLaunchKernelGGL(order_count_kernel, dim3(gq), dim3(256), 0, st, pr, nq, np, h->nl, qkey, ohist);
         hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(256), 0, st, ohist, h->nl + 1);
