@@ -212,32 +212,14 @@ __global__ __launch_bounds__(256) void batch_rerank_kernel(const BatchEntry *__r
                  e.S * e.KL, e.kk, e.n, e.eps_coef, e.out_idx, e.out_dist, e.flags, g - e.q_base);
 }
 
+// One workgroup per query: flagged queries run the exact scan in parallel, the others exit at once.
 __global__ __launch_bounds__(256) void batch_fallback_kernel(const BatchEntry *__restrict__ E,
                                                              const int64_t *__restrict__ q_base, int ne, int64_t nq) {
     __shared__ uint64_t wres[4][32];
-    __shared__ int32_t s_list[256];
-    __shared__ int s_n;
-    // chunks of 256 queries: the flags are read in parallel, then the workgroup scans each flagged
-    // query in turn (most chunks have none)
-    for (int64_t g0 = (int64_t)blockIdx.x * 256; g0 < nq; g0 += (int64_t)gridDim.x * 256) {
-        if (threadIdx.x == 0) s_n = 0;
-        __syncthreads();
-        const int64_t g = g0 + threadIdx.x;
-        if (g < nq) {
-            const BatchEntry &e = E[find_entry(q_base, ne, g)];
-            if (e.flags[g - e.q_base]) s_list[atomicAdd(&s_n, 1)] = (int32_t)threadIdx.x;
-        }
-        __syncthreads();
-        const int nf = s_n;
-        for (int i = 0; i < nf; ++i) {
-            const int64_t gq = g0 + s_list[i];
-            const BatchEntry &e = E[find_entry(q_base, ne, gq)];
-            fallback_query<32>(wres, e.X, e.xstride, e.n, e.d, e.Q, e.qstride, e.kk, e.out_idx, e.out_dist,
-                               gq - e.q_base);
-            __syncthreads(); // wres is reused by the next query
-        }
-        __syncthreads(); // s_n and s_list are reset for the next chunk
-    }
+    if (blockIdx.x >= nq) return;
+    const BatchEntry &e = E[find_entry(q_base, ne, blockIdx.x)];
+    const int64_t q = blockIdx.x - e.q_base;
+    if (e.flags[q]) fallback_query<32>(wres, e.X, e.xstride, e.n, e.d, e.Q, e.qstride, e.kk, e.out_idx, e.out_dist, q);
 }
 
 static int kl_of(int kk) { return kk <= 6 ? 8 : (kk <= 12 ? 16 : 32); } // as the single search
@@ -452,10 +434,8 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     if (!fused)
         hipLaunchKernelGGL(batch_rerank_kernel, dim3((unsigned)ceil_div(queries, (int64_t)4)), dim3(256), 0, st, dE,
                            dQ, ne, queries);
-    hipLaunchKernelGGL(batch_fallback_kernel,
-                       dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(queries, (int64_t)256),
-                                                                              2 * (int64_t)b->cus))),
-                       dim3(256), 0, st, dE, dQ, ne, queries);
+    hipLaunchKernelGGL(batch_fallback_kernel, dim3((unsigned)std::max<int64_t>(1, queries)), dim3(256), 0, st, dE,
+                       dQ, ne, queries);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(st, &g);
     PN_HIP(ce);

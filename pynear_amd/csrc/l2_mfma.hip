@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void l2_merge_rerank_kernel(
                      out_idx, out_dist, flags, q);
 }
 
-// A workgroup per flagged query (chunks of 256 queries, grid-stride; most are not flagged).
+// One workgroup per query: flagged queries run the exact scan in parallel, the others exit at once.
 template <int K>
 __global__ __launch_bounds__(256) void l2_fallback_kernel(const float *__restrict__ X, int64_t xstride, int64_t n,
                                                           int64_t d, const float *__restrict__ Q, int64_t qstride,
@@ -135,23 +135,8 @@ __global__ __launch_bounds__(256) void l2_fallback_kernel(const float *__restric
                                                           int32_t *__restrict__ out_idx, float *__restrict__ out_dist,
                                                           int64_t nq) {
     __shared__ uint64_t wres[4][K];
-    __shared__ int32_t s_list[256];
-    __shared__ int s_n;
-    // chunks of 256 queries: the flags are read in parallel, then the workgroup scans each flagged
-    // query in turn (most chunks have none)
-    for (int64_t q0 = (int64_t)blockIdx.x * 256; q0 < nq; q0 += (int64_t)gridDim.x * 256) {
-        if (threadIdx.x == 0) s_n = 0;
-        __syncthreads();
-        const int64_t q = q0 + threadIdx.x;
-        if (q < nq && flags[q]) s_list[atomicAdd(&s_n, 1)] = (int32_t)threadIdx.x;
-        __syncthreads();
-        const int nf = s_n;
-        for (int i = 0; i < nf; ++i) {
-            fallback_query<K>(wres, X, xstride, n, d, Q, qstride, kk, out_idx, out_dist, q0 + s_list[i]);
-            __syncthreads(); // wres is reused by the next query
-        }
-        __syncthreads(); // s_n and s_list are reset for the next chunk
-    }
+    if (blockIdx.x < nq && flags[blockIdx.x])
+        fallback_query<K>(wres, X, xstride, n, d, Q, qstride, kk, out_idx, out_dist, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- small-database path
@@ -458,8 +443,7 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
     int K2 = 2 * KL;
     int K2c = (int)std::min<int64_t>(K2, n);
     const int64_t gx = ceil_div(nq, 128);
-    // fallback: chunks of 256 queries, grid-stride
-    const unsigned fb_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nq, (int64_t)256), 2 * (int64_t)cus));
+    const unsigned fb_grid = (unsigned)std::max<int64_t>(1, nq); // fallback: a workgroup per query
     const int64_t nq_pad = gx * 128;
     // queries: split into hi/lo (padded to nq_pad rows of dp; padding rows are zero)
     L2Set &qs = wk.qsplit;
