@@ -64,8 +64,9 @@ pn_status pn_stage_timing(int32_t enable);
 pn_status pn_stage_times(double *ms /*[5]*/, int64_t *launches /*[5]*/);
 
 /* Launch-shape tuning parameters (no counterpart in the reference): named integers such as
- * "hamming.wg_per_cu" that the autotuner (pynear_amd/autotune.py) sets from its measured table.
- * Unknown names return PN_EINVAL. */
+ * "hamming.wg_per_cu" in the library's table (defaults measured on MI355X, overridable through
+ * the PN_TUNE environment variable "name=value,..."); pn_batch_tune keeps its own measured
+ * schedules per batch shape. Unknown names return PN_EINVAL. */
 pn_status pn_tune_set(const char *name, int64_t value);
 pn_status pn_tune_get(const char *name, int64_t *value);
 
@@ -236,7 +237,9 @@ typedef struct pn_batch pn_batch;
  * (int32[nq[i], min(k[i], n_i)]) and out_dist[i] (float32, sqrt L2, same shape). The plan binds
  * these pointers and captures the search (one split, one persistent scan over all tiles of all
  * indices, one merge, rerank and fallback launch) in a HIP graph; its schedule comes from the
- * tuning table (batch.slice_rows, batch.wg_per_cu, batch.order). */
+ * tuning table (batch.slice_rows, batch.wg_per_cu, batch.order). The plan also binds each index's
+ * device rows as they are now: after pn_exact_set on any of its indices, pn_batch_run and
+ * pn_batch_tune return PN_EINVAL (plan a new batch); the indices must outlive the plan. */
 pn_status pn_batch_create(int32_t n, pn_exact *const *indices, const void *const *queries, const int64_t *nq,
                           const int32_t *k, int32_t *const *out_idx, float *const *out_dist, pn_batch **out);
 void pn_batch_destroy(pn_batch *b);

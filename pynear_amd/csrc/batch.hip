@@ -216,10 +216,12 @@ __global__ __launch_bounds__(256) void batch_rerank_kernel(const BatchEntry *__r
 __global__ __launch_bounds__(256) void batch_fallback_kernel(const BatchEntry *__restrict__ E,
                                                              const int64_t *__restrict__ q_base, int ne, int64_t nq) {
     __shared__ uint64_t wres[4][32];
-    if (blockIdx.x >= nq) return;
-    const BatchEntry &e = E[find_entry(q_base, ne, blockIdx.x)];
-    const int64_t q = blockIdx.x - e.q_base;
-    if (e.flags[q]) fallback_query<32>(wres, e.X, e.xstride, e.n, e.d, e.Q, e.qstride, e.kk, e.out_idx, e.out_dist, q);
+    for (int64_t g = blockIdx.x; g < nq; g += gridDim.x) { // (the grid is capped: kFallbackMaxGrid)
+        const BatchEntry &e = E[find_entry(q_base, ne, g)];
+        const int64_t q = g - e.q_base;
+        if (e.flags[q])
+            fallback_query<32>(wres, e.X, e.xstride, e.n, e.d, e.Q, e.qstride, e.kk, e.out_idx, e.out_dist, q);
+    }
 }
 
 static int kl_of(int kk) { return kk <= 6 ? 8 : (kk <= 12 ? 16 : 32); } // as the single search
@@ -229,6 +231,7 @@ static int kl_of(int kk) { return kk <= 6 ? 8 : (kk <= 12 ? 16 : 32); } // as th
 struct pn_batch {
     int device = 0, cus = 256;
     std::vector<pn_exact *> idx;
+    std::vector<uint64_t> gen; // each index's set() generation when the batch was planned
     std::vector<const float *> q;
     std::vector<int64_t> nq;
     std::vector<int32_t> k;
@@ -434,7 +437,7 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     if (!fused)
         hipLaunchKernelGGL(batch_rerank_kernel, dim3((unsigned)ceil_div(queries, (int64_t)4)), dim3(256), 0, st, dE,
                            dQ, ne, queries);
-    hipLaunchKernelGGL(batch_fallback_kernel, dim3((unsigned)std::max<int64_t>(1, queries)), dim3(256), 0, st, dE,
+    hipLaunchKernelGGL(batch_fallback_kernel, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(queries, kFallbackMaxGrid))), dim3(256), 0, st, dE,
                        dQ, ne, queries);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(st, &g);
@@ -479,6 +482,7 @@ extern "C" pn_status pn_batch_create(int32_t n, pn_exact *const *indices, const 
             PN_REQUIRE(k[i] >= 1 && k[i] <= 32, PN_EINVAL, "batched search serves 1 <= k <= 32");
             PN_REQUIRE(nq[i] >= 1 && queries[i] && out_idx[i] && out_dist[i], PN_EINVAL, "empty or null sub-search");
             b->idx.push_back(h);
+            b->gen.push_back(h->generation);
             b->q.push_back((const float *)queries[i]);
             b->nq.push_back(nq[i]);
             b->k.push_back(k[i]);
@@ -508,9 +512,18 @@ extern "C" void pn_batch_destroy(pn_batch *b) {
     if (prev >= 0) (void)hipSetDevice(prev);
 }
 
+// The plan's graph holds each index's device rows and sizes as they were at planning time: an
+// index set() again since then (new rows, a new allocation, or emptied) makes the plan invalid.
+static void require_current(const pn_batch *b) {
+    for (size_t i = 0; i < b->idx.size(); ++i)
+        PN_REQUIRE(b->idx[i]->generation == b->gen[i], PN_EINVAL,
+                   "an index of this batch was set() again after the batch was planned; create a new batch");
+}
+
 extern "C" pn_status pn_batch_run(pn_batch *b, void *stream) {
     return guard([&] {
         PN_REQUIRE(b && b->exec, PN_EINVAL, "batch is not planned");
+        require_current(b);
         DeviceScope ds(b->device);
         PN_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
     });
@@ -531,6 +544,7 @@ extern "C" pn_status pn_batch_info(const pn_batch *b, int64_t *tiles, int32_t *c
 extern "C" pn_status pn_batch_tune(pn_batch *b, const char *table_path, int32_t reps) {
     return guard([&] {
         PN_REQUIRE(b, PN_EINVAL, "batch is null");
+        require_current(b);
         DeviceScope ds(b->device);
         const std::string sig = batch_signature(b);
         if (table_path && *table_path) { // a stored schedule for this shape wins

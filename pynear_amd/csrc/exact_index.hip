@@ -73,6 +73,7 @@ void exact_init(pn_exact *h, int32_t metric) {
 void exact_set_device(pn_exact *h, const void *rows_padded, int64_t n, int64_t dim, hipStream_t st) {
     bool scan_ok = false;
     int64_t re = padded_row_elems(h->metric, dim, &scan_ok);
+    ++h->generation;
     h->dim = dim;
     h->row_elems = re;
     h->scan_ok = scan_ok;
@@ -88,6 +89,54 @@ static int list_size_for(int kk) {
     return K;
 }
 
+static int out_kind_of(int32_t metric) {
+    return metric == PN_METRIC_HAMMING ? OUT_HAMMING_I32 : (metric == PN_METRIC_L2 ? OUT_L2_F32 : OUT_F32);
+}
+
+// Shape of a sliced scan: slices S of rps rows each, queries padded to nq_pad (whole 256 * qpl
+// groups). With part_cap > 0 the per-slice lists (S * K * nq_pad u64 keys) must fit part_cap
+// bytes: S is lowered as far as the slice length limit allows; false when even that does not fit.
+struct SlicePlan {
+    int64_t S, rps, nq_pad;
+    int qpl;
+};
+static bool plan_slices(const pn_exact *h, int64_t nq, int K, size_t part_cap, SlicePlan &p) {
+    p.qpl = (h->metric == PN_METRIC_HAMMING && tune_get("hamming.qpl") == 2 && nq > 256) ? 2 : 1;
+    const int64_t gx = ceil_div(nq, 256 * p.qpl);
+    p.nq_pad = gx * 256 * p.qpl;
+    // Whole multiples of the CU count: every CU holds the same number of resident workgroups.
+    const int64_t wpc = tune_get(h->metric == PN_METRIC_HAMMING ? "hamming.wg_per_cu" : "float.wg_per_cu");
+    const int64_t target = std::max<int64_t>(1, wpc) * h->cus;
+    int64_t S = std::max<int64_t>(1, target / gx);
+    S = std::min<int64_t>(S, std::max<int64_t>(1, h->n / 64));
+    const int64_t s_min = ceil_div(h->n, kMaxRowsPerSlice);
+    if (part_cap) {
+        const int64_t per_slice = (int64_t)K * p.nq_pad * (int64_t)sizeof(uint64_t);
+        S = std::min<int64_t>(S, (int64_t)(part_cap / (size_t)per_slice));
+        if (S < s_min) return false;
+    }
+    S = std::max<int64_t>(S, s_min);
+    p.rps = ceil_div(h->n, S);
+    p.S = ceil_div(h->n, p.rps);
+    return true;
+}
+
+// The sliced scan (Hamming and the float metrics up to kFloatScanMaxDim dims, K <= 32) and the
+// merge of its per-slice lists; part holds the plan's lists.
+static void sliced_search(const pn_exact *h, const void *dq, int64_t nq, int kk, int K, const SlicePlan &p,
+                          uint64_t *part, int32_t *oi, void *od, hipStream_t st) {
+    ScanArgs a{h->db.p, h->n, h->row_elems, h->dim, dq, nq, p.rps, p.S, part, p.nq_pad, p.qpl};
+    {
+        StageScope sc(st, STAGE_SCAN);
+        if (h->metric == PN_METRIC_HAMMING)
+            hamming_scan(a, (int)h->row_elems, K, st);
+        else
+            float_scan(a, h->metric, K, st);
+    }
+    StageScope sc(st, STAGE_SELECT);
+    merge_lists(part, p.S, K, nq, p.nq_pad, kk, out_kind_of(h->metric), oi, od, st);
+}
+
 // Enqueues the search of nq padded device queries; out_idx int32[nq, kk], out_dist int32/f32.
 void exact_search_enqueue(pn_exact *h, const void *dq, int64_t nq, int kk, int32_t *oi, void *od,
                                  hipStream_t st) {
@@ -97,31 +146,11 @@ void exact_search_enqueue(pn_exact *h, const void *dq, int64_t nq, int kk, int32
                        (float *)od, h->l2w, h->cus, st);
         return;
     }
-    int out_kind = h->metric == PN_METRIC_HAMMING ? OUT_HAMMING_I32
-                                                  : (h->metric == PN_METRIC_L2 ? OUT_L2_F32 : OUT_F32);
     if (h->scan_ok && K <= 32) {
-        const int qpl = (h->metric == PN_METRIC_HAMMING && tune_get("hamming.qpl") == 2 && nq > 256) ? 2 : 1;
-        int64_t gx = ceil_div(nq, 256 * qpl);
-        int64_t nq_pad = gx * 256 * qpl;
-        // Whole multiples of the CU count: every CU holds the same number of resident workgroups.
-        int64_t wpc = tune_get(h->metric == PN_METRIC_HAMMING ? "hamming.wg_per_cu" : "float.wg_per_cu");
-        int64_t target = std::max<int64_t>(1, wpc) * h->cus;
-        int64_t S = std::max<int64_t>(1, target / gx);
-        S = std::min<int64_t>(S, std::max<int64_t>(1, h->n / 64));
-        S = std::max<int64_t>(S, ceil_div(h->n, kMaxRowsPerSlice));
-        int64_t rps = ceil_div(h->n, S);
-        S = ceil_div(h->n, rps);
-        uint64_t *part = (uint64_t *)h->part.ensure((size_t)S * K * nq_pad * sizeof(uint64_t));
-        ScanArgs a{h->db.p, h->n, h->row_elems, h->dim, dq, nq, rps, S, part, nq_pad, qpl};
-        {
-            StageScope sc(st, STAGE_SCAN);
-            if (h->metric == PN_METRIC_HAMMING)
-                hamming_scan(a, (int)h->row_elems, K, st);
-            else
-                float_scan(a, h->metric, K, st);
-        }
-        StageScope sc(st, STAGE_SELECT);
-        merge_lists(part, S, K, nq, nq_pad, kk, out_kind, oi, od, st);
+        SlicePlan p;
+        plan_slices(h, nq, K, 0, p);
+        uint64_t *part = (uint64_t *)h->part.ensure((size_t)p.S * K * p.nq_pad * sizeof(uint64_t));
+        sliced_search(h, dq, nq, kk, K, p, part, oi, od, st);
         return;
     }
     // Distance-matrix path: bound the matrix to ~1 GiB of keys per chunk.
@@ -162,6 +191,7 @@ extern "C" pn_status pn_exact_set(pn_exact *h, const void *rows, int64_t n, int6
         PN_REQUIRE(h, PN_EINVAL, "index is null");
         PN_REQUIRE(n >= 0 && dim >= 0, PN_EINVAL, "negative shape");
         DeviceScope ds(h->device);
+        ++h->generation; // (whatever follows: the old rows are gone)
         if (n == 0) { // VPTree::set on an empty array leaves the index empty (VPTree.hpp:93-94)
             h->n = 0;
             h->dim = 0;

@@ -10,6 +10,7 @@ struct pn_exact {
     int64_t n = 0, dim = 0;      // rows; row length in floats (float metrics) or bytes (Hamming)
     int64_t row_elems = 0;       // padded row length on the device (floats or 32-bit words)
     bool scan_ok = false;        // the sliced scan kernel serves this row width
+    uint64_t generation = 0;     // bumped by every set(): plans that bound the rows check it
     pn::DevBuf db, qbuf, part, oidx, odist, ws;
     int cus = 256;
     pn::L2Set l2;                // bf16 split of the rows (L2 metric)

@@ -135,8 +135,8 @@ __global__ __launch_bounds__(256) void l2_fallback_kernel(const float *__restric
                                                           int32_t *__restrict__ out_idx, float *__restrict__ out_dist,
                                                           int64_t nq) {
     __shared__ uint64_t wres[4][K];
-    if (blockIdx.x < nq && flags[blockIdx.x])
-        fallback_query<K>(wres, X, xstride, n, d, Q, qstride, kk, out_idx, out_dist, blockIdx.x);
+    for (int64_t q = blockIdx.x; q < nq; q += gridDim.x) // (the grid is capped: kFallbackMaxGrid)
+        if (flags[q]) fallback_query<K>(wres, X, xstride, n, d, Q, qstride, kk, out_idx, out_dist, q);
 }
 
 // ---------------------------------------------------------------- small-database path
@@ -443,7 +443,9 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
     int K2 = 2 * KL;
     int K2c = (int)std::min<int64_t>(K2, n);
     const int64_t gx = ceil_div(nq, 128);
-    const unsigned fb_grid = (unsigned)std::max<int64_t>(1, nq); // fallback: a workgroup per query
+    // fallback: a workgroup per query, up to kFallbackMaxGrid workgroups (each then takes every
+    // kFallbackMaxGrid-th query), so the grid stays within the launch limits at any query count
+    const unsigned fb_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nq, kFallbackMaxGrid));
     const int64_t nq_pad = gx * 128;
     // queries: split into hi/lo (padded to nq_pad rows of dp; padding rows are zero)
     L2Set &qs = wk.qsplit;

@@ -617,7 +617,12 @@ __device__ __forceinline__ void rerank_regs(const float *__restrict__ X, int64_t
         if (n > K2c) {
             const uint32_t bnd = min(bmin, al == kKey32None ? kKey32None : (al & ~31u));
             const float bound = bnd == kKey32None ? INFINITY : key_f32(bnd);
-            flag = !(bound - eps > dk_core);
+            // every row outside the candidates has core >= bound - eps; the output orders by
+            // sqrtf(core), and a root equal to the k-th's would tie and could win on its lower
+            // index, so the roots must differ as well (taken from just below bound - eps, which
+            // absorbs the subtraction's rounding; the roots are monotone in their arguments)
+            const float lo = fmaxf(bound - eps, 0.f);
+            flag = !(lo > dk_core && sqrtf(nextafterf(lo, 0.f)) > sqrtf(dk_core));
         }
         flags[q] = flag;
     }
@@ -681,6 +686,7 @@ __device__ __forceinline__ void fallback_query(uint64_t (*wres)[K], const float 
             out_dist[q * kk + r] = key_f32((uint32_t)(m >> 32));
         }
     }
+    __syncthreads(); // wres is rewritten by the workgroup's next query
 }
 
 // ---------------------------------------------------------------- host side

@@ -129,3 +129,25 @@ def test_batch_rejects_unsupported():
         BatchSearch([(idx, Q, 33)])
     with pytest.raises(ValueError):
         BatchSearch([(idx, Q[:, :4], 3)])
+
+
+@pytest.mark.gpu
+def test_batch_refuses_to_run_after_an_index_is_set_again():
+    """The plan binds each index's device rows: set() on an index after planning (a larger set
+    reallocates the rows, an empty one releases them) makes run() and tune() raise instead of
+    replaying over stale memory; a new plan over the same indices serves the new rows."""
+    subs = _make(5, [(2000, 64, 20, 5), (1500, 128, 10, 10)])
+    b = BatchSearch([(idx, Q, k) for idx, X, Q, k in subs])
+    b.run()
+    idx0, X0, Q0, k0 = subs[0]
+    X_new = np.random.default_rng(6).standard_normal((5000, 64), dtype=np.float32)
+    idx0.set(X_new)
+    with pytest.raises(ValueError, match="set"):
+        b.run()
+    with pytest.raises(ValueError, match="set"):
+        b._plan.tune(None, reps=1)
+    b2 = BatchSearch([(idx, Q, k) for idx, X, Q, k in subs])
+    b2.run()
+    bi, bd = b2.results()[0]
+    ri, rd = oracle.knn_float(oracle.L2, X_new, Q0, k0)
+    assert np.array_equal(bi, ri) and np.array_equal(bd, rd)
