@@ -106,6 +106,17 @@ pn_status pn_exact_search(pn_exact *h, const void *queries, int64_t nq, int64_t 
 pn_status pn_exact_search_device(pn_exact *h, const void *queries, int64_t nq, int32_t k, int32_t *out_idx,
                                  void *out_dist, void *stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): pn_exact_search_device
+ * with no allocation at all. Its scratch (per-slice lists, zero-padded queries) is a fixed 32 MiB
+ * device array that each GPU holds for the life of the process, so calls that overlap on one GPU
+ * race, whatever their handles and streams. Serves k <= 32 for the rows the sliced scan takes:
+ * Hamming rows of up to 128 bytes, L2 / L1 / Chebyshev rows of up to 64 floats; other shapes, and
+ * query counts whose lists exceed the scratch, return PN_EINVAL. Results are identical to
+ * pn_exact_search_device's. */
+pn_status pn_exact_search_device_noalloc(pn_exact *h, const void *queries, int64_t nq, int32_t k, int32_t *out_idx,
+                                         void *out_dist, void *stream);
+
 /* ---------------------------------------------------------------- k-means
  * kmeans_l2(data, k, max_iter, seed) (PythonBindings.cpp:305-329, KMeans.hpp:25-126): host data
  * n x d float32; writes labels int32[n] and centroids float32[k, d]. The result is identical to

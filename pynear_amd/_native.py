@@ -40,6 +40,7 @@ SIGNATURES = {
     "pn_exact_info": (_i32, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "pn_exact_search": (_i32, [_vp, _vp, _i64, _i64, _i32, _vp, _vp]),
     "pn_exact_search_device": (_i32, [_vp, _vp, _i64, _i32, _vp, _vp, _vp]),
+    "pn_exact_search_device_noalloc": (_i32, [_vp, _vp, _i64, _i32, _vp, _vp, _vp]),
     "pn_kmeans_l2": (_i32, [_vp, _i64, _i64, _i64, _i64, _u32, _vp, _vp]),
     "pn_ivfl2_create": (_i32, [ctypes.POINTER(_vp)]),
     "pn_ivfl2_destroy": (None, [_vp]),
@@ -162,6 +163,11 @@ class ExactHandle:
     def search_device(self, q_ptr, nq, k, out_idx_ptr, out_dist_ptr, stream_ptr=None):
         check(lib().pn_exact_search_device(self._h, _vp(q_ptr), nq, int(k), _vp(out_idx_ptr),
                                            _vp(out_dist_ptr), _vp(stream_ptr) if stream_ptr else None))
+
+    def search_device_noalloc(self, q_ptr, nq, k, out_idx_ptr, out_dist_ptr, stream_ptr=None):
+        """search_device through the allocation-free entry point (fixed per-GPU scratch)."""
+        check(lib().pn_exact_search_device_noalloc(self._h, _vp(q_ptr), nq, int(k), _vp(out_idx_ptr),
+                                                   _vp(out_dist_ptr), _vp(stream_ptr) if stream_ptr else None))
 
 
 def tune_set(name, value):

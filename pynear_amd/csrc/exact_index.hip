@@ -9,6 +9,7 @@
 // float metrics; zero padding is neutral in every metric), contiguous, row-major.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -278,6 +279,79 @@ extern "C" pn_status pn_exact_search_device(pn_exact *h, const void *queries, in
             dq = tmp;
         }
         exact_search_enqueue(h, dq, nq, kk, out_idx, out_dist, st);
+    });
+}
+
+// ---------------------------------------------------------------- search without allocation
+// pn_exact_search_device_noalloc keeps its per-slice lists and padded queries in `scratch`, a
+// fixed device array (each GPU holds its own copy for the life of the process); its device
+// address on each GPU is taken once with hipGetSymbolAddress and used from then on.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+static void *noalloc_scratch(int dev) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void *ptrs[kMaxDevices] = {};
+    PN_REQUIRE(dev >= 0 && dev < kMaxDevices, PN_EDEVICE, "device id out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptrs[dev]) { // (the caller has made dev current)
+        void *p = nullptr;
+        PN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        PN_REQUIRE(p, PN_EDEVICE, "no device address for the search scratch");
+        ptrs[dev] = p;
+    }
+    return ptrs[dev];
+}
+
+// Rows of in_b bytes -> rows of out_b bytes (a multiple of 4, >= in_b) with a zero tail; one
+// thread per 32-bit output word. (A kernel, not a memset, so the call can sit in a captured graph.)
+__global__ void pad_rows_kernel(const uint8_t *__restrict__ src, int64_t n, int64_t in_b, int64_t out_w,
+                                uint32_t *__restrict__ dst) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * out_w) return;
+    const int64_t r = t / out_w, b0 = 4 * (t - r * out_w);
+    const uint8_t *s = src + r * in_b;
+    uint32_t v = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (b0 + j < in_b) v |= (uint32_t)s[b0 + j] << (8 * j);
+    dst[t] = v;
+}
+
+extern "C" pn_status pn_exact_search_device_noalloc(pn_exact *h, const void *queries, int64_t nq, int32_t k,
+                                                    int32_t *out_idx, void *out_dist, void *stream) {
+    return guard([&] {
+        PN_REQUIRE(h, PN_EINVAL, "index is null");
+        PN_REQUIRE(h->n > 0, PN_ERUNTIME, "index must be first initialized with .set() function and non empty dataset");
+        PN_REQUIRE(nq >= 0 && k >= 0, PN_EINVAL, "negative size");
+        const int kk = (int)std::min<int64_t>(k, h->n);
+        if (nq == 0 || kk == 0) return;
+        PN_REQUIRE(queries && out_idx && out_dist, PN_EINVAL, "null pointer");
+        const int K = list_size_for(kk);
+        PN_REQUIRE(h->scan_ok && K <= 32, PN_EINVAL,
+                   "search without allocation serves k <= 32 and rows the sliced scan takes (Hamming rows of the "
+                   "scan's widths, float rows of at most 64 dims)");
+        DeviceScope ds(h->device);
+        hipStream_t st = (hipStream_t)stream;
+        uint8_t *base = (uint8_t *)noalloc_scratch(h->device);
+        const size_t in_b = row_bytes_in(h->metric, h->dim);
+        const size_t pad_b = (size_t)h->row_elems * 4;
+        const size_t qbytes = in_b != pad_b ? (nq * pad_b + 255) / 256 * 256 : 0; // padded queries first
+        PN_REQUIRE(qbytes < kNoAllocScratchBytes, PN_EINVAL, "queries exceed the 32 MiB search scratch");
+        SlicePlan p;
+        PN_REQUIRE(plan_slices(h, nq, K, kNoAllocScratchBytes - qbytes, p), PN_EINVAL,
+                   "query count exceeds the 32 MiB search scratch of the allocation-free search");
+        const void *dq = queries;
+        if (qbytes) {
+            const int64_t words = (int64_t)(pad_b / 4);
+            hipLaunchKernelGGL(pad_rows_kernel, dim3((unsigned)ceil_div(nq * words, (int64_t)256)), dim3(256), 0, st,
+                               (const uint8_t *)queries, nq, (int64_t)in_b, words, (uint32_t *)base);
+            PN_HIP(hipGetLastError());
+            dq = base;
+        }
+        sliced_search(h, dq, nq, kk, K, p, (uint64_t *)(base + qbytes), out_idx, out_dist, st);
+        PN_HIP(hipGetLastError());
     });
 }
 
