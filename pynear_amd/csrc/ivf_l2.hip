@@ -116,6 +116,7 @@ constexpr int kIvfQS = 72;  // LDS row stride (bf16) of a staged query chunk: 16
 // first item, its item count and a zeroed work counter; bins[3 kXcds] the item total,
 // bins[3 kXcds + 1] the pair total and bins[3 kXcds + 2 ..] (as int64) the total work.
 constexpr int kXcds = 8;
+constexpr int kBinsInts = 3 * kXcds + 4;
 __global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restrict__ lcount,
                                                           const int32_t *__restrict__ corder,
                                                           const int64_t *__restrict__ loff, int nl,
@@ -313,6 +314,23 @@ __global__ __launch_bounds__(256) void pass_bound_kernel(const uint64_t *__restr
     }
     const float v = key_f32((uint32_t)(m >> 32) | 31u) * margin; // (the key's largest value)
     qthr[q] = isfinite(v) ? (f32_key(v) | 31u) + 1u : kKey32None;
+}
+
+// One wave per query: the bound of a later pass is the K2-th smallest candidate key of the
+// query's first probe (cap <= 64 slots, sorted or not). At least K2 candidates of the first probe
+// are at or below it, so no candidate above it is among the K2 the rerank takes; the scans keep
+// keys below qthr = its 32-bit key + 1. kKey32None when the first probe has fewer than K2.
+__global__ __launch_bounds__(256) void pass_bound_k2_kernel(const uint64_t *__restrict__ cand, int64_t nq, int np,
+                                                            int cap, const int32_t *__restrict__ qoff, int K2,
+                                                            uint32_t *__restrict__ qthr) {
+    const int lane = threadIdx.x & 63;
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= nq) return;
+    const uint64_t *src = cand + cand_slot(qoff, q, np, 0) * cap;
+    uint64_t v[1] = {lane < cap ? src[lane] : kKey64None};
+    wave_sort_u64<1, 2>(v, lane);
+    const uint64_t m = __shfl(v[0], K2 - 1, 64);
+    if (lane == 0) qthr[q] = m == kKey64None ? kKey32None : (uint32_t)(m >> 32) + 1u;
 }
 
 // The next work item of a workgroup of range xb: from its own range while that has items, then
@@ -519,6 +537,123 @@ union V8 {
     uint4 u[2];
 };
 
+// Row block done in the fp8 scan: keys of the wave's two 32-row tiles (rows tb0 .. tb0 + 63, tb0
+// a multiple of 32; rows outside [r0, r1) are masked, tiles at or past r1 skipped) for its two
+// query tiles, into the lane's lists. Lane (col, half) holds query column qcol of tile j and rows
+// 32 i + 8 (r / 4) + 4 half + r % 4 of element r. v = max(0, fma(m2, acc, xr) + qv) approximates
+// |q - x|^2 (m2 = -2 / scale^2, xr = |x - c|^2 from xs[row - tb0], qv = |q - c|^2). A (tile,
+// query) is done after one test when max(0, min_r fma(m2, acc_r, xr_r) + qv), a lower bound of
+// its 16 values whatever rows are masked, is at or above the threshold: most tiles, once the
+// lists hold good keys or under a bound from an earlier pass. A key is v's bits with the low 5
+// bits replaced by the row's offset in its tile. Thresholds: the lane's own, its half-wave
+// partner's and the smallest any list of the query posted in sthr[query] (LDS), where the lane
+// posts its own. FIRST: the lists are empty.
+//
+// Keys here carry the element index r (0 .. 15, an inline constant) in their low 4 bits instead
+// of the row offset (so no per-element offsets live in registers); the row, tb + 4 half +
+// 8 (r / 4) + r % 4, is formed only when a key enters a list (list_insert16r).
+template <int KL, bool FIRST>
+__device__ __forceinline__ void list_insert16r(uint32_t (&key)[16], uint32_t (&Lk)[KL], uint32_t (&Li)[KL],
+                                               uint32_t base_h, uint32_t &thr) {
+    auto row_of = [&](uint32_t k) { return base_h + (k & 3u) + ((k & 12u) << 1); };
+    if (FIRST) {
+        sort16(key);
+#pragma unroll
+        for (int j = 0; j < KL; ++j) {
+            Lk[j] = j < 16 && key[j] < thr ? key[j] : kKey32None;
+            Li[j] = j < 16 ? row_of(key[j]) : 0u;
+        }
+        thr = min(thr, Lk[KL - 1]);
+        return;
+    }
+    uint32_t mn = kKey32None;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) mn = min(mn, key[r]);
+    while (__any(mn < thr)) {
+        if (mn < thr) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) key[r] = key[r] == mn ? kKey32None : key[r];
+            const uint32_t ir = row_of(mn);
+#pragma unroll
+            for (int j = KL - 1; j > 0; --j) {
+                const bool up = mn < Lk[j - 1];
+                const bool here = mn < Lk[j];
+                Li[j] = up ? Li[j - 1] : (here ? ir : Li[j]);
+                Lk[j] = max(Lk[j - 1], min(mn, Lk[j]));
+            }
+            if (mn < Lk[0]) {
+                Li[0] = ir;
+                Lk[0] = mn;
+            }
+            thr = min(thr, Lk[KL - 1]);
+        }
+        mn = kKey32None;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) mn = min(mn, key[r]);
+    }
+}
+
+template <int KL>
+__device__ __forceinline__ void fp8_rowblock_keys(const f32x16 (&acc)[2][2], float m2, const float *xs, bool xs_lds,
+                                                  int64_t tb0, int64_t r0, int64_t r1, bool first, float qv0,
+                                                  float qv1, uint32_t (&Lk)[2][KL], uint32_t (&Li)[2][KL],
+                                                  uint32_t *sthr, int qcol, int half, bool two) {
+    uint32_t thr[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) thr[j] = min(Lk[j][KL - 1], sthr[qcol + 32 * j]);
+    const int h4 = 4 * half;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int64_t tb = tb0 + 32 * i;
+        if (tb >= r1) break; // (wave-uniform; so is the next tile)
+        // the tile's rows inside the list are offsets lo .. hi - 1 (all 32 except at a list's ends)
+        const int lo = (int)max<int64_t>(r0 - tb, 0), hi = (int)min<int64_t>(r1 - tb, 32);
+        const bool edge = lo > 0 || hi < 32;
+        float xr[16];
+        if (xs_lds) {
+            typedef const __attribute__((address_space(3))) float lds_float;
+            lds_float *x = (lds_float *)xs + 32 * i + 4 * half;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) xr[r] = x[8 * (r >> 2) + (r & 3)];
+        } else {
+            const float *x = xs + 32 * i + 4 * half;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) xr[r] = x[8 * (r >> 2) + (r & 3)];
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (j == 1 && !two) break;
+            const float qv = j == 0 ? qv0 : qv1;
+            float u[16];
+            float umin = INFINITY;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                u[r] = fmaf(m2, acc[i][j][r], xr[r]);
+                umin = fminf(umin, u[r]);
+            }
+            thr[j] = min(thr[j], (uint32_t)__shfl_xor((int)thr[j], 32, 64));
+            if (!__any((f32_key(fmaxf(0.f, umin + qv)) & ~15u) < thr[j])) continue;
+            uint32_t key[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) key[r] = (f32_key(fmaxf(0.f, u[r] + qv)) & ~15u) | (uint32_t)r;
+            if (edge) { // (uniform; a list's first and last tiles)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int off = h4 + 8 * (r >> 2) + (r & 3);
+                    key[r] = off >= lo && off < hi ? key[r] : kKey32None;
+                }
+            }
+            if (first && i == 0)
+                list_insert16r<KL, true>(key, Lk[j], Li[j], (uint32_t)tb + h4, thr[j]);
+            else
+                list_insert16r<KL, false>(key, Lk[j], Li[j], (uint32_t)tb + h4, thr[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+        if (j == 0 || two) atomicMin(&sthr[qcol + 32 * j], thr[j]);
+}
+
 // fp8 rows are stored by 32-row groups and 64-dim chunks: group g, chunk ch is one 2 KB block of
 // its 32 rows' 64 bytes, so a wave's fragment loads of one chunk read whole blocks.
 __device__ __forceinline__ int64_t fp8_row_chunk(int64_t row, int ch, int nch) {
@@ -668,32 +803,41 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
 // |q - c|^2 in fp32) into LDS once; the K loop then needs no staging: per 64-dim chunk one
 // v_mfma_scale_f32_32x32x64_f8f6f4 (unit scales) per (row tile, query tile), lane l holding
 // bytes 32 (l/32) .. +31 of row / query l % 32 of the chunk; the products are scaled back by
-// 1 / scale^2. Row fragments (fp8_row_chunk layout) are loaded two chunks ahead, across row
-// blocks. Keys, lists and the final merge are the bf16 scan's (8 lists per query in two groups,
-// 16 in one).
-template <int KL>
+// 1 / scale^2. The list's rows are taken as whole 32-row groups of the fp8_row_chunk layout (a
+// list's first and last groups may hold rows of its neighbours, which are masked), so a wave's
+// tile and chunk name one 2 KB block at a wave-uniform address; fragments are loaded two chunks
+// ahead, across row blocks. Keys: fp8_rowblock_keys (one threshold per query shared by its
+// lists); lists and the final merge are the bf16 scan's (8 lists per query in two groups, 16 in
+// one). PD: how many steps ahead the fragments are loaded (2 or 3). (Staging each step's rows once
+// in LDS for both query groups, which halves the L2 reads, measured slower: a barrier per step.)
+template <int KL, int PD>
 __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
     const uint8_t *__restrict__ L8, const float *__restrict__ rn, const int64_t *__restrict__ loff, int dp,
     float scale, const uint8_t *__restrict__ R8, const float *__restrict__ Rn,
     const int4 *__restrict__ items, int32_t *__restrict__ bins, int nranges, const int32_t *__restrict__ pair_q,
     const int32_t *__restrict__ pair_p, int np, const int32_t *__restrict__ qoff, const uint32_t *__restrict__ qthr,
     uint64_t *__restrict__ cand, int nmax) {
-    constexpr int IQ = kIvf8IQ, NL = 16;
-    extern __shared__ __align__(16) unsigned char dsm[]; // IQ x (dp + 16) bytes, then nmax floats
-    float *s_norm = (float *)(dsm + IQ * (dp + 16)); // the list's row norms when it has <= nmax rows
+    constexpr int IQ = kIvf8IQ;
+    // IQ x (dp + 16) bytes of query residuals, then the list's row norms (nmax floats)
+    extern __shared__ __align__(16) unsigned char dsm[];
+    float *s_norm = (float *)(dsm + IQ * (dp + 16)); // the list's groups' row norms (<= nmax rows)
     __shared__ int32_t s_q[IQ];
-    __shared__ uint32_t s_thr[NL * IQ / 2]; // wide: [8][128]; narrow: [16][64]
+    __shared__ uint32_t s_thr[IQ]; // per query: the smallest threshold any of its lists posted
     __shared__ float s_qr[IQ];
     __shared__ int s_slot;
     const int QS = dp + 16; // LDS row stride of the residual block (16 B of pad spreads the banks)
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5, col = lane & 31;
-    const float dscale = 1.f / (scale * scale);
+    const int tid = threadIdx.x, lane = tid & 63, half = lane >> 5, col = lane & 31;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6); // (wave-uniform: tile addresses in SGPRs)
+    const float m2 = -2.f / (scale * scale);
     const int xb = nranges == 1 ? 0 : blockIdx.x % kXcds;
     for (;;) {
         if (tid == 0) s_slot = take_item(bins, nranges, xb);
         __syncthreads();
-        if (s_slot < 0) break;
-        const int4 it = items[s_slot];
+        // (read into a scalar register: the item and every list bound and tile address derived
+        // from it are wave-uniform, computed on the scalar unit)
+        const int slot = __builtin_amdgcn_readfirstlane(s_slot);
+        if (slot < 0) break;
+        const int4 it = items[slot];
         const int c = it.x, start = it.y, cnt = it.z;
         const bool wide = cnt > 64;
         const int qg = wide ? w >> 2 : 0, wr = wide ? w & 3 : w; // query group, row group
@@ -701,9 +845,10 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
         const int q0 = 64 * qg;                 // the wave's first query
         const bool two = cnt - q0 > 32;         // (else its second query tile is padding)
         const int nrows = wide ? IQ : 64;       // query rows in LDS
-        if (tid < IQ) s_q[tid] = pair_q[start + min(tid, cnt - 1)];
-        for (int e = tid; e < NL * IQ / 2; e += 512)
-            s_thr[e] = qthr ? qthr[pair_q[start + min(e % tstride, cnt - 1)]] : kKey32None;
+        if (tid < IQ) {
+            s_q[tid] = pair_q[start + min(tid, cnt - 1)];
+            s_thr[tid] = qthr ? qthr[pair_q[start + min(tid, cnt - 1)]] : kKey32None;
+        }
         { // the item's query residuals (pair slots start ..) into LDS; padding rows repeat the last
             const int upr = dp / 16;
             for (int u = tid; u < nrows * upr; u += 512) {
@@ -714,24 +859,29 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
             if (tid < nrows) s_qr[tid] = Rn[start + min(tid, cnt - 1)];
         }
         const int64_t r0 = loff[c], r1 = loff[c + 1];
-        const bool norm_lds = r1 - r0 <= nmax;
+        // the list's 32-row groups g0 .. g1; a row block is 2 nrg of them, two per wave
+        const int64_t g0 = r0 >> 5, g1 = (r1 - 1) >> 5;
+        const int ngr = (int)(g1 - g0 + 1);
+        const bool norm_lds = 32 * ngr <= nmax; // (the norms are padded to whole groups)
         if (norm_lds)
-            for (int e = tid; e < (int)(r1 - r0); e += 512) s_norm[e] = rn[r0 + e];
+            for (int e = tid; e < 32 * ngr; e += 512) s_norm[e] = rn[32 * g0 + e];
         __syncthreads();
         const int nch = dp / kIvfKC;
-        const int RB = 64 * nrg; // rows per block
-        const int nrb = (int)((r1 - r0 + RB - 1) / RB);
+        const int GB = 2 * nrg; // groups per row block
+        const int nrb = (ngr + GB - 1) / GB;
         const float qr0 = s_qr[q0 + col], qr1 = s_qr[q0 + 32 + col];
         // row fragments of step g = (row block g / nch, chunk g % nch), loaded two steps ahead into
         // a ring of three register sets that rotate by name (three steps per loop trip): a copy
-        // between sets would wait for the loads in flight
+        // between sets would wait for the loads in flight. Groups past g1 are clamped to it (their
+        // keys are skipped).
         V8 fa[2], fb[2], fc[2];
+        const uint8_t *lane_l8 = L8 + col * 64 + 32 * half;
         auto load_a = [&](int g, V8 (&a)[2]) {
             const int rb = g / nch, ch = g - rb * nch;
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
-                const int64_t row = min(r0 + (int64_t)RB * rb + 64 * wr + 32 * i + col, r1 - 1);
-                const uint4 *src = (const uint4 *)(L8 + fp8_row_chunk(row, ch, nch) + 32 * half);
+                const int64_t grp = min(g0 + (int64_t)GB * rb + 2 * wr + i, g1);
+                const uint4 *src = (const uint4 *)(lane_l8 + ((grp * nch + ch) << 11));
                 a[i].u[0] = src[0];
                 a[i].u[1] = src[1];
             }
@@ -747,10 +897,9 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
             }
         f32x16 acc[2][2];
         int ch = 0, rb = 0;
-        // one step: the load two steps ahead, the chunk's products, and after a row block's last
-        // chunk its keys
-        auto step = [&](int g, const V8 (&cur)[2], V8 (&ahead)[2]) {
-            load_a(min(g + 2, nsteps - 1), ahead); // (unconditional, so the waits stay exact)
+        // one chunk: its products, and after a row block's last chunk its keys (row norms from
+        // xs_blk(rb): LDS or global)
+        auto chunk = [&](const V8 (&cur)[2], const float *xs_blk, bool xs_lds) {
             if (ch == 0)
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
@@ -771,20 +920,42 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
                         acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cur[i].v, b[j].v, acc[i][j], 0, 0,
                                                                                0, 127, 0, 127);
             if (++ch == nch) {
-                rowblock_keys<KL>(acc, dscale, dscale, rn, r0 + (int64_t)RB * rb + 64 * wr, r1, rb == 0, qr0, qr1, Lk,
-                                  Li, s_thr, tstride, nlist, wr * 2 + half, q0 + col, half, two, s_norm, norm_lds, r0);
+                const int64_t grp = g0 + (int64_t)GB * rb + 2 * wr; // the wave's first group of the block
+                fp8_rowblock_keys<KL>(acc, m2, xs_blk, xs_lds, 32 * grp, r0, r1, rb == 0, qr0, qr1, Lk, Li, s_thr,
+                                      q0 + col, half, two);
                 ch = 0;
                 ++rb;
             }
         };
-        // Whole trips: up to two steps past the end run on clamped rows of the last chunk and end
-        // in a row block past r1, whose keys are all masked.
-        load_a(0, fa);
-        load_a(min(1, nsteps - 1), fb);
-        for (int g = 0; g < nsteps; g += 3) {
-            step(g, fa, fc);
-            step(g + 1, fb, fa);
-            step(g + 2, fc, fb);
+        {
+            // one step: the load PD steps ahead, then the chunk. Whole trips: up to PD steps past
+            // the end run on clamped rows of the last chunk and end in a row block past r1, whose
+            // keys are skipped. (With three steps ahead in four sets, the compiler's waits, which
+            // across the trip's branches keep one step fewer in flight than the ring allows, still
+            // leave two.)
+            auto step = [&](int g, const V8 (&cur)[2], V8 (&ahead)[2]) {
+                load_a(min(g + PD, nsteps - 1), ahead); // (unconditional, so the waits stay exact)
+                const int64_t grp = g0 + (int64_t)GB * rb + 2 * wr;
+                chunk(cur, norm_lds ? s_norm + 32 * (grp - g0) : rn + 32 * grp, norm_lds);
+            };
+            load_a(0, fa);
+            load_a(min(1, nsteps - 1), fb);
+            if constexpr (PD == 2) {
+                for (int g = 0; g < nsteps; g += 3) {
+                    step(g, fa, fc);
+                    step(g + 1, fb, fa);
+                    step(g + 2, fc, fb);
+                }
+            } else {
+                V8 fd[2];
+                load_a(min(2, nsteps - 1), fc);
+                for (int g = 0; g < nsteps; g += 4) {
+                    step(g, fa, fd);
+                    step(g + 1, fb, fa);
+                    step(g + 2, fc, fb);
+                    step(g + 3, fd, fc);
+                }
+            }
         }
         // the lists go to LDS (the residual block is no longer read), then each query's nlist x KL
         // entries are sorted across one wave and its kIvf8Cap = 64 best go to its (query, probe)
@@ -803,21 +974,25 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
             const int l = m / KL, e = m % KL;
             return sl[(l * tstride + t) * KL + ((l & 1) ? KL - 1 - e : e)];
         };
+        // (a query whose lists are all empty, as under a bound most are, skips the network: the
+        // sorts are chains of dependent shuffles, as long as a good part of an item's products)
         for (int t = w; t < cnt; t += 8) {
             uint64_t *out = cand + cand_slot(qoff, s_q[t], np, pair_p[start + t]) * kIvf8Cap;
             if (nlist * KL <= 64) { // two query groups (8 lists), KL = 8: one entry per lane
                 uint64_t v[1] = {entry(lane, t)};
-                wave_sort_u64<1, 2 * KL>(v, lane);
+                if (__any(v[0] != kKey64None)) wave_sort_u64<1, 2 * KL>(v, lane);
                 out[lane] = v[0];
             } else {
                 constexpr int E = KL / 4; // 16 lists x KL entries, E per lane
                 uint64_t v[E];
+                bool any = false;
 #pragma unroll
                 for (int s2 = 0; s2 < E; ++s2) {
                     const int m = lane + 64 * s2;
                     v[s2] = m < nlist * KL ? entry(m, t) : kKey64None;
+                    any = any || v[s2] != kKey64None;
                 }
-                wave_sort_u64<E, 2 * KL>(v, lane);
+                if (__any(any)) wave_sort_u64<E, 2 * KL>(v, lane);
                 out[lane] = v[0]; // the 64 best
             }
         }
@@ -863,8 +1038,9 @@ __global__ __launch_bounds__(256, 2) void ivfl2_scan_kernel(
     for (;;) {
     if (tid == 0) s_slot = take_item(bins, nranges, xb);
     __syncthreads();
-    if (s_slot < 0) break;
-    const int4 it = items[s_slot];
+    const int slot = __builtin_amdgcn_readfirstlane(s_slot); // (wave-uniform item: scalar addressing)
+    if (slot < 0) break;
+    const int4 it = items[slot];
     const int c = it.x, start = it.y, cnt = it.z;
     const bool two = cnt > 32; // (else the second query tile is padding)
     if (tid < IQ) s_q[tid] = pair_q[start + min(tid, cnt - 1)];
@@ -1044,7 +1220,7 @@ __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restri
                                                            int cap, const int32_t *__restrict__ qoff,
                                                            int64_t nq, int k, int32_t *__restrict__ out_idx,
                                                            float *__restrict__ out_dist, int32_t *__restrict__ out_cnt,
-                                                           const int32_t *__restrict__ qorder, int64_t per_xcd) {
+                                                           const int32_t *__restrict__ qorder, int64_t per_xcd, int k2) {
     __shared__ uint64_t s_sort[4][256];
     const int lane = threadIdx.x & 63;
     // workgroup b runs on XCD b % 8: each XCD takes a contiguous run of qorder
@@ -1098,7 +1274,7 @@ __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restri
         wave_sort_u64<4, 2>(v, lane);
         mine = v[0];
     } else {
-        for (int r = 0; r < kIvfK2; ++r) {
+        for (int r = 0; r < k2; ++r) {
             const uint64_t m = wmin64(L[0]);
             if (m == kKey64None) break;
             if (L[0] == m) {
@@ -1109,13 +1285,14 @@ __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restri
             if (lane == r) mine = m;
         }
     }
+    if (lane >= k2) mine = kKey64None; // (the k2 best are reranked)
     const int32_t mypos = mine == kKey64None ? -1 : (int32_t)(uint32_t)(mine & 0xFFFFFFFFull);
     const int32_t myid = mypos < 0 ? -1 : lids[mypos];
     const float *qrow = Q + q * qstride;
     uint64_t key = kKey64None;
     const int g = lane >> 4, c = lane & 15;
 #pragma unroll 1
-    for (int round = 0; round < kIvfK2 / 4; ++round) {
+    for (int round = 0; round < (k2 + 3) / 4; ++round) {
         const int32_t pos = __shfl(mypos, 4 * round + g, 64); // group g: candidate 4 round + g
         float part = 0.f;
         if (pos >= 0) {
@@ -1243,8 +1420,12 @@ static void ivfl2_set_device(pn_ivfl2 *h, const float *dX, int64_t n, int64_t d,
         std::memcpy(&m, &bits, 4);
         h->sx8 = m > 0.f ? std::exp2(std::floor(std::log2(224.f / m))) : 1.f;
         const int dp = h->lrows.dp;
-        uint8_t *L8 = (uint8_t *)h->l8.ensure((size_t)ceil_div(n, (int64_t)32) * 32 * dp); // whole 32-row groups
-        float *rn = (float *)h->rn8.ensure((size_t)n * 4);
+        // whole 32-row groups (the scan reads a list's groups whole); the rows past n stay zero
+        const int64_t npad = ceil_div(n, (int64_t)32) * 32;
+        uint8_t *L8 = (uint8_t *)h->l8.ensure((size_t)npad * dp);
+        float *rn = (float *)h->rn8.ensure((size_t)npad * 4);
+        PN_HIP(hipMemsetAsync(L8, 0, (size_t)npad * dp, st)); // (the last group's rows interleave by chunk)
+        PN_HIP(hipMemsetAsync(rn + n, 0, (size_t)(npad - n) * 4, st));
         hipLaunchKernelGGL(residual_kernel<1>, g, dim3(256), 0, st, Xo, n, (int)d, dp, dof, nl, lc, h->sx8, dm, L8, rn);
         PN_HIP(hipGetLastError());
     }
@@ -1315,7 +1496,7 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     // each cluster has at most one partly filled item
     const int64_t max_items = ceil_div(total, kIvfIQ) + h->nl;
     int4 *items = (int4 *)h->items.ensure((size_t)max_items * 16);
-    int32_t *bins = (int32_t *)h->ntot.ensure((3 * kXcds + 4) * 4);
+    int32_t *bins = (int32_t *)h->ntot.ensure(kBinsInts * 4);
     int32_t *itoff = (int32_t *)h->itoff.ensure((size_t)h->nl * 4);
     int64_t *woff = (int64_t *)h->woff.ensure((size_t)h->nl * 8);
     const int nranges = tune_get("ivfl2.xcd_ranges") ? kXcds : 1;
@@ -1398,16 +1579,22 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
             PN_REQUIRE(rblk <= kLdsMax, PN_EINVAL, "fp8 precision serves d up to 1152");
             const int nmax = (int)std::min<size_t>(8192, (kLdsMax - rblk) / 4 / 64 * 64);
             const size_t lds = std::max(rblk + (size_t)nmax * 4, (size_t)16 * 64 * KLV * 8); // (and the lists)
-            if (lds > 64 * 1024)
-                PN_HIP(hipFuncSetAttribute((const void *)ivfl2_scan_fp8_kernel<KLV>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             const dim3 grid8((unsigned)std::min<int64_t>(ceil_div((int64_t)h->cus, (int64_t)kXcds) *
                                                              std::max<int64_t>(1, tune_get("ivfl2.fp8_wg_per_cu")),
                                                          max_items) * kXcds);
-            hipLaunchKernelGGL((ivfl2_scan_fp8_kernel<KLV>), grid8, dim3(512), lds, st, h->l8.as<uint8_t>(),
-                               h->rn8.as<float>(), h->loff.as<int64_t>(), dp, h->sx8, h->qres8.as<uint8_t>(),
-                               h->qresn.as<float>(), items, bins, nranges,
-                               pq, pp, np, qoff, qthr, cand, nmax);
+            auto go = [&](auto pdc) {
+                constexpr int PDV = decltype(pdc)::value;
+                if (lds > 64 * 1024)
+                    PN_HIP(hipFuncSetAttribute((const void *)ivfl2_scan_fp8_kernel<KLV, PDV>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                hipLaunchKernelGGL((ivfl2_scan_fp8_kernel<KLV, PDV>), grid8, dim3(512), lds, st, h->l8.as<uint8_t>(),
+                                   h->rn8.as<float>(), h->loff.as<int64_t>(), dp, h->sx8, h->qres8.as<uint8_t>(),
+                                   h->qresn.as<float>(), items, bins, nranges, pq, pp, np, qoff, qthr, cand, nmax);
+            };
+            if (tune_get("ivfl2.fp8_ahead") == 3)
+                go(std::integral_constant<int, 3>{});
+            else
+                go(std::integral_constant<int, 2>{});
         };
         if (prec == 8)
             kl == 8 ? launch8(K8{}) : launch8(K16{});
@@ -1417,17 +1604,27 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
             k <= 16 ? launch(P1{}, K8{}) : launch(P1{}, K16{});
         PN_HIP(hipGetLastError());
     };
-    // Two passes when a bound helps: the first probe of every query, then the others with the
-    // bound from the first (pass_bound_kernel); otherwise all probes in one pass.
+    // Two passes when a bound helps: the first probe of every query, then the others with a bound
+    // from the first. fp8 (cap = K2 slots per pair; the default): the K2-th candidate of the first
+    // probe, which no candidate the rerank takes can exceed (pass_bound_k2_kernel); optionally
+    // (pass_margin_pct > 0) the k-th approximation times a margin; otherwise one pass.
     const int64_t margin_pct = tune_get("ivfl2.pass_margin_pct");
-    if (np > 1 && k <= kIvfKP && margin_pct > 0) {
+    // candidates reranked per query: ivfl2.k2 (k' >= 4k is enough for the rerank to recover
+    // the exact top-k in practice), at least k, at most kIvfK2 = 64 (one per lane)
+    const int k2 = (int)std::min<int64_t>(kIvfK2, std::max<int64_t>({(int64_t)k, 4 * (int64_t)k, tune_get("ivfl2.k2")}));
+    const bool k2_pass = np > 1 && prec == 8 && cap >= k2 && tune_get("ivfl2.two_pass") != 0;
+    if (k2_pass || (np > 1 && k <= kIvfKP && margin_pct > 0)) {
         uint32_t *qthr = (uint32_t *)h->qthr.ensure((size_t)nq * 4);
         bucket(0, 1);
         scan(nullptr);
         {
             StageScope sc(st, STAGE_OTHER);
-            hipLaunchKernelGGL(pass_bound_kernel, dim3((unsigned)ceil_div(nq, (int64_t)4)), dim3(256), 0, st, cand, nq,
-                               np, cap, qoff, k, 1.f + (float)margin_pct / 100.f, qthr);
+            const dim3 g((unsigned)ceil_div(nq, (int64_t)4));
+            if (k2_pass)
+                hipLaunchKernelGGL(pass_bound_k2_kernel, g, dim3(256), 0, st, cand, nq, np, cap, qoff, k2, qthr);
+            else
+                hipLaunchKernelGGL(pass_bound_kernel, g, dim3(256), 0, st, cand, nq, np, cap, qoff, k,
+                                   1.f + (float)margin_pct / 100.f, qthr);
         }
         bucket(1, np - 1);
         scan(qthr);
@@ -1451,7 +1648,7 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         auto launch = [&](auto per, auto v) {
             hipLaunchKernelGGL((ivfl2_rerank_kernel<decltype(per)::value, decltype(v)::value>), g, dim3(256), 0, st,
                                h->X.as<float>(), d, h->lids.as<int32_t>(), Q, re, cand, np, cap, qoff, nq, k, oi, od,
-                               oc, qorder, per_xcd);
+                               oc, qorder, per_xcd, k2);
         };
         using T = std::true_type;
         using F = std::false_type;

@@ -49,7 +49,11 @@ static std::map<std::string, int64_t> &tune_table() {
         t["l2.kl4"] = 1;              // exact L2 scan: per-lane lists of 4 for k <= 6 (with >= 4 slices)
         t["l2.slice_major"] = 1;      // exact L2 scan grid: slices along x (rows of a slice stay in one XCD's L2)
         t["ivfl2.xcd_ranges"] = 0;     // 1: one item range per XCD; 0: one queue for all workgroups
-        t["ivfl2.fp8_wg_per_cu"] = 1;  // fp8 scan: workgroups of 8 waves per CU
+        t["ivfl2.fp8_wg_per_cu"] = 1;  // fp8 scan: workgroups per CU
+        t["ivfl2.two_pass"] = 0;       // fp8: first probe, then the rest under its K2-th candidate
+                                       // (the scan gains less than the second bucketing costs)
+        t["ivfl2.fp8_ahead"] = 3;      // fp8 scan: fragments loaded 2 or 3 steps ahead
+        t["ivfl2.k2"] = 40;            // candidates reranked in fp32 per query (max(4 k, this), <= 64)
         t["ivfl2.pass_margin_pct"] = 0; // > 0: scan the first probe, then the others bounded by its
                                          // k-th approximation x (1 + pct / 100); 0: one pass
         t["batch.slice_rows"] = 512;   // batched exact search: target rows per slice
