@@ -7,8 +7,9 @@ Writes
     (pynear/tests/VPTreeTests.cpp:51-98), as data;
   * ref_vectors.npz -- inputs and outputs of the reference's own module (built from its sources
     into oracle/_ref) on small seeded cases: VPTree k-NN / 1-NN for L2, L1, Chebyshev and Hamming,
-    the 5-point fixture of pynear/tests/test_vptree.py:282-301, IVFFlatBinaryIndex search and
-    kmeans_l2.
+    the 5-point fixture of pynear/tests/test_vptree.py:282-301, IVFFlatBinaryIndex search,
+    kmeans_l2, binary VP-trees at every width class (including the generic class at odd widths),
+    the free dist_* functions, BKTreeBinaryIndex128.find_threshold and MIHBinaryIndex.searchKNN.
 The fixtures are data only; nothing of the reference's sources is stored.
 """
 
@@ -89,6 +90,53 @@ def main():
     Xk = rng.random((800, 16)).astype(np.float32)
     lab, cen = R.kmeans_l2(Xk, 12, 100, 42)
     out.update({"km_X": Xk, "km_labels": lab, "km_centroids": cen})
+    # binary VP-trees at every width class the reference dispatches on: the 512/128/64-bit classes
+    # and the generic class at 12 and 6 bytes (int32 / int16 accumulators) and at 17 and 33 bytes,
+    # where its int8_t accumulator (DistanceFunctions.hpp:125-146) wraps above 127
+    for nb, cls in ((64, R.VPTreeBinaryIndex512), (16, R.VPTreeBinaryIndex128), (8, R.VPTreeBinaryIndex64),
+                    (12, R.VPTreeBinaryIndex), (6, R.VPTreeBinaryIndex), (17, R.VPTreeBinaryIndex),
+                    (33, R.VPTreeBinaryIndex)):
+        Xw = rng.integers(0, 256, (1500, nb), dtype=np.uint8)
+        Qw = np.concatenate([Xw[rng.integers(0, 1500, 5)] ^ np.uint8(3), rng.integers(0, 256, (4, nb), dtype=np.uint8)])
+        t = cls()
+        t.set(Xw)
+        i, d = t.searchKNN(Qw, 4)
+        i1, d1 = t.search1NN(Qw)
+        out.update({f"binw{nb}_X": Xw, f"binw{nb}_Q": Qw, f"binw{nb}_idx": np.array(i, np.int64),
+                    f"binw{nb}_dist": np.array(d, np.int64), f"binw{nb}_idx1": np.array(i1, np.int64),
+                    f"binw{nb}_dist1": np.array(d1, np.int64)})
+    # the free distance functions
+    fa = rng.standard_normal((7, 13)).astype(np.float32)
+    fb = rng.standard_normal((7, 13)).astype(np.float32)
+    out.update({"dist_fa": fa, "dist_fb": fb,
+                "dist_l2": np.array([R.dist_l2(fa[j], fb[j]) for j in range(7)], np.float32),
+                "dist_l1": np.array([R.dist_l1(fa[j], fb[j]) for j in range(7)], np.float32),
+                "dist_linf": np.array([R.dist_chebyshev(fa[j], fb[j]) for j in range(7)], np.float32)})
+    for nb, fn in ((64, R.dist_hamming_512), (32, R.dist_hamming_256), (16, R.dist_hamming_128), (8, R.dist_hamming_64)):
+        ha = rng.integers(0, 256, (5, nb), dtype=np.uint8)
+        hb = rng.integers(0, 256, (5, nb), dtype=np.uint8)
+        out.update({f"dist_h{nb}_a": ha, f"dist_h{nb}_b": hb,
+                    f"dist_h{nb}": np.array([fn(list(ha[j]), list(hb[j])) for j in range(5)], np.int64)})
+    # BK-tree range search (find_threshold; results in its tree order) and MIH top-k over the
+    # candidates of its sub-string tables
+    Xt = rng.integers(0, 256, (3000, 16), dtype=np.uint8)
+    Qt = np.concatenate([Xt[rng.integers(0, 3000, 12)] ^ np.uint8(0x11), rng.integers(0, 256, (4, 16), dtype=np.uint8)])
+    Xt[100] = Xt[7]  # an exact duplicate
+    for thr in (8, 40, 52):
+        t = R.BKTreeBinaryIndex128()
+        t.set(Xt)
+        ti, td, _ = t.find_threshold(Qt, thr)
+        out[f"bk_thr{thr}_count"] = np.array([len(r) for r in ti], np.int64)
+        out[f"bk_thr{thr}_idx"] = np.array([v for r in ti for v in r], np.int64)
+        out[f"bk_thr{thr}_dist"] = np.array([v for r in td for v in r], np.int64)
+    out["bk_X"], out["bk_Q"] = Xt, Qt
+    for m_, radius in ((4, 8), (2, 9), (8, 16)):
+        mi = R.MIHBinaryIndex(m_)
+        mi.set(Xt)
+        ii, dd = mi.searchKNN(Qt, 5, radius)
+        out[f"mih_m{m_}_r{radius}_count"] = np.array([len(r) for r in ii], np.int64)
+        out[f"mih_m{m_}_r{radius}_idx"] = np.array([v for r in ii for v in r], np.int64)
+        out[f"mih_m{m_}_r{radius}_dist"] = np.array([v for r in dd for v in r], np.int64)
     np.savez_compressed(os.path.join(HERE, "ref_vectors.npz"), **out)
     print("wrote", sorted(out))
 
