@@ -50,8 +50,10 @@ PEAK_INT32_VALU_TOPS = 39.3   # 256 CUs x 64 lanes x 2.4 GHz, one 32-bit op per 
 
 
 # ---------------------------------------------------------------- workloads
-def hamming_workload(n=1_000_000, nq=10_000, seed=1, nbytes=64, flip_p=0.1):
-    """Uniform random rows; each query = a random row with Binomial(nbits, flip_p) bits flipped."""
+def hamming_workload(n=1_000_000, nq=10_000, seed=1, nbytes=64, flip_p=0.1, with_source=False):
+    """Uniform random rows; each query = a random row with Binomial(nbits, flip_p) bits flipped
+    (distinct bits, so the query's distance to its source row is its flip count). with_source
+    also returns the source rows and the flip counts."""
     rng = np.random.default_rng(seed)
     X = rng.integers(0, 256, (n, nbytes), dtype=np.uint8)
     nbits = nbytes * 8
@@ -63,6 +65,8 @@ def hamming_workload(n=1_000_000, nq=10_000, seed=1, nbytes=64, flip_p=0.1):
     cols = order[np.arange(nbits)[None, :] < nflip[:, None]]
     mask_bits[rows, cols] = 1
     Q = X[src] ^ np.packbits(mask_bits, axis=1, bitorder="little")
+    if with_source:
+        return X, Q, src, nflip
     return X, Q
 
 
@@ -176,6 +180,18 @@ def _cores():
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
 
+def _cpu_model():
+    """The host CPU's model name (as lscpu prints it), for the CPU baselines."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def run_exact(workload, steps, warmup, cpu, settle=False):
     from pynear_amd import _native as N
     from pynear_amd import hipdev
@@ -223,6 +239,17 @@ def run_exact(workload, steps, warmup, cpu, settle=False):
     out = {"workload": desc, "value": round(nq * steps / (gpu_ms / 1e3), 1), "unit": "queries/s",
            "ms_per_step": round(gpu_ms / steps, 4), "steps": steps, "roofline": roof,
            "stages_ms_per_step": {s: round(v[0] / steps, 4) for s, v in stages.items() if v[1]}}
+    o1i = o1d = None
+    if workload == "l2":
+        # search1NN over the same rows and queries (k = 1), timed the same way
+        o1i = hipdev.DeviceArray((nq, 1), np.int32)
+        o1d = hipdev.DeviceArray((nq, 1), dt)
+        step1 = lambda: h.search_device(dq.ptr, nq, 1, o1i.ptr, o1d.ptr, stream.handle)
+        if settle:
+            _warm(step1, stream, 0.2)
+        ms1, _, _ = _time_device(N, hipdev, step1, steps, warmup, stream)
+        out["search1NN"] = {"value": round(nq * steps / (ms1 / 1e3), 1), "unit": "queries/s",
+                            "ms_per_step": round(ms1 / steps, 4), "steps": steps}
     if cpu:
         sys.path.insert(0, ROOT)
         import oracle
@@ -250,6 +277,15 @@ def run_exact(workload, steps, warmup, cpu, settle=False):
         out["cpu_baseline"] = {"value": round(sample / t, 2), "unit": "queries/s", "cores": _cores(), "kind": kind,
                                "sample": samp}
         out["cpu_sample_distances_equal"] = bool(np.array_equal(od.numpy()[:sample].astype(np.float64), ref_d))
+        if o1d is not None and R is not None:
+            t0 = time.perf_counter()
+            _, rd1 = idx.search1NN(Q[:sample])
+            t1 = time.perf_counter() - t0
+            out["search1NN"]["cpu_baseline"] = {"value": round(sample / t1, 2), "unit": "queries/s", "cores": _cores(),
+                                                "kind": "reference", "sample": f"{sample} of {nq} queries, "
+                                                f"reference {type(idx).__name__}.search1NN, same tree"}
+            out["search1NN"]["cpu_sample_distances_equal"] = bool(
+                np.array_equal(o1d.numpy()[:sample, 0], np.asarray(rd1, np.float32)))
     return out
 
 
@@ -589,6 +625,16 @@ def measured_traffic(kernel):
 
 
 # ---------------------------------------------------------------- main
+def _tag_cpu_model(node, model):
+    """Adds the host CPU model to every cpu_baseline object in the result."""
+    if isinstance(node, dict):
+        for key, v in node.items():
+            if key == "cpu_baseline" and isinstance(v, dict):
+                v["cpu_model"] = model
+            else:
+                _tag_cpu_model(v, model)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -660,6 +706,7 @@ def main():
             except Exception as e:  # noqa: BLE001
                 also[name] = {"error": f"{type(e).__name__}: {e}"}
         result["also"] = also
+    _tag_cpu_model(result, _cpu_model())
     result["tuning"] = {name: N.tune_get(name) for name in
                         ("hamming.wg_per_cu", "hamming.qpl", "float.wg_per_cu", "l2.mfma", "l2.wg_per_cu",
                          "l2.mfma_terms")}

@@ -117,6 +117,43 @@ pn_status pn_exact_search_device(pn_exact *h, const void *queries, int64_t nq, i
 pn_status pn_exact_search_device_noalloc(pn_exact *h, const void *queries, int64_t nq, int32_t k, int32_t *out_idx,
                                          void *out_dist, void *stream);
 
+/* BKTree::find_batch (BKTree.hpp:117-135) behind BKTreeBinaryIndex{64,128,256,512,}.find_threshold
+ * (PythonBindings.cpp:209-230, :408-451) on a Hamming index: every row within `threshold` of each
+ * query (dist <= threshold), in compressed rows. queries: host nq x dim bytes. offsets int64[nq + 1]
+ * and *total are always written; when *total <= capacity, out_idx / out_dist (int32[*total]) hold
+ * the matches of query q at [offsets[q], offsets[q + 1]), in ascending row order (the reference
+ * lists the same pairs in its tree's breadth-first order). When *total > capacity nothing else is
+ * written: call again with capacity >= *total. An empty index or a negative threshold finds nothing
+ * (BKTree.hpp:79-81); a metric other than Hamming returns PN_EINVAL. */
+pn_status pn_exact_range(pn_exact *h, const void *queries, int64_t nq, int64_t dim, int64_t threshold,
+                         int64_t *offsets, int32_t *out_idx, int32_t *out_dist, int64_t capacity, int64_t *total);
+
+/* ---------------------------------------------------------------- multi-index hashing
+ * MIHBinaryIndex (MIH.hpp:50-195) behind the pybind11 class of the same name
+ * (PythonBindings.cpp:253-272, :468-481). */
+typedef struct pn_mih pn_mih;
+
+/* MIHBinaryIndex(m) (MIH.hpp:57). m >= 1. */
+pn_status pn_mih_create(int32_t m, pn_mih **out);
+void pn_mih_destroy(pn_mih *h);
+
+/* MIHBinaryIndex::set (MIH.hpp:60-83): host rows n x nbytes, replacing the content; n == 0 clears.
+ * nbytes must be a multiple of m, at least m, with nbytes / m <= 8 (PN_EINVAL otherwise, as the
+ * reference's invalid_argument); on the GPU also m <= 32 for sub-strings of up to 4 bytes and
+ * m <= 16 for 5 to 8 bytes (PN_EINVAL beyond). */
+pn_status pn_mih_set(pn_mih *h, const uint8_t *rows, int64_t n, int64_t nbytes);
+
+/* m(), n(), nbytes() (MIH.hpp:154-156). */
+pn_status pn_mih_info(const pn_mih *h, int32_t *m, int64_t *n, int64_t *nbytes);
+
+/* MIHBinaryIndex::searchKNN (MIH.hpp:98-152): the kk = min(k, n) nearest candidates of each query,
+ * a candidate being a row with one of its m sub-strings within radius / m bits of the query's
+ * (the keys the reference enumerates, :102-122), ascending (distance, row). out_idx / out_dist
+ * int64[nq, kk] (slots past out_count[q] are -1). radius < 0 and a width other than the indexed
+ * one return PN_EINVAL; an empty index returns PN_ERUNTIME. */
+pn_status pn_mih_search(pn_mih *h, const uint8_t *queries, int64_t nq, int64_t nbytes, int32_t k, int32_t radius,
+                        int64_t *out_idx, int64_t *out_dist, int64_t *out_count);
+
 /* ---------------------------------------------------------------- k-means
  * kmeans_l2(data, k, max_iter, seed) (PythonBindings.cpp:305-329, KMeans.hpp:25-126): host data
  * n x d float32; writes labels int32[n] and centroids float32[k, d]. The result is identical to

@@ -8,7 +8,8 @@ Contents
 --------
 * ``liboracle.so`` (``oracle.cpp``): the reference's distance arithmetic in its own summation order,
   brute-force k-NN with (dist, idx) order, ``IVFFlatBinaryIndex`` build and search, ``kmeans_l2``.
-* numpy restatements: ``ivf_l2_search`` (``pynear/forest.py:63-111``), FP64 brute force.
+* numpy restatements: ``ivf_l2_search`` (``pynear/forest.py:63-111``), FP64 brute force,
+  ``hamming_range`` (BK-tree ``find_threshold``) and ``mih_search`` (``MIHBinaryIndex.searchKNN``).
 * ``load_ref()``: the reference's own ``_pynear`` module compiled by ``oracle/Makefile`` into
   ``oracle/_ref`` (absent when it was not built).
 """
@@ -202,6 +203,49 @@ def ivf_l2_search(centroids, cluster_data, cluster_norms, orig_indices, queries,
         all_i.append([b for _, b in res])
         all_d.append([a for a, _ in res])
     return all_i, all_d
+
+
+# ---------------------------------------------------------------- Hamming range / MIH (numpy)
+def _popcounts(X, Q):
+    """(nq, n, nbytes) per-byte popcounts of q xor x."""
+    return np.bitwise_count(np.asarray(Q, np.uint8)[:, None, :] ^ np.asarray(X, np.uint8)[None, :, :])
+
+
+def hamming_range(X, Q, threshold, chunk=64):
+    """Every row within ``threshold`` of each query: the set BKTree::find returns
+    (pynear/include/BKTree.hpp:71-115; the BK-tree is complete, a row enters iff dist <= threshold),
+    here in ascending row order. Returns (list of int64 row arrays, list of int64 distance arrays)."""
+    out_i, out_d = [], []
+    for s in range(0, len(Q), chunk):
+        d = _popcounts(X, Q[s:s + chunk]).sum(-1, dtype=np.int64)
+        for r in d:
+            rows = np.flatnonzero(r <= threshold)
+            out_i.append(rows.astype(np.int64))
+            out_d.append(r[rows])
+    return out_i, out_d
+
+
+def mih_search(X, Q, m, radius, k, chunk=64):
+    """MIHBinaryIndex::searchKNN (pynear/include/MIH.hpp:98-152) by brute force: a row is a candidate
+    when one of its m sub-strings (nbytes / m bytes each, :165-169) is within radius / m bits of the
+    query's (:102-122: the enumerated keys are exactly those within that distance); the k nearest
+    candidates by full Hamming distance, ascending (distance, row). The reference keeps ties at the
+    k-th distance in hash-set order; here the lower row wins."""
+    X = np.asarray(X, np.uint8)
+    nb = X.shape[1]
+    sb = nb // m
+    r_sub = radius // m
+    out_i, out_d = [], []
+    for s in range(0, len(Q), chunk):
+        pc = _popcounts(X, Q[s:s + chunk])
+        d = pc.sum(-1, dtype=np.int64)
+        cand = (pc[:, :, :m * sb].reshape(pc.shape[0], pc.shape[1], m, sb).sum(-1) <= r_sub).any(-1)
+        for dq, cq in zip(d, cand):
+            rows = np.flatnonzero(cq)
+            o = rows[np.lexsort((rows, dq[rows]))][:k]
+            out_i.append(o.astype(np.int64))
+            out_d.append(dq[o])
+    return out_i, out_d
 
 
 # ---------------------------------------------------------------- reference module

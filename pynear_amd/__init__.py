@@ -2,7 +2,8 @@
 
 The public surface follows ``pynear`` (pynear/__init__.py:1-160): the exact ``VPTree*Index``
 classes, ``VPTreeBinaryIndex`` with its width dispatch and input validation, ``IVFFlatBinaryIndex``,
-``IVFFlatL2Index``, the distance free functions, ``kmeans_l2`` and the scikit-learn adapters.
+``MIHBinaryIndex``, the ``BKTreeBinaryIndex*`` range-search classes, ``IVFFlatL2Index``, the
+distance free functions, ``kmeans_l2`` and the scikit-learn adapters.
 Every search runs in HIP kernels behind the C-ABI of ``include/pynear_amd.h``; there is no CPU
 fallback, and a missing library raises ``NativeLibraryError`` at first use.
 """
@@ -12,7 +13,9 @@ from typing import List, Tuple
 import numpy as np
 
 from ._native import NativeLibraryError  # noqa: F401
-from ._pynear import IVFFlatBinaryIndex
+from ._pynear import IVFFlatBinaryIndex, MIHBinaryIndex
+from ._pynear import BKTreeBinaryIndex as BKTreeBinaryIndexN
+from ._pynear import BKTreeBinaryIndex64, BKTreeBinaryIndex128, BKTreeBinaryIndex256, BKTreeBinaryIndex512
 from ._pynear import VPTreeBinaryIndex as VPTreeBinaryIndexN
 from ._pynear import VPTreeBinaryIndex64, VPTreeBinaryIndex128, VPTreeBinaryIndex256, VPTreeBinaryIndex512
 from ._pynear import VPTreeChebyshevIndex, VPTreeL1Index, VPTreeL2Index
@@ -79,7 +82,8 @@ class VPTreeBinaryIndex:
 
 
 __all__ = [
-    "IVFFlatBinaryIndex", "IVFFlatL2Index", "VPTreeL2Index", "VPTreeL1Index", "VPTreeChebyshevIndex", "VPTreeBinaryIndex", "VPTreeBinaryIndexN",
+    "IVFFlatBinaryIndex", "MIHBinaryIndex", "BKTreeBinaryIndexN", "BKTreeBinaryIndex64", "BKTreeBinaryIndex128",
+    "BKTreeBinaryIndex256", "BKTreeBinaryIndex512", "IVFFlatL2Index", "VPTreeL2Index", "VPTreeL1Index", "VPTreeChebyshevIndex", "VPTreeBinaryIndex", "VPTreeBinaryIndexN",
     "VPTreeBinaryIndex64", "VPTreeBinaryIndex128", "VPTreeBinaryIndex256", "VPTreeBinaryIndex512",
     "dist_l2", "dist_l1", "dist_chebyshev", "dist_hamming", "dist_hamming_64", "dist_hamming_128",
     "dist_hamming_256", "dist_hamming_512", "kmeans_l2", "NativeLibraryError",

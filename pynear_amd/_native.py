@@ -41,6 +41,12 @@ SIGNATURES = {
     "pn_exact_search": (_i32, [_vp, _vp, _i64, _i64, _i32, _vp, _vp]),
     "pn_exact_search_device": (_i32, [_vp, _vp, _i64, _i32, _vp, _vp, _vp]),
     "pn_exact_search_device_noalloc": (_i32, [_vp, _vp, _i64, _i32, _vp, _vp, _vp]),
+    "pn_exact_range": (_i32, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp, _i64, ctypes.POINTER(_i64)]),
+    "pn_mih_create": (_i32, [_i32, ctypes.POINTER(_vp)]),
+    "pn_mih_destroy": (None, [_vp]),
+    "pn_mih_set": (_i32, [_vp, _vp, _i64, _i64]),
+    "pn_mih_info": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "pn_mih_search": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _vp, _vp, _vp]),
     "pn_kmeans_l2": (_i32, [_vp, _i64, _i64, _i64, _i64, _u32, _vp, _vp]),
     "pn_ivfl2_create": (_i32, [ctypes.POINTER(_vp)]),
     "pn_ivfl2_destroy": (None, [_vp]),
@@ -168,6 +174,58 @@ class ExactHandle:
         """search_device through the allocation-free entry point (fixed per-GPU scratch)."""
         check(lib().pn_exact_search_device_noalloc(self._h, _vp(q_ptr), nq, int(k), _vp(out_idx_ptr),
                                                    _vp(out_dist_ptr), _vp(stream_ptr) if stream_ptr else None))
+
+    def range(self, queries, threshold, capacity=None):
+        """Hamming range search: (offsets int64[nq + 1], rows int32[total], distances int32[total]),
+        the matches of query q at [offsets[q], offsets[q + 1]) in ascending row order."""
+        nq, d = queries.shape
+        offsets = np.zeros(nq + 1, np.int64)
+        total = _i64(0)
+        cap = int(capacity) if capacity is not None else min(64 * nq, 1 << 22)
+        while True:
+            idx = np.empty(cap, np.int32)
+            dist = np.empty(cap, np.int32)
+            check(lib().pn_exact_range(self._h, ptr(queries), nq, d, int(threshold), ptr(offsets), ptr(idx), ptr(dist),
+                                       cap, ctypes.byref(total)))
+            if total.value <= cap:
+                return offsets, idx[:total.value], dist[:total.value]
+            cap = total.value  # too many matches for the first guess: once more with room for all
+
+
+class MihHandle:
+    """Owner of a ``pn_mih`` handle (MIHBinaryIndex on the current device)."""
+
+    def __init__(self, m):
+        h = _vp()
+        check(lib().pn_mih_create(int(m), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _LIB is not None:
+            _LIB.pn_mih_destroy(h)
+            self._h = None
+
+    def set(self, rows):
+        n, nb = rows.shape
+        check(lib().pn_mih_set(self._h, ptr(rows), n, nb))
+
+    def info(self):
+        m, n, nb = _i32(0), _i64(0), _i64(0)
+        check(lib().pn_mih_info(self._h, ctypes.byref(m), ctypes.byref(n), ctypes.byref(nb)))
+        return m.value, n.value, nb.value
+
+    def search(self, queries, k, radius):
+        """(idx int64[nq, kk], dist int64[nq, kk], counts int64[nq]); kk = min(k, n), slots past
+        the count are -1."""
+        nq, nb = queries.shape
+        _, n, _ = self.info()
+        kk = max(0, min(int(k), n))
+        idx = np.full((nq, kk), -1, np.int64)
+        dist = np.full((nq, kk), -1, np.int64)
+        cnt = np.zeros(nq, np.int64)
+        check(lib().pn_mih_search(self._h, ptr(queries), nq, nb, int(k), int(radius), ptr(idx), ptr(dist), ptr(cnt)))
+        return idx, dist, cnt
 
 
 def tune_set(name, value):

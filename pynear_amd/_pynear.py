@@ -179,6 +179,118 @@ class IVFFlatBinaryIndex:
             self.set(state["rows"])
 
 
+class _BKTreeIndex:
+    """Hamming range search over uint8 rows with the interface of ``BKTreeBinaryIndex*``
+    (PythonBindings.cpp:209-230, :408-451; BKTree.hpp:37-166).
+
+    ``set`` adds rows (BKTree::update: the index keeps earlier rows and numbers new ones after
+    them); ``find_threshold(vectors, threshold)`` returns, per query, every stored row within
+    ``threshold`` bits as (indices, distances, keys) lists. The rows are scored on the GPU by a
+    full scan instead of a tree walk, which finds the same set; matches come in ascending row
+    order (the reference's order is its tree's breadth-first order). ``values`` returns the
+    stored rows in insertion order (the reference: tree order; both documented as arbitrary).
+    """
+
+    _width = None
+
+    def __init__(self):
+        self._h = N.ExactHandle(N.METRIC_HAMMING)
+        self._rows = None
+
+    def _check_width(self, a):
+        want = self._width if self._width is not None else (None if self._rows is None else self._rows.shape[1])
+        if want is not None and a.shape[1] != want:
+            raise ValueError(f"{type(self).__name__} holds rows of {want} bytes, got {a.shape[1]}")
+
+    def set(self, vectors):
+        a = np.asarray(vectors)
+        if a.size == 0:
+            return
+        a = _as_2d_u8(a)
+        self._check_width(a)
+        self._rows = a.copy() if self._rows is None else np.concatenate([self._rows, a])
+        self._h.set(self._rows)
+
+    def find_threshold(self, vectors, threshold):
+        """Batch find all vectors below the distance threshold (inclusive)."""
+        a = np.asarray(vectors)
+        nq = a.shape[0] if a.ndim >= 1 else 0
+        if self._rows is None or nq == 0:
+            return [[] for _ in range(nq)], [[] for _ in range(nq)], [[] for _ in range(nq)]
+        q = _as_2d_u8(a)
+        self._check_width(q)
+        off, idx, dist = self._h.range(q, int(threshold))
+        rows = self._rows
+        ind = [idx[off[i]:off[i + 1]].tolist() for i in range(nq)]
+        return (ind, [dist[off[i]:off[i + 1]].tolist() for i in range(nq)], [rows[r].tolist() for r in ind])
+
+    def empty(self):
+        return self._rows is None
+
+    def size(self):
+        return 0 if self._rows is None else int(self._rows.shape[0])
+
+    def values(self):
+        """Return all stored vectors in arbitrary order (here: insertion order)."""
+        return [] if self._rows is None else self._rows.tolist()
+
+
+class BKTreeBinaryIndex512(_BKTreeIndex):
+    _width = 64
+
+
+class BKTreeBinaryIndex256(_BKTreeIndex):
+    _width = 32
+
+
+class BKTreeBinaryIndex128(_BKTreeIndex):
+    _width = 16
+
+
+class BKTreeBinaryIndex64(_BKTreeIndex):
+    _width = 8
+
+
+class BKTreeBinaryIndex(_BKTreeIndex):
+    _width = None
+
+
+class MIHBinaryIndex:
+    """Multi-Index Hashing for binary descriptors (approximate Hamming k-NN) on the GPU, with the
+    interface of ``_pynear.MIHBinaryIndex`` (PythonBindings.cpp:253-272, :468-481; MIH.hpp:50-195):
+    ``MIHBinaryIndex(m=8)``, ``set``, ``searchKNN(vectors, k, radius=8)`` -> (list[list[int]],
+    list[list[int]]) ascending by distance, possibly fewer than k per query, ``m()``, ``n()``,
+    ``nbytes()``. The candidates are the reference's (rows with a sub-string within radius // m
+    bits of the query's); every row is tested on the GPU instead of looking keys up in tables.
+    Ties at the k-th distance keep the lower row (the reference: hash-set order).
+    """
+
+    def __init__(self, m: int = 8):
+        self._h = N.MihHandle(int(m))
+
+    def set(self, vectors):
+        a = np.asarray(vectors)
+        if a.size == 0:
+            self._h.set(np.zeros((0, 0), np.uint8))
+            return
+        self._h.set(_as_2d_u8(a))
+
+    def searchKNN(self, vectors, k, radius=8):
+        """Batch find top-k vectors in index and return indices and distances."""
+        q = _as_2d_u8(vectors)
+        idx, dist, cnt = self._h.search(q, int(k), int(radius))
+        return ([idx[i, :c].tolist() for i, c in enumerate(cnt)], [dist[i, :c].tolist() for i, c in enumerate(cnt)])
+
+    def m(self):
+        return self._h.info()[0]
+
+    def n(self):
+        return self._h.info()[1]
+
+    def nbytes(self):
+        return self._h.info()[2]
+
+
 # ---------------------------------------------------------------- free functions
 def _pair_float(metric, a, b):
     a = np.ascontiguousarray(np.asarray(a), dtype=np.float32).reshape(1, -1)

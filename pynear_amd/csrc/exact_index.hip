@@ -101,8 +101,8 @@ struct SlicePlan {
     int64_t S, rps, nq_pad;
     int qpl;
 };
-static bool plan_slices(const pn_exact *h, int64_t nq, int K, size_t part_cap, SlicePlan &p) {
-    p.qpl = (h->metric == PN_METRIC_HAMMING && tune_get("hamming.qpl") == 2 && nq > 256) ? 2 : 1;
+static bool plan_slices(const pn_exact *h, int64_t nq, int K, size_t part_cap, SlicePlan &p, bool one_qpl = false) {
+    p.qpl = (!one_qpl && h->metric == PN_METRIC_HAMMING && tune_get("hamming.qpl") == 2 && nq > 256) ? 2 : 1;
     const int64_t gx = ceil_div(nq, 256 * p.qpl);
     p.nq_pad = gx * 256 * p.qpl;
     // Whole multiples of the CU count: every CU holds the same number of resident workgroups.
@@ -125,8 +125,8 @@ static bool plan_slices(const pn_exact *h, int64_t nq, int K, size_t part_cap, S
 // The sliced scan (Hamming and the float metrics up to kFloatScanMaxDim dims, K <= 32) and the
 // merge of its per-slice lists; part holds the plan's lists.
 static void sliced_search(const pn_exact *h, const void *dq, int64_t nq, int kk, int K, const SlicePlan &p,
-                          uint64_t *part, int32_t *oi, void *od, hipStream_t st) {
-    ScanArgs a{h->db.p, h->n, h->row_elems, h->dim, dq, nq, p.rps, p.S, part, p.nq_pad, p.qpl};
+                          uint64_t *part, int32_t *oi, void *od, hipStream_t st, const SubFilter &filt = SubFilter()) {
+    ScanArgs a{h->db.p, h->n, h->row_elems, h->dim, dq, nq, p.rps, p.S, part, p.nq_pad, p.qpl, filt};
     {
         StageScope sc(st, STAGE_SCAN);
         if (h->metric == PN_METRIC_HAMMING)
@@ -138,20 +138,29 @@ static void sliced_search(const pn_exact *h, const void *dq, int64_t nq, int kk,
     merge_lists(part, p.S, K, nq, p.nq_pad, kk, out_kind_of(h->metric), oi, od, st);
 }
 
+RangePlan range_plan(const pn_exact *h, int64_t nq) {
+    SlicePlan p;
+    plan_slices(h, nq, 1, 0, p, true);
+    return {p.S, p.rps, p.nq_pad};
+}
+
 // Enqueues the search of nq padded device queries; out_idx int32[nq, kk], out_dist int32/f32.
-void exact_search_enqueue(pn_exact *h, const void *dq, int64_t nq, int kk, int32_t *oi, void *od,
-                                 hipStream_t st) {
+void exact_search_enqueue(pn_exact *h, const void *dq, int64_t nq, int kk, int32_t *oi, void *od, hipStream_t st,
+                          const SubFilter &filt) {
+    const bool filtered = filt.group_words > 0;
+    PN_REQUIRE(!filtered || h->metric == PN_METRIC_HAMMING, PN_EINVAL, "the candidate filter is for Hamming rows");
     int K = list_size_for(kk);
+    if (filtered) K = kk <= 4 ? 4 : (kk <= 16 ? 16 : (kk <= 32 ? 32 : 64)); // the filtered scan's list sizes
     if (h->metric == PN_METRIC_L2 && kk <= 32 && tune_get("l2.mfma") != 0) {
         l2_mfma_search(h->l2, (const float *)h->db.p, h->row_elems, (const float *)dq, h->row_elems, nq, kk, oi,
                        (float *)od, h->l2w, h->cus, st);
         return;
     }
-    if (h->scan_ok && K <= 32) {
+    if (h->scan_ok && K <= 32 && (!filtered || h->row_elems % filt.group_words == 0)) {
         SlicePlan p;
-        plan_slices(h, nq, K, 0, p);
+        plan_slices(h, nq, K, 0, p, filtered);
         uint64_t *part = (uint64_t *)h->part.ensure((size_t)p.S * K * p.nq_pad * sizeof(uint64_t));
-        sliced_search(h, dq, nq, kk, K, p, part, oi, od, st);
+        sliced_search(h, dq, nq, kk, K, p, part, oi, od, st, filt);
         return;
     }
     // Distance-matrix path: bound the matrix to ~1 GiB of keys per chunk.
@@ -161,7 +170,7 @@ void exact_search_enqueue(pn_exact *h, const void *dq, int64_t nq, int kk, int32
     size_t wsb = matrix_search_workspace(h->n, q_chunk);
     void *ws = h->ws.ensure(wsb);
     StageScope sc(st, STAGE_OTHER);
-    matrix_search(h->metric, h->db.p, h->n, h->row_elems, h->dim, dq, nq, kk, oi, od, ws, wsb, q_chunk, st);
+    matrix_search(h->metric, h->db.p, h->n, h->row_elems, h->dim, dq, nq, kk, oi, od, ws, wsb, q_chunk, st, filt);
 }
 
 } // namespace pn

@@ -2,6 +2,7 @@
 // k-means assignment are exact nearest-centroid searches).
 #pragma once
 #include "common.hpp"
+#include "kernels.hpp"
 #include "l2_mfma.hpp"
 
 struct pn_exact {
@@ -12,6 +13,7 @@ struct pn_exact {
     bool scan_ok = false;        // the sliced scan kernel serves this row width
     uint64_t generation = 0;     // bumped by every set(): plans that bound the rows check it
     pn::DevBuf db, qbuf, part, oidx, odist, ws;
+    pn::DevBuf rws, rout;        // range search: counts and chunks; matches
     int cus = 256;
     pn::L2Set l2;                // bf16 split of the rows (L2 metric)
     pn::L2Work l2w;
@@ -29,7 +31,11 @@ void pad_rows_device(const void *src, int64_t n, size_t in_b, size_t out_b, void
 void exact_init(pn_exact *h, int32_t metric);
 // rows_padded: device rows already padded to padded_row_elems(metric, dim) 32-bit elements
 void exact_set_device(pn_exact *h, const void *rows_padded, int64_t n, int64_t dim, hipStream_t st);
-// nq padded device queries; out_idx int32[nq, kk]; out_dist int32 (Hamming) or float32 [nq, kk]
-void exact_search_enqueue(pn_exact *h, const void *dq, int64_t nq, int kk, int32_t *oi, void *od, hipStream_t st);
+// nq padded device queries; out_idx int32[nq, kk]; out_dist int32 (Hamming) or float32 [nq, kk].
+// filt (Hamming): only candidate rows are returned; missing results are index -1, distance -1.
+void exact_search_enqueue(pn_exact *h, const void *dq, int64_t nq, int kk, int32_t *oi, void *od, hipStream_t st,
+                          const SubFilter &filt = SubFilter());
+// Sliced-scan shape for a Hamming range search of nq queries over the index's rows.
+RangePlan range_plan(const pn_exact *h, int64_t nq);
 
 } // namespace pn

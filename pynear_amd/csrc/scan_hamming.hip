@@ -13,16 +13,22 @@
 //     lane qualifies, which is the common case once the lists fill.
 //   * slices of the database run in separate workgroups (gridDim.y); their lists are merged by
 //     the select kernel.
+//   * G > 0 (multi-index hashing, MIH.hpp:98-150): the row's sub-strings sit in groups of G words;
+//     a row enters only when one of its first `subs` groups is within `rsub` bits of the query's
+//     (the candidate set of the sub-string tables), with the full distance as its key.
+#include <algorithm>
+
 #include "common.hpp"
 #include "device_util.hpp"
 #include "kernels.hpp"
 
 namespace pn {
 
-template <int W, int K, int QPL, int RU>
+template <int W, int K, int QPL, int RU, int G = 0>
 __global__ __launch_bounds__(256) void hamming_scan_kernel(const uint32_t *__restrict__ db, int64_t n,
                                                            int64_t rows_per_slice, const uint32_t *__restrict__ qs,
-                                                           int64_t nq, uint64_t *__restrict__ part, int64_t nq_pad) {
+                                                           int64_t nq, uint64_t *__restrict__ part, int64_t nq_pad,
+                                                           int subs = 0, uint32_t rsub = 0) {
     // lane t of the workgroup serves queries base + t + 256 p, p < QPL (each wave stays dense)
     const int64_t qbase = (int64_t)blockIdx.x * (256 * QPL) + threadIdx.x;
     const int64_t s = blockIdx.y;
@@ -66,10 +72,23 @@ __global__ __launch_bounds__(256) void hamming_scan_kernel(const uint32_t *__res
         for (int u = 0; u < RU; ++u)
 #pragma unroll
             for (int p = 0; p < QPL; ++p) {
-                uint32_t acc = 0;
+                if constexpr (G == 0) {
+                    uint32_t acc = 0;
 #pragma unroll
-                for (int w = 0; w < W; ++w) acc = bcnt_acc(qv[p][w] ^ R[u][w], acc);
-                d[p][u] = acc;
+                    for (int w = 0; w < W; ++w) acc = bcnt_acc(qv[p][w] ^ R[u][w], acc);
+                    d[p][u] = acc;
+                } else {
+                    uint32_t acc = 0, mn = kKey32None;
+#pragma unroll
+                    for (int g = 0; g < W / G; ++g) {
+                        uint32_t s = 0;
+#pragma unroll
+                        for (int j = 0; j < G; ++j) s = bcnt_acc(qv[p][g * G + j] ^ R[u][g * G + j], s);
+                        acc += s;
+                        mn = g < subs ? min(mn, s) : mn;
+                    }
+                    d[p][u] = mn <= rsub ? acc : kKey32None; // a non-candidate never passes thr
+                }
             }
 #pragma unroll
         for (int p = 0; p < QPL; ++p) {
@@ -117,6 +136,23 @@ static void launch_w_k(const ScanArgs &a, hipStream_t st) {
     dim3 grid((unsigned)ceil_div(a.nq, 256 * a.qpl), (unsigned)a.slices);
     // SGPR budget: RU x W row words must fit next to the addressing registers.
     constexpr int RU = W >= 32 ? 2 : (W >= 16 ? 4 : 8);
+    if (a.filt.group_words > 0) {
+        PN_REQUIRE(a.qpl == 1, PN_EINVAL, "hamming scan: the candidate filter takes one query per lane");
+        const uint32_t rs = (uint32_t)std::max(0, a.filt.rsub);
+        if constexpr (K != 4 && K != 16 && K != 32)
+            fail(PN_EINVAL, "hamming scan: the candidate filter takes lists of 4, 16 or 32");
+        else if (a.filt.group_words == 1)
+            hipLaunchKernelGGL((hamming_scan_kernel<W, K, 1, RU, 1>), grid, dim3(256), 0, st, (const uint32_t *)a.db,
+                               a.n, a.rows_per_slice, (const uint32_t *)a.queries, a.nq, a.part, a.nq_pad, a.filt.subs,
+                               rs);
+        else if constexpr (W >= 2)
+            hipLaunchKernelGGL((hamming_scan_kernel<W, K, 1, RU, 2>), grid, dim3(256), 0, st, (const uint32_t *)a.db,
+                               a.n, a.rows_per_slice, (const uint32_t *)a.queries, a.nq, a.part, a.nq_pad, a.filt.subs,
+                               rs);
+        else
+            fail(PN_EINVAL, "hamming scan: groups of two words need rows of two words or more");
+        return;
+    }
     if (a.qpl == 2)
         hipLaunchKernelGGL((hamming_scan_kernel<W, K, 2, RU>), grid, dim3(256), 0, st, (const uint32_t *)a.db, a.n,
                            a.rows_per_slice, (const uint32_t *)a.queries, a.nq, a.part, a.nq_pad);

@@ -152,9 +152,11 @@ void merge_lists_wave(const uint64_t *part, int64_t slices, int klist, int cap, 
 
 // ---------------------------------------------------------------- distance matrix path
 // D[qi, i] = monotone u32 key of the final distance (bits of the float, or the Hamming count).
+// With a candidate filter (group words gw > 0) a Hamming row that is no candidate gets the empty
+// key, which sorts last and comes back as index -1.
 __global__ void dist_matrix_kernel(int metric, const void *__restrict__ db, int64_t n, int64_t row_elems,
                                    int64_t dim, const void *__restrict__ qs, int64_t q0, int64_t nqc,
-                                   uint32_t *__restrict__ D, int32_t *__restrict__ V) {
+                                   uint32_t *__restrict__ D, int32_t *__restrict__ V, int gw, int subs, int rsub) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t qi = blockIdx.y;
     if (i >= n || qi >= nqc) return;
@@ -163,8 +165,19 @@ __global__ void dist_matrix_kernel(int metric, const void *__restrict__ db, int6
         const uint32_t *a = (const uint32_t *)qs + (q0 + qi) * row_elems;
         const uint32_t *b = (const uint32_t *)db + i * row_elems;
         uint32_t acc = 0;
-        for (int64_t w = 0; w < row_elems; ++w) acc = __popc(a[w] ^ b[w]) + acc;
-        key = acc;
+        if (gw > 0) {
+            int mn = INT32_MAX;
+            for (int64_t g = 0; g * gw < row_elems; ++g) {
+                uint32_t s = 0;
+                for (int j = 0; j < gw; ++j) s += __popc(a[g * gw + j] ^ b[g * gw + j]);
+                acc += s;
+                if (g < subs) mn = min(mn, (int)s);
+            }
+            key = mn <= rsub ? acc : kKey32None;
+        } else {
+            for (int64_t w = 0; w < row_elems; ++w) acc = __popc(a[w] ^ b[w]) + acc;
+            key = acc;
+        }
     } else {
         const float *a = (const float *)qs + (q0 + qi) * row_elems;
         const float *b = (const float *)db + i * row_elems;
@@ -186,8 +199,8 @@ __global__ void gather_topk_kernel(const uint32_t *__restrict__ Ds, const int32_
     if (t >= nqc * kk) return;
     const int64_t qi = t / kk, j = t % kk;
     const int64_t o = (q0 + qi) * kk + j;
-    out_idx[o] = Vs[qi * n + j];
     uint32_t key = Ds[qi * n + j];
+    out_idx[o] = metric == PN_METRIC_HAMMING && key == kKey32None ? -1 : Vs[qi * n + j];
     if (metric == PN_METRIC_HAMMING)
         ((int32_t *)out_dist)[o] = (int32_t)key;
     else
@@ -213,8 +226,10 @@ size_t matrix_search_workspace(int64_t n, int64_t q_chunk) {
 
 void matrix_search(int metric, const void *db, int64_t n, int64_t row_elems, int64_t dim, const void *queries,
                    int64_t nq, int kk, int32_t *out_idx, void *out_dist, void *ws, size_t ws_bytes, int64_t q_chunk,
-                   hipStream_t st) {
+                   hipStream_t st, const SubFilter &filt) {
     PN_REQUIRE(n * q_chunk < (int64_t)INT32_MAX, PN_EINVAL, "matrix search: chunk too large");
+    PN_REQUIRE(filt.group_words == 0 || (metric == PN_METRIC_HAMMING && row_elems % filt.group_words == 0), PN_EINVAL,
+               "matrix search: the candidate filter needs Hamming rows of whole groups");
     size_t items = (size_t)n * q_chunk;
     char *p = (char *)ws;
     uint32_t *D = (uint32_t *)p;          p += align256(items * 4);
@@ -229,7 +244,7 @@ void matrix_search(int metric, const void *db, int64_t n, int64_t row_elems, int
         int64_t nqc = std::min(q_chunk, nq - q0);
         dim3 g1((unsigned)ceil_div(n, 256), (unsigned)nqc);
         hipLaunchKernelGGL(dist_matrix_kernel, g1, dim3(256), 0, st, metric, db, n, row_elems, dim, queries, q0, nqc,
-                           D, V);
+                           D, V, filt.group_words, filt.subs, filt.rsub);
         hipLaunchKernelGGL(fill_offsets_kernel, dim3((unsigned)ceil_div(nqc + 1, 256)), dim3(256), 0, st, off, nqc, n);
         size_t tb = tmp_bytes;
         PN_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp, tb, D, Ds, V, Vs, (int)(n * nqc), (int)nqc, off,

@@ -120,8 +120,13 @@ def main():
     # BK-tree range search (find_threshold; results in its tree order) and MIH top-k over the
     # candidates of its sub-string tables
     Xt = rng.integers(0, 256, (3000, 16), dtype=np.uint8)
-    Qt = np.concatenate([Xt[rng.integers(0, 3000, 12)] ^ np.uint8(0x11), rng.integers(0, 256, (4, 16), dtype=np.uint8)])
     Xt[100] = Xt[7]  # an exact duplicate
+    src = np.concatenate([[7], rng.integers(0, 3000, 13)])
+    Qt = Xt[src].copy()
+    for j, nf in enumerate((0, 1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 30)):  # distinct bits flipped
+        bits = rng.permutation(128)[:nf]
+        Qt[j, bits // 8] ^= (np.uint8(1) << (bits % 8).astype(np.uint8))
+    Qt = np.concatenate([Qt, rng.integers(0, 256, (4, 16), dtype=np.uint8)])
     for thr in (8, 40, 52):
         t = R.BKTreeBinaryIndex128()
         t.set(Xt)

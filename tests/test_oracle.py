@@ -179,3 +179,160 @@ def test_ivf_l2_numpy_restatement_exact_at_full_probe():
     fi, fd = oracle.knn_fp64(oracle.L2, X, Q, 5)
     assert [sorted(r) for r in ii] == [sorted(r) for r in fi.tolist()]
     np.testing.assert_allclose(np.sqrt(np.array(dd)), fd, rtol=1e-4)
+
+
+# ---------------------------------------------------------------- binary widths, dist_*, range, MIH
+def _true_hamming(X, Q):
+    return np.bitwise_count(Q[:, None, :] ^ X[None, :, :]).sum(-1).astype(np.int64)
+
+
+@pytest.mark.parametrize("nb", [64, 16, 8, 12, 6, 17])
+def test_knn_hamming_vs_reference_width_classes(g, nb):
+    """The fixed 512/128/64-bit classes and the generic class (int32 / int16 / int8 accumulators,
+    none of which wraps here): distances equal, rows equal below the k-th distance."""
+    X, Q = g[f"binw{nb}_X"], g[f"binw{nb}_Q"]
+    oi, od = oracle.knn_hamming(X, Q, 4)
+    ri, rd = g[f"binw{nb}_idx"][:, ::-1], g[f"binw{nb}_dist"][:, ::-1]
+    np.testing.assert_array_equal(od, rd)
+    true = _true_hamming(X, Q)
+    for q in range(len(Q)):
+        assert (true[q, ri[q]] == rd[q]).all()
+        below = od[q] < od[q].max()
+        assert sorted(oi[q][below]) == sorted(ri[q][rd[q] < rd[q].max()])
+    np.testing.assert_array_equal(oracle.knn_hamming(X, Q, 1)[1][:, 0], g[f"binw{nb}_dist1"])
+
+
+def test_generic_width_33_bytes_reference_wraps_int8(g):
+    """At 33 bytes the reference's generic distance sums into int8_t (DistanceFunctions.hpp:125-146):
+    its recorded distances are the int8 wrap of the true popcount of the rows it returned, and
+    differ from the true k-NN, which the oracle returns."""
+    X, Q = g["binw33_X"], g["binw33_Q"]
+    true = _true_hamming(X, Q)
+    ri, rd = g["binw33_idx"], g["binw33_dist"]
+    np.testing.assert_array_equal(np.take_along_axis(true, ri, 1).astype(np.int8), rd)
+    oi, od = oracle.knn_hamming(X, Q, 4)
+    np.testing.assert_array_equal(od, np.sort(true, 1)[:, :4])
+    assert (rd < 0).all() and (od > 0).all()
+
+
+def test_free_distance_functions_vs_reference(g):
+    fa, fb = g["dist_fa"], g["dist_fb"]
+    for name, m in (("l2", oracle.L2), ("l1", oracle.L1), ("linf", oracle.LINF)):
+        got = np.array([oracle.dist(m, fa[j], fb[j]) for j in range(len(fa))], np.float32)
+        np.testing.assert_array_equal(got, g[f"dist_{name}"])
+    for nb in (64, 32, 16, 8):
+        a, b = g[f"dist_h{nb}_a"], g[f"dist_h{nb}_b"]
+        assert [oracle.dist_hamming(a[j], b[j]) for j in range(len(a))] == g[f"dist_h{nb}"].tolist()
+
+
+def _csr(g, key):
+    c = g[f"{key}_count"]
+    off = np.concatenate([[0], np.cumsum(c)])
+    return [(g[f"{key}_idx"][off[i]:off[i + 1]], g[f"{key}_dist"][off[i]:off[i + 1]]) for i in range(len(c))]
+
+
+@pytest.mark.parametrize("thr", [8, 40, 52])
+def test_hamming_range_vs_reference_bktree(g, thr):
+    """BKTreeBinaryIndex128.find_threshold: the same (row, distance) pairs, in any order (the
+    reference lists them in its tree's breadth-first order)."""
+    oi, od = oracle.hamming_range(g["bk_X"], g["bk_Q"], thr)
+    for (ri, rd), a, b in zip(_csr(g, f"bk_thr{thr}"), oi, od):
+        assert sorted(zip(ri.tolist(), rd.tolist())) == list(zip(a.tolist(), b.tolist()))
+    assert any(len(a) > 1 for a in oi)  # the duplicated row comes back twice
+
+
+def test_hamming_range_brute_force_small():
+    rng = np.random.default_rng(11)
+    X = rng.integers(0, 256, (300, 5), dtype=np.uint8)
+    Q = np.concatenate([X[:3] ^ np.uint8(1), rng.integers(0, 256, (3, 5), dtype=np.uint8)])
+    for thr in (-1, 0, 3, 12, 40):
+        oi, od = oracle.hamming_range(X, Q, thr)
+        for q in range(len(Q)):
+            exp = [(r, bin(int.from_bytes(Q[q].tobytes(), "little") ^ int.from_bytes(X[r].tobytes(), "little")).count("1"))
+                   for r in range(len(X))]
+            exp = [(r, d) for r, d in exp if d <= thr]
+            assert list(zip(oi[q].tolist(), od[q].tolist())) == exp
+
+
+@pytest.mark.parametrize("m,radius", [(4, 8), (2, 9), (8, 16)])
+def test_mih_vs_reference(g, m, radius):
+    """MIHBinaryIndex.searchKNN(k=5): distances equal; rows below the k-th distance equal in
+    order; rows at the k-th distance are candidates of that distance (the reference keeps them in
+    hash-set order)."""
+    X, Q = g["bk_X"], g["bk_Q"]
+    oi, od = oracle.mih_search(X, Q, m, radius, 5)
+    true = _true_hamming(X, Q)
+    sb = X.shape[1] // m
+    for q, ((ri, rd), a, b) in enumerate(zip(_csr(g, f"mih_m{m}_r{radius}"), oi, od)):
+        np.testing.assert_array_equal(rd, b)
+        if len(b):
+            below = b < b.max()
+            np.testing.assert_array_equal(ri[below], a[below])
+            assert (true[q, ri] == rd).all()
+            sub = np.bitwise_count(X[ri] ^ Q[q]).reshape(len(ri), m, sb).sum(-1)
+            assert ((sub <= radius // m).any(1)).all()
+    assert sum(len(a) for a in oi) >= 10
+
+
+def test_mih_invariants_and_planted_mistakes(g):
+    """Pigeonhole: every row within the radius is a candidate, so when the k nearest rows are
+    within it MIH equals the exact k-NN; a sub-string radius of all bits makes every row a
+    candidate. Restatements that use the full radius per sub-string, or sub-strings shifted by
+    a byte, disagree with the reference's recorded results."""
+    X, Q = g["bk_X"], g["bk_Q"]
+    ei, ed = oracle.knn_hamming(X, Q, 5)
+    for m, radius in ((4, 8), (2, 9), (8, 16), (16, 15)):
+        oi, od = oracle.mih_search(X, Q, m, radius, 5)
+        for q in range(len(Q)):
+            within = ed[q] <= radius
+            if within.all():
+                np.testing.assert_array_equal(oi[q], ei[q])
+            else:
+                assert set(ei[q][within].tolist()) <= set(oi[q].tolist())
+    ai, ad = oracle.mih_search(X, Q, 4, 4 * 128, 5)
+    np.testing.assert_array_equal(np.array(ai), ei)
+    # planted mistakes, checked against the m = 8, radius 16 record
+    rec = _csr(g, "mih_m8_r16")
+    true = _true_hamming(X, Q)
+    sub = np.bitwise_count(Q[:, None, :] ^ X[None]).reshape(len(Q), len(X), 8, 2).sum(-1)
+    wrong_radius = (sub <= 16).any(-1)
+    shifted = np.bitwise_count(np.roll(Q, 1, 1)[:, None, :] ^ np.roll(X, 1, 1)[None])
+    shifted = (shifted[:, :, :16].reshape(len(Q), len(X), 8, 2).sum(-1) <= 2).any(-1)
+    for cand in (wrong_radius, shifted):
+        bad = 0
+        for q, (ri, rd) in enumerate(rec):
+            rows = np.flatnonzero(cand[q])
+            o = rows[np.lexsort((rows, true[q, rows]))][:5]
+            bad += not np.array_equal(true[q, o], rd)
+        assert bad > 0
+
+
+def test_ivf_l2_restatement_partial_probe_vs_fp64():
+    """At n_probe < nlist the restatement's neighbours are the fp64 exact top-k over the union of
+    the n_probe nearest clusters (fp64 centroid distances), up to fp32 near ties."""
+    from neighbours import near_tie_check, nearest_lists, sq_l2_rows
+    rng = np.random.default_rng(12)
+    centres = rng.standard_normal((12, 10)).astype(np.float32)
+    X = (centres[rng.integers(0, 12, 1500)] + 0.4 * rng.standard_normal((1500, 10))).astype(np.float32)
+    Q = (centres[rng.integers(0, 12, 40)] + 0.4 * rng.standard_normal((40, 10))).astype(np.float32)
+    lab, cen = oracle.kmeans_l2(X, 12, 100, 42)
+    keep = [c for c in range(12) if (lab == c).any()]
+    data = [X[lab == c] for c in keep]
+    norms = [(d * d).sum(1) for d in data]
+    ids = [np.where(lab == c)[0] for c in keep]
+    for n_probe in (1, 2, 5):
+        ii, dd = oracle.ivf_l2_search(cen[keep], data, norms, ids, Q, n_probe, 7)
+        chosen, clear = nearest_lists(Q, cen[keep], np.arange(len(keep)), n_probe)
+        assert clear.all()
+        for q in range(len(Q)):
+            rows = np.sort(np.concatenate([ids[c] for c in chosen[q]]))
+            # the expanded form |q|^2 + |x|^2 - 2 q.x in fp32 costs ~1e-6 of |x|^2 absolute
+            near_tie_check(ii[q], dd[q], rows, sq_l2_rows(X, Q[q], rows), 7, 1e-4, rtol_d=1e-3, atol_d=1e-4)
+        # a restatement that probed the farthest clusters instead fails the same check
+    far_i, far_d = oracle.ivf_l2_search(cen[keep], data, norms, ids, -Q * 50, 1, 7)
+    chosen, _ = nearest_lists(Q, cen[keep], np.arange(len(keep)), 1)
+    with pytest.raises(AssertionError):
+        for q in range(len(Q)):
+            rows = np.sort(ids[chosen[q][0]])
+            near_tie_check(far_i[q], np.sort(sq_l2_rows(X, Q[q], np.array(far_i[q]))), rows,
+                           sq_l2_rows(X, Q[q], rows), 7, 1e-4, rtol_d=1e-3, atol_d=1e-4)
