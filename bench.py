@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
 """Benchmark of the pynear_amd search path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ivf|hamming|l2] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload ivf|hamming|l2|batch|learned|ivf_binary|range|mih] [--no-cpu-baseline]
 
 A step is one search of the whole query batch with the database and the queries already resident
 in HBM (the device entry points on a HIP stream of the library's own runtime). Workloads:
@@ -22,6 +23,11 @@ in HBM (the device entry points on a HIP stream of the library's own runtime). W
   on 200,000 queries' exact top-10) and adaptive n_probe, N = 2,000,000 x 512 from a Zipf(1.1)
   weighted 4,096-component mixture (sigma 0.25), nlist = 4,096, 10,000 held-out queries, k = 10;
   compared with fixed-n_probe IVF (bf16x3 ~ fp32 candidate distances, and bf16) at Recall@10 >= 0.95.
+* ivf_binary: IVFFlatBinaryIndex over the hamming rows and queries, nlist 512, n_probe 16, k = 10.
+* range: Hamming range search (the BK-tree classes' find_threshold) over the hamming rows, 10,000
+  queries with Binomial(512, 0.06) flips, threshold 60 (host entry point; device time reported).
+* mih: MIHBinaryIndex over the hamming rows, m = 8, radius 16, k = 10, queries with
+  Binomial(512, 0.02) flips (host entry point; device time reported).
 
 The roofline of the dominant kernel is measured with HIP events around each launch on its
 stream. The CPU baseline runs on this host: the reference's own VP-tree (built from its sources
@@ -196,7 +202,7 @@ def run_exact(workload, steps, warmup, cpu, settle=False):
     from pynear_amd import _native as N
     from pynear_amd import hipdev
     if workload == "hamming":
-        X, Q = hamming_workload()
+        X, Q = _hamming_data()[:2]
         k, metric, dt = 10, N.METRIC_HAMMING, np.int32
         desc = f"exact Hamming kNN, N={len(X):,} x 512-bit, Q={len(Q):,}, k={k}"
     else:
@@ -413,6 +419,204 @@ def run_ivf(steps, warmup, cpu, probes=(8, 16, 32), precisions=("bf16", "fp8")):
         hi, _, hc = h.search(Q[:sample], chosen, k, prec)
         out["cpu_sample_recall_vs_port"] = round(float(np.mean(
             [len(set(hi[i, :hc[i]].tolist()) & set(ci[i])) / k for i in range(sample)])), 4)
+    return out
+
+
+_HAM_CACHE = {}
+
+
+def _hamming_data(flip_p=0.1):
+    """The 1M x 512-bit rows and 10k queries (flip probability flip_p), generated once per value."""
+    if flip_p not in _HAM_CACHE:
+        _HAM_CACHE[flip_p] = hamming_workload(flip_p=flip_p, with_source=True)
+    return _HAM_CACHE[flip_p]
+
+
+def _host_stage_timing(N, call, reps):
+    """Runs a host entry point `reps` times with per-stage HIP events; returns (wall s per call,
+    {stage: ms per call})."""
+    call()
+    N.stage_timing(True)
+    N.stage_times()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    wall = (time.perf_counter() - t0) / reps
+    st = N.stage_times()
+    N.stage_timing(False)
+    return wall, {s: v[0] / reps for s, v in st.items() if v[1]}
+
+
+def run_ivf_binary(steps, warmup, cpu, nlist=512, nprobe=16, k=10):
+    """IVFFlatBinaryIndex: N = 1M x 512-bit, nlist 512, n_probe 16, k = 10, 10k queries."""
+    from pynear_amd import _native as N
+    from pynear_amd import hipdev
+    X, Q, src, _ = _hamming_data()
+    n, nq = len(X), len(Q)
+    t0 = time.perf_counter()
+    h = N.IvfBinHandle(nlist, nprobe, 20, 42)
+    h.set(X)
+    t_build = time.perf_counter() - t0
+    cent, labels = h.export()
+    _log(f"ivf_binary: built in {t_build:.1f} s")
+    # rows scanned per query: the n_probe nearest centroids by (distance, id), summed list sizes
+    sizes = np.bincount(labels, minlength=len(cent))
+    probed = 0
+    for s in range(0, nq, 500):
+        dc = np.bitwise_count(Q[s:s + 500, None, :] ^ cent[None]).sum(-1)
+        order = np.lexsort((np.broadcast_to(np.arange(len(cent)), dc.shape), dc), axis=1)[:, :nprobe]
+        probed += int(sizes[order].sum())
+    dq = hipdev.DeviceArray.from_numpy(Q)
+    oi = hipdev.DeviceArray((nq, k), np.int32)
+    od = hipdev.DeviceArray((nq, k), np.int32)
+    oc = hipdev.DeviceArray((nq,), np.int32)
+    stream = hipdev.Stream()
+    step = lambda: h.search_device(dq.ptr, nq, k, oi.ptr, od.ptr, oc.ptr, stream.handle)
+    gpu_ms, _, stages = _time_device(N, hipdev, step, steps, warmup, stream)
+    scan_ms, scan_n = stages["scan"]
+    scan_s = scan_ms / max(scan_n, 1) / 1e3
+    ops = 2.0 * (X.shape[1] // 4) * probed
+    ach = ops / scan_s / 1e12
+    ex = N.ExactHandle(N.METRIC_HAMMING)
+    ex.set(X)
+    gt = ex.search(Q, k)[0]
+    got = oi.numpy()
+    recall = float(np.mean([len(set(got[i].tolist()) & set(gt[i].tolist())) / k for i in range(nq)]))
+    # uniform random rows have no cluster structure: beyond the query's source row its exact
+    # neighbours are arbitrary rows ~230 bits away, so Recall@10 says little; the source row
+    # (the true nearest, ~51 bits away) is what the probe must find
+    found = float(np.mean(got[:, 0] == src))
+    out = {"workload": f"IVFFlatBinaryIndex N={n:,} x 512-bit, nlist={nlist}, n_probe={nprobe}, Q={nq:,}, k={k}",
+           "value": round(nq * steps / (gpu_ms / 1e3), 1), "unit": "queries/s", "ms_per_step": round(gpu_ms / steps, 4),
+           "steps": steps, "recall_at_10": round(recall, 4), "source_found": round(found, 4), "setup_s": round(t_build, 2),
+           "stages_ms_per_step": {s: round(v[0] / steps, 4) for s, v in stages.items() if v[1]},
+           "roofline": {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_INT32_VALU_TOPS, "unit": "TOP/s",
+                        "frac": round(ach / PEAK_INT32_VALU_TOPS, 4), "traffic": measured_traffic("ivfbin_fine_kernel<16>"),
+                        "kernel": "ivfbin_fine_kernel<16>", "kernel_ms": round(scan_s * 1e3, 4),
+                        "work": (f"{ops:.3e} int32 ops per launch: 16 words x (xor + popcount-accumulate) x "
+                                 f"{probed:,} probed rows (all queries), one distance per candidate")}}
+    if cpu:
+        sys.path.insert(0, ROOT)
+        import oracle
+        R = oracle.load_ref()
+        sample = 1000
+        if R is not None:
+            t0 = time.perf_counter()
+            ref = R.IVFFlatBinaryIndex(nlist=nlist, nprobe=nprobe, max_iter=20, seed=42)
+            ref.set(X)
+            tb = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            ri, rd = ref.searchKNN(Q[:sample], k)
+            t = time.perf_counter() - t0
+            kind, samp = "reference", (f"{sample} of {nq} queries; reference IVFFlatBinaryIndex (serial query loop), "
+                                       f"search only, build {tb:.1f} s excluded")
+        else:
+            off, ids = oracle.lists_from_labels(labels, len(cent))
+            t0 = time.perf_counter()
+            ri, rd = oracle.ivfbin_search(cent, off, ids, X, Q[:sample], nprobe, k)
+            t = time.perf_counter() - t0
+            kind, samp = "port", f"{sample} of {nq} queries; C++ restatement of the reference search (serial)"
+        out["cpu_baseline"] = {"value": round(sample / t, 2), "unit": "queries/s", "cores": 1, "kind": kind,
+                               "sample": samp}
+        gc = oc.numpy()
+        out["cpu_sample_results_equal"] = bool(all(got[i, :gc[i]].tolist() == list(ri[i]) for i in range(sample)))
+    return out
+
+
+def run_hamming_range(cpu, threshold=60, reps=5):
+    """Hamming range search (BK-tree find_threshold) over the 1M x 512-bit rows, 10k queries with
+    6% of their bits flipped, threshold 60."""
+    from pynear_amd import _native as N
+    X, Q, src, nflip = _hamming_data(0.06)
+    n, nq = len(X), len(Q)
+    h = N.ExactHandle(N.METRIC_HAMMING)
+    h.set(X)
+    res = [None]
+
+    def call():
+        res[0] = h.range(Q, threshold)
+
+    wall, st = _host_stage_timing(N, call, reps)
+    off, idx, dist = res[0]
+    dev_ms = st.get("scan", 0.0) + st.get("select", 0.0)
+    ops = 2.0 * (X.shape[1] // 4) * n * nq
+    ach = ops / (st["scan"] / 1e3) / 1e12
+    found = float(np.mean([src[q] in idx[off[q]:off[q + 1]] for q in range(nq) if nflip[q] <= threshold]))
+    out = {"workload": f"Hamming range search N={n:,} x 512-bit, Q={nq:,}, threshold {threshold}",
+           "value": round(nq / (dev_ms / 1e3), 1), "unit": "queries/s", "ms_per_call_device": round(dev_ms, 4),
+           "host_call_qps": round(nq / wall, 1), "matches": int(off[-1]), "source_found": round(found, 4),
+           "stages_ms_per_call": {s: round(v, 4) for s, v in st.items()},
+           "roofline": {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_INT32_VALU_TOPS, "unit": "TOP/s",
+                        "frac": round(ach / PEAK_INT32_VALU_TOPS, 4), "traffic": None,
+                        "kernel": "range_scan_kernel<16, 4, false>", "kernel_ms": round(st["scan"], 4),
+                        "work": f"{ops:.3e} int32 ops per count pass: 16 words x 2 ops x Q x N"}}
+    if cpu:
+        sys.path.insert(0, ROOT)
+        import oracle
+        R = oracle.load_ref()
+        sample = 200
+        if R is not None:
+            t0 = time.perf_counter()
+            bk = R.BKTreeBinaryIndex512()
+            bk.set(X)
+            tb = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            ri, rd, _ = bk.find_threshold(Q[:sample], threshold)
+            t = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": round(sample / t, 2), "unit": "queries/s", "cores": _cores(),
+                                   "kind": "reference",
+                                   "sample": (f"{sample} of {nq} queries; reference BKTreeBinaryIndex512.find_threshold "
+                                              f"(OpenMP), build {tb:.1f} s excluded")}
+            out["cpu_sample_sets_equal"] = bool(all(sorted(ri[q]) == idx[off[q]:off[q + 1]].tolist()
+                                                    for q in range(sample)))
+    return out
+
+
+def run_mih(cpu, m=8, radius=16, k=10, reps=5):
+    """MIHBinaryIndex over the 1M x 512-bit rows: m = 8, radius 16, k = 10, 10k queries with 2% of
+    their bits flipped."""
+    from pynear_amd import _native as N
+    X, Q, src, nflip = _hamming_data(0.02)
+    n, nq = len(X), len(Q)
+    h = N.MihHandle(m)
+    h.set(X)
+    res = [None]
+
+    def call():
+        res[0] = h.search(Q, k, radius)
+
+    wall, st = _host_stage_timing(N, call, reps)
+    idx, dist, cnt = res[0]
+    dev_ms = sum(st.values())
+    scan = st["scan"]
+    ops = (2.0 * (X.shape[1] // 4) + 2.0 * m) * n * nq  # xor + popcount per word, add + min per sub-string
+    ach = ops / (scan / 1e3) / 1e12
+    found = float(np.mean([idx[q, 0] == src[q] for q in range(nq) if nflip[q] <= radius]))
+    out = {"workload": f"MIHBinaryIndex N={n:,} x 512-bit, m={m}, radius {radius}, Q={nq:,}, k={k}",
+           "value": round(nq / (dev_ms / 1e3), 1), "unit": "queries/s", "ms_per_call_device": round(dev_ms, 4),
+           "host_call_qps": round(nq / wall, 1), "source_first": round(found, 4),
+           "stages_ms_per_call": {s: round(v, 4) for s, v in st.items()},
+           "roofline": {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_INT32_VALU_TOPS, "unit": "TOP/s",
+                        "frac": round(ach / PEAK_INT32_VALU_TOPS, 4), "traffic": None,
+                        "kernel": "hamming_scan_kernel<16, 16, 1, 4, 2>", "kernel_ms": round(scan, 4),
+                        "work": f"{ops:.3e} int32 ops per launch: (16 words x 2 + 8 groups x 2) x Q x N"}}
+    if cpu:
+        sys.path.insert(0, ROOT)
+        import oracle
+        R = oracle.load_ref()
+        sample = 4000
+        if R is not None:
+            t0 = time.perf_counter()
+            ref = R.MIHBinaryIndex(m)
+            ref.set(X)
+            tb = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            ri, rd = ref.searchKNN(Q[:sample], k, radius)
+            t = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": round(sample / t, 2), "unit": "queries/s", "cores": 1, "kind": "reference",
+                                   "sample": (f"{sample} of {nq} queries; reference MIHBinaryIndex.searchKNN (serial), "
+                                              f"build {tb:.1f} s excluded")}
+            out["cpu_sample_distances_equal"] = bool(all(list(rd[q]) == dist[q, :cnt[q]].tolist() for q in range(sample)))
     return out
 
 
@@ -640,7 +844,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ivf", choices=["ivf", "hamming", "l2", "batch", "learned"])
+    ap.add_argument("--workload", default="ivf",
+                    choices=["ivf", "hamming", "l2", "batch", "learned", "ivf_binary", "range", "mih"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="ivf only: skip the exact workloads")
     ap.add_argument("--ivf-probes", default="8,16,32", help="ivf only: n_probe values swept")
@@ -667,6 +872,11 @@ def main():
         main_r = run_learned(args.steps, args.warmup, cpu)
         main_r.setdefault("roofline", None)
         dtype = "bf16"
+    elif args.workload in ("ivf_binary", "range", "mih"):
+        main_r = {"ivf_binary": lambda: run_ivf_binary(args.steps, args.warmup, cpu),
+                  "range": lambda: run_hamming_range(cpu), "mih": lambda: run_mih(cpu)}[args.workload]()
+        main_r.setdefault("ms_per_step", main_r.get("ms_per_call_device"))
+        dtype = "u32"
     else:
         main_r = run_exact(args.workload, args.steps, args.warmup, cpu)
         dtype = "u32" if args.workload == "hamming" else "f32"
@@ -692,13 +902,18 @@ def main():
     for key in ("cpu_baseline", "precision", "recall_at_10", "n_probe", "fp64_rel_err", "stages_ms_per_step", "sweep", "setup_s",
                 "cpu_sample_recall_vs_port", "cpu_sample_distances_equal", "schedule", "scan_tiles",
                 "equal_to_plain_searches", "plain_searches_ms", "sweep_fp8", "learned", "fixed_nprobe_fp32", "fixed_nprobe_bf16",
-                "speedup_vs_fixed_fp32", "speedup_vs_fixed_bf16", "train", "learned_by_precision", "fixed_nprobe_fp8"):
+                "speedup_vs_fixed_fp32", "speedup_vs_fixed_bf16", "train", "learned_by_precision", "fixed_nprobe_fp8",
+                "cpu_sample_results_equal", "host_call_qps", "matches", "source_found", "source_first",
+                "stages_ms_per_call", "cpu_sample_sets_equal", "search1NN", "ms_per_call_device"):
         if key in main_r:
             result[key] = main_r[key]
     if args.workload == "ivf" and not args.no_secondary:
         also = {}
         for name, fn in (("exact_hamming", lambda: run_exact("hamming", args.steps, args.warmup, cpu, settle=True)),
                          ("exact_l2", lambda: run_exact("l2", max(args.steps, 50), args.warmup, cpu, settle=True)),
+                         ("ivf_binary", lambda: run_ivf_binary(args.steps, args.warmup, cpu)),
+                         ("hamming_range", lambda: run_hamming_range(cpu)),
+                         ("mih", lambda: run_mih(cpu)),
                          ("batched_exact_l2", lambda: run_batch(args.steps, args.warmup, cpu)),
                          ("learned_probe_ivf", lambda: run_learned(args.steps, args.warmup, cpu))):
             try:  # a secondary workload that fails is reported, not fatal to the headline line
