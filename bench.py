@@ -230,8 +230,9 @@ def run_exact(workload, steps, warmup, cpu, settle=False):
         ops = 2.0 * (X.shape[1] // 4) * nq * n
         ach = ops / scan_s / 1e12
         roof = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_INT32_VALU_TOPS, "unit": "TOP/s",
-                "frac": round(ach / PEAK_INT32_VALU_TOPS, 4), "traffic": measured_traffic("hamming_scan_kernel<16,16,1,4>"),
-                "kernel": "hamming_scan_kernel<16,16,1,4>", "kernel_ms": round(scan_s * 1e3, 4),
+                "frac": round(ach / PEAK_INT32_VALU_TOPS, 4),
+                "traffic": measured_traffic("hamming_scan_kernel<16, 16, 1, 4, 0>"),
+                "kernel": "hamming_scan_kernel<16, 16, 1, 4, 0>", "kernel_ms": round(scan_s * 1e3, 4),
                 "work": f"{ops:.3e} int32 ops per launch: 16 words x (xor + popcount-accumulate) x Q x N"}
     else:
         flops = 2.0 * X.shape[1] * nq * n
@@ -239,8 +240,9 @@ def run_exact(workload, steps, warmup, cpu, settle=False):
         # the result is fp32-exact (bit-identical to the reference), so the work is priced as fp32
         # flops against the fp32 peak; the kernel gets there with 3 bf16 MFMA products per term
         roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic("l2_mfma_scan_small_kernel<4,3,2>"),
-                "kernel": "l2_mfma_scan_small_kernel<4,3,2>", "kernel_ms": round(scan_s * 1e3, 4),
+                "frac": round(ach / PEAK_FP32_TFLOPS, 4),
+                "traffic": measured_traffic("l2_mfma_scan_small_kernel<4, 3, 2>"),
+                "kernel": "l2_mfma_scan_small_kernel<4, 3, 2>", "kernel_ms": round(scan_s * 1e3, 4),
                 "work": f"{flops:.3e} flop per launch (2 d Q N)"}
     out = {"workload": desc, "value": round(nq * steps / (gpu_ms / 1e3), 1), "unit": "queries/s",
            "ms_per_step": round(gpu_ms / steps, 4), "steps": steps, "roofline": roof,
@@ -355,7 +357,7 @@ def run_ivf(steps, warmup, cpu, probes=(8, 16, 32), precisions=("bf16", "fp8")):
     scan_s = S["scan_launch_ms"] / 1e3
     dp = (d + 63) // 64 * 64
     ebytes = 1 if prec == "fp8" else 2
-    kname = "ivfl2_scan_fp8_kernel<8>" if prec == "fp8" else "ivfl2_scan_kernel<1, 8>"
+    kname = "ivfl2_scan_fp8_kernel<8, 3>" if prec == "fp8" else "ivfl2_scan_kernel<1, 8>"
     peak_mm = PEAK_FP8_TFLOPS if prec == "fp8" else PEAK_BF16_TFLOPS
     # algorithmic work of one scan launch: the rows of every probed cluster once (bf16: 2 B, fp8: 1 B
     # per dim) and 2 d flops per (query, probed row); the probes are recomputed here in fp64
@@ -491,8 +493,9 @@ def run_ivf_binary(steps, warmup, cpu, nlist=512, nprobe=16, k=10):
            "steps": steps, "recall_at_10": round(recall, 4), "source_found": round(found, 4), "setup_s": round(t_build, 2),
            "stages_ms_per_step": {s: round(v[0] / steps, 4) for s, v in stages.items() if v[1]},
            "roofline": {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_INT32_VALU_TOPS, "unit": "TOP/s",
-                        "frac": round(ach / PEAK_INT32_VALU_TOPS, 4), "traffic": measured_traffic("ivfbin_fine_kernel<16>"),
-                        "kernel": "ivfbin_fine_kernel<16>", "kernel_ms": round(scan_s * 1e3, 4),
+                        "frac": round(ach / PEAK_INT32_VALU_TOPS, 4),
+                        "traffic": measured_traffic("ivfbin_list_scan_kernel<16, 16, 4>"),
+                        "kernel": "ivfbin_list_scan_kernel<16, 16, 4>", "kernel_ms": round(scan_s * 1e3, 4),
                         "work": (f"{ops:.3e} int32 ops per launch: 16 words x (xor + popcount-accumulate) x "
                                  f"{probed:,} probed rows (all queries), one distance per candidate")}}
     if cpu:
@@ -547,7 +550,8 @@ def run_hamming_range(cpu, threshold=60, reps=5):
            "host_call_qps": round(nq / wall, 1), "matches": int(off[-1]), "source_found": round(found, 4),
            "stages_ms_per_call": {s: round(v, 4) for s, v in st.items()},
            "roofline": {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_INT32_VALU_TOPS, "unit": "TOP/s",
-                        "frac": round(ach / PEAK_INT32_VALU_TOPS, 4), "traffic": None,
+                        "frac": round(ach / PEAK_INT32_VALU_TOPS, 4),
+                        "traffic": measured_traffic("range_scan_kernel<16, 4, false>"),
                         "kernel": "range_scan_kernel<16, 4, false>", "kernel_ms": round(st["scan"], 4),
                         "work": f"{ops:.3e} int32 ops per count pass: 16 words x 2 ops x Q x N"}}
     if cpu:
@@ -597,7 +601,8 @@ def run_mih(cpu, m=8, radius=16, k=10, reps=5):
            "host_call_qps": round(nq / wall, 1), "source_first": round(found, 4),
            "stages_ms_per_call": {s: round(v, 4) for s, v in st.items()},
            "roofline": {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_INT32_VALU_TOPS, "unit": "TOP/s",
-                        "frac": round(ach / PEAK_INT32_VALU_TOPS, 4), "traffic": None,
+                        "frac": round(ach / PEAK_INT32_VALU_TOPS, 4),
+                        "traffic": measured_traffic("hamming_scan_kernel<16, 16, 1, 4, 2>"),
                         "kernel": "hamming_scan_kernel<16, 16, 1, 4, 2>", "kernel_ms": round(scan, 4),
                         "work": f"{ops:.3e} int32 ops per launch: (16 words x 2 + 8 groups x 2) x Q x N"}}
     if cpu:
@@ -660,7 +665,8 @@ def run_batch(steps, warmup, cpu):
     out = {"workload": "64 exact L2 indices, N~LogU[1k,200k], d in {64,128,256}, Q~LogU[1,4096], k in {1,10,32}",
            "value": round(nq_tot / (ms / 1e3), 1), "unit": "queries/s", "ms_per_step": round(ms, 4),
            "roofline": {"bound": "mfma", "achieved": round(flops / (ms / 1e3) / 1e12, 3), "peak": PEAK_FP32_TFLOPS,
-                        "unit": "TFLOP/s", "frac": round(ideal_ms / ms, 4), "traffic": None,
+                        "unit": "TFLOP/s", "frac": round(ideal_ms / ms, 4),
+                        "traffic": measured_traffic("batch graph (all kernels of one replay)"),
                         "kernel": "whole batch graph (split, persistent scan, merge, rerank, fallback)",
                         "kernel_ms": round(ms, 4),
                         "work": (f"{flops:.3e} flop (sum 2 Q N d); frac = sum over indices of max(flops/fp32 peak, "

@@ -13,7 +13,9 @@
 // Search (BinaryIVF.hpp:65-111):
 //   * coarse: exact top-nprobe centroids per query in (distance, centroid id) order -- the
 //     reference's partial_sort over (dist, cid) pairs;
-//   * fine: one workgroup per query scans its probed lists twice. Pass 1 builds a histogram of
+//   * fine, list-major (k <= 32, rows of up to 128 bytes): each inverted list is scanned once for
+//     all queries that probe it, then one wave per query merges its per-probe lists (below);
+//   * fine, query-major (the other shapes): one workgroup per query scans its probed lists twice. Pass 1 builds a histogram of
 //     distances, giving T (the k-th smallest distance) and m = #{d < T}. Pass 2 walks the
 //     candidates in scan order (probe rank, then ascending id) with a block-wide prefix count of
 //     d <= T, keeping every d < T candidate and the d == T candidates among the first k with
@@ -35,8 +37,10 @@ struct pn_ivfbin {
     int device = 0;
     int64_t n = 0, nbytes = 0, words = 0; // words: padded row width in 32-bit words
     int32_t nc = 0;                        // clusters actually built = min(nlist, n)
+    int64_t max_list = 0;                  // rows of the longest inverted list
     pn_exact cent;                         // centroid index (coarse assignment)
     pn::DevBuf lrows, lids, loff, qbuf, probes, pdist, oidx, odist, ocnt;
+    pn::DevBuf lm;                         // list-major search: buckets, work items, per-probe lists
     std::vector<int32_t> labels;           // final labels (host), for export
 };
 
@@ -263,8 +267,399 @@ __global__ __launch_bounds__(256) void ivfbin_fine_kernel(const uint32_t *__rest
 
 static size_t fine_smem(int nbins, int k) { return ((size_t)nbins * 4 + 7) / 8 * 8 + (size_t)k * 20; }
 
-static void launch_fine(const pn_ivfbin *h, const uint32_t *dq, int64_t nq, int np, int k, int32_t *oi, int32_t *od,
+// ---------------------------------------------------------------- list-major fine search
+// The query-major kernel above gives each query one workgroup that walks its probed lists twice.
+// The list-major search instead scans each inverted list once for all the queries that probe it:
+//   1. bucket: the (query, probe rank) pairs are counted and scattered per list, and every list
+//      gets ceil(pairs / 64) work items of 64 pairs;
+//   2. scan (one workgroup of four waves per item): lane = one (query, rank) pair with the query's
+//      words in VGPRs, each wave streaming a quarter of the list's rows through SGPRs; each lane
+//      keeps the K smallest (distance, position) keys -- position = row position in the list
+//      storage, ascending id -- and wave 0 merges the four quarters' lists through LDS;
+//   3. merge (one wave per query): from the n_probe lists of K keys, the reference heap's result.
+// Why the merge is exact (with T the k-th smallest distance, m = #{d < T} < k): every d < T
+// candidate sits in its list's K smallest; the reference keeps all of them plus, among the
+// candidates F = the first k with d <= T in scan order (probe rank, then position), the k - m
+// smallest ids of those with d == T. A list's members of F are its first d <= T rows by position;
+// its ties among them that can be kept are its earliest (= lowest-id) ties, at most k - m <= K - m_p
+// of them, and the list's K smallest hold its first K - m_p ties. The count of d <= T rows a list
+// adds to F is exact from its K keys unless all K are <= T, in which case F ends inside it.
+constexpr int kLmGroup = 64; // (query, rank) pairs per work item
+
+__global__ void zero_i32_kernel(int32_t *p, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0;
+}
+
+__global__ void lm_count_kernel(const int32_t *__restrict__ probes, int64_t npairs, int32_t *__restrict__ cnt) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < npairs) atomicAdd(&cnt[probes[t]], 1);
+}
+
+// One workgroup of 1024: exclusive scans of the per-list pair counts (boff) and item counts
+// (ioff); n_items = total items.
+__global__ __launch_bounds__(1024) void lm_scan_kernel(const int32_t *__restrict__ cnt, int nc,
+                                                       int64_t *__restrict__ boff, int32_t *__restrict__ ioff,
+                                                       int32_t *__restrict__ n_items) {
+    __shared__ int64_t sb[1024];
+    __shared__ int32_t si[1024];
+    __shared__ int64_t carry_b;
+    __shared__ int32_t carry_i;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        carry_b = 0;
+        carry_i = 0;
+    }
+    __syncthreads();
+    for (int base = 0; base < nc; base += 1024) {
+        const int c = base + t;
+        const int32_t v = c < nc ? cnt[c] : 0;
+        sb[t] = v;
+        si[t] = (v + kLmGroup - 1) / kLmGroup;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) { // inclusive Hillis-Steele scan
+            const int64_t xb = t >= off ? sb[t - off] : 0;
+            const int32_t xi = t >= off ? si[t - off] : 0;
+            __syncthreads();
+            sb[t] += xb;
+            si[t] += xi;
+            __syncthreads();
+        }
+        if (c < nc) {
+            boff[c] = carry_b + sb[t] - v;
+            ioff[c] = carry_i + si[t] - (v + kLmGroup - 1) / kLmGroup;
+        }
+        __syncthreads();
+        if (t == 1023) {
+            carry_b += sb[1023];
+            carry_i += si[1023];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        boff[nc] = carry_b;
+        ioff[nc] = carry_i;
+        n_items[0] = carry_i;
+    }
+}
+
+__global__ void lm_scatter_kernel(const int32_t *__restrict__ probes, int64_t npairs,
+                                  const int64_t *__restrict__ boff, int32_t *__restrict__ cursor,
+                                  int32_t *__restrict__ bucket) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs) return;
+    const int c = probes[t];
+    bucket[boff[c] + atomicAdd(&cursor[c], 1)] = (int32_t)t; // t = query * n_probe + rank
+}
+
+// items[ioff[c] + i] = (c, 64 i) for the lists with pairs
+__global__ void lm_items_kernel(const int64_t *__restrict__ boff, const int32_t *__restrict__ ioff, int nc,
+                                int64_t *__restrict__ items) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int32_t ni = ioff[c + 1] - ioff[c];
+    for (int32_t i = 0; i < ni; ++i) items[ioff[c] + i] = ((int64_t)c << 32) | (int64_t)(i * kLmGroup);
+}
+
+template <int W, int K, int RU>
+__global__ __launch_bounds__(256) void ivfbin_list_scan_kernel(const uint32_t *__restrict__ lrows,
+                                                               const int64_t *__restrict__ loff,
+                                                               const int64_t *__restrict__ boff,
+                                                               const int32_t *__restrict__ bucket,
+                                                               const int64_t *__restrict__ items,
+                                                               const int32_t *__restrict__ n_items,
+                                                               const uint32_t *__restrict__ qs, int np,
+                                                               uint64_t *__restrict__ part) {
+    // one work item per workgroup: its 64 (query, rank) pairs on the lanes of every wave, the
+    // list's rows split into four contiguous quarters, one per wave; wave 0 merges the four lists
+    __shared__ uint32_t red[3][K][64];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t it = blockIdx.x;
+    if (it >= n_items[0]) return; // (whole workgroup)
+    const int64_t item = items[it];
+    const int c = __builtin_amdgcn_readfirstlane((int)(item >> 32));
+    const int s = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFF));
+    const int64_t b0 = boff[c], nb = boff[c + 1] - b0;
+    const bool live = s + lane < nb;
+    const int32_t e = live ? bucket[b0 + s + lane] : 0;
+    const int64_t q = e / np;
+    uint32_t qv[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) qv[w] = live ? qs[q * W + w] : 0u;
+    const int64_t r0 = loff[c];
+    const int nall = (int)(loff[c + 1] - r0);
+    const int quarter = (nall + 3) / 4;
+    const int lr0 = min(nall, wv * quarter), lr1 = min(nall, lr0 + quarter);
+    uint32_t L[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) L[j] = kKey32None;
+    uint32_t thr = kKey32None >> kHamIdxBits;
+    const uint32_t *sdb = lrows + r0 * W;
+    uint32_t R[RU][W];
+    for (int lr = lr0; lr < lr1; lr += RU) {
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            const int rr = min(lr + u, lr1 - 1); // (clamped rows are loaded, never inserted)
+#pragma unroll
+            for (int w = 0; w < W; ++w) R[u][w] = sdb[rr * W + w];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const int cnt = min(RU, lr1 - lr);
+        uint32_t d[RU];
+        uint32_t mn = kKey32None;
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) acc = bcnt_acc(qv[w] ^ R[u][w], acc);
+            d[u] = u < cnt ? acc : kKey32None;
+            mn = min(mn, d[u]);
+        }
+        if (mn < thr) {
+#pragma unroll
+            for (int u = 0; u < RU; ++u) {
+                if (d[u] < thr) {
+                    list_insert_u32<K>(L, (d[u] << kHamIdxBits) | (uint32_t)(lr + u));
+                    thr = L[K - 1] >> kHamIdxBits;
+                }
+            }
+        }
+    }
+    // keys hold the row's position in the whole list, so the four lists merge by key alone
+    if (wv > 0) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) red[wv - 1][j][lane] = L[j];
+    }
+    __syncthreads();
+    if (wv > 0 || !live) return;
+#pragma unroll
+    for (int o = 0; o < 3; ++o)
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const uint32_t key = red[o][j][lane];
+            if (key < L[K - 1]) list_insert_u32<K>(L, key);
+        }
+    uint64_t *out = part + (int64_t)e * K;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const uint32_t key = L[j];
+        out[j] = key == kKey32None ? kKey64None
+                                   : ((uint64_t)(key >> kHamIdxBits) << 32) | (uint64_t)(r0 + (key & kHamIdxMask));
+    }
+}
+
+// One workgroup of one wave per query: the reference heap's result from the query's n_probe lists
+// of K keys (d << 32 | storage position), np * K <= 64 E. Every branch below is uniform across
+// the wave, so the barriers only order its lanes' LDS accesses.
+template <int E>
+__global__ __launch_bounds__(64) void ivfbin_lm_merge_kernel(const uint64_t *__restrict__ part, int64_t nq, int np,
+                                                             int K, int k, const int32_t *__restrict__ lids,
+                                                             int32_t *__restrict__ out_idx,
+                                                             int32_t *__restrict__ out_dist,
+                                                             int32_t *__restrict__ out_cnt) {
+    __shared__ uint64_t ent[1][64 * E];
+    __shared__ uint64_t fin[1][64];
+    __shared__ int32_t tie[1][64];
+    constexpr int wv = 0;
+    const int lane = threadIdx.x;
+    const int64_t q = blockIdx.x;
+    if (q >= nq) return;
+    const int total = np * K;
+    const uint64_t *src = part + q * total;
+    uint64_t v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int i = lane + 64 * s;
+        v[s] = i < total ? src[i] : kKey64None;
+        ent[wv][i] = v[s];
+    }
+    __syncthreads();
+    auto wsum = [&](int x) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        return x;
+    };
+    auto count_le = [&](uint32_t t) { // valid entries with d <= t
+        int c = 0;
+#pragma unroll
+        for (int s = 0; s < E; ++s) c += v[s] != kKey64None && (uint32_t)(v[s] >> 32) <= t;
+        return wsum(c);
+    };
+    int nvalid = 0;
+#pragma unroll
+    for (int s = 0; s < E; ++s) nvalid += v[s] != kKey64None;
+    nvalid = wsum(nvalid);
+    uint32_t T = 0xFFFFFFFFu; // everything is kept when there are at most k candidates
+    int m = nvalid;
+    if (nvalid > k) {
+        uint32_t lo = 0, hi = 1024; // distances of rows of <= 32 words are <= 1024
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (count_le(mid) >= k) hi = mid;
+            else lo = mid + 1;
+        }
+        T = lo;
+        m = T > 0 ? count_le(T - 1) : 0;
+    }
+    // the d < T candidates (all d when everything is kept) go to fin[0 .. m)
+    int nf = 0;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const bool take = v[s] != kKey64None && (uint32_t)(v[s] >> 32) < T;
+        const unsigned long long bal = __ballot(take);
+        if (take) {
+            const int slot = nf + __popcll(bal & ((1ull << lane) - 1ull));
+            const int32_t id = lids[(int64_t)(v[s] & 0xFFFFFFFFull)];
+            fin[wv][slot] = (v[s] & 0xFFFFFFFF00000000ull) | (uint32_t)id;
+        }
+        nf += __popcll(bal);
+    }
+    int cnt = nf;
+    if (nvalid > k) {
+        // ties: walk the lists in probe order, F = the first k candidates with d <= T
+        int cum = 0, ntie = 0;
+        for (int p = 0; p < np && cum < k; ++p) {
+            const uint64_t key = lane < K ? ent[wv][p * K + lane] : kKey64None;
+            const bool le = key != kKey64None && (uint32_t)(key >> 32) <= T;
+            const uint32_t pos = (uint32_t)(key & 0xFFFFFFFFull);
+            const int cp = __popcll(__ballot(le));
+            const int qp = cp == K ? k - cum : min(cp, k - cum);
+            int rank = 0;
+            for (int i = 0; i < K; ++i) {
+                const bool lei = __shfl(le ? 1 : 0, i, 64) != 0;
+                const uint32_t posi = __shfl(pos, i, 64);
+                rank += lei && posi < pos;
+            }
+            const bool tk = le && rank < qp && (uint32_t)(key >> 32) == T;
+            const unsigned long long bal = __ballot(tk);
+            if (tk) tie[wv][ntie + __popcll(bal & ((1ull << lane) - 1ull))] = lids[(int64_t)pos];
+            ntie += __popcll(bal);
+            cum += qp;
+        }
+        __syncthreads();
+        // the k - m lowest ids among the ties
+        const int keep = k - m;
+        if (lane < ntie) {
+            const int32_t id = tie[wv][lane];
+            int r = 0;
+            for (int i = 0; i < ntie; ++i) r += tie[wv][i] < id;
+            if (r < keep) fin[wv][m + r] = ((uint64_t)T << 32) | (uint32_t)id;
+        }
+        cnt = m + min(keep, ntie);
+    }
+    __syncthreads();
+    // ascending (distance, id): rank of each final key
+    if (lane < cnt) {
+        const uint64_t key = fin[wv][lane];
+        int r = 0;
+        for (int i = 0; i < cnt; ++i) r += fin[wv][i] < key;
+        out_idx[q * k + r] = (int32_t)(uint32_t)(key & 0xFFFFFFFFull);
+        out_dist[q * k + r] = (int32_t)(key >> 32);
+    }
+    for (int i = cnt + lane; i < k; i += 64) {
+        out_idx[q * k + i] = -1;
+        out_dist[q * k + i] = -1;
+    }
+    if (lane == 0) out_cnt[q] = cnt;
+}
+
+static int lm_list_size(int k) { return k <= 4 ? 4 : (k <= 8 ? 8 : (k <= 16 ? 16 : 32)); }
+
+// The list-major path serves k <= 32, rows of 1 .. 32 words, lists shorter than 2^21 rows and
+// n_probe * K <= 1024; the query-major kernel takes the rest.
+static bool lm_ok(const pn_ivfbin *h, int np, int k) {
+    const int w = (int)h->words;
+    return tune_get("ivfbin.list_major") != 0 && k <= 32 && (w == 1 || w == 2 || w == 4 || w == 8 || w == 16 || w == 32) &&
+           h->max_list < kMaxRowsPerSlice && (int64_t)np * lm_list_size(k) <= 1024;
+}
+
+template <int W, int K>
+static void launch_lm_scan(const pn_ivfbin *h, const int64_t *boff, const int32_t *bucket, const int64_t *items,
+                           const int32_t *n_items, int64_t max_items, const uint32_t *dq, int np, uint64_t *part,
+                           hipStream_t st) {
+    constexpr int RU = W >= 32 ? 2 : (W >= 16 ? 4 : 8);
+    hipLaunchKernelGGL((ivfbin_list_scan_kernel<W, K, RU>), dim3((unsigned)max_items), dim3(256), 0, st,
+                       h->lrows.as<uint32_t>(), h->loff.as<int64_t>(), boff, bucket, items, n_items, dq, np, part);
+}
+
+template <int W>
+static void launch_lm_scan_w(int K, const pn_ivfbin *h, const int64_t *boff, const int32_t *bucket,
+                             const int64_t *items, const int32_t *n_items, int64_t max_items, const uint32_t *dq,
+                             int np, uint64_t *part, hipStream_t st) {
+    switch (K) {
+    case 4: launch_lm_scan<W, 4>(h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
+    case 8: launch_lm_scan<W, 8>(h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
+    case 16: launch_lm_scan<W, 16>(h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
+    default: launch_lm_scan<W, 32>(h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
+    }
+}
+
+static void launch_fine_lm(pn_ivfbin *h, const uint32_t *dq, int64_t nq, int np, int k, int32_t *oi, int32_t *od,
+                           int32_t *oc, hipStream_t st) {
+    const int K = lm_list_size(k);
+    const int nc = h->nc;
+    const int64_t npairs = nq * np;
+    const int64_t max_items = npairs / kLmGroup + nc + 1;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t b_cnt = al((size_t)nc * 4), b_boff = al((size_t)(nc + 1) * 8), b_ioff = al((size_t)(nc + 1) * 4),
+                 b_ni = al(4), b_bucket = al((size_t)npairs * 4), b_items = al((size_t)max_items * 8),
+                 b_part = al((size_t)npairs * K * 8);
+    uint8_t *w = (uint8_t *)h->lm.ensure(2 * b_cnt + b_boff + b_ioff + b_ni + b_bucket + b_items + b_part);
+    int32_t *cnt = (int32_t *)w;
+    int32_t *cursor = (int32_t *)(w + b_cnt);
+    int64_t *boff = (int64_t *)(w + 2 * b_cnt);
+    int32_t *ioff = (int32_t *)(w + 2 * b_cnt + b_boff);
+    int32_t *n_items = (int32_t *)(w + 2 * b_cnt + b_boff + b_ioff);
+    int32_t *bucket = (int32_t *)(w + 2 * b_cnt + b_boff + b_ioff + b_ni);
+    int64_t *items = (int64_t *)(w + 2 * b_cnt + b_boff + b_ioff + b_ni + b_bucket);
+    uint64_t *part = (uint64_t *)(w + 2 * b_cnt + b_boff + b_ioff + b_ni + b_bucket + b_items);
+    const int32_t *pr = h->probes.as<int32_t>();
+    const unsigned gp = (unsigned)ceil_div(npairs, 256);
+    {
+        StageScope sc(st, STAGE_OTHER);
+        hipLaunchKernelGGL(zero_i32_kernel, dim3((unsigned)ceil_div(2 * b_cnt / 4, 256)), dim3(256), 0, st, cnt,
+                           (int64_t)(2 * b_cnt / 4));
+        hipLaunchKernelGGL(lm_count_kernel, dim3(gp), dim3(256), 0, st, pr, npairs, cnt);
+        hipLaunchKernelGGL(lm_scan_kernel, dim3(1), dim3(1024), 0, st, cnt, nc, boff, ioff, n_items);
+        hipLaunchKernelGGL(lm_scatter_kernel, dim3(gp), dim3(256), 0, st, pr, npairs, boff, cursor, bucket);
+        hipLaunchKernelGGL(lm_items_kernel, dim3((unsigned)ceil_div(nc, 256)), dim3(256), 0, st, boff, ioff, nc,
+                           items);
+        PN_HIP(hipGetLastError());
+    }
+    {
+        StageScope sc(st, STAGE_SCAN);
+        switch (h->words) {
+        case 1: launch_lm_scan_w<1>(K, h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
+        case 2: launch_lm_scan_w<2>(K, h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
+        case 4: launch_lm_scan_w<4>(K, h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
+        case 8: launch_lm_scan_w<8>(K, h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
+        case 16: launch_lm_scan_w<16>(K, h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
+        default: launch_lm_scan_w<32>(K, h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
+        }
+        PN_HIP(hipGetLastError());
+    }
+    StageScope sc_sel(st, STAGE_SELECT);
+    const int E = (int)std::max<int64_t>(1, ceil_div((int64_t)np * K, 64));
+    const dim3 gm((unsigned)nq);
+#define PN_LMM(EV) \
+    hipLaunchKernelGGL((ivfbin_lm_merge_kernel<EV>), gm, dim3(64), 0, st, part, nq, np, K, k, h->lids.as<int32_t>(), oi, od, oc)
+    if (E <= 1) PN_LMM(1);
+    else if (E <= 2) PN_LMM(2);
+    else if (E <= 4) PN_LMM(4);
+    else if (E <= 8) PN_LMM(8);
+    else PN_LMM(16);
+#undef PN_LMM
+    PN_HIP(hipGetLastError());
+}
+
+static void launch_fine(pn_ivfbin *h, const uint32_t *dq, int64_t nq, int np, int k, int32_t *oi, int32_t *od,
                         int32_t *oc, hipStream_t st) {
+    if (lm_ok(h, np, k)) {
+        launch_fine_lm(h, dq, nq, np, k, oi, od, oc, st);
+        return;
+    }
+    StageScope sc(st, STAGE_SCAN);
     const int nbins = (int)(h->nbytes * 8) + 1;
     const size_t sm = fine_smem(nbins, k);
     PN_REQUIRE(sm <= 64 * 1024, PN_EINVAL, "IVFFlatBinaryIndex search: k or row width too large");
@@ -397,6 +792,8 @@ static void ivfbin_build(pn_ivfbin *h, const uint8_t *rows, int64_t n, int64_t n
     h->nbytes = nbytes;
     h->words = words;
     h->nc = k;
+    h->max_list = 0;
+    for (int32_t c = 0; c < k; ++c) h->max_list = std::max<int64_t>(h->max_list, off[c + 1] - off[c]);
 }
 
 // Search of nq padded device queries; outputs int32 [nq, k] (-1 padded) and counts.
@@ -415,7 +812,6 @@ static void ivfbin_search_enqueue(pn_ivfbin *h, const uint32_t *dq, int64_t nq, 
         StageScope sc(st, STAGE_COARSE);
         exact_search_enqueue(&h->cent, dq, nq, np, pr, pd, st);
     }
-    StageScope sc(st, STAGE_SCAN);
     launch_fine(h, dq, nq, np, k, oi, od, oc, st);
 }
 

@@ -56,6 +56,7 @@ static std::map<std::string, int64_t> &tune_table() {
         t["ivfl2.k2"] = 40;            // candidates reranked in fp32 per query (max(4 k, this), <= 64)
         t["ivfl2.pass_margin_pct"] = 0; // > 0: scan the first probe, then the others bounded by its
                                          // k-th approximation x (1 + pct / 100); 0: one pass
+        t["ivfbin.list_major"] = 1;    // IVF-binary fine search: 1 list-major scan + merge, 0 query-major
         t["batch.slice_rows"] = 512;   // batched exact search: target rows per slice
         t["batch.wg_per_cu"] = 2;      // batched exact search: persistent scan workgroups per CU
         t["batch.order"] = 1;          // batched exact search: 1 = tiles by decreasing cost
