@@ -219,15 +219,17 @@ pn_status pn_ivfl2_set(pn_ivfl2 *h, const float *rows, int64_t n, int64_t d, con
 pn_status pn_ivfl2_info(const pn_ivfl2 *h, int64_t *n, int64_t *d, int32_t *n_lists);
 
 /* _IVFFlatIndex.searchKNN (forest.py:63-111): host queries nq x d; nprobe clusters per query
- * (capped at the non-empty cluster count); k <= 64. precision 0: bf16 candidate distances,
- * 1: three-product bf16 (~fp32) candidate distances, 2: fp8 e4m3 candidate distances on the
- * residuals to the list centroid (one power-of-two scale; d <= 1152); all are followed by an fp32
- * squared-L2 rerank of 64 candidates. out_idx int64[nq, k], out_dist float32[nq, k] (squared
- * L2), out_count. */
+ * (capped at the non-empty cluster count); any k. For k <= 64, precision 0: bf16 candidate
+ * distances, 1: three-product bf16 (~fp32) candidate distances, 2: fp8 e4m3 candidate distances
+ * on the residuals to the list centroid (one power-of-two scale; d <= 1152); all are followed by
+ * an fp32 squared-L2 rerank of max(4 k, 40) (at most 64) candidates. For k > 64 every row of the
+ * probed lists is scored in fp32 with the rerank's arithmetic (precision does not apply) and the
+ * k smallest are kept. out_idx int64[nq, k], out_dist float32[nq, k] (squared L2), out_count. */
 pn_status pn_ivfl2_search(pn_ivfl2 *h, const float *queries, int64_t nq, int64_t d, int32_t nprobe, int32_t k,
                           int32_t precision, int64_t *out_idx, float *out_dist, int32_t *out_count);
 
-/* Device-resident form: device queries (nq x d, contiguous), int32/float32 outputs [nq, k]. */
+/* Device-resident form: device queries (nq x d, contiguous), int32/float32 outputs [nq, k];
+ * k <= 64 (larger k: pn_ivfl2_search). */
 pn_status pn_ivfl2_search_device(pn_ivfl2 *h, const float *queries, int64_t nq, int32_t nprobe, int32_t k,
                                  int32_t precision, int32_t *out_idx, float *out_dist, int32_t *out_count,
                                  void *stream);

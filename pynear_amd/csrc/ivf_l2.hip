@@ -22,6 +22,8 @@
 #include <type_traits>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "common.hpp"
 #include "device_util.hpp"
 #include "exact_index.hpp"
@@ -876,17 +878,26 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
         // keys are skipped).
         V8 fa[2], fb[2], fc[2];
         const uint8_t *lane_l8 = L8 + col * 64 + 32 * half;
-        auto load_a = [&](int g, V8 (&a)[2]) {
-            const int rb = g / nch, ch = g - rb * nch;
+        const int nsteps = nrb * nch;
+        // the loads walk the steps in order, each trip's last ones clamped to the final step:
+        // a cursor (row block lrb, chunk lch) replaces a division per load
+        int lrb = 0, lch = 0;
+        const int last_rb = nrb - 1, last_ch = nch - 1;
+        auto load_a = [&](V8 (&a)[2]) {
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
-                const int64_t grp = min(g0 + (int64_t)GB * rb + 2 * wr + i, g1);
-                const uint4 *src = (const uint4 *)(lane_l8 + ((grp * nch + ch) << 11));
+                const int64_t grp = min(g0 + (int64_t)GB * lrb + 2 * wr + i, g1);
+                const uint4 *src = (const uint4 *)(lane_l8 + ((grp * nch + lch) << 11));
                 a[i].u[0] = src[0];
                 a[i].u[1] = src[1];
             }
+            if (lrb < last_rb || lch < last_ch) {
+                if (++lch == nch) {
+                    lch = 0;
+                    ++lrb;
+                }
+            }
         };
-        const int nsteps = nrb * nch;
         uint32_t Lk[2][KL], Li[2][KL];
 #pragma unroll
         for (int j = 0; j < 2; ++j)
@@ -934,12 +945,12 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
             // across the trip's branches keep one step fewer in flight than the ring allows, still
             // leave two.)
             auto step = [&](int g, const V8 (&cur)[2], V8 (&ahead)[2]) {
-                load_a(min(g + PD, nsteps - 1), ahead); // (unconditional, so the waits stay exact)
+                load_a(ahead); // step min(g + PD, nsteps - 1) (unconditional, so the waits stay exact)
                 const int64_t grp = g0 + (int64_t)GB * rb + 2 * wr;
                 chunk(cur, norm_lds ? s_norm + 32 * (grp - g0) : rn + 32 * grp, norm_lds);
             };
-            load_a(0, fa);
-            load_a(min(1, nsteps - 1), fb);
+            load_a(fa);
+            load_a(fb);
             if constexpr (PD == 2) {
                 for (int g = 0; g < nsteps; g += 3) {
                     step(g, fa, fc);
@@ -948,7 +959,7 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
                 }
             } else {
                 V8 fd[2];
-                load_a(min(2, nsteps - 1), fc);
+                load_a(fc);
                 for (int g = 0; g < nsteps; g += 4) {
                     step(g, fa, fd);
                     step(g + 1, fb, fa);
@@ -1208,6 +1219,40 @@ __global__ void order_scatter_kernel(const int32_t *__restrict__ key, int64_t nq
     qorder[atomicAdd(&cursor[key[q]], 1)] = (int32_t)q;
 }
 
+// sum (q - x)^2 over d dims by 16 lanes (c = lane % 16; every lane of the 16 gets the sum): with
+// VEC, lane c takes the float4 pieces 4 c, 4 c + 64, ... into four chains joined as (0+1)+(2+3);
+// otherwise dims c, c + 16, ... in one chain; then a butterfly over the 16 lanes. x = nullptr
+// gives 0 (the lanes still take part in the butterfly). Every fp32 distance the IVF-L2 search
+// returns is this sum.
+template <bool VEC>
+__device__ __forceinline__ float row_sq_dist16(const float *__restrict__ qrow, const float *__restrict__ xr,
+                                               int64_t d, int c) {
+    float part = 0.f;
+    if (xr) {
+        if (VEC) {
+            float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
+#pragma unroll 6
+            for (int64_t i = 4 * c; i < d; i += 64) {
+                const float4 a = *(const float4 *)(qrow + i), b = *(const float4 *)(xr + i);
+                const float t0 = a.x - b.x, t1 = a.y - b.y, t2 = a.z - b.z, t3 = a.w - b.w;
+                p0 = fmaf(t0, t0, p0);
+                p1 = fmaf(t1, t1, p1);
+                p2 = fmaf(t2, t2, p2);
+                p3 = fmaf(t3, t3, p3);
+            }
+            part = (p0 + p1) + (p2 + p3);
+        } else {
+            for (int64_t i = c; i < d; i += 16) {
+                const float t = qrow[i] - xr[i];
+                part = fmaf(t, t, part);
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) part += __shfl_xor(part, off, 16);
+    return part;
+}
+
 // One wave per query: the K2 best candidates by the approximation are recomputed in fp32 as
 // sum (q - x)^2 by 16 lanes each (four candidates per round; with VEC, 16-byte loads, so a wave
 // reads four rows per step in 256-byte pieces), then the k smallest (distance, id) are written
@@ -1294,30 +1339,7 @@ __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restri
 #pragma unroll 1
     for (int round = 0; round < (k2 + 3) / 4; ++round) {
         const int32_t pos = __shfl(mypos, 4 * round + g, 64); // group g: candidate 4 round + g
-        float part = 0.f;
-        if (pos >= 0) {
-            const float *xr = X + (int64_t)pos * d; // (rows in cluster order)
-            if (VEC) {
-                float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
-#pragma unroll 6
-                for (int64_t i = 4 * c; i < d; i += 64) {
-                    const float4 a = *(const float4 *)(qrow + i), b = *(const float4 *)(xr + i);
-                    const float t0 = a.x - b.x, t1 = a.y - b.y, t2 = a.z - b.z, t3 = a.w - b.w;
-                    p0 = fmaf(t0, t0, p0);
-                    p1 = fmaf(t1, t1, p1);
-                    p2 = fmaf(t2, t2, p2);
-                    p3 = fmaf(t3, t3, p3);
-                }
-                part = (p0 + p1) + (p2 + p3);
-            } else {
-                for (int64_t i = c; i < d; i += 16) {
-                    const float t = qrow[i] - xr[i];
-                    part = fmaf(t, t, part);
-                }
-            }
-        }
-#pragma unroll
-        for (int off = 8; off > 0; off >>= 1) part += __shfl_xor(part, off, 16);
+        const float part = row_sq_dist16<VEC>(qrow, pos >= 0 ? X + (int64_t)pos * d : nullptr, d, c);
         const bool owner = lane >= 4 * round && lane < 4 * round + 4;
         const float v = __shfl(part, owner ? 16 * (lane - 4 * round) : 0, 64);
         if (owner && myid >= 0) key = ((uint64_t)f32_key(v) << 32) | (uint32_t)myid;
@@ -1435,6 +1457,130 @@ static void ivfl2_set_device(pn_ivfl2 *h, const float *dX, int64_t n, int64_t d,
 static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t nq, const int32_t *pr, int np,
                                 bool sparse, int k, int prec, int32_t *oi, float *od, int32_t *oc, hipStream_t st);
 
+// ---------------------------------------------------------------- k > 64: every probed row
+// For k above the candidate pipeline's 64 (forest.py:84-105 has no limit), every row of every
+// probed list is scored in fp32 (row_sq_dist16, the rerank's arithmetic, so the distances are the
+// ones smaller k returns) and each query's keys (distance bits << 32 | id) are sorted; the first
+// k are the result, ascending (distance, id).
+
+// rows of the probed lists of query q (probes of -1 count nothing)
+__global__ void exh_count_kernel(const int32_t *__restrict__ pr, int64_t nq, int np,
+                                 const int64_t *__restrict__ loff, int64_t *__restrict__ rows) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    int64_t t = 0;
+    for (int p = 0; p < np; ++p) {
+        const int c = pr[q * np + p];
+        if (c >= 0) t += loff[c + 1] - loff[c];
+    }
+    rows[q] = t;
+}
+
+// One workgroup per (query, probe) of queries q0 .. q0 + nqc - 1: keys of the list's rows at
+// koff[q - q0] + (rows of the query's earlier probes); four rows per wave per round.
+template <bool VEC>
+__global__ __launch_bounds__(256) void exh_keys_kernel(const float *__restrict__ X, int64_t d,
+                                                       const int32_t *__restrict__ lids, const float *__restrict__ Q,
+                                                       int64_t qstride, const int32_t *__restrict__ pr, int np,
+                                                       const int64_t *__restrict__ loff, int64_t q0,
+                                                       const int64_t *__restrict__ koff, uint64_t *__restrict__ keys) {
+    __shared__ int64_t s_base;
+    const int64_t pair = blockIdx.x;
+    const int64_t q = q0 + pair / np;
+    const int p = (int)(pair % np);
+    const int c = pr[q * np + p];
+    if (c < 0) return; // (whole workgroup)
+    if (threadIdx.x == 0) {
+        int64_t b = koff[q - q0];
+        for (int pp = 0; pp < p; ++pp) {
+            const int cc = pr[q * np + pp];
+            if (cc >= 0) b += loff[cc + 1] - loff[cc];
+        }
+        s_base = b;
+    }
+    __syncthreads();
+    const int64_t r0 = loff[c], nr = loff[c + 1] - r0, base = s_base;
+    const int lane = threadIdx.x & 63, g = lane >> 4, cl = lane & 15, w = threadIdx.x >> 6;
+    const float *qrow = Q + q * qstride;
+    for (int64_t j0 = 16 * w; j0 < nr; j0 += 64) { // (wave-uniform)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t j = j0 + 4 * u + g;
+            const float v = row_sq_dist16<VEC>(qrow, j < nr ? X + (r0 + j) * d : nullptr, d, cl);
+            if (cl == 0 && j < nr) keys[base + j] = ((uint64_t)f32_key(v) << 32) | (uint32_t)lids[r0 + j];
+        }
+    }
+}
+
+__global__ void exh_out_kernel(const uint64_t *__restrict__ sorted, const int64_t *__restrict__ koff, int64_t q0,
+                               int64_t nqc, int k, int32_t *__restrict__ oi, float *__restrict__ od,
+                               int32_t *__restrict__ oc) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nqc * k) return;
+    const int64_t qi = t / k, j = t - qi * k;
+    const int64_t n = koff[qi + 1] - koff[qi];
+    const int64_t o = (q0 + qi) * k + j;
+    if (j < n) {
+        const uint64_t key = sorted[koff[qi] + j];
+        oi[o] = (int32_t)(uint32_t)(key & 0xFFFFFFFFull);
+        od[o] = key_f32((uint32_t)(key >> 32));
+    } else {
+        oi[o] = -1;
+        od[o] = 0.f;
+    }
+    if (j == 0) oc[q0 + qi] = (int32_t)min<int64_t>(n, k);
+}
+
+// Host-synchronous (the row counts size the key buffers); queries in chunks of at most
+// kExhKeys keys (or one query).
+constexpr int64_t kExhKeys = int64_t(1) << 26;
+static void ivfl2_exhaustive(pn_ivfl2 *h, const float *Q, int64_t re, int64_t nq, const int32_t *pr, int np, int k,
+                             int32_t *oi, float *od, int32_t *oc, hipStream_t st) {
+    StageScope sc(st, STAGE_RERANK);
+    const int64_t d = h->d;
+    DevBuf rows_d, koff_d, keys_a, keys_b, tmp;
+    int64_t *rows = (int64_t *)rows_d.ensure((size_t)nq * 8);
+    hipLaunchKernelGGL(exh_count_kernel, dim3((unsigned)ceil_div(nq, (int64_t)256)), dim3(256), 0, st, pr, nq, np,
+                       h->loff.as<int64_t>(), rows);
+    std::vector<int64_t> hr(nq);
+    PN_HIP(hipMemcpyAsync(hr.data(), rows, (size_t)nq * 8, hipMemcpyDeviceToHost, st));
+    PN_HIP(hipStreamSynchronize(st));
+    const bool vec = d % 4 == 0 && re % 4 == 0;
+    int64_t qa = 0;
+    while (qa < nq) {
+        int64_t qb = qa, tot = 0;
+        while (qb < nq && (qb == qa || tot + hr[qb] <= kExhKeys)) tot += hr[qb++];
+        PN_REQUIRE(tot < (int64_t)INT32_MAX, PN_EINVAL, "IVF search: too many probed rows for one query");
+        const int64_t nqc = qb - qa;
+        std::vector<int64_t> ko(nqc + 1, 0);
+        for (int64_t i = 0; i < nqc; ++i) ko[i + 1] = ko[i] + hr[qa + i];
+        int64_t *koff = (int64_t *)koff_d.ensure((size_t)(nqc + 1) * 8);
+        PN_HIP(hipMemcpyAsync(koff, ko.data(), (size_t)(nqc + 1) * 8, hipMemcpyHostToDevice, st));
+        uint64_t *ka = (uint64_t *)keys_a.ensure((size_t)std::max<int64_t>(tot, 1) * 8);
+        uint64_t *kb = (uint64_t *)keys_b.ensure((size_t)std::max<int64_t>(tot, 1) * 8);
+        if (tot > 0) {
+            const dim3 gk((unsigned)(nqc * np));
+            if (vec)
+                hipLaunchKernelGGL(exh_keys_kernel<true>, gk, dim3(256), 0, st, h->X.as<float>(), d,
+                                   h->lids.as<int32_t>(), Q, re, pr, np, h->loff.as<int64_t>(), qa, koff, ka);
+            else
+                hipLaunchKernelGGL(exh_keys_kernel<false>, gk, dim3(256), 0, st, h->X.as<float>(), d,
+                                   h->lids.as<int32_t>(), Q, re, pr, np, h->loff.as<int64_t>(), qa, koff, ka);
+            size_t tb = 0;
+            PN_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, ka, kb, (int)tot, (int)nqc, koff, koff + 1,
+                                                              0, 64, st));
+            void *tp = tmp.ensure(tb);
+            PN_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(tp, tb, ka, kb, (int)tot, (int)nqc, koff, koff + 1, 0, 64,
+                                                              st));
+        }
+        hipLaunchKernelGGL(exh_out_kernel, dim3((unsigned)ceil_div(nqc * k, (int64_t)256)), dim3(256), 0, st, kb, koff,
+                           qa, nqc, k, oi, od, oc);
+        PN_HIP(hipGetLastError());
+        PN_HIP(hipStreamSynchronize(st)); // (the chunk's host offsets and buffers are reused)
+        qa = qb;
+    }
+}
+
 // ABI precision -> scan variant: 0 bf16 (1 product), 1 bf16 hi/lo (3 products), 2 fp8 e4m3
 static int prec_of(int32_t precision) {
     PN_REQUIRE(precision >= 0 && precision <= 2, PN_EINVAL, "precision must be 0 (bf16), 1 (bf16x3) or 2 (fp8)");
@@ -1474,6 +1620,10 @@ static void ivfl2_search_enqueue(pn_ivfl2 *h, const float *dq, int64_t qstride_i
 // empty slot). Q has row stride re.
 static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t nq, const int32_t *pr, int np,
                                 bool sparse, int k, int prec, int32_t *oi, float *od, int32_t *oc, hipStream_t st) {
+    if (k > kIvfK2) { // (host entry point only: the device ones take k <= 64)
+        ivfl2_exhaustive(h, Q, re, nq, pr, np, k, oi, od, oc, st);
+        return;
+    }
     const int64_t d = h->d;
     // bf16 split of the queries
     const int dp = h->lrows.dp;
@@ -1745,7 +1895,6 @@ extern "C" pn_status pn_ivfl2_search(pn_ivfl2 *h, const float *queries, int64_t 
         PN_REQUIRE(h, PN_EINVAL, "index is null");
         PN_REQUIRE(h->n > 0, PN_ERUNTIME, "Index is empty - call set() first");
         PN_REQUIRE(nq >= 0 && k >= 0, PN_EINVAL, "negative size");
-        PN_REQUIRE(k <= 64, PN_EINVAL, "IVFFlatL2Index search supports k <= 64");
         PN_REQUIRE(d == h->d, PN_EINVAL, "query dimension differs from the index dimension");
         if (nq == 0 || k == 0) {
             if (nq && out_count) std::memset(out_count, 0, (size_t)nq * 4);

@@ -250,3 +250,39 @@ def test_device_entry_point_matches_host():
     np.testing.assert_array_equal(oc.numpy(), hc)
     np.testing.assert_array_equal(oi.numpy(), hi)
     np.testing.assert_array_equal(od.numpy(), hd)
+
+
+@pytest.mark.parametrize("k", [65, 100, 500])
+@pytest.mark.parametrize("n_probe", [1, 5, 40])
+def test_k_above_64_scores_every_probed_row(k, n_probe):
+    """k > 64 (no limit in forest.py:84-105): every row of the probed lists is scored in fp32, so
+    the result is the exact top-k of those rows up to fp32 near ties, the same for every
+    precision; a query with fewer probed rows than k gets all of them."""
+    X = _gmm(8000, 48, 40, seed=41)
+    Q = _gmm(40, 48, 40, seed=42)
+    idx = IVFFlatL2Index(n_clusters=40, n_probe=n_probe)
+    idx.set(X)
+    gi, gd = idx.searchKNN(Q, k)
+    idx.precision = "fp8"
+    fi, fd = idx.searchKNN(Q, k)
+    assert gi == fi and gd == fd
+    lists = np.arange(len(idx._list_centroids))
+    chosen, clear = nearest_lists(Q, idx._centroids[idx._list_centroids], lists, n_probe)
+    assert clear.mean() > 0.9
+    for i in np.flatnonzero(clear):
+        rows = np.sort(np.concatenate([idx._orig_indices[c] for c in chosen[i]]))
+        assert len(gi[i]) == min(k, len(rows))
+        near_tie_check(gi[i], gd[i], rows, sq_l2_rows(X, Q[i], rows), k, 1e-5)
+
+
+def test_k_above_64_device_entry_rejects():
+    from pynear_amd import _native as N, hipdev
+    X = _gmm(2000, 16, 10, seed=43)
+    h = N.IvfL2Handle()
+    h.build(X, 10, 10, 42)
+    dq = hipdev.DeviceArray.from_numpy(X[:4])
+    o = hipdev.DeviceArray((4, 100), np.int32)
+    with pytest.raises(ValueError):
+        h.search_device(dq.ptr, 4, 3, 100, o.ptr, o.ptr, o.ptr)
+    idx, dist, cnt = h.search(X[:4], 10, 2000)  # every row, k = n
+    assert (cnt == 2000).all() and (idx[:, 0] == np.arange(4)).all() and (dist[:, 0] == 0).all()
