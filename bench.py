@@ -494,8 +494,8 @@ def run_ivf_binary(steps, warmup, cpu, nlist=512, nprobe=16, k=10):
            "stages_ms_per_step": {s: round(v[0] / steps, 4) for s, v in stages.items() if v[1]},
            "roofline": {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_INT32_VALU_TOPS, "unit": "TOP/s",
                         "frac": round(ach / PEAK_INT32_VALU_TOPS, 4),
-                        "traffic": measured_traffic("ivfbin_list_scan_kernel<16, 16, 4>"),
-                        "kernel": "ivfbin_list_scan_kernel<16, 16, 4>", "kernel_ms": round(scan_s * 1e3, 4),
+                        "traffic": measured_traffic("ivfbin_list_scan_kernel<16, 10, 4>"),
+                        "kernel": "ivfbin_list_scan_kernel<16, 10, 4>", "kernel_ms": round(scan_s * 1e3, 4),
                         "work": (f"{ops:.3e} int32 ops per launch: 16 words x (xor + popcount-accumulate) x "
                                  f"{probed:,} probed rows (all queries), one distance per candidate")}}
     if cpu:

@@ -296,6 +296,39 @@ __global__ void lm_count_kernel(const int32_t *__restrict__ probes, int64_t npai
     if (t < npairs) atomicAdd(&cnt[probes[t]], 1);
 }
 
+// Two-level bucketing for nc <= kLmLdsLists: a workgroup counts its kLmChunk pairs per list in
+// LDS, then takes one range per list from the global counts (wgb[block][list] = its start within
+// the list's bucket); the scatter places each pair at boff + wgb + its rank from an LDS counter.
+// (One global atomic per workgroup and list instead of one per pair.)
+constexpr int kLmLdsLists = 8192;
+constexpr int kLmChunk = 8192;
+__global__ __launch_bounds__(256) void lm_count_lds_kernel(const int32_t *__restrict__ probes, int64_t npairs, int nc,
+                                                           int32_t *__restrict__ cnt, int32_t *__restrict__ wgb) {
+    __shared__ int32_t hist[kLmLdsLists];
+    for (int c = threadIdx.x; c < nc; c += 256) hist[c] = 0;
+    __syncthreads();
+    const int64_t t0 = (int64_t)blockIdx.x * kLmChunk, t1 = min(npairs, t0 + kLmChunk);
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) atomicAdd(&hist[probes[t]], 1);
+    __syncthreads();
+    int32_t *my = wgb + (int64_t)blockIdx.x * nc;
+    for (int c = threadIdx.x; c < nc; c += 256) my[c] = hist[c] ? atomicAdd(&cnt[c], hist[c]) : 0;
+}
+
+__global__ __launch_bounds__(256) void lm_scatter_lds_kernel(const int32_t *__restrict__ probes, int64_t npairs, int nc,
+                                                             const int64_t *__restrict__ boff,
+                                                             const int32_t *__restrict__ wgb,
+                                                             int32_t *__restrict__ bucket) {
+    __shared__ int32_t cur[kLmLdsLists];
+    for (int c = threadIdx.x; c < nc; c += 256) cur[c] = 0;
+    __syncthreads();
+    const int32_t *my = wgb + (int64_t)blockIdx.x * nc;
+    const int64_t t0 = (int64_t)blockIdx.x * kLmChunk, t1 = min(npairs, t0 + kLmChunk);
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
+        const int c = probes[t];
+        bucket[boff[c] + my[c] + atomicAdd(&cur[c], 1)] = (int32_t)t;
+    }
+}
+
 // One workgroup of 1024: exclusive scans of the per-list pair counts (boff) and item counts
 // (ioff); n_items = total items.
 __global__ __launch_bounds__(1024) void lm_scan_kernel(const int32_t *__restrict__ cnt, int nc,
@@ -564,7 +597,11 @@ __global__ __launch_bounds__(64) void ivfbin_lm_merge_kernel(const uint64_t *__r
     if (lane == 0) out_cnt[q] = cnt;
 }
 
-static int lm_list_size(int k) { return k <= 4 ? 4 : (k <= 8 ? 8 : (k <= 16 ? 16 : 32)); }
+// K >= k keys per (query, probe) list; the insert into a lane's list costs K operations, so K
+// follows k closely (10 and 20 for the common k = 10 and 20)
+static int lm_list_size(int k) {
+    return k <= 4 ? 4 : (k <= 8 ? 8 : (k <= 10 ? 10 : (k <= 16 ? 16 : (k <= 20 ? 20 : 32))));
+}
 
 // The list-major path serves k <= 32, rows of 1 .. 32 words, lists shorter than 2^21 rows and
 // n_probe * K <= 1024; the query-major kernel takes the rest.
@@ -590,7 +627,9 @@ static void launch_lm_scan_w(int K, const pn_ivfbin *h, const int64_t *boff, con
     switch (K) {
     case 4: launch_lm_scan<W, 4>(h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
     case 8: launch_lm_scan<W, 8>(h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
+    case 10: launch_lm_scan<W, 10>(h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
     case 16: launch_lm_scan<W, 16>(h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
+    case 20: launch_lm_scan<W, 20>(h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
     default: launch_lm_scan<W, 32>(h, boff, bucket, items, n_items, max_items, dq, np, part, st); break;
     }
 }
@@ -605,7 +644,11 @@ static void launch_fine_lm(pn_ivfbin *h, const uint32_t *dq, int64_t nq, int np,
     const size_t b_cnt = al((size_t)nc * 4), b_boff = al((size_t)(nc + 1) * 8), b_ioff = al((size_t)(nc + 1) * 4),
                  b_ni = al(4), b_bucket = al((size_t)npairs * 4), b_items = al((size_t)max_items * 8),
                  b_part = al((size_t)npairs * K * 8);
-    uint8_t *w = (uint8_t *)h->lm.ensure(2 * b_cnt + b_boff + b_ioff + b_ni + b_bucket + b_items + b_part);
+    const bool lds = nc <= kLmLdsLists;
+    const int64_t nwg = ceil_div(npairs, (int64_t)kLmChunk);
+    const size_t b_wgb = lds ? al((size_t)nwg * nc * 4) : 0;
+    uint8_t *w = (uint8_t *)h->lm.ensure(2 * b_cnt + b_boff + b_ioff + b_ni + b_bucket + b_items + b_part + b_wgb);
+    int32_t *wgb = (int32_t *)(w + 2 * b_cnt + b_boff + b_ioff + b_ni + b_bucket + b_items + b_part);
     int32_t *cnt = (int32_t *)w;
     int32_t *cursor = (int32_t *)(w + b_cnt);
     int64_t *boff = (int64_t *)(w + 2 * b_cnt);
@@ -620,9 +663,16 @@ static void launch_fine_lm(pn_ivfbin *h, const uint32_t *dq, int64_t nq, int np,
         StageScope sc(st, STAGE_OTHER);
         hipLaunchKernelGGL(zero_i32_kernel, dim3((unsigned)ceil_div(2 * b_cnt / 4, 256)), dim3(256), 0, st, cnt,
                            (int64_t)(2 * b_cnt / 4));
-        hipLaunchKernelGGL(lm_count_kernel, dim3(gp), dim3(256), 0, st, pr, npairs, cnt);
+        if (lds)
+            hipLaunchKernelGGL(lm_count_lds_kernel, dim3((unsigned)nwg), dim3(256), 0, st, pr, npairs, nc, cnt, wgb);
+        else
+            hipLaunchKernelGGL(lm_count_kernel, dim3(gp), dim3(256), 0, st, pr, npairs, cnt);
         hipLaunchKernelGGL(lm_scan_kernel, dim3(1), dim3(1024), 0, st, cnt, nc, boff, ioff, n_items);
-        hipLaunchKernelGGL(lm_scatter_kernel, dim3(gp), dim3(256), 0, st, pr, npairs, boff, cursor, bucket);
+        if (lds)
+            hipLaunchKernelGGL(lm_scatter_lds_kernel, dim3((unsigned)nwg), dim3(256), 0, st, pr, npairs, nc, boff, wgb,
+                               bucket);
+        else
+            hipLaunchKernelGGL(lm_scatter_kernel, dim3(gp), dim3(256), 0, st, pr, npairs, boff, cursor, bucket);
         hipLaunchKernelGGL(lm_items_kernel, dim3((unsigned)ceil_div(nc, 256)), dim3(256), 0, st, boff, ioff, nc,
                            items);
         PN_HIP(hipGetLastError());

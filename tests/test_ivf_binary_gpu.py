@@ -137,7 +137,7 @@ def test_larger_index_sample():
 
 @pytest.mark.parametrize("list_major", [1, 0])
 @pytest.mark.parametrize("nbytes", [4, 8, 20, 64, 128, 130])
-@pytest.mark.parametrize("k,nprobe", [(1, 1), (3, 7), (17, 5), (32, 40)])
+@pytest.mark.parametrize("k,nprobe", [(1, 1), (3, 7), (10, 4), (12, 3), (17, 5), (32, 40)])
 def test_list_major_and_query_major_paths(list_major, nbytes, k, nprobe):
     """Both fine searches (list-major: per-list top lists and a per-query merge; query-major: one
     workgroup per query) equal the literal heap restatement, on rows with many distance ties."""
