@@ -398,20 +398,54 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     const int64_t *nc = b->ncls;
     const unsigned grid_scan =
         (unsigned)std::max<int64_t>(1, std::min<int64_t>(nc[0], (int64_t)b->cus * std::max(1, cfg.wg_per_cu)));
-    // the wide scans hold one wave per SIMD: one workgroup per CU
+    // the wide scans hold one wave per SIMD (their query fragments in registers): one workgroup
+    // per CU
     const unsigned grid_wide = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nc[1], (int64_t)b->cus));
     const unsigned grid_wd = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nc[2], (int64_t)b->cus));
+    // The three scans are independent branches of the graph (a fork after the split, a join before
+    // the merge): the wide class holds one workgroup -- one wave per SIMD -- per CU for its LDS
+    // tiles, and the other two classes' workgroups run beside it instead of after it. The wide
+    // scan is launched first so that its workgroups find their LDS free.
+    const bool fork = tune_get("batch.concurrent_scans") != 0 && nc[1] > 0 && (nc[0] > 0 || nc[2] > 0);
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    struct Cleanup {
+        hipStream_t *s;
+        hipEvent_t *e;
+        ~Cleanup() {
+            for (int i = 0; i < 2; ++i)
+                if (s[i]) (void)hipStreamDestroy(s[i]);
+            for (int i = 0; i < 3; ++i)
+                if (e[i]) (void)hipEventDestroy(e[i]);
+        }
+    } cleanup{side, ev};
+    if (fork) {
+        for (int i = 0; i < 2; ++i) PN_HIP(hipStreamCreateWithFlags(&side[i], hipStreamNonBlocking));
+        for (int i = 0; i < 3; ++i) PN_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+    }
     PN_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(batch_split_kernel, dim3((unsigned)ceil_div(rows, (int64_t)4)), dim3(256), 0, st, dE, dR, ne,
                        rows, ctr);
     const int2 *dt = b->tiles.as<int2>();
-    if (nc[0] > 0)
-        hipLaunchKernelGGL(batch_scan_kernel<0>, dim3(grid_scan), dim3(256), 0, st, dE, dt, nc[0], ctr);
+    if (fork) {
+        PN_HIP(hipEventRecord(ev[0], st));
+        PN_HIP(hipStreamWaitEvent(side[0], ev[0], 0));
+        PN_HIP(hipStreamWaitEvent(side[1], ev[0], 0));
+    }
     if (nc[1] > 0)
         hipLaunchKernelGGL(batch_scan_kernel<1>, dim3(grid_wide), dim3(256), 0, st, dE, dt + nc[0], nc[1], ctr + 1);
+    if (nc[0] > 0)
+        hipLaunchKernelGGL(batch_scan_kernel<0>, dim3(grid_scan), dim3(256), 0, fork ? side[0] : st, dE, dt, nc[0],
+                           ctr);
     if (nc[2] > 0)
-        hipLaunchKernelGGL(batch_scan_kernel<2>, dim3(grid_wd), dim3(256), 0, st, dE, dt + nc[0] + nc[1], nc[2],
-                           ctr + 2);
+        hipLaunchKernelGGL(batch_scan_kernel<2>, dim3(grid_wd), dim3(256), 0, fork ? side[1] : st, dE,
+                           dt + nc[0] + nc[1], nc[2], ctr + 2);
+    if (fork) {
+        PN_HIP(hipEventRecord(ev[1], side[0]));
+        PN_HIP(hipEventRecord(ev[2], side[1]));
+        PN_HIP(hipStreamWaitEvent(st, ev[1], 0));
+        PN_HIP(hipStreamWaitEvent(st, ev[2], 0));
+    }
     const bool fused = tune_get("merge.lean") != 0 && tune_get("l2.fuse_rerank") != 0;
     if (fused) {
         hipLaunchKernelGGL(batch_merge_rerank_kernel, dim3(xcd_grid(ceil_div(queries, (int64_t)4))), dim3(256), 0, st,

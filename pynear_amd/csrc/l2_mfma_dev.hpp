@@ -53,7 +53,38 @@ __device__ __forceinline__ void split_row(const float *__restrict__ X, int64_t n
 // its row, and (key & ~31) is a lower bound of the approximation. Passing keys (below thr) are
 // inserted in ascending order; thr may come from the other half-wave's list of the same query:
 // rows at or above it cannot reach the KL best of the two lists together.
-template <int KL, bool MASK, bool FIRST>
+// Merges 16 ascending keys b (of the tile at row t) into the ascending list (Lk, Li) of KL = 16 or
+// 32, keeping the KL smallest: the list followed by b reversed is bitonic, so one half-cleaner
+// step (entry i takes the smaller of Lk[i] and the key facing it) leaves the KL smallest as a
+// bitonic sequence, which a bitonic merge sorts. KL - 1 + KL log2(KL) / 2 compare-exchanges.
+template <int KL>
+__device__ __forceinline__ void merge_sorted16(const uint32_t (&b)[16], uint32_t t, uint32_t (&Lk)[KL],
+                                               uint32_t (&Li)[KL]) {
+    static_assert(KL == 16 || KL == 32, "lists of 16 or 32 keys");
+#pragma unroll
+    for (int i = KL - 16; i < KL; ++i) {
+        const uint32_t bk = b[KL - 1 - i];
+        const bool take = bk < Lk[i];
+        Li[i] = take ? t + (bk & 31u) : Li[i];
+        Lk[i] = take ? bk : Lk[i];
+    }
+#pragma unroll
+    for (int s = KL / 2; s > 0; s >>= 1)
+#pragma unroll
+        for (int i = 0; i < KL; ++i)
+            if ((i & s) == 0) {
+                const bool sw = Lk[i + s] < Lk[i];
+                const uint32_t ka = Lk[i], kb = Lk[i + s], ia = Li[i], ib = Li[i + s];
+                Lk[i] = sw ? kb : ka;
+                Lk[i + s] = sw ? ka : kb;
+                Li[i] = sw ? ib : ia;
+                Li[i + s] = sw ? ia : ib;
+            }
+}
+
+// NET: with many keys to insert, take the network path below (the LDS-staged wide scan; in the
+// other scans its registers would spill)
+template <int KL, bool MASK, bool FIRST, bool NET = false>
 __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4 (&xv)[4], float qnv, int64_t t,
                                                  int64_t r1, int half, uint32_t (&Lk)[KL], uint32_t (&Li)[KL],
                                                  uint32_t &thr) {
@@ -90,6 +121,22 @@ __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4
         }
         thr = min(thr, Lk[KL - 1]);
         return;
+    }
+    // many keys below thr (a list still filling, as over short slices with k = 32): the tile's keys
+    // under thr are sorted and merged by networks in one go -- the same KL smallest as the rounds
+    // below take, whose cost grows with the number inserted
+    if constexpr (NET && (KL == 16 || KL == 32)) {
+        int below = 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) below += key[r] < thr ? 1 : 0;
+        if (__any(below >= 4)) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) key[r] = key[r] < thr ? key[r] : kKey32None;
+            sort16(key);
+            merge_sorted16<KL>(key, (uint32_t)t, Lk, Li);
+            thr = min(thr, Lk[KL - 1]);
+            return;
+        }
     }
     // keys below thr go in ascending order (thr falls as the list fills), one round each among the
     // keys above the last taken (keys are unique); with FAST2 the two smallest go in first
@@ -454,8 +501,8 @@ __device__ __forceinline__ void scan_wide(uint64_t (*lds)[32][KL], __bf16 *stage
                 t + 32 <= r1 ? tile_keys_insert<KL, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
                              : tile_keys_insert<KL, true, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
             else
-                t + 32 <= r1 ? tile_keys_insert<KL, false, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
-                             : tile_keys_insert<KL, true, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+                t + 32 <= r1 ? tile_keys_insert<KL, false, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
+                             : tile_keys_insert<KL, true, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
 #pragma unroll
             for (int g = 0; g < 4; ++g) xv[g] = *(const float4 *)(xn + tn + 8 * g + 4 * half);
             // the other buffer was last read for tile i - 1, which every wave finished before the

@@ -60,6 +60,7 @@ static std::map<std::string, int64_t> &tune_table() {
         t["batch.slice_rows"] = 512;   // batched exact search: target rows per slice
         t["batch.wg_per_cu"] = 2;      // batched exact search: persistent scan workgroups per CU
         t["batch.order"] = 1;          // batched exact search: 1 = tiles by decreasing cost
+        t["batch.concurrent_scans"] = 1; // batched exact search: the three scan classes as parallel graph branches
         if (const char *env = std::getenv("PN_TUNE")) {
             std::string e(env);
             size_t pos = 0;
