@@ -382,7 +382,7 @@ def run_ivf(steps, warmup, cpu, probes=(8, 16, 32), precisions=("bf16", "fp8")):
                  "other_bound": {"hbm_GBs": round(bw, 1), "mfma_TFLOPs": round(tf, 2)}})
     out = {"workload": (f"IVFFlatL2Index GMM N={n:,} x d={d} (1,024 comps, sigma 0.3), nlist={nlist}, "
                         f"Q={nq:,}, k={k}, n_probe={chosen} (smallest with Recall@10 >= 0.95), "
-                        f"{prec} candidate distances + fp32 rerank of 64"),
+                        f"{prec} candidate distances + fp32 rerank of max(4k, 40)"),
            "value": S["qps"], "unit": "queries/s", "ms_per_step": S["ms_per_step"], "steps": S["steps"],
            "roofline": roof,
            "dtype": "fp8e4m3+f32" if prec == "fp8" else "bf16+f32",
@@ -767,19 +767,39 @@ def run_learned(steps, warmup, cpu, target=0.95):
             stream.synchronize()
             r = recall()
             if r >= target:
-                _, counts = probe.select(dq.ptr, nq, mass, 1, np_max)
-                ms = _time_device(N, hipdev, learned(mass, prec), steps, warmup, stream)[0] / steps
-                by_prec[prec] = {"precision": prec, "mass": mass, "mean_n_probe": round(float(counts.numpy().mean()), 2),
+                probes, counts = probe.select(dq.ptr, nq, mass, 1, np_max)
+                hp, hc = probes.numpy(), counts.numpy()
+                gms, _, stages = _time_device(N, hipdev, learned(mass, prec), steps, warmup, stream)
+                ms = gms / steps
+                # the list scan's algorithmic flops: 2 d per (query, row of a selected list)
+                sizes = np.bincount(labels, minlength=nlist)
+                sel = np.arange(np_max)[None, :] < hc[:, None]
+                rows = int(np.where(sel, sizes[np.maximum(hp, 0)] * (hp >= 0), 0).sum())
+                scan_ms, scan_n = stages["scan"]
+                scan_s = scan_ms / max(scan_n, 1) / 1e3
+                peak = PEAK_FP8_TFLOPS if prec == "fp8" else PEAK_BF16_TFLOPS
+                tf = 2.0 * X.shape[1] * rows / scan_s / 1e12
+                by_prec[prec] = {"precision": prec, "mass": mass, "mean_n_probe": round(float(hc.mean()), 2),
                                  "recall_at_10": round(r, 4), "ms_per_step": round(ms, 4),
-                                 "qps": round(nq / (ms / 1e3), 1)}
+                                 "qps": round(nq / (ms / 1e3), 1),
+                                 "stages_ms_per_step": {s: round(v[0] / steps, 4) for s, v in stages.items() if v[1]},
+                                 "roofline": {"bound": "mfma", "achieved": round(tf, 2), "peak": peak,
+                                              "unit": "TFLOP/s", "frac": round(tf / peak, 4), "traffic": None,
+                                              "kernel": "ivfl2_scan_fp8_kernel<8, 3>" if prec == "fp8"
+                                              else "ivfl2_scan_kernel<1, 8>",
+                                              "kernel_ms": round(scan_s * 1e3, 4),
+                                              "work": f"{2.0 * X.shape[1] * rows:.3e} flop (2 d x {rows:,} rows of the "
+                                                      f"selected lists)"}}
                 break
         _log(f"learned: adaptive {prec} {by_prec.get(prec)}")
     lrn = max(by_prec.values(), key=lambda v: v["qps"]) if by_prec else None
     out = {"workload": (f"learned-probe IVF-L2, Zipf(1.1) GMM N={len(X):,} x d={X.shape[1]}, nlist={nlist}, "
                         f"MLP {X.shape[1]}->256->{nlist} bf16 trained on {len(QT):,} queries, {nq:,} held-out, k={k}; "
-                        f"list scan in {lrn['precision'] if lrn else '-'}, fp32 rerank of 64"),
+                        f"list scan in {lrn['precision'] if lrn else '-'}, fp32 rerank of max(4k, 40)"),
            "value": lrn["qps"] if lrn else None, "unit": "queries/s",
            "ms_per_step": lrn["ms_per_step"] if lrn else None,
+           "roofline": lrn["roofline"] if lrn else None,
+           "kmeans_iters": 20,  # (the fixed and learned searches share this index; forest.py's set uses 100)
            "learned": lrn, "learned_by_precision": by_prec, "fixed_nprobe_fp32": base, "fixed_nprobe_bf16": fast,
            "fixed_nprobe_fp8": fast8,
            "speedup_vs_fixed_fp32": round(lrn["qps"] / base["qps"], 3) if lrn and base else None,
