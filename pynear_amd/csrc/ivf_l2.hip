@@ -56,12 +56,16 @@ struct pn_ivfl2 {
     int32_t nc = 0;          // k-means clusters (empty ones included)
     pn::DevBuf pmap;         // per search: given probes mapped to list indices
     pn::DevBuf l8;           // fp8 e4m3 residuals x - c (cluster order) scaled by sx8, dp bytes per row
+    // segments (row ranges: a list's parts, or the list): fp8 centre per segment (l8's layout, one
+    // row per segment), {R, X, |centre|, min |x - c|^2}, row offsets [nseg + 1], first segment per list
+    pn::DevBuf c8, smeta, soff, sbase;
+    int64_t nseg = 0;
     pn::DevBuf rn8;          // |x - c|^2 per row (cluster order)
     pn::DevBuf lcent;        // list centroids, fp32 nl x d (list order)
     float sx8 = 1.f;
     pn::DevBuf qpad, qsplit_h, qsplit_l, qn, probes, pdist, lcount, cursor, pair_q, pair_p, pair_c, pslot;
     pn::DevBuf items, ntot /* item ranges */, itoff, woff, cand, qoff, qthr, qres8, qresn, qkey, qorder, ohist, oidx,
-        odist, ocnt;
+        odist, ocnt, qres8n, pact;
 };
 
 namespace pn {
@@ -595,30 +599,29 @@ __device__ __forceinline__ void list_insert16r(uint32_t (&key)[16], uint32_t (&L
     }
 }
 
+// Row norms come from LDS only: a global load here would enter the vector-memory counter behind
+// the fragment loads in flight, and waiting for it would drain them at every row block. Rows are
+// 32-bit (a list's rows and the database's fit in int32).
 template <int KL>
-__device__ __forceinline__ void fp8_rowblock_keys(const f32x16 (&acc)[2][2], float m2, const float *xs, bool xs_lds,
-                                                  int64_t tb0, int64_t r0, int64_t r1, bool first, float qv0,
-                                                  float qv1, uint32_t (&Lk)[2][KL], uint32_t (&Li)[2][KL],
-                                                  uint32_t *sthr, int qcol, int half, bool two) {
+__device__ __forceinline__ void fp8_rowblock_keys(const f32x16 (&acc)[2][2], float m2, const float *xs, int tb0,
+                                                  int r0, int r1, bool first, float qv0, float qv1,
+                                                  uint32_t (&Lk)[2][KL], uint32_t (&Li)[2][KL], uint32_t *sthr,
+                                                  int qcol, int half, bool two) {
     uint32_t thr[2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) thr[j] = min(Lk[j][KL - 1], sthr[qcol + 32 * j]);
     const int h4 = 4 * half;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-        const int64_t tb = tb0 + 32 * i;
+        const int tb = tb0 + 32 * i;
         if (tb >= r1) break; // (wave-uniform; so is the next tile)
         // the tile's rows inside the list are offsets lo .. hi - 1 (all 32 except at a list's ends)
-        const int lo = (int)max<int64_t>(r0 - tb, 0), hi = (int)min<int64_t>(r1 - tb, 32);
+        const int lo = max(r0 - tb, 0), hi = min(r1 - tb, 32);
         const bool edge = lo > 0 || hi < 32;
         float xr[16];
-        if (xs_lds) {
+        {
             typedef const __attribute__((address_space(3))) float lds_float;
             lds_float *x = (lds_float *)xs + 32 * i + 4 * half;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) xr[r] = x[8 * (r >> 2) + (r & 3)];
-        } else {
-            const float *x = xs + 32 * i + 4 * half;
 #pragma unroll
             for (int r = 0; r < 16; ++r) xr[r] = x[8 * (r >> 2) + (r & 3)];
         }
@@ -709,7 +712,8 @@ __global__ __launch_bounds__(256) void residual_kernel(const float *__restrict__
 }
 
 // Query residuals of every pair, (q - c) * scale as fp8 e4m3 (clamped to +-448) into R8[pair
-// slot] (dp bytes, zero padding) and |q - c|^2 into Rn[pair slot]. One wave per query: its row is
+// slot] (dp bytes, zero padding), |q - c|^2 into Rn[pair slot] and, when Rq is given, the norm of
+// the fp8 residual (its fp32 sum rounded up by 2^-10) into Rq[pair slot]. One wave per query: its row is
 // read once (dims 4 l + 256 t .. +3 in lane l), then each of its probes p_lo .. p_lo + npw - 1
 // (list probes[q np + p], slot pslot[q np + p]) is converted; slots of absent probes are skipped.
 template <int NV>
@@ -717,7 +721,8 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
                                                              int dp, const float *__restrict__ lcent, float scale,
                                                              const int32_t *__restrict__ probes, int64_t nq, int np,
                                                              int p_lo, int npw, const int32_t *__restrict__ pslot,
-                                                             uint8_t *__restrict__ R8, float *__restrict__ Rn) {
+                                                             uint8_t *__restrict__ R8, float *__restrict__ Rn,
+                                                             float *__restrict__ Rq) {
     const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (q >= nq) return;
@@ -756,7 +761,7 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
     auto emit = [&](int c, int64_t sl, const Row &cv) __attribute__((always_inline)) {
         if (c < 0) return; // (wave-uniform)
         uint8_t *out = R8 + sl * dp;
-        float ss = 0.f;
+        float ss = 0.f, s8 = 0.f;
 #pragma unroll
         for (int t = 0; t < NV; ++t) {
             const int e0 = 4 * lane + 256 * t;
@@ -772,11 +777,21 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
                 wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], wq, false);
                 wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], wq, true);
                 *(int *)(out + e0) = wq;
+                if (Rq) {
+                    const float a0 = __builtin_amdgcn_cvt_f32_fp8(wq, 0), a1 = __builtin_amdgcn_cvt_f32_fp8(wq, 1);
+                    const float a2 = __builtin_amdgcn_cvt_f32_fp8(wq, 2), a3 = __builtin_amdgcn_cvt_f32_fp8(wq, 3);
+                    s8 = fmaf(a0, a0, fmaf(a1, a1, fmaf(a2, a2, fmaf(a3, a3, s8))));
+                }
             }
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
         if (lane == 0) Rn[sl] = ss;
+        if (Rq) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) s8 += __shfl_xor(s8, off, 64);
+            if (lane == 0) Rq[sl] = sqrtf(s8) * (1.f + 1.f / 1024.f);
+        }
     };
     for (int pb = p_lo; pb < p_lo + npw; pb += 64) {
         const int nj = min(64, p_lo + npw - pb);
@@ -793,6 +808,115 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
             }
         }
     }
+}
+
+// ---------------------------------------------------------------- group pruning
+// For the second pass of the fp8 scan (queries hold a bound from their first probe): per work
+// item (a list and up to 128 of its queries) and per 32-query tile qt, which 32-row groups of the
+// list some query of the tile may need. The list's rows are cut into segments (its parts, or the
+// whole list); for every row x of a segment and query residual q8,
+// q8.x8 <= U = q8.centre + |q8| (R + 2^-11 (X + |centre|)) (seg_meta_kernel; the 2^-11 covers
+// the fp32 rounding of both products' sums), so the scan's approximation of any row of the
+// segment is at least lb = qv + mxr + m2 U (m2 < 0), less 2^-12 of its terms' magnitude for the
+// fp32 rounding of both sides. A segment whose lb key is at or above the query's threshold holds
+// no row the query keeps; a group is needed when a needed segment overlaps it. Bit G & 31 of
+// act[(item 4 + qt) kPruneWords + (G >> 5) - (g0 >> 5)] is set for a needed group G, and
+// act[any_at + item 4 + qt] says whether any is. Tiles without a query write zeros; padding
+// columns repeat the item's last query. Lists spanning more than kPruneWords words of groups are
+// not pruned. One workgroup of four waves (one per query tile) per item; the tile's query
+// fragments are held in registers across the centre tiles (NC = dp / 64 chunks at most).
+constexpr int kPruneWords = 64;
+template <int NC>
+__global__ __launch_bounds__(256) void prune_groups_kernel(const uint8_t *__restrict__ C8,
+                                                           const float4 *__restrict__ smeta,
+                                                           const int64_t *__restrict__ soff,
+                                                           const int32_t *__restrict__ sbase,
+                                                           const int64_t *__restrict__ loff, int dp, float scale,
+                                                           const uint8_t *__restrict__ R8, const float *__restrict__ Rn,
+                                                           const float *__restrict__ Rq, const int4 *__restrict__ items,
+                                                           const int32_t *__restrict__ bins,
+                                                           const int32_t *__restrict__ pair_q,
+                                                           const uint32_t *__restrict__ qthr,
+                                                           uint32_t *__restrict__ act, int64_t any_at) {
+    const int item = blockIdx.x;
+    if (item >= bins[3 * kXcds]) return; // (the item total)
+    const int4 it = items[item];
+    const int c = it.x, start = it.y, cnt = it.z;
+    const int lane = threadIdx.x & 63, half = lane >> 5, col = lane & 31;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int q0 = 32 * w;
+    const int r0 = (int)loff[c], r1 = (int)loff[c + 1];
+    const int g0 = r0 >> 5, g1 = (r1 - 1) >> 5, cg0 = g0 >> 5, nwd = (g1 >> 5) - cg0 + 1;
+    uint32_t *out = act + ((int64_t)item * 4 + w) * kPruneWords;
+    uint32_t *any = act + any_at + (int64_t)item * 4 + w;
+    if (nwd > kPruneWords) { // (the scan scans such lists whole)
+        if (lane == 0) *any = 1u;
+        return;
+    }
+    if (q0 >= cnt) {
+        for (int wi = lane; wi < nwd; wi += 64) out[wi] = 0u;
+        if (lane == 0) *any = 0u;
+        return;
+    }
+    const int nch = dp / kIvfKC;
+    const int64_t slot = start + min(q0 + col, cnt - 1); // the lane's query (pair slot)
+    const float qv = Rn[slot], qn = Rq[slot];
+    const uint32_t th = qthr[pair_q[slot]];
+    const float m2 = -2.f / (scale * scale);
+    V8 b[NC];
+    const uint8_t *qrow = R8 + slot * dp + 32 * half;
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch)
+        if (ch < nch) {
+            b[ch].u[0] = *(const uint4 *)(qrow + ch * kIvfKC);
+            b[ch].u[1] = *(const uint4 *)(qrow + ch * kIvfKC + 16);
+        }
+    const int sb0 = sbase[c], sb1 = sbase[c + 1]; // the list's segments (<= 64)
+    uint64_t smask = 0;                            // bit s - sb0: segment s needed by some query
+    const uint8_t *lane_c8 = C8 + col * 64 + 32 * half;
+    for (int ct = sb0 >> 5; ct <= (sb1 - 1) >> 5; ++ct) {
+        const uint8_t *cb = lane_c8 + (((int64_t)ct * nch) << 11);
+        V8 a[NC];
+#pragma unroll
+        for (int ch = 0; ch < NC; ++ch)
+            if (ch < nch) {
+                const uint4 *src = (const uint4 *)(cb + ((int64_t)ch << 11));
+                a[ch].u[0] = src[0];
+                a[ch].u[1] = src[1];
+            }
+        f32x16 acc = {};
+#pragma unroll
+        for (int ch = 0; ch < NC; ++ch)
+            if (ch < nch)
+                acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[ch].v, b[ch].v, acc, 0, 0, 0, 127, 0, 127);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int sg = 32 * ct + 8 * (r >> 2) + 4 * half + (r & 3);
+            const float4 sv = smeta[min(max(sg, sb0), sb1 - 1)];
+            const float U = acc[r] + qn * (sv.x + (sv.y + sv.z) * (1.f / 2048.f));
+            float lb = fmaf(m2, U, qv + sv.w);
+            lb -= (qv + sv.w - m2 * fabsf(U)) * (1.f / 4096.f);
+            const bool need = sg >= sb0 && sg < sb1 && (f32_key(fmaxf(0.f, lb)) & ~15u) < th;
+            const uint64_t bal = __ballot(need);
+            const int s0 = 32 * ct + 8 * (r >> 2) + (r & 3) - sb0; // half 0's segment, relative
+            if ((uint32_t)bal && s0 >= 0 && s0 < 64) smask |= 1ull << s0;
+            if ((uint32_t)(bal >> 32) && s0 + 4 >= 0 && s0 + 4 < 64) smask |= 1ull << (s0 + 4);
+        }
+    }
+    // needed segments -> needed groups (word wi: groups 32 (cg0 + wi) ..)
+    for (int wi = lane; wi < nwd; wi += 64) {
+        const int wg0 = 32 * (cg0 + wi);
+        uint32_t m32 = 0;
+        for (uint64_t m = smask; m; m &= m - 1) {
+            const int sg = sb0 + __builtin_ctzll(m);
+            const int ga = (int)(soff[sg] >> 5) - wg0, gb = (int)((soff[sg + 1] - 1) >> 5) - wg0;
+            if (gb < 0 || ga > 31) continue;
+            const int lo = max(ga, 0), hi = min(gb, 31);
+            m32 |= (hi == 31 ? 0xFFFFFFFFu : ((2u << hi) - 1u)) & ~((1u << lo) - 1u);
+        }
+        out[wi] = m32;
+    }
+    if (lane == 0) *any = smask != 0 ? 1u : 0u;
 }
 
 // The list scan in fp8 e4m3 on residuals to the list's centroid c: |q - x|^2 =
@@ -818,7 +942,7 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
     float scale, const uint8_t *__restrict__ R8, const float *__restrict__ Rn,
     const int4 *__restrict__ items, int32_t *__restrict__ bins, int nranges, const int32_t *__restrict__ pair_q,
     const int32_t *__restrict__ pair_p, int np, const int32_t *__restrict__ qoff, const uint32_t *__restrict__ qthr,
-    uint64_t *__restrict__ cand, int nmax) {
+    uint64_t *__restrict__ cand, int nmax, const uint32_t *__restrict__ act, int64_t any_at) {
     constexpr int IQ = kIvf8IQ;
     // IQ x (dp + 16) bytes of query residuals, then the list's row norms (nmax floats)
     extern __shared__ __align__(16) unsigned char dsm[];
@@ -826,21 +950,43 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
     __shared__ int32_t s_q[IQ];
     __shared__ uint32_t s_thr[IQ]; // per query: the smallest threshold any of its lists posted
     __shared__ float s_qr[IQ];
-    __shared__ int s_slot;
+    __shared__ int s_slot[2];
     const int QS = dp + 16; // LDS row stride of the residual block (16 B of pad spreads the banks)
     const int tid = threadIdx.x, lane = tid & 63, half = lane >> 5, col = lane & 31;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6); // (wave-uniform: tile addresses in SGPRs)
     const float m2 = -2.f / (scale * scale);
     const int xb = nranges == 1 ? 0 : blockIdx.x % kXcds;
-    for (;;) {
-        if (tid == 0) s_slot = take_item(bins, nranges, xb);
-        __syncthreads();
+    // Work items are taken one ahead: thread 0 takes the next item when an item starts and posts
+    // it in the other s_slot entry at the item's end, so the queue's atomic overlaps the item.
+    // (one range: the queue's item count and first item are fixed for the launch; the atomic's
+    // result is first needed at the item's end)
+    const int cnt0 = bins[kXcds], first0 = bins[0];
+    if (tid == 0) s_slot[0] = take_item(bins, nranges, xb);
+    for (int par = 0;; par ^= 1) {
+        __syncthreads(); // (s_slot[par] posted; the previous item's LDS is no longer read)
         // (read into a scalar register: the item and every list bound and tile address derived
         // from it are wave-uniform, computed on the scalar unit)
-        const int slot = __builtin_amdgcn_readfirstlane(s_slot);
+        const int slot = __builtin_amdgcn_readfirstlane(s_slot[par]);
         if (slot < 0) break;
+        int next = -1;
+        if (tid == 0) next = nranges == 1 ? atomicAdd(&bins[2 * kXcds], 1) : take_item(bins, nranges, xb);
         const int4 it = items[slot];
         const int c = it.x, start = it.y, cnt = it.z;
+        // rows and groups in 32 bits (the database has fewer than 2^31 rows)
+        const int r0 = (int)loff[c], r1 = (int)loff[c + 1];
+        if (act) {
+            // an item no query of which needs any group (prune_groups_kernel) keeps no candidate
+            const uint4 an = *(const uint4 *)(act + any_at + (int64_t)slot * 4);
+            if ((an.x | an.y | an.z | an.w) == 0u) {
+                for (int e = tid; e < cnt * kIvf8Cap; e += 512) {
+                    const int t = e / kIvf8Cap;
+                    cand[cand_slot(qoff, pair_q[start + t], np, pair_p[start + t]) * kIvf8Cap + e % kIvf8Cap] =
+                        kKey64None;
+                }
+                if (tid == 0) s_slot[par ^ 1] = nranges == 1 ? (next < cnt0 ? first0 + next : -1) : next;
+                continue;
+            }
+        }
         const bool wide = cnt > 64;
         const int qg = wide ? w >> 2 : 0, wr = wide ? w & 3 : w; // query group, row group
         const int nrg = wide ? 4 : 8, nlist = 2 * nrg, tstride = wide ? IQ : 64;
@@ -851,53 +997,34 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
             s_q[tid] = pair_q[start + min(tid, cnt - 1)];
             s_thr[tid] = qthr ? qthr[pair_q[start + min(tid, cnt - 1)]] : kKey32None;
         }
-        { // the item's query residuals (pair slots start ..) into LDS; padding rows repeat the last
-            const int upr = dp / 16;
-            for (int u = tid; u < nrows * upr; u += 512) {
-                const int t = u / upr, e = u - t * upr;
-                *(uint4 *)(dsm + t * QS + 16 * e) =
-                    *(const uint4 *)(R8 + (int64_t)(start + min(t, cnt - 1)) * dp + 16 * e);
+        { // the item's query residuals (pair slots start ..) into LDS, four loads in flight per
+          // thread; padding rows repeat the last
+            const int upr = dp / 16, tot = nrows * upr;
+            for (int u0 = tid; u0 < tot; u0 += 4 * 512) {
+                uint4 v[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int u = min(u0 + 512 * k, tot - 1), t = u / upr, e = u - t * upr;
+                    v[k] = *(const uint4 *)(R8 + (int64_t)(start + min(t, cnt - 1)) * dp + 16 * e);
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int u = u0 + 512 * k, t = u / upr, e = u - t * upr;
+                    if (u < tot) *(uint4 *)(dsm + t * QS + 16 * e) = v[k];
+                }
             }
             if (tid < nrows) s_qr[tid] = Rn[start + min(tid, cnt - 1)];
         }
-        const int64_t r0 = loff[c], r1 = loff[c + 1];
         // the list's 32-row groups g0 .. g1; a row block is 2 nrg of them, two per wave
-        const int64_t g0 = r0 >> 5, g1 = (r1 - 1) >> 5;
-        const int ngr = (int)(g1 - g0 + 1);
-        const bool norm_lds = 32 * ngr <= nmax; // (the norms are padded to whole groups)
-        if (norm_lds)
-            for (int e = tid; e < 32 * ngr; e += 512) s_norm[e] = rn[32 * g0 + e];
-        __syncthreads();
+        const int g0 = r0 >> 5, g1 = (r1 - 1) >> 5;
+        const int ngr = g1 - g0 + 1;
         const int nch = dp / kIvfKC;
         const int GB = 2 * nrg; // groups per row block
-        const int nrb = (ngr + GB - 1) / GB;
-        const float qr0 = s_qr[q0 + col], qr1 = s_qr[q0 + 32 + col];
-        // row fragments of step g = (row block g / nch, chunk g % nch), loaded two steps ahead into
-        // a ring of three register sets that rotate by name (three steps per loop trip): a copy
-        // between sets would wait for the loads in flight. Groups past g1 are clamped to it (their
-        // keys are skipped).
-        V8 fa[2], fb[2], fc[2];
+        // the list is scanned in segments of whole row blocks whose row norms fit in LDS (nmax is
+        // a multiple of 512 rows, 16 groups: every segment but the last is whole row blocks); a
+        // list of at most nmax rows is one segment
+        const int segg = nmax / 32;
         const uint8_t *lane_l8 = L8 + col * 64 + 32 * half;
-        const int nsteps = nrb * nch;
-        // the loads walk the steps in order, each trip's last ones clamped to the final step:
-        // a cursor (row block lrb, chunk lch) replaces a division per load
-        int lrb = 0, lch = 0;
-        const int last_rb = nrb - 1, last_ch = nch - 1;
-        auto load_a = [&](V8 (&a)[2]) {
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const int64_t grp = min(g0 + (int64_t)GB * lrb + 2 * wr + i, g1);
-                const uint4 *src = (const uint4 *)(lane_l8 + ((grp * nch + lch) << 11));
-                a[i].u[0] = src[0];
-                a[i].u[1] = src[1];
-            }
-            if (lrb < last_rb || lch < last_ch) {
-                if (++lch == nch) {
-                    lch = 0;
-                    ++lrb;
-                }
-            }
-        };
         uint32_t Lk[2][KL], Li[2][KL];
 #pragma unroll
         for (int j = 0; j < 2; ++j)
@@ -906,65 +1033,127 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
                 Lk[j][e] = kKey32None;
                 Li[j][e] = 0u;
             }
-        f32x16 acc[2][2];
-        int ch = 0, rb = 0;
-        // one chunk: its products, and after a row block's last chunk its keys (row norms from
-        // xs_blk(rb): LDS or global)
-        auto chunk = [&](const V8 (&cur)[2], const float *xs_blk, bool xs_lds) {
-            if (ch == 0)
+        bool fresh = true; // (the item's lists are empty until its first row block's keys)
+        for (int sg = 0; sg < ngr; sg += segg) {
+            const int sng = min(segg, ngr - sg);
+            const int sg0 = g0 + sg, sg1 = sg0 + sng - 1; // the segment's groups
+            if (sg > 0) __syncthreads(); // (every wave has read the previous segment's norms)
+            for (int e0 = tid; e0 < 32 * sng; e0 += 4 * 512) {
+                float v[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) v[k] = rn[32 * sg0 + min(e0 + 512 * k, 32 * sng - 1)];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (e0 + 512 * k < 32 * sng) s_norm[e0 + 512 * k] = v[k];
+            }
+            __syncthreads(); // (and, for the first segment, the item's query block)
+            const float qr0 = s_qr[q0 + col], qr1 = s_qr[q0 + 32 + col];
+            const int nrb = (sng + GB - 1) / GB; // (<= 32: segg <= 256 groups, GB >= 8)
+            // the wave's row blocks to scan: all, or with group pruning (act) those with a group that
+            // some query of the wave's query group may need
+            uint32_t rbmask = nrb >= 32 ? 0xFFFFFFFFu : ((1u << nrb) - 1u);
+            if (act && (g1 >> 5) - (g0 >> 5) < kPruneWords) {
+                // groups that some query of the wave's two query tiles may need (prune_groups_kernel)
+                const uint32_t *aq = act + ((int64_t)slot * 4 + 2 * qg) * kPruneWords;
+                const int cgl = g0 >> 5;
+                rbmask = 0;
+                for (int rb2 = 0; rb2 < nrb; ++rb2)
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) {
+                        const int G = sg0 + GB * rb2 + 2 * wr + i;
+                        if (G <= sg1) {
+                            const int wi = (G >> 5) - cgl;
+                            if (((aq[wi] | aq[kPruneWords + wi]) >> (G & 31)) & 1u) rbmask |= 1u << rb2;
+                        }
+                    }
+            }
+            if (rbmask == 0) continue; // (wave-uniform; no barrier follows inside the segment)
+            // the active row block after b, or -1
+            auto next_rb = [&](int b) { return b >= 31 ? -1 : __ffs(rbmask & (0xFFFFFFFEu << b)) - 1; };
+            // row fragments of step g (the g / nch-th active row block, chunk g % nch), loaded PD
+            // steps ahead into a ring of PD + 1 register sets that rotate by name (PD + 1 steps per
+            // loop trip): a copy between sets would wait for the loads in flight. Groups past the
+            // segment's last are clamped to it (they lie past g1: their keys are skipped).
+            V8 fa[2], fb[2], fc[2];
+            const int nsteps = __popc(rbmask) * nch;
+            // the loads walk the steps in order, each trip's last ones clamped to the final step:
+            // a cursor (row block lrb, chunk lch) replaces a division per load
+            int lrb = __ffs(rbmask) - 1, lch = 0;
+            const int last_rb = 31 - __clz(rbmask), last_ch = nch - 1;
+            auto load_a = [&](V8 (&a)[2]) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const int64_t grp = min(sg0 + GB * lrb + 2 * wr + i, sg1);
+                    const uint4 *src = (const uint4 *)(lane_l8 + ((grp * nch + lch) << 11));
+                    a[i].u[0] = src[0];
+                    a[i].u[1] = src[1];
+                }
+                if (lrb < last_rb || lch < last_ch) {
+                    if (++lch == nch) {
+                        lch = 0;
+                        lrb = next_rb(lrb);
+                    }
+                }
+            };
+            f32x16 acc[2][2] = {}; // (defined before the loop: no undefined values live across it)
+            int ch = 0, rb = __ffs(rbmask) - 1;
+            // one chunk: its products, and after a row block's last chunk its keys
+            auto chunk = [&](const V8 (&cur)[2]) {
+                if (ch == 0)
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+#pragma unroll
+                        for (int j = 0; j < 2; ++j) acc[i][j] = f32x16{};
+                V8 b[2];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const uint4 *sb = (const uint4 *)(dsm + (q0 + 32 * j + col) * QS + ch * kIvfKC + 32 * half);
+                    b[j].u[0] = sb[0];
+                    b[j].u[1] = sb[1];
+                }
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
 #pragma unroll
-                    for (int j = 0; j < 2; ++j) acc[i][j] = f32x16{};
-            V8 b[2];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const uint4 *sb = (const uint4 *)(dsm + (q0 + 32 * j + col) * QS + ch * kIvfKC + 32 * half);
-                b[j].u[0] = sb[0];
-                b[j].u[1] = sb[1];
-            }
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    if (j == 0 || two)
-                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cur[i].v, b[j].v, acc[i][j], 0, 0,
-                                                                               0, 127, 0, 127);
-            if (++ch == nch) {
-                const int64_t grp = g0 + (int64_t)GB * rb + 2 * wr; // the wave's first group of the block
-                fp8_rowblock_keys<KL>(acc, m2, xs_blk, xs_lds, 32 * grp, r0, r1, rb == 0, qr0, qr1, Lk, Li, s_thr,
-                                      q0 + col, half, two);
-                ch = 0;
-                ++rb;
-            }
-        };
-        {
+                    for (int j = 0; j < 2; ++j)
+                        if (j == 0 || two)
+                            acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cur[i].v, b[j].v, acc[i][j], 0,
+                                                                                   0, 0, 127, 0, 127);
+                if (++ch == nch) {
+                    const int grp = sg0 + GB * rb + 2 * wr; // the wave's first group of the block
+                    // (a row block past the segment, from a trip's last steps, is skipped whole)
+                    fp8_rowblock_keys<KL>(acc, m2, s_norm + 32 * (grp - sg0), 32 * grp, r0, min(r1, 32 * (sg1 + 1)),
+                                          fresh, qr0, qr1, Lk, Li, s_thr, q0 + col, half, two);
+                    fresh = false;
+                    ch = 0;
+                    const int nx = next_rb(rb);
+                    rb = nx < 0 ? nrb : nx; // (past the last active block: nrb, a block past the segment)
+                }
+            };
             // one step: the load PD steps ahead, then the chunk. Whole trips: up to PD steps past
-            // the end run on clamped rows of the last chunk and end in a row block past r1, whose
-            // keys are skipped. (With three steps ahead in four sets, the compiler's waits, which
-            // across the trip's branches keep one step fewer in flight than the ring allows, still
-            // leave two.)
-            auto step = [&](int g, const V8 (&cur)[2], V8 (&ahead)[2]) {
+            // the end run on clamped rows of the last chunk and end in a row block past the
+            // segment, whose keys are skipped. (With three steps ahead in four sets, the compiler's
+            // waits, which across the trip's branches keep one step fewer in flight than the ring
+            // allows, still leave two.)
+            auto step = [&](const V8 (&cur)[2], V8 (&ahead)[2]) {
                 load_a(ahead); // step min(g + PD, nsteps - 1) (unconditional, so the waits stay exact)
-                const int64_t grp = g0 + (int64_t)GB * rb + 2 * wr;
-                chunk(cur, norm_lds ? s_norm + 32 * (grp - g0) : rn + 32 * grp, norm_lds);
+                chunk(cur);
             };
             load_a(fa);
             load_a(fb);
             if constexpr (PD == 2) {
                 for (int g = 0; g < nsteps; g += 3) {
-                    step(g, fa, fc);
-                    step(g + 1, fb, fa);
-                    step(g + 2, fc, fb);
+                    step(fa, fc);
+                    step(fb, fa);
+                    step(fc, fb);
                 }
             } else {
                 V8 fd[2];
                 load_a(fc);
                 for (int g = 0; g < nsteps; g += 4) {
-                    step(g, fa, fd);
-                    step(g + 1, fb, fa);
-                    step(g + 2, fc, fb);
-                    step(g + 3, fd, fc);
+                    step(fa, fd);
+                    step(fb, fa);
+                    step(fc, fb);
+                    step(fd, fc);
                 }
             }
         }
@@ -1007,6 +1196,7 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
                 out[lane] = v[0]; // the 64 best
             }
         }
+        if (tid == 0) s_slot[par ^ 1] = nranges == 1 ? (next < cnt0 ? first0 + next : -1) : next;
         // (the next item's LDS writes follow the loop's first barrier, which every wave reaches
         // only after this item's sorts)
     }
@@ -1370,6 +1560,230 @@ __global__ void gather_rows_f32_kernel(const float *__restrict__ X, int64_t d, c
     if (t < n * d) out[t] = X[(int64_t)ids[t / d] * d + t % d];
 }
 
+// ---------------------------------------------------------------- row order inside a list
+// A list's rows are ordered by a secondary partition of the list, ids ascending inside a part:
+// kSubIters Lloyd iterations over m = min(kSubMax, n_list / kSubRows) part centroids seeded with
+// evenly spaced rows of the list (lists of fewer than 2 kSubRows rows, and rows of more than
+// kSubDims dims, keep id order). Candidates are named by id and reranked in fp32, so the order
+// changes no result beyond ties of the approximation; it keeps the rows of a 32-row group close
+// together, which lets the group's fp8 centre and radius bound its distances (group pruning in the
+// fp8 scan). All sums run in a fixed order: the order is a function of the rows and lists alone.
+constexpr int kSubRows = 384;
+constexpr int kSubMax = 64;
+constexpr int kSubIters = 8;
+constexpr int kSubDims = 2048;
+
+// the list of row `row` (rows in list order): the last list with loff[list] <= row
+__device__ __forceinline__ int list_of_row(const int64_t *__restrict__ loff, int nl, int64_t row) {
+    int lo = 0, hi = nl - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (loff[mid] <= row) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Part centroid p of list l starts as the list's row at (2 j + 1) n_l / (2 m) (j = p - pbase[l]).
+__global__ __launch_bounds__(256) void sub_init_kernel(const float *__restrict__ X, int d,
+                                                       const int64_t *__restrict__ loff,
+                                                       const int32_t *__restrict__ plist,
+                                                       const int32_t *__restrict__ pbase, float *__restrict__ P) {
+    const int part = blockIdx.x, l = plist[part];
+    const int64_t j = part - pbase[l], m = pbase[l + 1] - pbase[l];
+    const int64_t nlr = loff[l + 1] - loff[l];
+    const float *x = X + (loff[l] + (2 * j + 1) * nlr / (2 * m)) * d;
+    for (int e = threadIdx.x; e < d; e += 256) P[(int64_t)part * d + e] = x[e];
+}
+
+// One wave per row (list order): the nearest part centroid of its list (lower index on ties);
+// part 0 for lists without parts.
+__global__ __launch_bounds__(256) void sub_assign_kernel(const float *__restrict__ X, int64_t n, int d,
+                                                         const int64_t *__restrict__ loff, int nl,
+                                                         const int32_t *__restrict__ pbase,
+                                                         const float *__restrict__ P, int32_t *__restrict__ sub) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const int l = list_of_row(loff, nl, row);
+    const int b = pbase[l], m = pbase[l + 1] - b;
+    int bj = 0;
+    if (m > 1) {
+        const float *x = X + row * d;
+        float best = INFINITY;
+        for (int j = 0; j < m; ++j) {
+            const float *p = P + (int64_t)(b + j) * d;
+            float s = 0.f;
+            for (int e = lane; e < d; e += 64) {
+                const float t = x[e] - p[e];
+                s = fmaf(t, t, s);
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            s = __shfl(s, 0, 64); // (lane 0's sum: every lane compares the same value)
+            if (s < best) {
+                best = s;
+                bj = j;
+            }
+        }
+    }
+    if (lane == 0) sub[row] = bj;
+}
+
+// One workgroup per part: the mean of the list's rows assigned to it, rows taken in ascending
+// order (a part that holds no row keeps its centroid). d <= kSubDims.
+__global__ __launch_bounds__(256) void sub_update_kernel(const float *__restrict__ X, int d,
+                                                         const int64_t *__restrict__ loff,
+                                                         const int32_t *__restrict__ plist,
+                                                         const int32_t *__restrict__ pbase,
+                                                         const int32_t *__restrict__ sub, float *__restrict__ P) {
+    __shared__ int32_t s_rows[256];
+    __shared__ int s_wc[4];
+    const int part = blockIdx.x, l = plist[part];
+    const int j = part - pbase[l];
+    const int64_t r0 = loff[l], r1 = loff[l + 1];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    constexpr int T = kSubDims / 256;
+    float acc[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) acc[t] = 0.f;
+    int64_t cnt = 0;
+    for (int64_t c0 = r0; c0 < r1; c0 += 256) {
+        const int64_t r = c0 + tid;
+        const bool hit = r < r1 && sub[r] == j;
+        const uint64_t bal = __ballot(hit);
+        if (lane == 0) s_wc[w] = __popcll(bal);
+        __syncthreads();
+        int off = 0;
+        for (int u = 0; u < w; ++u) off += s_wc[u];
+        const int nn = s_wc[0] + s_wc[1] + s_wc[2] + s_wc[3];
+        if (hit) s_rows[off + __popcll(bal & ((1ull << lane) - 1))] = (int32_t)(r - r0);
+        __syncthreads();
+        for (int i = 0; i < nn; ++i) {
+            const float *x = X + (r0 + s_rows[i]) * d;
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int e = tid + 256 * t;
+                if (e < d) acc[t] += x[e];
+            }
+        }
+        cnt += nn;
+        __syncthreads();
+    }
+    if (cnt > 0)
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int e = tid + 256 * t;
+            if (e < d) P[(int64_t)part * d + e] = acc[t] / (float)cnt;
+        }
+}
+
+// Reorders the rows of each list (ids: list-order position -> id, X in that order) by the
+// secondary partition; rewrites ids (host and device) and regathers X from the original rows dX.
+static void sub_partition_order(const float *dX, int64_t n, int64_t d, const std::vector<int64_t> &off,
+                                std::vector<int32_t> &ids, int32_t *dids, const int64_t *dloff, float *Xo,
+                                std::vector<int64_t> &soff, std::vector<int32_t> &sbase, hipStream_t st);
+
+// ---------------------------------------------------------------- segment bounds of the fp8 rows
+// One wave per segment sg (rows soff[sg] .. soff[sg + 1] - 1 of the fp8 rows L8, fp8_row_chunk
+// layout, dp bytes per row): the centre, the segment's rows' fp8 values averaged and rounded to
+// fp8 as the rows are, goes to centre row sg of C8 (the same layout); sm[sg] = {R, X, |centre|,
+// mxr} with R >= max |x8 - centre| and X >= max |x8| over the segment's rows (their fp32 sums
+// rounded up by 2^-10) and mxr the least rn over them. With these, for every row x of the segment
+// and any query residual q8, q8.x8 <= q8.centre + |q8| R, which bounds the scan's approximation
+// from below (prune_groups_kernel).
+template <int NS>
+__global__ __launch_bounds__(256) void seg_meta_kernel(const uint8_t *__restrict__ L8, const float *__restrict__ rn,
+                                                       const int64_t *__restrict__ soff, int64_t nseg, int dp,
+                                                       uint8_t *__restrict__ C8, float4 *__restrict__ sm) {
+    const int64_t sg = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (sg >= nseg) return;
+    const int64_t s0 = soff[sg], s1 = soff[sg + 1];
+    const int nch = dp / kIvfKC;
+    auto at = [&](int64_t row, int b) { return fp8_row_chunk(row, b / kIvfKC, nch) + b % kIvfKC; };
+    auto dec = [&](int w, float (&v)[4]) {
+        v[0] = __builtin_amdgcn_cvt_f32_fp8(w, 0);
+        v[1] = __builtin_amdgcn_cvt_f32_fp8(w, 1);
+        v[2] = __builtin_amdgcn_cvt_f32_fp8(w, 2);
+        v[3] = __builtin_amdgcn_cvt_f32_fp8(w, 3);
+    };
+    float sum[NS][4];
+#pragma unroll
+    for (int t = 0; t < NS; ++t)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sum[t][u] = 0.f;
+    float mxr = INFINITY;
+#pragma unroll 4
+    for (int64_t row = s0; row < s1; ++row) {
+        mxr = fminf(mxr, rn[row]);
+#pragma unroll
+        for (int t = 0; t < NS; ++t) {
+            const int b = 4 * lane + 256 * t;
+            if (b < dp) {
+                float v[4];
+                dec(*(const int *)(L8 + at(row, b)), v);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) sum[t][u] += v[u];
+            }
+        }
+    }
+    const float inv = 1.f / (float)max<int64_t>(s1 - s0, 1);
+    float mq[NS][4];
+    float mn = 0.f;
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        const int b = 4 * lane + 256 * t;
+        if (b < dp) {
+            float v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = fminf(fmaxf(sum[t][u] * inv, -448.f), 448.f);
+            int wq = 0;
+            wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], wq, false);
+            wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], wq, true);
+            *(int *)(C8 + at(sg, b)) = wq;
+            dec(wq, mq[t]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) mn = fmaf(mq[t][u], mq[t][u], mn);
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) mq[t][u] = 0.f;
+        }
+    }
+    float r2max = 0.f, x2max = 0.f;
+#pragma unroll 4
+    for (int64_t row = s0; row < s1; ++row) {
+        float e2 = 0.f, x2 = 0.f;
+#pragma unroll
+        for (int t = 0; t < NS; ++t) {
+            const int b = 4 * lane + 256 * t;
+            if (b < dp) {
+                float v[4];
+                dec(*(const int *)(L8 + at(row, b)), v);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const float e = v[u] - mq[t][u];
+                    e2 = fmaf(e, e, e2);
+                    x2 = fmaf(v[u], v[u], x2);
+                }
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            e2 += __shfl_xor(e2, off, 64);
+            x2 += __shfl_xor(x2, off, 64);
+        }
+        r2max = fmaxf(r2max, __shfl(e2, 0, 64));
+        x2max = fmaxf(x2max, __shfl(x2, 0, 64));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mn += __shfl_xor(mn, off, 64);
+    mn = __shfl(mn, 0, 64);
+    if (lane == 0) {
+        constexpr float up = 1.f + 1.f / 1024.f;
+        sm[sg] = make_float4(sqrtf(r2max) * up, sqrtf(x2max) * up, sqrtf(mn) * up, mxr);
+    }
+}
+
 // Builds the device index from device rows X (n x d, contiguous), centroids C (nc x d) and labels.
 static void ivfl2_set_device(pn_ivfl2 *h, const float *dX, int64_t n, int64_t d, const float *dC, int64_t nc,
                              const int32_t *hlabels, hipStream_t st) {
@@ -1415,6 +1829,15 @@ static void ivfl2_set_device(pn_ivfl2 *h, const float *dX, int64_t n, int64_t d,
     // see the scan kernel)
     float *Xo = (float *)h->X.ensure((size_t)n * d * 4);
     hipLaunchKernelGGL(gather_rows_f32_kernel, dim3((unsigned)ceil_div(n * d, 256)), dim3(256), 0, st, dX, d, dl, n, Xo);
+    std::vector<int64_t> soff;
+    std::vector<int32_t> sbase;
+    sub_partition_order(dX, n, d, off, ids, dl, dof, Xo, soff, sbase, st);
+    const int64_t nseg = (int64_t)soff.size() - 1;
+    h->nseg = nseg;
+    PN_HIP(hipMemcpyAsync(h->soff.ensure((size_t)(nseg + 1) * 8), soff.data(), (size_t)(nseg + 1) * 8,
+                          hipMemcpyHostToDevice, st));
+    PN_HIP(hipMemcpyAsync(h->sbase.ensure((size_t)(nl + 1) * 4), sbase.data(), (size_t)(nl + 1) * 4,
+                          hipMemcpyHostToDevice, st));
     l2_prepare(h->lrows, Xo, n, d, d, st, kIvfKC); // whole K chunks: the scan has no dim tail
     // coarse index over the kept centroids
     bool scan_ok = false;
@@ -1449,13 +1872,152 @@ static void ivfl2_set_device(pn_ivfl2 *h, const float *dX, int64_t n, int64_t d,
         PN_HIP(hipMemsetAsync(L8, 0, (size_t)npad * dp, st)); // (the last group's rows interleave by chunk)
         PN_HIP(hipMemsetAsync(rn + n, 0, (size_t)(npad - n) * 4, st));
         hipLaunchKernelGGL(residual_kernel<1>, g, dim3(256), 0, st, Xo, n, (int)d, dp, dof, nl, lc, h->sx8, dm, L8, rn);
+        // segment centres and radii (centre rows padded to whole 32-row groups of centres)
+        const int64_t nseg = h->nseg, ncpad = ceil_div(nseg, (int64_t)32) * 32;
+        uint8_t *C8 = (uint8_t *)h->c8.ensure((size_t)ncpad * dp);
+        float4 *sm = (float4 *)h->smeta.ensure((size_t)nseg * 16);
+        PN_HIP(hipMemsetAsync(C8, 0, (size_t)ncpad * dp, st));
+        const int64_t *so = h->soff.as<int64_t>();
+        const dim3 gs((unsigned)ceil_div(nseg, (int64_t)4));
+        if (dp <= 256)
+            hipLaunchKernelGGL(seg_meta_kernel<1>, gs, dim3(256), 0, st, L8, rn, so, nseg, dp, C8, sm);
+        else if (dp <= 512)
+            hipLaunchKernelGGL(seg_meta_kernel<2>, gs, dim3(256), 0, st, L8, rn, so, nseg, dp, C8, sm);
+        else if (dp <= 768)
+            hipLaunchKernelGGL(seg_meta_kernel<3>, gs, dim3(256), 0, st, L8, rn, so, nseg, dp, C8, sm);
+        else
+            hipLaunchKernelGGL(seg_meta_kernel<5>, gs, dim3(256), 0, st, L8, rn, so, nseg, dp, C8, sm);
         PN_HIP(hipGetLastError());
     }
     PN_HIP(hipStreamSynchronize(st)); // host vectors and temporaries are released on return
 }
 
-static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t nq, const int32_t *pr, int np,
+static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t nq, int32_t *pr, int np,
                                 bool sparse, int k, int prec, int32_t *oi, float *od, int32_t *oc, hipStream_t st);
+
+// The first probe of each query becomes its nearest probed list of at least k2 rows (swapped
+// with slot 0): the bounded second pass of the fp8 scan takes each query's bound from the k2-th
+// candidate of its first probe, which a shorter list cannot give. The probe set is unchanged and
+// the rerank takes a query's candidates by key from all its probes, so the order of a query's
+// probes changes no result.
+__global__ void first_probe_kernel(int32_t *__restrict__ pr, int64_t nq, int np, const int64_t *__restrict__ loff,
+                                   int k2) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    int32_t *p = pr + q * np;
+    for (int j = 0; j < np; ++j) {
+        const int c = p[j];
+        if (c >= 0 && loff[c + 1] - loff[c] >= k2) {
+            if (j > 0) {
+                p[j] = p[0];
+                p[0] = c;
+            }
+            return;
+        }
+    }
+}
+
+static void sub_partition_order(const float *dX, int64_t n, int64_t d, const std::vector<int64_t> &off,
+                                std::vector<int32_t> &ids, int32_t *dids, const int64_t *dloff, float *Xo,
+                                std::vector<int64_t> &soff, std::vector<int32_t> &sbase, hipStream_t st) {
+    const int32_t nl = (int32_t)off.size() - 1;
+    // segments: the non-empty parts of each partitioned list in chain order, else the whole list
+    soff.assign(1, 0);
+    sbase.assign(nl + 1, 0);
+    auto whole = [&]() {
+        soff.assign(1, 0);
+        for (int32_t l = 0; l < nl; ++l) {
+            soff.push_back(off[l + 1]);
+            sbase[l + 1] = l + 1;
+        }
+    };
+    if (d > kSubDims || nl <= 0) {
+        whole();
+        return;
+    }
+    std::vector<int32_t> pbase(nl + 1, 0), plist;
+    for (int32_t l = 0; l < nl; ++l) {
+        const int64_t nr = off[l + 1] - off[l];
+        const int32_t m = nr >= 2 * kSubRows ? (int32_t)std::min<int64_t>(kSubMax, nr / kSubRows) : 0;
+        pbase[l + 1] = pbase[l] + m;
+        for (int32_t j = 0; j < m; ++j) plist.push_back(l);
+    }
+    const int32_t np_ = pbase[nl];
+    if (np_ == 0) {
+        whole();
+        return;
+    }
+    DevBuf pb, pl, P, sb;
+    int32_t *dpb = (int32_t *)pb.ensure((size_t)(nl + 1) * 4);
+    int32_t *dpl = (int32_t *)pl.ensure((size_t)np_ * 4);
+    float *dP = (float *)P.ensure((size_t)np_ * d * 4);
+    int32_t *dsub = (int32_t *)sb.ensure((size_t)n * 4);
+    PN_HIP(hipMemcpyAsync(dpb, pbase.data(), (size_t)(nl + 1) * 4, hipMemcpyHostToDevice, st));
+    PN_HIP(hipMemcpyAsync(dpl, plist.data(), (size_t)np_ * 4, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(sub_init_kernel, dim3((unsigned)np_), dim3(256), 0, st, Xo, (int)d, dloff, dpl, dpb, dP);
+    const dim3 ga((unsigned)ceil_div(n, (int64_t)4));
+    for (int it = 0; it < kSubIters; ++it) {
+        hipLaunchKernelGGL(sub_assign_kernel, ga, dim3(256), 0, st, Xo, n, (int)d, dloff, nl, dpb, dP, dsub);
+        hipLaunchKernelGGL(sub_update_kernel, dim3((unsigned)np_), dim3(256), 0, st, Xo, (int)d, dloff, dpl, dpb, dsub,
+                           dP);
+    }
+    hipLaunchKernelGGL(sub_assign_kernel, ga, dim3(256), 0, st, Xo, n, (int)d, dloff, nl, dpb, dP, dsub);
+    PN_HIP(hipGetLastError());
+    std::vector<int32_t> sub(n);
+    std::vector<float> hp((size_t)np_ * d);
+    PN_HIP(hipMemcpyAsync(sub.data(), dsub, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    PN_HIP(hipMemcpyAsync(hp.data(), dP, hp.size() * 4, hipMemcpyDeviceToHost, st));
+    PN_HIP(hipStreamSynchronize(st));
+    // Parts in chain order: from part 0, each next part is the nearest unvisited one (lower index
+    // on ties), so parts of one clump of the list sit next to each other and the group that
+    // straddles two of them stays compact. Then a stable counting sort of the list's positions by
+    // the part's place in the chain.
+    std::vector<int32_t> nids(ids);
+    for (int32_t l = 0; l < nl; ++l) {
+        const int32_t m = pbase[l + 1] - pbase[l];
+        if (m < 2) {
+            soff.push_back(off[l + 1]);
+            sbase[l + 1] = (int32_t)soff.size() - 1;
+            continue;
+        }
+        const float *pc = hp.data() + (size_t)pbase[l] * d;
+        std::vector<int32_t> place(m, -1);
+        int32_t cur_p = 0;
+        place[0] = 0;
+        for (int32_t t = 1; t < m; ++t) {
+            int32_t best = -1;
+            double bd = 0.0;
+            for (int32_t j = 0; j < m; ++j) {
+                if (place[j] >= 0) continue;
+                double dd = 0.0;
+                for (int64_t e = 0; e < d; ++e) {
+                    const double v = (double)pc[(size_t)cur_p * d + e] - (double)pc[(size_t)j * d + e];
+                    dd += v * v;
+                }
+                if (best < 0 || dd < bd) {
+                    best = j;
+                    bd = dd;
+                }
+            }
+            place[best] = t;
+            cur_p = best;
+        }
+        std::vector<int64_t> cur(m + 1, 0);
+        for (int64_t r = off[l]; r < off[l + 1]; ++r) ++cur[place[sub[r]] + 1];
+        for (int32_t j = 0; j < m; ++j) cur[j + 1] += cur[j];
+        for (int32_t j = 0; j < m; ++j)
+            if (cur[j + 1] > cur[j]) soff.push_back(off[l] + cur[j + 1]);
+        sbase[l + 1] = (int32_t)soff.size() - 1;
+        for (int64_t r = off[l]; r < off[l + 1]; ++r) nids[off[l] + cur[place[sub[r]]]++] = ids[r];
+    }
+    ids.swap(nids);
+    PN_HIP(hipMemcpyAsync(dids, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(gather_rows_f32_kernel, dim3((unsigned)ceil_div(n * d, 256)), dim3(256), 0, st, dX, d, dids, n,
+                       Xo);
+    PN_HIP(hipGetLastError());
+    PN_HIP(hipStreamSynchronize(st)); // (host vectors and temporaries are released on return)
+}
+
 
 // ---------------------------------------------------------------- k > 64: every probed row
 // For k above the candidate pipeline's 64 (forest.py:84-105 has no limit), every row of every
@@ -1618,7 +2180,9 @@ static void ivfl2_search_enqueue(pn_ivfl2 *h, const float *dq, int64_t qstride_i
 
 // The list scan and rerank for given probes pr[nq x np] (list indices; with `sparse`, -1 marks an
 // empty slot). Q has row stride re.
-static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t nq, const int32_t *pr, int np,
+// pr: the handle's own probe buffer (list indices), reordered per query by first_probe_kernel in
+// the bounded two-pass fp8 scan.
+static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t nq, int32_t *pr, int np,
                                 bool sparse, int k, int prec, int32_t *oi, float *od, int32_t *oc, hipStream_t st) {
     if (k > kIvfK2) { // (host entry point only: the device ones take k <= 64)
         ivfl2_exhaustive(h, Q, re, nq, pr, np, k, oi, od, oc, st);
@@ -1643,14 +2207,18 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     int32_t *pp = (int32_t *)h->pair_p.ensure((size_t)total * 4);
     int32_t *pc = (int32_t *)h->pair_c.ensure((size_t)total * 4);
     int32_t *psl = (int32_t *)h->pslot.ensure((size_t)total * 4);
+    // queries per item: the fp8 scan takes up to 128 (two query groups share each row fetched),
+    // fewer in its bounded second pass (ivfl2.fp8_iq2: most items there need no row at all, and
+    // the few that do spread over more workgroups); the bf16 scans 64
+    const int iq_b = prec == 8 ? (int)std::min<int64_t>(kIvf8IQ, std::max<int64_t>(32, tune_get("ivfl2.fp8_iq2")))
+                               : kIvfIQ;
     // each cluster has at most one partly filled item
-    const int64_t max_items = ceil_div(total, kIvfIQ) + h->nl;
+    const int64_t max_items = ceil_div(total, (int64_t)std::min(kIvfIQ, iq_b)) + h->nl;
     int4 *items = (int4 *)h->items.ensure((size_t)max_items * 16);
     int32_t *bins = (int32_t *)h->ntot.ensure(kBinsInts * 4);
     int32_t *itoff = (int32_t *)h->itoff.ensure((size_t)h->nl * 4);
     int64_t *woff = (int64_t *)h->woff.ensure((size_t)h->nl * 8);
     const int nranges = tune_get("ivfl2.xcd_ranges") ? kXcds : 1;
-    const int iq = prec == 8 ? kIvf8IQ : kIvfIQ; // queries per item
     // candidate slots per (query, probe): the bf16 scan merges its lists into the KP = 32 best,
     // the fp8 scan (more lists, coarser keys) into the 64 best
     const int kl = k <= 16 ? 8 : 16;
@@ -1672,11 +2240,22 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         if (prec != 8) l2_split_rows(Q, nq, d, re, dp, Qh, Ql, qn, st); // (the fp8 scan converts in LDS)
     }
     // pairs of probe slots p_lo .. p_lo + npw - 1 -> items of at most 64 pairs of one cluster
+    // candidates reranked per query: ivfl2.k2 (k' >= 4k is enough for the rerank to recover
+    // the exact top-k in practice), at least k, at most kIvfK2 = 64 (one per lane)
+    const int k2 = (int)std::min<int64_t>(kIvfK2, std::max<int64_t>({(int64_t)k, 4 * (int64_t)k, tune_get("ivfl2.k2")}));
+    const bool k2_pass = np > 1 && prec == 8 && cap >= k2 && tune_get("ivfl2.two_pass") != 0;
+    // the second pass of the fp8 scan skips the row groups that no query of a query group can need
+    // (ivfl2.prune; the results are those of the same pass without it)
+    const bool prune = k2_pass && tune_get("ivfl2.prune") != 0;
+    if (k2_pass)
+        hipLaunchKernelGGL(first_probe_kernel, dim3((unsigned)ceil_div(nq, (int64_t)256)), dim3(256), 0, st, pr, nq, np,
+                           h->loff.as<int64_t>(), k2);
     auto bucket = [&](int p_lo, int npw) {
         StageScope sc(st, STAGE_OTHER);
         const int64_t tw = nq * npw;
         const unsigned pchunks = (unsigned)ceil_div(tw, (int64_t)kPairChunk);
         hipLaunchKernelGGL(probe_count_kernel, dim3(pchunks), dim3(256), 0, st, pr, tw, np, p_lo, npw, h->nl, lc);
+        const int iq = prec == 8 ? (p_lo > 0 ? iq_b : kIvf8IQ) : kIvfIQ; // queries per item
         hipLaunchKernelGGL(build_items_kernel, dim3(1), dim3(256), 0, st, lc, h->corder.as<int32_t>(),
                            h->loff.as<int64_t>(), h->nl, cur, itoff, woff, bins, nranges, iq, ohist, h->nl + 1);
         hipLaunchKernelGGL(expand_items_kernel, dim3((unsigned)ceil_div(max_items, (int64_t)256)), dim3(256), 0, st, lc,
@@ -1687,21 +2266,23 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         if (prec == 8) { // fp8 query residuals of every pair
             uint8_t *R8 = (uint8_t *)h->qres8.ensure((size_t)total * dp);
             float *Rn = (float *)h->qresn.ensure((size_t)total * 4);
+            // norms of the fp8 residuals for the pruned pass (the pairs after the first probe)
+            float *Rq = prune && p_lo > 0 ? (float *)h->qres8n.ensure((size_t)total * 4) : nullptr;
             const dim3 gq((unsigned)ceil_div(nq, (int64_t)4));
             const float *lcp = h->lcent.as<float>();
             PN_REQUIRE(dp <= 1152, PN_EINVAL, "fp8 precision serves d up to 1152");
             if (dp <= 256)
                 hipLaunchKernelGGL(query_residual_kernel<1>, gq, dim3(256), 0, st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq,
-                                   np, p_lo, npw, psl, R8, Rn);
+                                   np, p_lo, npw, psl, R8, Rn, Rq);
             else if (dp <= 512)
                 hipLaunchKernelGGL(query_residual_kernel<2>, gq, dim3(256), 0, st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq,
-                                   np, p_lo, npw, psl, R8, Rn);
+                                   np, p_lo, npw, psl, R8, Rn, Rq);
             else if (dp <= 768)
                 hipLaunchKernelGGL(query_residual_kernel<3>, gq, dim3(256), 0, st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq,
-                                   np, p_lo, npw, psl, R8, Rn);
+                                   np, p_lo, npw, psl, R8, Rn, Rq);
             else
                 hipLaunchKernelGGL(query_residual_kernel<5>, gq, dim3(256), 0, st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq,
-                                   np, p_lo, npw, psl, R8, Rn);
+                                   np, p_lo, npw, psl, R8, Rn, Rq);
         }
         PN_HIP(hipGetLastError());
     };
@@ -1727,11 +2308,33 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
             constexpr size_t kLdsMax = 148 * 1024;
             const size_t rblk = (size_t)kIvf8IQ * (dp + 16);
             PN_REQUIRE(rblk <= kLdsMax, PN_EINVAL, "fp8 precision serves d up to 1152");
-            const int nmax = (int)std::min<size_t>(8192, (kLdsMax - rblk) / 4 / 64 * 64);
+            const int nmax = (int)std::min<size_t>(8192, (kLdsMax - rblk) / 4 / 512 * 512); // (>= 512 at dp <= 1152)
+            PN_REQUIRE(nmax >= 512, PN_EINVAL, "fp8 precision serves d up to 1152");
             const size_t lds = std::max(rblk + (size_t)nmax * 4, (size_t)16 * 64 * KLV * 8); // (and the lists)
             const dim3 grid8((unsigned)std::min<int64_t>(ceil_div((int64_t)h->cus, (int64_t)kXcds) *
                                                              std::max<int64_t>(1, tune_get("ivfl2.fp8_wg_per_cu")),
                                                          max_items) * kXcds);
+            // group pruning of the bounded pass (d <= 768: the query tile's fragments in registers)
+            const uint32_t *act = nullptr;
+            // group words per (item, query tile), then one "any group needed" word per (item, tile)
+            const int64_t any_at = max_items * 4 * kPruneWords;
+            if (prune && qthr && dp <= 768) {
+                uint32_t *pa = (uint32_t *)h->pact.ensure((size_t)(any_at + max_items * 4) * 4);
+                auto pg = [&](auto ncc) {
+                    hipLaunchKernelGGL((prune_groups_kernel<decltype(ncc)::value>), dim3((unsigned)max_items),
+                                       dim3(256), 0, st, h->c8.as<uint8_t>(), h->smeta.as<float4>(),
+                                       h->soff.as<int64_t>(), h->sbase.as<int32_t>(), h->loff.as<int64_t>(), dp,
+                                       h->sx8, h->qres8.as<uint8_t>(), h->qresn.as<float>(), h->qres8n.as<float>(),
+                                       items, bins, pq, qthr, pa, any_at);
+                };
+                if (dp <= 256)
+                    pg(std::integral_constant<int, 4>{});
+                else if (dp <= 512)
+                    pg(std::integral_constant<int, 8>{});
+                else
+                    pg(std::integral_constant<int, 12>{});
+                act = pa;
+            }
             auto go = [&](auto pdc) {
                 constexpr int PDV = decltype(pdc)::value;
                 if (lds > 64 * 1024)
@@ -1739,7 +2342,8 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                 hipLaunchKernelGGL((ivfl2_scan_fp8_kernel<KLV, PDV>), grid8, dim3(512), lds, st, h->l8.as<uint8_t>(),
                                    h->rn8.as<float>(), h->loff.as<int64_t>(), dp, h->sx8, h->qres8.as<uint8_t>(),
-                                   h->qresn.as<float>(), items, bins, nranges, pq, pp, np, qoff, qthr, cand, nmax);
+                                   h->qresn.as<float>(), items, bins, nranges, pq, pp, np, qoff, qthr, cand, nmax, act,
+                                   any_at);
             };
             if (tune_get("ivfl2.fp8_ahead") == 3)
                 go(std::integral_constant<int, 3>{});
@@ -1759,10 +2363,6 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     // probe, which no candidate the rerank takes can exceed (pass_bound_k2_kernel); optionally
     // (pass_margin_pct > 0) the k-th approximation times a margin; otherwise one pass.
     const int64_t margin_pct = tune_get("ivfl2.pass_margin_pct");
-    // candidates reranked per query: ivfl2.k2 (k' >= 4k is enough for the rerank to recover
-    // the exact top-k in practice), at least k, at most kIvfK2 = 64 (one per lane)
-    const int k2 = (int)std::min<int64_t>(kIvfK2, std::max<int64_t>({(int64_t)k, 4 * (int64_t)k, tune_get("ivfl2.k2")}));
-    const bool k2_pass = np > 1 && prec == 8 && cap >= k2 && tune_get("ivfl2.two_pass") != 0;
     if (k2_pass || (np > 1 && k <= kIvfKP && margin_pct > 0)) {
         uint32_t *qthr = (uint32_t *)h->qthr.ensure((size_t)nq * 4);
         bucket(0, 1);

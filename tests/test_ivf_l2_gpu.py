@@ -118,6 +118,29 @@ def test_bounded_second_pass_keeps_results(precision):
         N.tune_set("ivfl2.pass_margin_pct", old)
 
 
+@pytest.mark.parametrize("k,n_probe,nlist", [(10, 8, 40), (20, 6, 40), (1, 3, 40), (10, 3, 3), (40, 5, 12)])
+def test_pruned_second_pass_equals_unpruned(k, n_probe, nlist):
+    """The fp8 scan's second pass skips the row groups whose lower bound (the group's fp8 centre
+    and radius) rules out every query of a query group (ivfl2.prune). Its neighbours and
+    distances must equal those of the same pass without the skipping, bit for bit; nlist = 3 gives
+    lists longer than one LDS segment of row norms (8,192 rows)."""
+    from pynear_amd import _native as N
+    X = _gmm(30000, 96, 24, seed=41, sigma=0.3)
+    Q = _gmm(2000, 96, 24, seed=42, sigma=0.3)
+    h = N.IvfL2Handle()
+    h.build(X, nlist, 10, 3)
+    old = N.tune_get("ivfl2.prune")
+    try:
+        N.tune_set("ivfl2.prune", 1)
+        a = h.search(Q, n_probe, k, "fp8")
+        N.tune_set("ivfl2.prune", 0)
+        b = h.search(Q, n_probe, k, "fp8")
+    finally:
+        N.tune_set("ivfl2.prune", old)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y), (k, n_probe, nlist)
+
+
 @pytest.mark.parametrize("precision", ["bf16", "bf16x3", "fp8"])
 def test_repeated_searches_identical(precision):
     """Lists probed by a few hundred queries each split into several work items that workgroups
