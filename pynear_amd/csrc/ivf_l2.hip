@@ -352,6 +352,12 @@ __device__ __forceinline__ int take_item(int32_t *__restrict__ bins, int nranges
     return -1;
 }
 
+// Fills n words with v (grid-stride).
+__global__ void fill_u32_kernel(uint32_t *__restrict__ p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 // Candidate slots of (query, probe) pairs without a probe stay empty.
 __global__ void fill_u64_kernel(uint64_t *__restrict__ p, int64_t n, uint64_t v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -810,34 +816,45 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
     }
 }
 
-// ---------------------------------------------------------------- group pruning
-// For the second pass of the fp8 scan (queries hold a bound from their first probe): per work
-// item (a list and up to 128 of its queries) and per 32-query tile qt, which 32-row groups of the
-// list some query of the tile may need. The list's rows are cut into segments (its parts, or the
-// whole list); for every row x of a segment and query residual q8,
-// q8.x8 <= U = q8.centre + |q8| (R + 2^-11 (X + |centre|)) (seg_meta_kernel; the 2^-11 covers
-// the fp32 rounding of both products' sums), so the scan's approximation of any row of the
-// segment is at least lb = qv + mxr + m2 U (m2 < 0), less 2^-12 of its terms' magnitude for the
-// fp32 rounding of both sides. A segment whose lb key is at or above the query's threshold holds
-// no row the query keeps; a group is needed when a needed segment overlaps it. Bit G & 31 of
-// act[(item 4 + qt) kPruneWords + (G >> 5) - (g0 >> 5)] is set for a needed group G, and
-// act[any_at + item 4 + qt] says whether any is. Tiles without a query write zeros; padding
-// columns repeat the item's last query. Lists spanning more than kPruneWords words of groups are
-// not pruned. One workgroup of four waves (one per query tile) per item; the tile's query
-// fragments are held in registers across the centre tiles (NC = dp / 64 chunks at most).
+// ---------------------------------------------------------------- segment bounds per query
+// The list scan in fp8 keeps, per (query, probe), keys below a per-query threshold; a query's
+// threshold is the key just above T, an upper bound of the approximation of at least k2 rows of
+// its first probe (so no row above T is among the k2 candidates the rerank takes), and the scan
+// skips the 32-row groups in which no query of a query group can find a row below its threshold.
+// Both come from the list's segments (its parts, or the whole list): for every row x of a segment
+// and query residual q8, |q8.x8 - q8.centre| <= |q8| E with E = R + 3 X 2^-11 (seg_meta_kernel;
+// the 3 X 2^-11 covers the fp32 rounding of both products' sums), so the scan's approximation
+// qv + xr + m2 q8.x8 (m2 < 0) of any row of the segment lies in
+//   [qv + mxr + m2 (q8.centre + |q8| E), qv + Mxr + m2 (q8.centre - |q8| E)],
+// each end moved outward by 2^-12 of its terms' magnitude for the fp32 rounding of both sides.
+// One workgroup of four waves (one per 32-query tile) per work item (a list and up to 128 of its
+// queries); the tile's query fragments are held in registers across the list's centre tiles (NC =
+// dp / 64 chunks at most), lane (col, half) holding query col and segments 8 (r / 4) + 4 half + r % 4
+// of each centre tile.
+// MODE 0 (bounds): for the item's pairs of probe rank 0, T = the least segment upper bound u such
+// that the segments whose upper bound is at most u hold at least k2 rows, into qthr[query]
+// (queries without such a T keep the value the caller filled, kKey32None: no bound).
+// MODE 1 (groups): bit G & 31 of act[(item 4 + qt) kPruneWords + (G >> 5) - (g0 >> 5)] is set when
+// some query of tile qt may need group G (a segment overlapping G with its lower bound's key below
+// the query's threshold), and act[any_at + item 4 + qt] says whether any group is needed. Tiles
+// without a query write zeros; padding columns repeat the item's last query. Lists spanning more
+// than kPruneWords words of groups are not pruned (their "any" word is set).
 constexpr int kPruneWords = 64;
-template <int NC>
-__global__ __launch_bounds__(256) void prune_groups_kernel(const uint8_t *__restrict__ C8,
-                                                           const float4 *__restrict__ smeta,
-                                                           const int64_t *__restrict__ soff,
-                                                           const int32_t *__restrict__ sbase,
-                                                           const int64_t *__restrict__ loff, int dp, float scale,
-                                                           const uint8_t *__restrict__ R8, const float *__restrict__ Rn,
-                                                           const float *__restrict__ Rq, const int4 *__restrict__ items,
-                                                           const int32_t *__restrict__ bins,
-                                                           const int32_t *__restrict__ pair_q,
-                                                           const uint32_t *__restrict__ qthr,
-                                                           uint32_t *__restrict__ act, int64_t any_at) {
+constexpr int kSegMax = 64; // segments per list at most (the parts of a list: kSubMax)
+template <int NC, int MODE>
+__global__ __launch_bounds__(256) void seg_bound_kernel(const uint8_t *__restrict__ C8,
+                                                        const float4 *__restrict__ smeta,
+                                                        const int64_t *__restrict__ soff,
+                                                        const int32_t *__restrict__ sbase,
+                                                        const int64_t *__restrict__ loff, int dp, float scale,
+                                                        const uint8_t *__restrict__ R8, const float *__restrict__ Rn,
+                                                        const float *__restrict__ Rq, const int4 *__restrict__ items,
+                                                        const int32_t *__restrict__ bins,
+                                                        const int32_t *__restrict__ pair_q,
+                                                        const int32_t *__restrict__ pair_p,
+                                                        uint32_t *__restrict__ qthr, uint32_t *__restrict__ act,
+                                                        int64_t any_at, int k2) {
+    __shared__ float s_ub[4][32][kSegMax]; // MODE 0: per wave, query column and segment: upper bound
     const int item = blockIdx.x;
     if (item >= bins[3 * kXcds]) return; // (the item total)
     const int4 it = items[item];
@@ -849,19 +866,23 @@ __global__ __launch_bounds__(256) void prune_groups_kernel(const uint8_t *__rest
     const int g0 = r0 >> 5, g1 = (r1 - 1) >> 5, cg0 = g0 >> 5, nwd = (g1 >> 5) - cg0 + 1;
     uint32_t *out = act + ((int64_t)item * 4 + w) * kPruneWords;
     uint32_t *any = act + any_at + (int64_t)item * 4 + w;
-    if (nwd > kPruneWords) { // (the scan scans such lists whole)
+    if (MODE == 1 && nwd > kPruneWords) { // (the scan scans such lists whole)
         if (lane == 0) *any = 1u;
         return;
     }
     if (q0 >= cnt) {
-        for (int wi = lane; wi < nwd; wi += 64) out[wi] = 0u;
-        if (lane == 0) *any = 0u;
+        if (MODE == 1) {
+            for (int wi = lane; wi < nwd; wi += 64) out[wi] = 0u;
+            if (lane == 0) *any = 0u;
+        }
         return;
     }
-    const int nch = dp / kIvfKC;
     const int64_t slot = start + min(q0 + col, cnt - 1); // the lane's query (pair slot)
+    const bool first = pair_p[slot] == 0;               // (MODE 0: the pairs to bound)
+    if (MODE == 0 && !__any(first)) return;
+    const int nch = dp / kIvfKC;
     const float qv = Rn[slot], qn = Rq[slot];
-    const uint32_t th = qthr[pair_q[slot]];
+    const uint32_t th = MODE == 1 ? qthr[pair_q[slot]] : 0u;
     const float m2 = -2.f / (scale * scale);
     V8 b[NC];
     const uint8_t *qrow = R8 + slot * dp + 32 * half;
@@ -871,8 +892,8 @@ __global__ __launch_bounds__(256) void prune_groups_kernel(const uint8_t *__rest
             b[ch].u[0] = *(const uint4 *)(qrow + ch * kIvfKC);
             b[ch].u[1] = *(const uint4 *)(qrow + ch * kIvfKC + 16);
         }
-    const int sb0 = sbase[c], sb1 = sbase[c + 1]; // the list's segments (<= 64)
-    uint64_t smask = 0;                            // bit s - sb0: segment s needed by some query
+    const int sb0 = sbase[c], sb1 = sbase[c + 1]; // the list's segments (<= kSegMax)
+    uint64_t smask = 0;                            // MODE 1, bit s - sb0: segment s needed
     const uint8_t *lane_c8 = C8 + col * 64 + 32 * half;
     for (int ct = sb0 >> 5; ct <= (sb1 - 1) >> 5; ++ct) {
         const uint8_t *cb = lane_c8 + (((int64_t)ct * nch) << 11);
@@ -892,16 +913,45 @@ __global__ __launch_bounds__(256) void prune_groups_kernel(const uint8_t *__rest
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int sg = 32 * ct + 8 * (r >> 2) + 4 * half + (r & 3);
+            const bool in = sg >= sb0 && sg < sb1;
             const float4 sv = smeta[min(max(sg, sb0), sb1 - 1)];
-            const float U = acc[r] + qn * (sv.x + (sv.y + sv.z) * (1.f / 2048.f));
-            float lb = fmaf(m2, U, qv + sv.w);
-            lb -= (qv + sv.w - m2 * fabsf(U)) * (1.f / 4096.f);
-            const bool need = sg >= sb0 && sg < sb1 && (f32_key(fmaxf(0.f, lb)) & ~15u) < th;
-            const uint64_t bal = __ballot(need);
-            const int s0 = 32 * ct + 8 * (r >> 2) + (r & 3) - sb0; // half 0's segment, relative
-            if ((uint32_t)bal && s0 >= 0 && s0 < 64) smask |= 1ull << s0;
-            if ((uint32_t)(bal >> 32) && s0 + 4 >= 0 && s0 + 4 < 64) smask |= 1ull << (s0 + 4);
+            const float e = qn * (sv.x + 3.f * sv.y * (1.f / 2048.f));
+            if (MODE == 0) {
+                const float U = acc[r] - e; // (<= q8.x8 of every row of the segment)
+                float ub = fmaf(m2, U, qv + sv.z);
+                ub += (qv + sv.z - m2 * fabsf(U)) * (1.f / 4096.f);
+                if (in) s_ub[w][col][sg - sb0] = ub;
+            } else {
+                const float U = acc[r] + e; // (>= q8.x8 of every row of the segment)
+                float lb = fmaf(m2, U, qv + sv.w);
+                lb -= (qv + sv.w - m2 * fabsf(U)) * (1.f / 4096.f);
+                const bool need = in && (f32_key(fmaxf(0.f, lb)) & ~15u) < th;
+                const uint64_t bal = __ballot(need);
+                const int s0 = 32 * ct + 8 * (r >> 2) + (r & 3) - sb0; // half 0's segment, relative
+                if ((uint32_t)bal && s0 >= 0 && s0 < 64) smask |= 1ull << s0;
+                if ((uint32_t)(bal >> 32) && s0 + 4 >= 0 && s0 + 4 < 64) smask |= 1ull << (s0 + 4);
+            }
         }
+    }
+    if (MODE == 0) {
+        // lane (col, half) tries the upper bounds of segments half, half + 2, ... of query col as T;
+        // the two halves' least valid T is taken
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int ns = sb1 - sb0;
+        float best = INFINITY;
+        for (int t = half; t < ns; t += 2) {
+            const float u = s_ub[w][col][t];
+            int64_t rows = 0;
+            for (int s2 = 0; s2 < ns; ++s2)
+                if (s_ub[w][col][s2] <= u) rows += soff[sb0 + s2 + 1] - soff[sb0 + s2];
+            if (rows >= k2) best = fminf(best, u);
+        }
+        best = fminf(best, __shfl_xor(best, 32, 64));
+        if (half == 0 && first && q0 + col < cnt && best < INFINITY)
+            qthr[pair_q[slot]] = (f32_key(fmaxf(0.f, best)) & ~15u) + 16u;
+        return;
     }
     // needed segments -> needed groups (word wi: groups 32 (cg0 + wi) ..)
     for (int wi = lane; wi < nwd; wi += 64) {
@@ -1053,19 +1103,21 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
             // some query of the wave's query group may need
             uint32_t rbmask = nrb >= 32 ? 0xFFFFFFFFu : ((1u << nrb) - 1u);
             if (act && (g1 >> 5) - (g0 >> 5) < kPruneWords) {
-                // groups that some query of the wave's two query tiles may need (prune_groups_kernel)
+                // groups that some query of the wave's two query tiles may need (seg_bound_kernel):
+                // lane l tests row block l
                 const uint32_t *aq = act + ((int64_t)slot * 4 + 2 * qg) * kPruneWords;
                 const int cgl = g0 >> 5;
-                rbmask = 0;
-                for (int rb2 = 0; rb2 < nrb; ++rb2)
+                bool on = false;
+                if (lane < nrb)
 #pragma unroll
                     for (int i = 0; i < 2; ++i) {
-                        const int G = sg0 + GB * rb2 + 2 * wr + i;
+                        const int G = sg0 + GB * lane + 2 * wr + i;
                         if (G <= sg1) {
                             const int wi = (G >> 5) - cgl;
-                            if (((aq[wi] | aq[kPruneWords + wi]) >> (G & 31)) & 1u) rbmask |= 1u << rb2;
+                            on = on || (((aq[wi] | aq[kPruneWords + wi]) >> (G & 31)) & 1u);
                         }
                     }
+                rbmask = (uint32_t)__ballot(on);
             }
             if (rbmask == 0) continue; // (wave-uniform; no barrier follows inside the segment)
             // the active row block after b, or -1
@@ -1686,11 +1738,12 @@ static void sub_partition_order(const float *dX, int64_t n, int64_t d, const std
 // ---------------------------------------------------------------- segment bounds of the fp8 rows
 // One wave per segment sg (rows soff[sg] .. soff[sg + 1] - 1 of the fp8 rows L8, fp8_row_chunk
 // layout, dp bytes per row): the centre, the segment's rows' fp8 values averaged and rounded to
-// fp8 as the rows are, goes to centre row sg of C8 (the same layout); sm[sg] = {R, X, |centre|,
-// mxr} with R >= max |x8 - centre| and X >= max |x8| over the segment's rows (their fp32 sums
-// rounded up by 2^-10) and mxr the least rn over them. With these, for every row x of the segment
-// and any query residual q8, q8.x8 <= q8.centre + |q8| R, which bounds the scan's approximation
-// from below (prune_groups_kernel).
+// fp8 as the rows are, goes to centre row sg of C8 (the same layout); sm[sg] = {R, X, Mxr, mxr}
+// with R >= max |x8 - centre| and X >= max |x8| over the segment's rows (their fp32 sums rounded
+// up by 2^-10) and Mxr / mxr the largest / least rn over them. With these, for every row x of the
+// segment and any query residual q8, |q8.x8 - q8.centre| <= |q8| R, which bounds the scan's
+// approximation of the segment's rows from both sides (seg_bound_kernel); |centre| <= 1.0625 X
+// (the mean's norm is at most X before its fp8 rounding).
 template <int NS>
 __global__ __launch_bounds__(256) void seg_meta_kernel(const uint8_t *__restrict__ L8, const float *__restrict__ rn,
                                                        const int64_t *__restrict__ soff, int64_t nseg, int dp,
@@ -1712,10 +1765,11 @@ __global__ __launch_bounds__(256) void seg_meta_kernel(const uint8_t *__restrict
     for (int t = 0; t < NS; ++t)
 #pragma unroll
         for (int u = 0; u < 4; ++u) sum[t][u] = 0.f;
-    float mxr = INFINITY;
+    float mxr = INFINITY, Mxr = 0.f;
 #pragma unroll 4
     for (int64_t row = s0; row < s1; ++row) {
         mxr = fminf(mxr, rn[row]);
+        Mxr = fmaxf(Mxr, rn[row]);
 #pragma unroll
         for (int t = 0; t < NS; ++t) {
             const int b = 4 * lane + 256 * t;
@@ -1729,7 +1783,6 @@ __global__ __launch_bounds__(256) void seg_meta_kernel(const uint8_t *__restrict
     }
     const float inv = 1.f / (float)max<int64_t>(s1 - s0, 1);
     float mq[NS][4];
-    float mn = 0.f;
 #pragma unroll
     for (int t = 0; t < NS; ++t) {
         const int b = 4 * lane + 256 * t;
@@ -1742,8 +1795,6 @@ __global__ __launch_bounds__(256) void seg_meta_kernel(const uint8_t *__restrict
             wq = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], wq, true);
             *(int *)(C8 + at(sg, b)) = wq;
             dec(wq, mq[t]);
-#pragma unroll
-            for (int u = 0; u < 4; ++u) mn = fmaf(mq[t][u], mq[t][u], mn);
         } else {
 #pragma unroll
             for (int u = 0; u < 4; ++u) mq[t][u] = 0.f;
@@ -1775,12 +1826,9 @@ __global__ __launch_bounds__(256) void seg_meta_kernel(const uint8_t *__restrict
         r2max = fmaxf(r2max, __shfl(e2, 0, 64));
         x2max = fmaxf(x2max, __shfl(x2, 0, 64));
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) mn += __shfl_xor(mn, off, 64);
-    mn = __shfl(mn, 0, 64);
     if (lane == 0) {
         constexpr float up = 1.f + 1.f / 1024.f;
-        sm[sg] = make_float4(sqrtf(r2max) * up, sqrtf(x2max) * up, sqrtf(mn) * up, mxr);
+        sm[sg] = make_float4(sqrtf(r2max) * up, sqrtf(x2max) * up, Mxr, mxr);
     }
 }
 
@@ -2243,11 +2291,17 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     // candidates reranked per query: ivfl2.k2 (k' >= 4k is enough for the rerank to recover
     // the exact top-k in practice), at least k, at most kIvfK2 = 64 (one per lane)
     const int k2 = (int)std::min<int64_t>(kIvfK2, std::max<int64_t>({(int64_t)k, 4 * (int64_t)k, tune_get("ivfl2.k2")}));
-    const bool k2_pass = np > 1 && prec == 8 && cap >= k2 && tune_get("ivfl2.two_pass") != 0;
-    // the second pass of the fp8 scan skips the row groups that no query of a query group can need
-    // (ivfl2.prune; the results are those of the same pass without it)
-    const bool prune = k2_pass && tune_get("ivfl2.prune") != 0;
-    if (k2_pass)
+    // fp8 with segment bounds (the default; d <= 768, the query tile's fragments in registers):
+    // each query's threshold from its first probe's segments, then one pass over all probes that
+    // skips the row groups no query of a query group can need (ivfl2.prune; the results are those
+    // of the same pass without the skipping)
+    const int dp8 = h->lrows.dp;
+    const bool bounded = np > 1 && prec == 8 && cap >= k2 && dp8 <= 768 && h->nseg > 0 &&
+                         tune_get("ivfl2.bound") != 0;
+    const bool prune = bounded && tune_get("ivfl2.prune") != 0;
+    // else, fp8: optionally two passes, the first probe scanned first for the bound
+    const bool k2_pass = !bounded && np > 1 && prec == 8 && cap >= k2 && tune_get("ivfl2.two_pass") != 0;
+    if (bounded || k2_pass)
         hipLaunchKernelGGL(first_probe_kernel, dim3((unsigned)ceil_div(nq, (int64_t)256)), dim3(256), 0, st, pr, nq, np,
                            h->loff.as<int64_t>(), k2);
     auto bucket = [&](int p_lo, int npw) {
@@ -2266,8 +2320,8 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         if (prec == 8) { // fp8 query residuals of every pair
             uint8_t *R8 = (uint8_t *)h->qres8.ensure((size_t)total * dp);
             float *Rn = (float *)h->qresn.ensure((size_t)total * 4);
-            // norms of the fp8 residuals for the pruned pass (the pairs after the first probe)
-            float *Rq = prune && p_lo > 0 ? (float *)h->qres8n.ensure((size_t)total * 4) : nullptr;
+            // norms of the fp8 residuals for the segment bounds
+            float *Rq = bounded ? (float *)h->qres8n.ensure((size_t)total * 4) : nullptr;
             const dim3 gq((unsigned)ceil_div(nq, (int64_t)4));
             const float *lcp = h->lcent.as<float>();
             PN_REQUIRE(dp <= 1152, PN_EINVAL, "fp8 precision serves d up to 1152");
@@ -2284,6 +2338,28 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
                 hipLaunchKernelGGL(query_residual_kernel<5>, gq, dim3(256), 0, st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq,
                                    np, p_lo, npw, psl, R8, Rn, Rq);
         }
+        PN_HIP(hipGetLastError());
+    };
+    // segment bounds over the work items: MODE 0 thresholds into qthr, MODE 1 group masks
+    auto seg_bounds = [&](int mode, uint32_t *qthr, uint32_t *pa, int64_t any_at) {
+        auto go = [&](auto ncc, auto mc) {
+            hipLaunchKernelGGL((seg_bound_kernel<decltype(ncc)::value, decltype(mc)::value>), dim3((unsigned)max_items),
+                               dim3(256), 0, st, h->c8.as<uint8_t>(), h->smeta.as<float4>(), h->soff.as<int64_t>(),
+                               h->sbase.as<int32_t>(), h->loff.as<int64_t>(), dp8, h->sx8, h->qres8.as<uint8_t>(),
+                               h->qresn.as<float>(), h->qres8n.as<float>(), items, bins, pq, pp, qthr, pa, any_at, k2);
+        };
+        auto by_mode = [&](auto ncc) {
+            if (mode == 0)
+                go(ncc, std::integral_constant<int, 0>{});
+            else
+                go(ncc, std::integral_constant<int, 1>{});
+        };
+        if (dp8 <= 256)
+            by_mode(std::integral_constant<int, 4>{});
+        else if (dp8 <= 512)
+            by_mode(std::integral_constant<int, 8>{});
+        else
+            by_mode(std::integral_constant<int, 12>{});
         PN_HIP(hipGetLastError());
     };
     auto scan = [&](const uint32_t *qthr) {
@@ -2318,21 +2394,9 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
             const uint32_t *act = nullptr;
             // group words per (item, query tile), then one "any group needed" word per (item, tile)
             const int64_t any_at = max_items * 4 * kPruneWords;
-            if (prune && qthr && dp <= 768) {
+            if (prune && qthr) {
                 uint32_t *pa = (uint32_t *)h->pact.ensure((size_t)(any_at + max_items * 4) * 4);
-                auto pg = [&](auto ncc) {
-                    hipLaunchKernelGGL((prune_groups_kernel<decltype(ncc)::value>), dim3((unsigned)max_items),
-                                       dim3(256), 0, st, h->c8.as<uint8_t>(), h->smeta.as<float4>(),
-                                       h->soff.as<int64_t>(), h->sbase.as<int32_t>(), h->loff.as<int64_t>(), dp,
-                                       h->sx8, h->qres8.as<uint8_t>(), h->qresn.as<float>(), h->qres8n.as<float>(),
-                                       items, bins, pq, qthr, pa, any_at);
-                };
-                if (dp <= 256)
-                    pg(std::integral_constant<int, 4>{});
-                else if (dp <= 512)
-                    pg(std::integral_constant<int, 8>{});
-                else
-                    pg(std::integral_constant<int, 12>{});
+                seg_bounds(1, const_cast<uint32_t *>(qthr), pa, any_at); // (MODE 1 only reads it)
                 act = pa;
             }
             auto go = [&](auto pdc) {
@@ -2363,7 +2427,17 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     // probe, which no candidate the rerank takes can exceed (pass_bound_k2_kernel); optionally
     // (pass_margin_pct > 0) the k-th approximation times a margin; otherwise one pass.
     const int64_t margin_pct = tune_get("ivfl2.pass_margin_pct");
-    if (k2_pass || (np > 1 && k <= kIvfKP && margin_pct > 0)) {
+    if (bounded) {
+        uint32_t *qthr = (uint32_t *)h->qthr.ensure((size_t)nq * 4);
+        bucket(0, np);
+        {
+            StageScope sc(st, STAGE_OTHER);
+            hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)std::min<int64_t>(1024, ceil_div(nq, (int64_t)256))),
+                               dim3(256), 0, st, qthr, nq, kKey32None);
+            seg_bounds(0, qthr, h->pact.as<uint32_t>(), 0);
+        }
+        scan(qthr);
+    } else if (k2_pass || (np > 1 && k <= kIvfKP && margin_pct > 0)) {
         uint32_t *qthr = (uint32_t *)h->qthr.ensure((size_t)nq * 4);
         bucket(0, 1);
         scan(nullptr);

@@ -50,9 +50,10 @@ static std::map<std::string, int64_t> &tune_table() {
         t["l2.slice_major"] = 1;      // exact L2 scan grid: slices along x (rows of a slice stay in one XCD's L2)
         t["ivfl2.xcd_ranges"] = 0;     // 1: one item range per XCD; 0: one queue for all workgroups
         t["ivfl2.fp8_wg_per_cu"] = 1;  // fp8 scan: workgroups per CU
-        t["ivfl2.two_pass"] = 1;       // fp8: first probe, then the rest under its K2-th candidate
-        t["ivfl2.prune"] = 1;          // fp8 second pass: skip row groups that no query can need
-        t["ivfl2.fp8_iq2"] = 128;      // fp8 second pass: queries per work item (32 .. 128)
+        t["ivfl2.bound"] = 1;          // fp8: per-query thresholds from the first probe's segment bounds
+        t["ivfl2.prune"] = 1;          // fp8 with bounds: skip row groups that no query can need
+        t["ivfl2.two_pass"] = 0;       // fp8 without bounds: first probe, then the rest under its K2-th candidate
+        t["ivfl2.fp8_iq2"] = 128;      // fp8 second pass of two: queries per work item (32 .. 128)
                                        // (the scan gains less than the second bucketing costs)
         t["ivfl2.fp8_ahead"] = 3;      // fp8 scan: fragments loaded 2 or 3 steps ahead
         t["ivfl2.k2"] = 40;            // candidates reranked in fp32 per query (max(4 k, this), <= 64)
