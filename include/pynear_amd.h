@@ -242,6 +242,13 @@ pn_status pn_ivfl2_search_probes_device(pn_ivfl2 *h, const float *queries, int64
                                         const int32_t *counts, int32_t np_max, int32_t k, int32_t precision,
                                         int32_t *out_idx, float *out_dist, int32_t *out_count, void *stream);
 
+/* Work of the last fp8 list scan on this handle (precision 2): the matrix-core instructions it
+ * issued (v_mfma_scale_f32_32x32x64_f8f6f4, 131072 flops each) and the bytes of fp8 rows its work
+ * items read (rows that both query groups of an item scan counted once), after group pruning.
+ * Read after the search's stream has completed. No reference counterpart: measurement only (the
+ * roofline of the scan, whose work depends on the data). */
+pn_status pn_ivfl2_scan_work(pn_ivfl2 *h, int64_t *mfma_ops, int64_t *row_bytes);
+
 /* ---------------------------------------------------------------- learned probe predictor
  * An MLP d -> hidden (ReLU) -> nlist in bf16 (fp32 accumulation and masters) that ranks a query's
  * IVF clusters, trained on the device on exact neighbours; its softmax picks an adaptive number

@@ -53,6 +53,7 @@ SIGNATURES = {
     "pn_ivfl2_build": (_i32, [_vp, _vp, _i64, _i64, _i64, _i64, _u32, _vp, _vp]),
     "pn_ivfl2_set": (_i32, [_vp, _vp, _i64, _i64, _vp, _i64, _vp]),
     "pn_ivfl2_info": (_i32, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i32)]),
+    "pn_ivfl2_scan_work": (_i32, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "pn_ivfl2_search": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _i32, _vp, _vp, _vp]),
     "pn_ivfl2_search_device": (_i32, [_vp, _vp, _i64, _i32, _i32, _i32, _vp, _vp, _vp, _vp]),
     "pn_ivfbin_create": (_i32, [_i32, _i32, _i32, _u32, ctypes.POINTER(_vp)]),
@@ -322,6 +323,12 @@ class IvfL2Handle:
         n, d, nl = _i64(0), _i64(0), _i32(0)
         check(lib().pn_ivfl2_info(self._h, ctypes.byref(n), ctypes.byref(d), ctypes.byref(nl)))
         return n.value, d.value, nl.value
+
+    def scan_work(self):
+        """(MFMA instructions, fetched fp8 row bytes) of the last fp8 list scan."""
+        m, b = _i64(0), _i64(0)
+        check(lib().pn_ivfl2_scan_work(self._h, ctypes.byref(m), ctypes.byref(b)))
+        return m.value, b.value
 
     def search(self, queries, nprobe, k, precision="bf16"):
         nq, d = queries.shape

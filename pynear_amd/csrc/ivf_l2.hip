@@ -66,6 +66,8 @@ struct pn_ivfl2 {
     pn::DevBuf qpad, qsplit_h, qsplit_l, qn, probes, pdist, lcount, cursor, pair_q, pair_p, pair_c, pslot;
     pn::DevBuf items, ntot /* item ranges */, itoff, woff, cand, qoff, qthr, qres8, qresn, qkey, qorder, ohist, oidx,
         odist, ocnt, qres8n, pact;
+    pn::DevBuf work;         // the last fp8 scan launch: per workgroup MFMA instructions and fetched row bytes
+    int64_t work_n = 0;      // its workgroups
 };
 
 namespace pn {
@@ -992,7 +994,9 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
     float scale, const uint8_t *__restrict__ R8, const float *__restrict__ Rn,
     const int4 *__restrict__ items, int32_t *__restrict__ bins, int nranges, const int32_t *__restrict__ pair_q,
     const int32_t *__restrict__ pair_p, int np, const int32_t *__restrict__ qoff, const uint32_t *__restrict__ qthr,
-    uint64_t *__restrict__ cand, int nmax, const uint32_t *__restrict__ act, int64_t any_at) {
+    uint64_t *__restrict__ cand, int nmax, const uint32_t *__restrict__ act, int64_t any_at,
+    unsigned long long *__restrict__ work) {
+    // work: per workgroup, the MFMA instructions its waves issued and the fp8 row bytes they fetched
     constexpr int IQ = kIvf8IQ;
     // IQ x (dp + 16) bytes of query residuals, then the list's row norms (nmax floats)
     extern __shared__ __align__(16) unsigned char dsm[];
@@ -1011,6 +1015,7 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
     // (one range: the queue's item count and first item are fixed for the launch; the atomic's
     // result is first needed at the item's end)
     const int cnt0 = bins[kXcds], first0 = bins[0];
+    unsigned long long wk_mfma = 0, wk_bytes = 0; // (work[2 blockIdx.x ..]: this workgroup's work)
     if (tid == 0) s_slot[0] = take_item(bins, nranges, xb);
     for (int par = 0;; par ^= 1) {
         __syncthreads(); // (s_slot[par] posted; the previous item's LDS is no longer read)
@@ -1102,22 +1107,33 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
             // the wave's row blocks to scan: all, or with group pruning (act) those with a group that
             // some query of the wave's query group may need
             uint32_t rbmask = nrb >= 32 ? 0xFFFFFFFFu : ((1u << nrb) - 1u);
+            uint32_t uniq = qg == 0 ? rbmask : 0u; // (work count: blocks whose rows this wave reads first)
             if (act && (g1 >> 5) - (g0 >> 5) < kPruneWords) {
                 // groups that some query of the wave's two query tiles may need (seg_bound_kernel):
                 // lane l tests row block l
-                const uint32_t *aq = act + ((int64_t)slot * 4 + 2 * qg) * kPruneWords;
                 const int cgl = g0 >> 5;
-                bool on = false;
-                if (lane < nrb)
+                auto mask_of = [&](int g) { // query group g's row blocks
+                    const uint32_t *aq = act + ((int64_t)slot * 4 + 2 * g) * kPruneWords;
+                    bool on = false;
+                    if (lane < nrb)
 #pragma unroll
-                    for (int i = 0; i < 2; ++i) {
-                        const int G = sg0 + GB * lane + 2 * wr + i;
-                        if (G <= sg1) {
-                            const int wi = (G >> 5) - cgl;
-                            on = on || (((aq[wi] | aq[kPruneWords + wi]) >> (G & 31)) & 1u);
+                        for (int i = 0; i < 2; ++i) {
+                            const int G = sg0 + GB * lane + 2 * wr + i;
+                            if (G <= sg1) {
+                                const int wi = (G >> 5) - cgl;
+                                on = on || (((aq[wi] | aq[kPruneWords + wi]) >> (G & 31)) & 1u);
+                            }
                         }
-                    }
-                rbmask = (uint32_t)__ballot(on);
+                    return (uint32_t)__ballot(on);
+                };
+                rbmask = mask_of(qg);
+                // (the rows of blocks that query group 0 also scans come from L2: not counted twice)
+                if (work && qg == 1) uniq = rbmask & ~mask_of(0);
+            }
+            if (act && qg == 0) uniq = rbmask;
+            { // the wave's products and the row bytes the item reads (summed per workgroup at the end)
+                wk_mfma += (unsigned long long)__popc(rbmask) * (unsigned long long)(nch * (two ? 4 : 2));
+                wk_bytes += (unsigned long long)__popc(uniq) * 64ull * (unsigned long long)dp;
             }
             if (rbmask == 0) continue; // (wave-uniform; no barrier follows inside the segment)
             // the active row block after b, or -1
@@ -1251,6 +1267,19 @@ __global__ __launch_bounds__(512, 1) void ivfl2_scan_fp8_kernel(
         if (tid == 0) s_slot[par ^ 1] = nranges == 1 ? (next < cnt0 ? first0 + next : -1) : next;
         // (the next item's LDS writes follow the loop's first barrier, which every wave reaches
         // only after this item's sorts)
+    }
+    if (work) { // (every wave left the loop after the same barrier: the LDS below is free)
+        unsigned long long *s_wk = (unsigned long long *)dsm;
+        if (lane == 0) {
+            s_wk[2 * w] = wk_mfma;
+            s_wk[2 * w + 1] = wk_bytes;
+        }
+        __syncthreads();
+        if (tid < 2) {
+            unsigned long long t = 0;
+            for (int v = 0; v < 8; ++v) t += s_wk[2 * v + tid];
+            work[2 * blockIdx.x + tid] = t;
+        }
     }
 }
 
@@ -2340,6 +2369,7 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         }
         PN_HIP(hipGetLastError());
     };
+    const int64_t any_at = max_items * 4 * kPruneWords; // (the "any" words after the group words)
     // segment bounds over the work items: MODE 0 thresholds into qthr, MODE 1 group masks
     auto seg_bounds = [&](int mode, uint32_t *qthr, uint32_t *pa, int64_t any_at) {
         auto go = [&](auto ncc, auto mc) {
@@ -2390,15 +2420,10 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
             const dim3 grid8((unsigned)std::min<int64_t>(ceil_div((int64_t)h->cus, (int64_t)kXcds) *
                                                              std::max<int64_t>(1, tune_get("ivfl2.fp8_wg_per_cu")),
                                                          max_items) * kXcds);
-            // group pruning of the bounded pass (d <= 768: the query tile's fragments in registers)
-            const uint32_t *act = nullptr;
-            // group words per (item, query tile), then one "any group needed" word per (item, tile)
-            const int64_t any_at = max_items * 4 * kPruneWords;
-            if (prune && qthr) {
-                uint32_t *pa = (uint32_t *)h->pact.ensure((size_t)(any_at + max_items * 4) * 4);
-                seg_bounds(1, const_cast<uint32_t *>(qthr), pa, any_at); // (MODE 1 only reads it)
-                act = pa;
-            }
+            // group pruning of the bounded pass (the masks written by seg_bound_kernel, mode 1)
+            const uint32_t *act = prune && qthr ? h->pact.as<uint32_t>() : nullptr;
+            unsigned long long *wk = (unsigned long long *)h->work.ensure((size_t)grid8.x * 16);
+            h->work_n = (int64_t)grid8.x;
             auto go = [&](auto pdc) {
                 constexpr int PDV = decltype(pdc)::value;
                 if (lds > 64 * 1024)
@@ -2407,7 +2432,7 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
                 hipLaunchKernelGGL((ivfl2_scan_fp8_kernel<KLV, PDV>), grid8, dim3(512), lds, st, h->l8.as<uint8_t>(),
                                    h->rn8.as<float>(), h->loff.as<int64_t>(), dp, h->sx8, h->qres8.as<uint8_t>(),
                                    h->qresn.as<float>(), items, bins, nranges, pq, pp, np, qoff, qthr, cand, nmax, act,
-                                   any_at);
+                                   any_at, wk);
             };
             if (tune_get("ivfl2.fp8_ahead") == 3)
                 go(std::integral_constant<int, 3>{});
@@ -2434,7 +2459,10 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
             StageScope sc(st, STAGE_OTHER);
             hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)std::min<int64_t>(1024, ceil_div(nq, (int64_t)256))),
                                dim3(256), 0, st, qthr, nq, kKey32None);
-            seg_bounds(0, qthr, h->pact.as<uint32_t>(), 0);
+            // group words per (item, query tile), then one "any group needed" word per (item, tile)
+            uint32_t *pa = (uint32_t *)h->pact.ensure((size_t)(any_at + max_items * 4) * 4);
+            seg_bounds(0, qthr, pa, any_at);
+            if (prune) seg_bounds(1, qthr, pa, any_at);
         }
         scan(qthr);
     } else if (k2_pass || (np > 1 && k <= kIvfKP && margin_pct > 0)) {
@@ -2550,6 +2578,22 @@ extern "C" pn_status pn_ivfl2_set(pn_ivfl2 *h, const float *rows, int64_t n, int
         PN_HIP(hipMemcpyAsync(dX, rows, (size_t)n * d * 4, hipMemcpyHostToDevice, st));
         PN_HIP(hipMemcpyAsync(dC, centroids, (size_t)nc * d * 4, hipMemcpyHostToDevice, st));
         ivfl2_set_device(h, dX, n, d, dC, nc, labels, st);
+    });
+}
+
+extern "C" pn_status pn_ivfl2_scan_work(pn_ivfl2 *h, int64_t *mfma_ops, int64_t *row_bytes) {
+    return guard([&] {
+        PN_REQUIRE(h && mfma_ops && row_bytes, PN_EINVAL, "null pointer");
+        DeviceScope ds(h->device);
+        std::vector<uint64_t> v((size_t)h->work_n * 2, 0);
+        if (h->work_n) PN_HIP(hipMemcpy(v.data(), h->work.p, v.size() * 8, hipMemcpyDeviceToHost));
+        int64_t m = 0, b = 0;
+        for (int64_t i = 0; i < h->work_n; ++i) {
+            m += (int64_t)v[2 * i];
+            b += (int64_t)v[2 * i + 1];
+        }
+        *mfma_ops = m;
+        *row_bytes = b;
     });
 }
 
