@@ -1529,7 +1529,7 @@ __device__ __forceinline__ float row_sq_dist16(const float *__restrict__ qrow, c
 // reads four rows per step in 256-byte pieces), then the k smallest (distance, id) are written
 // ascending. X holds the rows in cluster order (a candidate's position indexes it; lids maps a
 // position to its id), so the rows that queries of one cluster share are near each other.
-template <int PER, bool VEC>
+template <bool VEC>
 __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restrict__ X, int64_t d,
                                                            const int32_t *__restrict__ lids, const float *__restrict__ Q,
                                                            int64_t qstride, const uint64_t *__restrict__ cand, int np,
@@ -1537,68 +1537,45 @@ __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restri
                                                            int64_t nq, int k, int32_t *__restrict__ out_idx,
                                                            float *__restrict__ out_dist, int32_t *__restrict__ out_cnt,
                                                            const int32_t *__restrict__ qorder, int64_t per_xcd, int k2) {
-    __shared__ uint64_t s_sort[4][256];
     const int lane = threadIdx.x & 63;
     // workgroup b runs on XCD b % 8: each XCD takes a contiguous run of qorder
     const int64_t pos = ((int64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds) * 4 + (threadIdx.x >> 6);
     if (pos >= nq) return;
     const int64_t q = qorder[pos];
-    uint64_t L[PER]; // this lane's candidates, sorted
-#pragma unroll
-    for (int j = 0; j < PER; ++j) L[j] = kKey64None;
     const uint64_t *src = cand + cand_slot(qoff, q, np, 0) * cap;
-    const int ncand = (qoff ? qoff[q + 1] - qoff[q] : np) * cap;
-    // bound: when every lane holds a candidate, the largest of the 64 lane minima has at least
-    // K2 = 64 candidates at or below it, so nothing above it is among the K2 best
-    uint64_t lmin = kKey64None;
-    for (int e = lane; e < ncand; e += 64) lmin = min(lmin, src[e]);
-    uint64_t bound = lmin;
+    const int npair = qoff ? qoff[q + 1] - qoff[q] : np;
+    // Each (query, probe) slot holds its candidates ascending, empty entries last (the scans merge
+    // them so). The query's 64 best so far are held across the wave, lane r holding the r-th
+    // (`mine`, ascending over lanes); a slot whose first entry is below the current k2-th best is
+    // merged in: its first min(cap, 64) entries, reversed across the lanes, against `mine` -- the
+    // lane-wise minimum is a bitonic sequence of the 64 smallest of both, which six compare-exchange
+    // stages sort. Slots that the scan's bounds left empty cost one read.
+    uint64_t mine = kKey64None;
+    const int mc = min(cap, 64);
+    auto merge = [&](uint64_t b) { // b: lane j holds entry j of a slot (ascending)
+        const uint64_t rb = __shfl(b, 63 - lane, 64);
+        uint64_t c = rb < mine ? rb : mine;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const uint64_t o = __shfl_xor(bound, off, 64);
-        bound = o > bound ? o : bound; // (kKey64None when a lane has none: no bound)
-    }
-    for (int e = lane; e < ncand; e += 64) {
-        const uint64_t v = src[e];
-        if (v <= bound && v < L[PER - 1]) list_insert_u64<PER>(L, v);
-    }
-    uint64_t mine = kKey64None; // lane r takes the r-th best candidate
-    // at most 256 held in all: packed into the wave's LDS and sorted by one bitonic network; else
-    // K2 rounds of a wave-wide minimum
-    int held = 0;
-#pragma unroll
-    for (int j = 0; j < PER; ++j) held += L[j] != kKey64None ? 1 : 0;
-    int off = held;
-#pragma unroll
-    for (int dd = 1; dd < 64; dd <<= 1) {
-        const int t = __shfl_up(off, dd, 64);
-        if (lane >= dd) off += t;
-    }
-    const int total = __shfl(off, 63, 64);
-    off -= held;
-    if (total <= 256) {
-        uint64_t *wb = s_sort[threadIdx.x >> 6];
-#pragma unroll
-        for (int j = 0; j < PER; ++j)
-            if (j < held) wb[off + j] = L[j];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        uint64_t v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = lane + 64 * u < total ? wb[lane + 64 * u] : kKey64None;
-        wave_sort_u64<4, 2>(v, lane);
-        mine = v[0];
-    } else {
-        for (int r = 0; r < k2; ++r) {
-            const uint64_t m = wmin64(L[0]);
-            if (m == kKey64None) break;
-            if (L[0] == m) {
-#pragma unroll
-                for (int j = 0; j < PER - 1; ++j) L[j] = L[j + 1];
-                L[PER - 1] = kKey64None;
-            }
-            if (lane == r) mine = m;
+        for (int j = 32; j > 0; j >>= 1) {
+            const uint64_t o = __shfl_xor(c, j, 64);
+            const bool lower = (lane & j) == 0;
+            c = lower ? (o < c ? o : c) : (o < c ? c : o);
+        }
+        mine = c;
+    };
+    for (int p0 = 0; p0 < npair; p0 += 64) {
+        const uint64_t first = p0 + lane < npair ? src[(int64_t)(p0 + lane) * cap] : kKey64None;
+        uint64_t bal = __ballot(first != kKey64None);
+        // two slots per round: both loads in flight before the merges
+        while (bal) {
+            const int pa = __builtin_ctzll(bal);
+            bal &= bal - 1;
+            const int pb = bal ? __builtin_ctzll(bal) : -1;
+            if (pb >= 0) bal &= bal - 1;
+            const uint64_t ba = lane < mc ? src[(int64_t)(p0 + pa) * cap + lane] : kKey64None;
+            const uint64_t bb = pb >= 0 && lane < mc ? src[(int64_t)(p0 + pb) * cap + lane] : kKey64None;
+            if (__shfl(first, pa, 64) < __shfl(mine, k2 - 1, 64)) merge(ba);
+            if (pb >= 0 && __shfl(first, pb, 64) < __shfl(mine, k2 - 1, 64)) merge(bb);
         }
     }
     if (lane >= k2) mine = kKey64None; // (the k2 best are reranked)
@@ -2486,7 +2463,6 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     }
     {
         StageScope sc(st, STAGE_RERANK);
-        const int ncand = np * cap;
         const bool vec = d % 4 == 0 && re % 4 == 0;
         // queries by first probed list
         int32_t *qkey = (int32_t *)h->qkey.ensure((size_t)nq * 4);
@@ -2497,21 +2473,12 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         hipLaunchKernelGGL(order_scatter_kernel, dim3(gq), dim3(256), 0, st, qkey, nq, ohist, qorder);
         const int64_t per_xcd = ceil_div(ceil_div(nq, (int64_t)4), (int64_t)kXcds);
         dim3 g((unsigned)(per_xcd * kXcds));
-        auto launch = [&](auto per, auto v) {
-            hipLaunchKernelGGL((ivfl2_rerank_kernel<decltype(per)::value, decltype(v)::value>), g, dim3(256), 0, st,
-                               h->X.as<float>(), d, h->lids.as<int32_t>(), Q, re, cand, np, cap, qoff, nq, k, oi, od,
-                               oc, qorder, per_xcd, k2);
+        auto launch = [&](auto v) {
+            hipLaunchKernelGGL((ivfl2_rerank_kernel<decltype(v)::value>), g, dim3(256), 0, st, h->X.as<float>(), d,
+                               h->lids.as<int32_t>(), Q, re, cand, np, cap, qoff, nq, k, oi, od, oc, qorder, per_xcd,
+                               k2);
         };
-        using T = std::true_type;
-        using F = std::false_type;
-        if (ncand <= 64 * 8)
-            vec ? launch(std::integral_constant<int, 8>{}, T{}) : launch(std::integral_constant<int, 8>{}, F{});
-        else if (ncand <= 64 * 16)
-            vec ? launch(std::integral_constant<int, 16>{}, T{}) : launch(std::integral_constant<int, 16>{}, F{});
-        else if (ncand <= 64 * 32)
-            vec ? launch(std::integral_constant<int, 32>{}, T{}) : launch(std::integral_constant<int, 32>{}, F{});
-        else // a lane may hold all K2 = 64 best
-            vec ? launch(std::integral_constant<int, 64>{}, T{}) : launch(std::integral_constant<int, 64>{}, F{});
+        vec ? launch(std::true_type{}) : launch(std::false_type{});
         PN_HIP(hipGetLastError());
     }
 }
