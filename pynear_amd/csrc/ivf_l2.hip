@@ -66,6 +66,7 @@ struct pn_ivfl2 {
     pn::DevBuf qpad, qsplit_h, qsplit_l, qn, probes, pdist, lcount, cursor, pair_q, pair_p, pair_c, pslot;
     pn::DevBuf items, ntot /* item ranges */, itoff, woff, cand, qoff, qthr, qres8, qresn, qkey, qorder, ohist, oidx,
         odist, ocnt, qres8n, pact;
+    bool ohist_dirty = false; // the rerank order's histogram may be non-zero (a search stopped early)
     pn::DevBuf work;         // the last fp8 scan launch: per workgroup MFMA instructions and fetched row bytes
     int64_t work_n = 0;      // its workgroups
 };
@@ -130,11 +131,10 @@ __global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restr
                                                           const int64_t *__restrict__ loff, int nl,
                                                           int32_t *__restrict__ cursor, int32_t *__restrict__ itoff,
                                                           int64_t *__restrict__ woff, int32_t *__restrict__ bins,
-                                                          int nranges, int iq, int32_t *__restrict__ zero, int nzero) {
+                                                          int nranges, int iq, int32_t *__restrict__ ohist, int nh) {
     __shared__ int32_t s_pairs[256], s_items[256];
     __shared__ int64_t s_work[256];
     const int tid = threadIdx.x;
-    for (int i = tid; i < nzero; i += 256) zero[i] = 0; // (the rerank's histogram, cleared here for free)
     const int per = (nl + 255) / 256;
     const int o0 = min(nl, tid * per), o1 = min(nl, o0 + per);
     int pairs = 0, nit = 0;
@@ -182,6 +182,26 @@ __global__ __launch_bounds__(256) void build_items_kernel(const int32_t *__restr
         bins[3 * kXcds + 1] = s_pairs[255];
         *(int64_t *)(bins + 3 * kXcds + 2) = s_work[255];
     }
+    if (ohist) { // the rerank's query order: exclusive scan of the first-probe histogram, in place
+        __syncthreads();
+        const int hp = (nh + 255) / 256, h0 = min(nh, tid * hp), h1 = min(nh, h0 + hp);
+        int sum = 0;
+        for (int o = h0; o < h1; ++o) sum += ohist[o];
+        s_pairs[tid] = sum;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const int a = tid >= off ? s_pairs[tid - off] : 0;
+            __syncthreads();
+            s_pairs[tid] += a;
+            __syncthreads();
+        }
+        int run = s_pairs[tid] - sum;
+        for (int o = h0; o < h1; ++o) {
+            const int v = ohist[o];
+            ohist[o] = run;
+            run += v;
+        }
+    }
 }
 
 // One thread per item g: its cluster (the last corder position o with itoff[o] <= g), its pairs
@@ -190,7 +210,16 @@ __global__ void expand_items_kernel(const int32_t *__restrict__ lcount, const in
                                     const int64_t *__restrict__ loff, int nl, const int32_t *__restrict__ cursor,
                                     const int32_t *__restrict__ itoff, const int64_t *__restrict__ woff,
                                     int64_t max_items, int nranges, int iq, int4 *__restrict__ items,
-                                    int32_t *__restrict__ bins) {
+                                    int32_t *__restrict__ bins, int eblocks, const int32_t *__restrict__ qkey,
+                                    int64_t nq, int32_t *__restrict__ ocur, int32_t *__restrict__ qorder) {
+    if ((int)blockIdx.x >= eblocks) { // the rerank's query order: queries by first probe
+        const int64_t q = (int64_t)(blockIdx.x - eblocks) * blockDim.x + threadIdx.x;
+        if (q < nq) {
+            const int at = atomicAdd(&ocur[qkey[q]], 1);
+            if (at < nq) qorder[at] = (int32_t)q;
+        }
+        return;
+    }
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= max_items || g >= bins[3 * kXcds]) return;
     int lo = 0, hi = nl - 1;
@@ -354,11 +383,6 @@ __device__ __forceinline__ int take_item(int32_t *__restrict__ bins, int nranges
     return -1;
 }
 
-// Fills n words with v (grid-stride).
-__global__ void fill_u32_kernel(uint32_t *__restrict__ p, int64_t n, uint32_t v) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = v;
-}
 
 // Candidate slots of (query, probe) pairs without a probe stay empty.
 __global__ void fill_u64_kernel(uint64_t *__restrict__ p, int64_t n, uint64_t v) {
@@ -1444,21 +1468,6 @@ __device__ __forceinline__ uint64_t wmin64(uint64_t v) {
 // Rerank order: queries grouped by their first probed list (a counting sort), so that queries
 // sharing candidate rows are reranked together. key = the list of the first valid probe slot
 // (nl: none).
-__global__ void order_count_kernel(const int32_t *__restrict__ probes, int64_t nq, int np, int nl,
-                                   int32_t *__restrict__ key, int32_t *__restrict__ hist) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
-    int c = nl;
-    for (int j = 0; j < np; ++j) {
-        const int v = probes[q * np + j];
-        if (v >= 0) {
-            c = v;
-            break;
-        }
-    }
-    key[q] = c;
-    atomicAdd(&hist[c], 1);
-}
 
 // Exclusive scan of hist[0 .. m) in place (one workgroup of 256).
 __global__ __launch_bounds__(256) void order_scan_kernel(int32_t *__restrict__ hist, int m) {
@@ -1483,12 +1492,6 @@ __global__ __launch_bounds__(256) void order_scan_kernel(int32_t *__restrict__ h
     }
 }
 
-__global__ void order_scatter_kernel(const int32_t *__restrict__ key, int64_t nq, int32_t *__restrict__ cursor,
-                                     int32_t *__restrict__ qorder) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
-    qorder[atomicAdd(&cursor[key[q]], 1)] = (int32_t)q;
-}
 
 // sum (q - x)^2 over d dims by 16 lanes (c = lane % 16; every lane of the 16 gets the sum): with
 // VEC, lane c takes the float4 pieces 4 c, 4 c + 64, ... into four chains joined as (0+1)+(2+3);
@@ -1536,8 +1539,11 @@ __global__ __launch_bounds__(256) void ivfl2_rerank_kernel(const float *__restri
                                                            int cap, const int32_t *__restrict__ qoff,
                                                            int64_t nq, int k, int32_t *__restrict__ out_idx,
                                                            float *__restrict__ out_dist, int32_t *__restrict__ out_cnt,
-                                                           const int32_t *__restrict__ qorder, int64_t per_xcd, int k2) {
+                                                           const int32_t *__restrict__ qorder, int64_t per_xcd, int k2,
+                                                           int32_t *__restrict__ ohist, int nh) {
     const int lane = threadIdx.x & 63;
+    // the query order's histogram (read before this launch) is left zero for the next search
+    for (int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x; b < nh; b += (int64_t)gridDim.x * 256) ohist[b] = 0;
     // workgroup b runs on XCD b % 8: each XCD takes a contiguous run of qorder
     const int64_t pos = ((int64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds) * 4 + (threadIdx.x >> 6);
     if (pos >= nq) return;
@@ -1949,26 +1955,39 @@ static void ivfl2_set_device(pn_ivfl2 *h, const float *dX, int64_t n, int64_t d,
 static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t nq, int32_t *pr, int np,
                                 bool sparse, int k, int prec, int32_t *oi, float *od, int32_t *oc, hipStream_t st);
 
-// The first probe of each query becomes its nearest probed list of at least k2 rows (swapped
-// with slot 0): the bounded second pass of the fp8 scan takes each query's bound from the k2-th
-// candidate of its first probe, which a shorter list cannot give. The probe set is unchanged and
-// the rerank takes a query's candidates by key from all its probes, so the order of a query's
-// probes changes no result.
-__global__ void first_probe_kernel(int32_t *__restrict__ pr, int64_t nq, int np, const int64_t *__restrict__ loff,
-                                   int k2) {
+// Per query, before the list scan: with `swap`, its first probe becomes its nearest probed list
+// of at least k2 rows (swapped with slot 0: the bounded fp8 scan takes each query's threshold from
+// its first probe, which a shorter list cannot give; the probe set is unchanged and the rerank
+// takes a query's candidates by key from all its probes, so the order of a query's probes changes
+// no result); qthr (if given) starts at kKey32None (no bound); and the query's first probe (nl for
+// none) goes to qkey and is counted in ohist, for the rerank's query order. ohist is zero on entry
+// (the rerank kernel clears it after use).
+__global__ void query_prep_kernel(int32_t *__restrict__ pr, int64_t nq, int np, const int64_t *__restrict__ loff,
+                                  int k2, int swap, uint32_t *__restrict__ qthr, int32_t *__restrict__ qkey,
+                                  int32_t *__restrict__ ohist, int nl) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
     int32_t *p = pr + q * np;
-    for (int j = 0; j < np; ++j) {
-        const int c = p[j];
-        if (c >= 0 && loff[c + 1] - loff[c] >= k2) {
-            if (j > 0) {
-                p[j] = p[0];
-                p[0] = c;
+    if (swap)
+        for (int j = 0; j < np; ++j) {
+            const int c = p[j];
+            if (c >= 0 && loff[c + 1] - loff[c] >= k2) {
+                if (j > 0) {
+                    p[j] = p[0];
+                    p[0] = c;
+                }
+                break;
             }
-            return;
         }
-    }
+    if (qthr) qthr[q] = kKey32None;
+    int key = nl;
+    for (int j = 0; j < np; ++j)
+        if (p[j] >= 0) {
+            key = p[j];
+            break;
+        }
+    qkey[q] = key;
+    atomicAdd(&ohist[key], 1);
 }
 
 static void sub_partition_order(const float *dX, int64_t n, int64_t d, const std::vector<int64_t> &off,
@@ -2234,8 +2253,8 @@ static void ivfl2_search_enqueue(pn_ivfl2 *h, const float *dq, int64_t qstride_i
 
 // The list scan and rerank for given probes pr[nq x np] (list indices; with `sparse`, -1 marks an
 // empty slot). Q has row stride re.
-// pr: the handle's own probe buffer (list indices), reordered per query by first_probe_kernel in
-// the bounded two-pass fp8 scan.
+// pr: the handle's own probe buffer (list indices), reordered per query by query_prep_kernel for
+// the bounded fp8 scans.
 static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t nq, int32_t *pr, int np,
                                 bool sparse, int k, int prec, int32_t *oi, float *od, int32_t *oc, hipStream_t st) {
     if (k > kIvfK2) { // (host entry point only: the device ones take k <= 64)
@@ -2254,8 +2273,16 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     const bool lc_new = !h->lcount.p || h->lcount.bytes < (size_t)h->nl * 4;
     int32_t *lc = (int32_t *)h->lcount.ensure((size_t)h->nl * 4);
     if (lc_new) PN_HIP(hipMemsetAsync(lc, 0, (size_t)h->nl * 4, st));
-    // the rerank's histogram of first probes (cleared by build_items_kernel)
+    // the rerank's query order: a histogram of first probes (zero between searches: the rerank
+    // kernel clears it after use, so a new allocation is cleared once here; a search that stopped on
+    // an error before its rerank leaves it dirty: cleared here then), keys and the order
+    if (prec == 8) PN_REQUIRE(dp <= 1152, PN_EINVAL, "fp8 precision serves d up to 1152");
+    const bool oh_new = !h->ohist.p || h->ohist.bytes < (size_t)(h->nl + 1) * 4;
     int32_t *ohist = (int32_t *)h->ohist.ensure((size_t)(h->nl + 1) * 4);
+    if (oh_new || h->ohist_dirty) PN_HIP(hipMemsetAsync(ohist, 0, (size_t)(h->nl + 1) * 4, st));
+    h->ohist_dirty = true;
+    int32_t *qkey = (int32_t *)h->qkey.ensure((size_t)nq * 4);
+    int32_t *qorder = (int32_t *)h->qorder.ensure((size_t)nq * 4);
     int32_t *cur = (int32_t *)h->cursor.ensure((size_t)h->nl * 4);
     int32_t *pq = (int32_t *)h->pair_q.ensure((size_t)total * 4);
     int32_t *pp = (int32_t *)h->pair_p.ensure((size_t)total * 4);
@@ -2293,7 +2320,6 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         }
         if (prec != 8) l2_split_rows(Q, nq, d, re, dp, Qh, Ql, qn, st); // (the fp8 scan converts in LDS)
     }
-    // pairs of probe slots p_lo .. p_lo + npw - 1 -> items of at most 64 pairs of one cluster
     // candidates reranked per query: ivfl2.k2 (k' >= 4k is enough for the rerank to recover
     // the exact top-k in practice), at least k, at most kIvfK2 = 64 (one per lane)
     const int k2 = (int)std::min<int64_t>(kIvfK2, std::max<int64_t>({(int64_t)k, 4 * (int64_t)k, tune_get("ivfl2.k2")}));
@@ -2307,9 +2333,18 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     const bool prune = bounded && tune_get("ivfl2.prune") != 0;
     // else, fp8: optionally two passes, the first probe scanned first for the bound
     const bool k2_pass = !bounded && np > 1 && prec == 8 && cap >= k2 && tune_get("ivfl2.two_pass") != 0;
-    if (bounded || k2_pass)
-        hipLaunchKernelGGL(first_probe_kernel, dim3((unsigned)ceil_div(nq, (int64_t)256)), dim3(256), 0, st, pr, nq, np,
-                           h->loff.as<int64_t>(), k2);
+    uint32_t *qthr = bounded || k2_pass || (np > 1 && k <= kIvfKP && tune_get("ivfl2.pass_margin_pct") > 0)
+                         ? (uint32_t *)h->qthr.ensure((size_t)nq * 4)
+                         : nullptr;
+    {
+        StageScope sc(st, STAGE_OTHER);
+        hipLaunchKernelGGL(query_prep_kernel, dim3((unsigned)ceil_div(nq, (int64_t)256)), dim3(256), 0, st, pr, nq, np,
+                           h->loff.as<int64_t>(), k2, (bounded || k2_pass) ? 1 : 0, bounded ? qthr : nullptr, qkey,
+                           ohist, h->nl);
+    }
+    // pairs of probe slots p_lo .. p_lo + npw - 1 -> items of at most iq pairs of one cluster; the
+    // first call also finishes the rerank's query order
+    bool first_bucket = true;
     auto bucket = [&](int p_lo, int npw) {
         StageScope sc(st, STAGE_OTHER);
         const int64_t tw = nq * npw;
@@ -2317,10 +2352,14 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         hipLaunchKernelGGL(probe_count_kernel, dim3(pchunks), dim3(256), 0, st, pr, tw, np, p_lo, npw, h->nl, lc);
         const int iq = prec == 8 ? (p_lo > 0 ? iq_b : kIvf8IQ) : kIvfIQ; // queries per item
         hipLaunchKernelGGL(build_items_kernel, dim3(1), dim3(256), 0, st, lc, h->corder.as<int32_t>(),
-                           h->loff.as<int64_t>(), h->nl, cur, itoff, woff, bins, nranges, iq, ohist, h->nl + 1);
-        hipLaunchKernelGGL(expand_items_kernel, dim3((unsigned)ceil_div(max_items, (int64_t)256)), dim3(256), 0, st, lc,
+                           h->loff.as<int64_t>(), h->nl, cur, itoff, woff, bins, nranges, iq,
+                           first_bucket ? ohist : nullptr, h->nl + 1);
+        const int eblocks = (int)ceil_div(max_items, (int64_t)256);
+        const int oblocks = first_bucket ? (int)ceil_div(nq, (int64_t)256) : 0;
+        hipLaunchKernelGGL(expand_items_kernel, dim3((unsigned)(eblocks + oblocks)), dim3(256), 0, st, lc,
                            h->corder.as<int32_t>(), h->loff.as<int64_t>(), h->nl, cur, itoff, woff, max_items, nranges,
-                           iq, items, bins);
+                           iq, items, bins, eblocks, qkey, nq, ohist, qorder);
+        first_bucket = false;
         hipLaunchKernelGGL(probe_scatter_kernel, dim3(pchunks), dim3(256), 0, st, pr, tw, np, p_lo, npw, h->nl, cur, pq,
                            pp, pc, psl, lc);
         if (prec == 8) { // fp8 query residuals of every pair
@@ -2330,7 +2369,6 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
             float *Rq = bounded ? (float *)h->qres8n.ensure((size_t)total * 4) : nullptr;
             const dim3 gq((unsigned)ceil_div(nq, (int64_t)4));
             const float *lcp = h->lcent.as<float>();
-            PN_REQUIRE(dp <= 1152, PN_EINVAL, "fp8 precision serves d up to 1152");
             if (dp <= 256)
                 hipLaunchKernelGGL(query_residual_kernel<1>, gq, dim3(256), 0, st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq,
                                    np, p_lo, npw, psl, R8, Rn, Rq);
@@ -2430,12 +2468,9 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     // (pass_margin_pct > 0) the k-th approximation times a margin; otherwise one pass.
     const int64_t margin_pct = tune_get("ivfl2.pass_margin_pct");
     if (bounded) {
-        uint32_t *qthr = (uint32_t *)h->qthr.ensure((size_t)nq * 4);
         bucket(0, np);
         {
             StageScope sc(st, STAGE_OTHER);
-            hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)std::min<int64_t>(1024, ceil_div(nq, (int64_t)256))),
-                               dim3(256), 0, st, qthr, nq, kKey32None);
             // group words per (item, query tile), then one "any group needed" word per (item, tile)
             uint32_t *pa = (uint32_t *)h->pact.ensure((size_t)(any_at + max_items * 4) * 4);
             seg_bounds(0, qthr, pa, any_at);
@@ -2443,7 +2478,6 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
         }
         scan(qthr);
     } else if (k2_pass || (np > 1 && k <= kIvfKP && margin_pct > 0)) {
-        uint32_t *qthr = (uint32_t *)h->qthr.ensure((size_t)nq * 4);
         bucket(0, 1);
         scan(nullptr);
         {
@@ -2464,22 +2498,17 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
     {
         StageScope sc(st, STAGE_RERANK);
         const bool vec = d % 4 == 0 && re % 4 == 0;
-        // queries by first probed list
-        int32_t *qkey = (int32_t *)h->qkey.ensure((size_t)nq * 4);
-        int32_t *qorder = (int32_t *)h->qorder.ensure((size_t)nq * 4);
-        const unsigned gq = (unsigned)ceil_div(nq, (int64_t)256);
-        hipLaunchKernelGGL(order_count_kernel, dim3(gq), dim3(256), 0, st, pr, nq, np, h->nl, qkey, ohist);
-        hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(256), 0, st, ohist, h->nl + 1);
-        hipLaunchKernelGGL(order_scatter_kernel, dim3(gq), dim3(256), 0, st, qkey, nq, ohist, qorder);
+        // queries by first probed list (qorder, from the first bucketing)
         const int64_t per_xcd = ceil_div(ceil_div(nq, (int64_t)4), (int64_t)kXcds);
         dim3 g((unsigned)(per_xcd * kXcds));
         auto launch = [&](auto v) {
             hipLaunchKernelGGL((ivfl2_rerank_kernel<decltype(v)::value>), g, dim3(256), 0, st, h->X.as<float>(), d,
                                h->lids.as<int32_t>(), Q, re, cand, np, cap, qoff, nq, k, oi, od, oc, qorder, per_xcd,
-                               k2);
+                               k2, ohist, h->nl + 1);
         };
         vec ? launch(std::true_type{}) : launch(std::false_type{});
         PN_HIP(hipGetLastError());
+        h->ohist_dirty = false;
     }
 }
 
