@@ -42,12 +42,14 @@ struct BatchEntry {
     int64_t xstride, n, d;
     int dp, kk, KL, K2, K2c;
     int G; // 32-query groups per scan workgroup: 4, or 2 / 1 for at most 64 / 32 queries
+    uint32_t lowm; // row bits of the scan's keys-only lists (rows per slice <= lowm + 1)
     float eps_coef;
     const float *Q;
     int64_t qstride, nq, nq_pad, gx, S, rps;
     __bf16 *Qh, *Ql;
     float *qn;
     uint64_t *part;
+    uint32_t *gthr; // per padded query: the bound its slices share (null: not shared)
     int32_t *cidx, *flags;
     float *capx;
     int32_t *out_idx;
@@ -55,10 +57,33 @@ struct BatchEntry {
     int64_t row_base, q_base; // prefixes over the entries: padded query rows, queries
 };
 
+// The entry with every pointer marked as global memory (as_global): kernels take their entry
+// through this copy.
+__device__ __forceinline__ BatchEntry global_entry(const BatchEntry &src) {
+    BatchEntry e = src;
+    e.Xh = as_global(e.Xh);
+    e.Xl = as_global(e.Xl);
+    e.xn = as_global(e.xn);
+    e.X = as_global(e.X);
+    e.xn_max = as_global(e.xn_max);
+    e.Q = as_global(e.Q);
+    e.Qh = as_global(e.Qh);
+    e.Ql = as_global(e.Ql);
+    e.qn = as_global(e.qn);
+    e.part = as_global(e.part);
+    e.gthr = e.gthr ? as_global(e.gthr) : nullptr;
+    e.cidx = as_global(e.cidx);
+    e.flags = as_global(e.flags);
+    e.capx = as_global(e.capx);
+    e.out_idx = as_global(e.out_idx);
+    e.out_dist = as_global(e.out_dist);
+    return e;
+}
+
 struct BatchCfg {
     int slice_rows = 512; // target rows per slice (rounded to 32-row tiles)
     int wg_per_cu = 2;    // persistent scan workgroups per CU
-    int order = 1;        // 1: tiles by decreasing cost; 0: index order
+    int order = 1;        // 1: tiles by decreasing cost; 0: index order; 2: by slice, then by cost
 };
 
 __device__ __forceinline__ int find_entry(const int64_t *base, int n, int64_t x) {
@@ -75,20 +100,21 @@ __global__ __launch_bounds__(256) void batch_split_kernel(const BatchEntry *__re
     const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (blockIdx.x == 0 && threadIdx.x < 3) counter[threadIdx.x] = 0; // the scan queues of this run
     if (r >= rows) return;
-    const BatchEntry &e = E[find_entry(row_base, ne, r)];
+    const BatchEntry e = global_entry(E[find_entry(row_base, ne, r)]);
     split_row(e.Q, e.nq, e.d, e.qstride, e.dp, e.Qh, e.Ql, e.qn, r - e.row_base);
+    if (e.gthr && (threadIdx.x & 63) == 0) e.gthr[r - e.row_base] = kKey32None;
 }
 
 template <int KL>
 __device__ __forceinline__ void batch_tile(uint64_t *lds_raw, const BatchEntry &e, int64_t qb, int64_t s) {
     auto lds = reinterpret_cast<uint64_t (*)[32][KL]>(lds_raw);
     switch (e.dp) {
-    case 16: scan_small<KL, 3, 1>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G); break;
-    case 32: scan_small<KL, 3, 2>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G); break;
-    case 48: scan_small<KL, 3, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G); break;
-    case 64: scan_small<KL, 3, 4>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G); break;
+    case 16: scan_small<KL, 3, 1, true>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G, e.gthr); break;
+    case 32: scan_small<KL, 3, 2, true>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G, e.gthr); break;
+    case 48: scan_small<KL, 3, 3, true>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G, e.gthr); break;
+    case 64: scan_small<KL, 3, 4, true>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G, e.gthr); break;
     default:
-        scan_generic<KL, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.dp, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s);
+        scan_generic<KL, 3, true>(lds, e.Xh, e.Xl, e.xn, e.n, e.dp, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.gthr);
     }
 }
 
@@ -98,11 +124,11 @@ __device__ __forceinline__ void batch_tile_wide(uint64_t *lds_raw, __bf16 *stage
                                                 int64_t s) {
     auto lds = reinterpret_cast<uint64_t (*)[32][KL]>(lds_raw);
     if (e.dp == 128)
-        scan_wide<KL, 3, 2>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s);
+        scan_wide<KL, 3, 2, true>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.gthr);
     else if (e.dp == 192)
-        scan_wide<KL, 3, 3>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s);
+        scan_wide<KL, 3, 3, true>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.gthr);
     else
-        scan_wide<KL, 3, 4>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s);
+        scan_wide<KL, 3, 4, true>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.gthr);
 }
 
 // Wide-row tiles of indices with at most 64 queries: the waves scan different slices.
@@ -110,11 +136,11 @@ template <int KL>
 __device__ __forceinline__ void batch_tile_wide_direct(uint64_t *lds_raw, const BatchEntry &e, int64_t qb, int64_t s) {
     auto lds = reinterpret_cast<uint64_t (*)[32][KL]>(lds_raw);
     if (e.dp == 128)
-        scan_wide_direct<KL, 3, 2>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G);
+        scan_wide_direct<KL, 3, 2, true>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G, e.gthr);
     else if (e.dp == 192)
-        scan_wide_direct<KL, 3, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G);
+        scan_wide_direct<KL, 3, 3, true>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G, e.gthr);
     else
-        scan_wide_direct<KL, 3, 4>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G);
+        scan_wide_direct<KL, 3, 4, true>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G, e.gthr);
 }
 
 static bool wide_dp(int dp) { return dp == 128 || dp == 192 || dp == 256; }
@@ -136,7 +162,7 @@ __global__ __launch_bounds__(256) void batch_scan_kernel(const BatchEntry *__res
         const int64_t slot = s_slot;
         if (slot >= ntiles) break;
         const int2 t = tiles[slot];
-        const BatchEntry &e = E[t.x];
+        const BatchEntry e = global_entry(E[t.x]);
         const int64_t qb = t.y / e.S, s = t.y % e.S;
         if (CLS == 1) {
             if (e.KL == 8) batch_tile_wide<8>(lds_raw, stage, e, qb, s);
@@ -166,7 +192,7 @@ __global__ __launch_bounds__(256) void batch_merge_kernel(const BatchEntry *__re
     const int64_t i = xcd_contiguous_block() * 4 + (threadIdx.x >> 6);
     if (i >= count) return;
     const int64_t g = gids[i];
-    const BatchEntry &e = E[find_entry(q_base, ne, g)];
+    const BatchEntry e = global_entry(E[find_entry(q_base, ne, g)]);
     uint64_t unused;
     merge_wave_query<K>(e.part, e.S, e.KL, e.S * e.KL, 1, g - e.q_base, e.K2c, OUT_F32, e.cidx, e.capx, wbuf, unused);
 }
@@ -178,7 +204,7 @@ __global__ __launch_bounds__(256) void batch_merge_lean_kernel(const BatchEntry 
     __shared__ uint64_t wbuf_all[4][256];
     const int64_t g = xcd_contiguous_block() * 4 + (threadIdx.x >> 6);
     if (g >= nq) return;
-    const BatchEntry &e = E[find_entry(q_base, ne, g)];
+    const BatchEntry e = global_entry(E[find_entry(q_base, ne, g)]);
     uint64_t unused;
     merge_wave_lean(e.part, e.S, e.KL, e.S * e.KL, 1, g - e.q_base, e.K2c, OUT_F32, e.cidx, e.capx,
                     wbuf_all[threadIdx.x >> 6], rounds_only, unused);
@@ -191,13 +217,13 @@ __global__ __launch_bounds__(256) void batch_merge_rerank_kernel(const BatchEntr
     __shared__ uint64_t wbuf_all[4][256];
     const int64_t g = xcd_contiguous_block() * 4 + (threadIdx.x >> 6);
     if (g >= nq) return;
-    const BatchEntry &e = E[find_entry(q_base, ne, g)];
+    const BatchEntry e = global_entry(E[find_entry(q_base, ne, g)]);
     const int64_t q = g - e.q_base;
     uint64_t mine;
     merge_wave_lean<true>(e.part, e.S, e.KL, e.S * e.KL, 1, q, e.K2c, OUT_F32, nullptr, nullptr,
                           wbuf_all[threadIdx.x >> 6], rounds_only, mine);
     rerank_regs<false>(e.X, e.xstride, e.d, e.Q, e.qstride, e.qn, e.xn_max, mine, e.K2c, e.part, e.S, e.KL,
-                       e.S * e.KL, e.kk, e.n, e.eps_coef, e.out_idx, e.out_dist, e.flags, q);
+                       e.S * e.KL, e.kk, e.n, e.eps_coef, e.out_idx, e.out_dist, e.flags, q, e.lowm);
 }
 
 // a lane that merges one slice holds at most KL keys; with more slices per lane it needs K2c
@@ -207,9 +233,9 @@ __global__ __launch_bounds__(256) void batch_rerank_kernel(const BatchEntry *__r
                                                            const int64_t *__restrict__ q_base, int ne, int64_t nq) {
     const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (g >= nq) return;
-    const BatchEntry &e = E[find_entry(q_base, ne, g)];
+    const BatchEntry e = global_entry(E[find_entry(q_base, ne, g)]);
     rerank_query<false>(e.X, e.xstride, e.d, e.Q, e.qstride, e.qn, e.xn_max, e.cidx, e.capx, e.K2c, e.part, e.S, e.KL,
-                 e.S * e.KL, e.kk, e.n, e.eps_coef, e.out_idx, e.out_dist, e.flags, g - e.q_base);
+                 e.S * e.KL, e.kk, e.n, e.eps_coef, e.out_idx, e.out_dist, e.flags, g - e.q_base, e.lowm);
 }
 
 // One workgroup per query: flagged queries run the exact scan in parallel, the others exit at once.
@@ -217,7 +243,7 @@ __global__ __launch_bounds__(256) void batch_fallback_kernel(const BatchEntry *_
                                                              const int64_t *__restrict__ q_base, int ne, int64_t nq) {
     __shared__ uint64_t wres[4][32];
     for (int64_t g = blockIdx.x; g < nq; g += gridDim.x) { // (the grid is capped: kFallbackMaxGrid)
-        const BatchEntry &e = E[find_entry(q_base, ne, g)];
+        const BatchEntry e = global_entry(E[find_entry(q_base, ne, g)]);
         const int64_t q = g - e.q_base;
         if (e.flags[q])
             fallback_query<32>(wres, e.X, e.xstride, e.n, e.d, e.Q, e.qstride, e.kk, e.out_idx, e.out_dist, q);
@@ -277,11 +303,17 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
         bytes += (n + 255) / 256 * 256;
         return o;
     };
-    struct Off { size_t qh, ql, qn, part, cidx, capx, flags; };
+    struct Off { size_t qh, ql, qn, part, gthr, cidx, capx, flags; };
     std::vector<Off> O(ne);
     int64_t rows = 0, queries = 0;
-    std::vector<std::pair<double, int2>> tc[3]; // tiles of each scan class
+    struct TileRec {
+        double cost;
+        int64_t round; // the tile's slice (or group of slices) within its query block
+        int2 t;
+    };
+    std::vector<TileRec> tc[3]; // tiles of each scan class
     const bool wide_on = tune_get("l2.wide") != 0;
+    const bool share = tune_get("batch.share") != 0;
     for (int i = 0; i < ne; ++i) {
         pn_exact *h = b->idx[i];
         BatchEntry &e = E[i];
@@ -298,7 +330,10 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
         e.qstride = e.d;
         e.gx = ceil_div(e.nq, (int64_t)128);
         e.nq_pad = e.gx * 128;
-        int64_t S = std::max<int64_t>(1, ceil_div(e.n, (int64_t)std::max(32, cfg.slice_rows)));
+        // slices of at most 4096 rows: a keys-only list key keeps at most 12 row bits, so at least
+        // 19 bits of the approximation (a relative step of at most 2^-11; a coarser key only
+        // widens the rerank's candidate filter and its completeness bound)
+        int64_t S = std::max<int64_t>(1, ceil_div(e.n, (int64_t)std::min(4096, std::max(32, cfg.slice_rows))));
         // few queries: fewer 32-query groups per workgroup and its waves on consecutive slices, so
         // no wave multiplies padding queries (S rounded up to whole groups of 4 / G slices; the
         // trailing ones may hold no rows)
@@ -307,6 +342,9 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
         S = ceil_div(S, m) * m;
         e.rps = ceil_div(ceil_div(e.n, S), (int64_t)32) * 32;
         e.S = e.G == 4 ? ceil_div(e.n, e.rps) : S;
+        int rb = 5;
+        while ((int64_t)1 << rb < e.rps) ++rb;
+        e.lowm = (1u << rb) - 1u;
         e.row_base = rows;
         e.q_base = queries;
         rbase[i] = rows;
@@ -317,6 +355,7 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
         O[i].ql = take((size_t)e.nq_pad * e.dp * 2);
         O[i].qn = take((size_t)e.nq_pad * 4);
         O[i].part = take((size_t)e.S * e.KL * e.nq_pad * 8);
+        O[i].gthr = take((size_t)e.nq_pad * 4);
         O[i].cidx = take((size_t)e.nq * e.K2c * 4);
         O[i].capx = take((size_t)e.nq * e.K2c * 4);
         O[i].flags = take((size_t)e.nq * 4);
@@ -324,12 +363,19 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
         const int cls = wide_on && wide_dp(e.dp) ? (e.G == 4 ? 1 : 2) : 0;
         // tiles: (query block, slice) for G = 4; (the one query block, group of 4 / G slices) else
         const int64_t ntile = e.G == 4 ? e.gx * e.S : e.S / m;
-        for (int64_t t = 0; t < ntile; ++t) tc[cls].push_back({cost, make_int2(i, (int)t)});
+        for (int64_t t = 0; t < ntile; ++t) tc[cls].push_back({cost, e.G == 4 ? t % e.S : t, make_int2(i, (int)t)});
     }
-    std::vector<std::pair<double, int2>> tl;
+    // order 1: longest tiles first. Order 2: by slice, then longest first -- a query block's
+    // slices run in different rounds of the persistent grid, and the later ones start from the
+    // bound the earlier ones published.
+    std::vector<TileRec> tl;
     for (int c = 0; c < 3; ++c) {
         if (cfg.order == 1)
-            std::stable_sort(tc[c].begin(), tc[c].end(), [](const auto &a, const auto &x) { return a.first > x.first; });
+            std::stable_sort(tc[c].begin(), tc[c].end(), [](const auto &a, const auto &x) { return a.cost > x.cost; });
+        else if (cfg.order == 2)
+            std::stable_sort(tc[c].begin(), tc[c].end(), [](const auto &a, const auto &x) {
+                return a.round != x.round ? a.round < x.round : a.cost > x.cost;
+            });
         tl.insert(tl.end(), tc[c].begin(), tc[c].end());
     }
     char *base = (char *)b->scratch.ensure(bytes);
@@ -346,6 +392,8 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
         e.Ql = (__bf16 *)(base + O[i].ql);
         e.qn = (float *)(base + O[i].qn);
         e.part = (uint64_t *)(base + O[i].part);
+        // slices of one query share their bound only when there are several
+        e.gthr = share && e.S > 1 ? (uint32_t *)(base + O[i].gthr) : nullptr;
         e.cidx = (int32_t *)(base + O[i].cidx);
         e.capx = (float *)(base + O[i].capx);
         e.flags = (int32_t *)(base + O[i].flags);
@@ -353,7 +401,7 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
         e.out_dist = b->od[i];
     }
     std::vector<int2> tiles(tl.size());
-    for (size_t t = 0; t < tl.size(); ++t) tiles[t] = tl[t].second;
+    for (size_t t = 0; t < tl.size(); ++t) tiles[t] = tl[t].t;
     b->ntiles = (int64_t)tiles.size();
     for (int c = 0; c < 3; ++c) b->ncls[c] = (int64_t)tc[c].size();
     b->rows = rows;
@@ -598,7 +646,7 @@ extern "C" pn_status pn_batch_tune(pn_batch *b, const char *table_path, int32_t 
         float best_ms = batch_time(b, std::max(1, (int)reps));
         for (int rows : {128, 256, 512, 1024, 2048, 4096})
             for (int w : {1, 2, 4})
-                for (int o : {0, 1}) {
+                for (int o : {0, 1, 2}) {
                     BatchCfg c{rows, w, o};
                     batch_plan(b, c);
                     const float ms = batch_time(b, std::max(1, (int)reps));

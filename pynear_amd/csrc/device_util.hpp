@@ -34,6 +34,15 @@ __device__ __forceinline__ int64_t xcd_contiguous_block() {
 // Grid size for xcd_contiguous_block: blocks rounded up to a multiple of 8.
 inline unsigned xcd_grid(int64_t blocks) { return (unsigned)((blocks + 7) / 8 * 8); }
 
+// A pointer loaded from device memory (a field of a work descriptor) is generic to the compiler,
+// whose flat loads and stores count against both the vector-memory and the LDS wait counters, so
+// an LDS read then waits for every global load in flight. Every pointer a descriptor holds points
+// to global memory: rebuilt through an integer as a global-address-space pointer, the accesses
+// through it compile to global loads and stores.
+template <class T> __device__ __forceinline__ T *as_global(T *p) {
+    return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
+}
+
 // Insert `key` into the ascending list L[0..K-1] (L[K-1] is dropped). With L sorted,
 // max(L[j-1], min(key, L[j])) == med3(L[j-1], key, L[j]), one v_med3_u32 per slot.
 template <int K> __device__ __forceinline__ void list_insert_u32(uint32_t (&L)[K], uint32_t key) {

@@ -48,46 +48,60 @@ __device__ __forceinline__ void split_row(const float *__restrict__ X, int64_t n
 
 
 // Keys of one 32 x 32 tile for this lane's query (rows t + 8g + 4 half + {0..3}, g < 4) into
-// the lane's sorted list (Lk, Li) of KL. A key is the approximation's float bits with the low 5
-// bits replaced by the row's offset in the tile: keys stay unique inside a tile, the minimum names
-// its row, and (key & ~31) is a lower bound of the approximation. Passing keys (below thr) are
-// inserted in ascending order; thr may come from the other half-wave's list of the same query:
-// rows at or above it cannot reach the KL best of the two lists together.
+// the lane's sorted list (Lk, Li) of KL. A key is the approximation's float bits with the low bits
+// (mask lowm) replaced by the row: keys stay unique, the minimum names its row, and (key & ~lowm)
+// is a lower bound of the approximation. With KO = false the low 5 bits hold the row's offset in
+// its tile and Li the rows; with KO ("keys only", slices of at most lowm + 1 rows) they hold the
+// row's offset in its slice, the row is r0 + (key & lowm), and the lists carry no rows at all, so
+// an insertion costs one three-way median per entry instead of five operations. Passing keys
+// (below thr) are inserted in ascending order; thr may come from the other half-wave's list of the
+// same query or from the query's other slices: rows at or above it cannot reach the KL best.
 // Merges 16 ascending keys b (of the tile at row t) into the ascending list (Lk, Li) of KL = 16 or
 // 32, keeping the KL smallest: the list followed by b reversed is bitonic, so one half-cleaner
 // step (entry i takes the smaller of Lk[i] and the key facing it) leaves the KL smallest as a
 // bitonic sequence, which a bitonic merge sorts. KL - 1 + KL log2(KL) / 2 compare-exchanges.
-template <int KL>
+template <int KL, bool KO = false>
 __device__ __forceinline__ void merge_sorted16(const uint32_t (&b)[16], uint32_t t, uint32_t (&Lk)[KL],
                                                uint32_t (&Li)[KL]) {
     static_assert(KL == 16 || KL == 32, "lists of 16 or 32 keys");
 #pragma unroll
     for (int i = KL - 16; i < KL; ++i) {
         const uint32_t bk = b[KL - 1 - i];
-        const bool take = bk < Lk[i];
-        Li[i] = take ? t + (bk & 31u) : Li[i];
-        Lk[i] = take ? bk : Lk[i];
+        if (KO) {
+            Lk[i] = min(bk, Lk[i]);
+        } else {
+            const bool take = bk < Lk[i];
+            Li[i] = take ? t + (bk & 31u) : Li[i];
+            Lk[i] = take ? bk : Lk[i];
+        }
     }
 #pragma unroll
     for (int s = KL / 2; s > 0; s >>= 1)
 #pragma unroll
         for (int i = 0; i < KL; ++i)
             if ((i & s) == 0) {
-                const bool sw = Lk[i + s] < Lk[i];
-                const uint32_t ka = Lk[i], kb = Lk[i + s], ia = Li[i], ib = Li[i + s];
-                Lk[i] = sw ? kb : ka;
-                Lk[i + s] = sw ? ka : kb;
-                Li[i] = sw ? ib : ia;
-                Li[i + s] = sw ? ia : ib;
+                const uint32_t ka = Lk[i], kb = Lk[i + s];
+                if (KO) {
+                    Lk[i] = min(ka, kb);
+                    Lk[i + s] = max(ka, kb);
+                } else {
+                    const bool sw = kb < ka;
+                    const uint32_t ia = Li[i], ib = Li[i + s];
+                    Lk[i] = sw ? kb : ka;
+                    Lk[i + s] = sw ? ka : kb;
+                    Li[i] = sw ? ib : ia;
+                    Li[i + s] = sw ? ia : ib;
+                }
             }
 }
 
 // NET: with many keys to insert, take the network path below (the LDS-staged wide scan; in the
 // other scans its registers would spill)
-template <int KL, bool MASK, bool FIRST, bool NET = false>
+// rb: the slice's first row (KO: rows are numbered from it in the keys); lowm: the row bits.
+template <int KL, bool MASK, bool FIRST, bool NET = false, bool KO = false>
 __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4 (&xv)[4], float qnv, int64_t t,
                                                  int64_t r1, int half, uint32_t (&Lk)[KL], uint32_t (&Li)[KL],
-                                                 uint32_t &thr) {
+                                                 uint32_t &thr, int64_t rb = 0, uint32_t lowm = 31u) {
     float v[16];
     float vmin = INFINITY;
 #pragma unroll
@@ -98,13 +112,14 @@ __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4
     }
     // every key of the tile is >= bits(vmin) with the offset bits cleared (masked rows only lower
     // vmin): a tile no lane can take from is done
-    if (!__any((f32_key(vmin) & ~31u) < thr)) return;
+    if (!__any((f32_key(vmin) & ~lowm) < thr)) return;
     uint32_t key[16];
     uint32_t m1 = kKey32None, m2 = kKey32None; // the lane's two smallest keys (FAST2)
+    const uint32_t tb = KO ? (uint32_t)(t - rb) : 0u; // the tile's first row in the key's row bits
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int off = 8 * (r >> 2) + 4 * half + (r & 3);
-        key[r] = !MASK || t + off < r1 ? ((f32_key(v[r]) & ~31u) | (uint32_t)off) : kKey32None;
+        key[r] = !MASK || t + off < r1 ? ((f32_key(v[r]) & ~lowm) | (tb + (uint32_t)off)) : kKey32None;
         m2 = min(m2, max(m1, key[r]));
         m1 = min(m1, key[r]);
     }
@@ -117,7 +132,7 @@ __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4
 #pragma unroll
         for (int j = 0; j < KL; ++j) {
             Lk[j] = j < 16 ? key[j] : kKey32None;
-            Li[j] = j < 16 ? (uint32_t)t + (key[j] & 31u) : 0u;
+            if (!KO) Li[j] = j < 16 ? (uint32_t)t + (key[j] & 31u) : 0u;
         }
         thr = min(thr, Lk[KL - 1]);
         return;
@@ -133,7 +148,7 @@ __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4
 #pragma unroll
             for (int r = 0; r < 16; ++r) key[r] = key[r] < thr ? key[r] : kKey32None;
             sort16(key);
-            merge_sorted16<KL>(key, (uint32_t)t, Lk, Li);
+            merge_sorted16<KL, KO>(key, (uint32_t)t, Lk, Li);
             thr = min(thr, Lk[KL - 1]);
             return;
         }
@@ -141,17 +156,21 @@ __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4
     // keys below thr go in ascending order (thr falls as the list fills), one round each among the
     // keys above the last taken (keys are unique); with FAST2 the two smallest go in first
     auto insert = [&](uint32_t mk) __attribute__((always_inline)) {
-        const uint32_t ir = (uint32_t)t + (mk & 31u);
+        if (KO) {
+            list_insert_u32<KL>(Lk, mk);
+        } else {
+            const uint32_t ir = (uint32_t)t + (mk & 31u);
 #pragma unroll
-        for (int j = KL - 1; j > 0; --j) {
-            const bool up = mk < Lk[j - 1];
-            const bool here = mk < Lk[j];
-            Li[j] = up ? Li[j - 1] : (here ? ir : Li[j]);
-            Lk[j] = max(Lk[j - 1], min(mk, Lk[j]));
-        }
-        if (mk < Lk[0]) {
-            Li[0] = ir;
-            Lk[0] = mk;
+            for (int j = KL - 1; j > 0; --j) {
+                const bool up = mk < Lk[j - 1];
+                const bool here = mk < Lk[j];
+                Li[j] = up ? Li[j - 1] : (here ? ir : Li[j]);
+                Lk[j] = max(Lk[j - 1], min(mk, Lk[j]));
+            }
+            if (mk < Lk[0]) {
+                Li[0] = ir;
+                Lk[0] = mk;
+            }
         }
         thr = min(thr, Lk[KL - 1]);
     };
@@ -176,13 +195,20 @@ __device__ __forceinline__ void tile_keys_insert(const f32x16 &acc, const float4
 // Half-wave merge: lanes l and l+32 hold lists of the same query; lane l writes the KL smallest
 // of both to out[j] (the query's list for this slice, contiguous). Keys a list dropped are >= its
 // last key >= the merged last key, so
-// the merged last key still bounds every dropped candidate from below.
-template <int KL>
+// the merged last key still bounds every dropped candidate from below. With gq (the query's shared
+// bound, see share_issue) a full merged list publishes its last key there.
+template <int KL, bool KO = false>
 __device__ __forceinline__ void half_merge_out(uint64_t (*lds)[KL], int half, const uint32_t (&Lk)[KL],
-                                               const uint32_t (&Li)[KL], uint64_t *out) {
+                                               const uint32_t (&Li)[KL], uint64_t *out, uint32_t *gq = nullptr,
+                                               int64_t rb = 0, uint32_t lowm = 31u) {
+    // list entry e of this lane as a 64-bit list key (approximation word, row)
+    auto entry = [&](int e) __attribute__((always_inline)) {
+        const uint32_t row = KO ? (uint32_t)rb + (Lk[e] & lowm) : Li[e];
+        return Lk[e] == kKey32None ? kKey64None : (((uint64_t)Lk[e] << 32) | row);
+    };
     if (half == 1) {
 #pragma unroll
-        for (int j = 0; j < KL; ++j) lds[0][j] = Lk[j] == kKey32None ? kKey64None : (((uint64_t)Lk[j] << 32) | Li[j]);
+        for (int j = 0; j < KL; ++j) lds[0][j] = entry(j);
     }
     __syncthreads();
     if (half == 0) {
@@ -192,22 +218,49 @@ __device__ __forceinline__ void half_merge_out(uint64_t (*lds)[KL], int half, co
             uint64_t va = kKey64None;
 #pragma unroll
             for (int e = 0; e < KL; ++e) // own list entry pa, selected without dynamic register indexing
-                if (e == pa) va = Lk[e] == kKey32None ? kKey64None : (((uint64_t)Lk[e] << 32) | Li[e]);
+                if (e == pa) va = entry(e);
             const uint64_t vb = pb < KL ? lds[0][pb] : kKey64None;
             const bool takea = va <= vb;
             out[j] = takea ? va : vb;
             pa += takea ? 1 : 0;
             pb += takea ? 0 : 1;
+            if (gq && j == KL - 1 && (takea ? va : vb) != kKey64None) atomicMin(gq, (uint32_t)((takea ? va : vb) >> 32));
         }
     }
 }
 
-template <int KL, int PREC>
+// Bound shared between the slices of one query: gthr[q] holds the smallest list bound any slice of
+// query q has reached (the KL-th key of a full list of the query's rows, so all of the query's KL
+// best keys lie below it). Every kShareTiles tiles a slice publishes its own bound there and takes
+// the smaller of the two: a slice then skips the rows that the query's other slices have already
+// beaten instead of filling its list with them. Dropped rows stay bounded by what the merge sees:
+// a published bound is at or above the final last key of the publishing slice's full list. The
+// atomic is issued before the tile's loads (its result is waited for after the products without
+// draining them) and only by the first half-wave; both halves take the result.
+constexpr int kShareTiles = 4;
+
+// Row bits of keys-only lists: the mask of the smallest power of two holding rows_per_slice rows
+// (at least the 5 bits of a tile offset).
+__device__ __forceinline__ uint32_t slice_row_mask(int64_t rows_per_slice) {
+    const uint32_t bits = 64 - __clzll((unsigned long long)max<int64_t>(rows_per_slice - 1, 31));
+    return (1u << bits) - 1u;
+}
+
+__device__ __forceinline__ uint32_t share_issue(uint32_t *gthr, int64_t q, int half, uint32_t thr) {
+    return half == 0 ? atomicMin(gthr + q, thr) : kKey32None;
+}
+
+__device__ __forceinline__ uint32_t share_take(uint32_t thr, uint32_t old) {
+    return min(thr, min(old, (uint32_t)__shfl_xor((int)old, 32, 64)));
+}
+
+template <int KL, int PREC, bool KO = false>
 __device__ __forceinline__ void scan_generic(uint64_t (*lds)[32][KL], const __bf16 *__restrict__ Xh,
                                              const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
                                              int dp, const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
                                              const float *__restrict__ qn, int64_t rows_per_slice,
-                                             uint64_t *__restrict__ part, int64_t qsl, int64_t qb, int64_t s) {
+                                             uint64_t *__restrict__ part, int64_t qsl, int64_t qb, int64_t s,
+                                             uint32_t *__restrict__ gthr = nullptr) {
     // A workgroup owns 128 queries (wave w: queries 32w..32w+31 of the block) and one slice of
     // rows; the four waves walk the same 32-row tiles, so each tile is fetched from L2 once per
     // workgroup and served to the other waves from L1. Row arrays are padded with zero rows to a
@@ -216,6 +269,7 @@ __device__ __forceinline__ void scan_generic(uint64_t (*lds)[32][KL], const __bf
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
     const int64_t q = qb * 128 + 32 * w + col; // query buffers are padded to whole 128-query blocks
     const int64_t r0 = s * rows_per_slice, r1 = min(n, r0 + rows_per_slice);
+    const uint32_t lowm = KO ? slice_row_mask(rows_per_slice) : 31u;
     const float qnv = qn[q];
     const __bf16 *qh = Qh + q * dp + 8 * half;
     const __bf16 *ql = Ql + q * dp + 8 * half;
@@ -229,6 +283,9 @@ __device__ __forceinline__ void scan_generic(uint64_t (*lds)[32][KL], const __bf
     uint32_t thr = kKey32None;
 
     for (int64_t t = r0; t < r1; t += 32) {
+        const bool share = gthr && (((t - r0) >> 5) % kShareTiles) == 0;
+        uint32_t old = kKey32None;
+        if (share) old = share_issue(gthr, q, half, thr);
         // |x|^2 of this lane's 16 output rows: rows t + 8g + 4 half + {0..3}
         float4 xv[4];
 #pragma unroll
@@ -238,27 +295,28 @@ __device__ __forceinline__ void scan_generic(uint64_t (*lds)[32][KL], const __bf
         f32x16 acc = {};
         mfma_tile_bf16<PREC, PREC == 3 ? 4 : 8>(acc, xh, xl, qh, ql, dp);
         thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64)); // the other half-wave's list of this query
+        if (share) thr = share_take(thr, old);
         if (t == r0) // only a slice's last tile can hold rows past its end
-            t + 32 <= r1 ? tile_keys_insert<KL, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
-                         : tile_keys_insert<KL, true, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+            t + 32 <= r1 ? tile_keys_insert<KL, false, true, false, KO>(acc, xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm)
+                         : tile_keys_insert<KL, true, true, false, KO>(acc, xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm);
         else
-            t + 32 <= r1 ? tile_keys_insert<KL, false, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
-                         : tile_keys_insert<KL, true, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+            t + 32 <= r1 ? tile_keys_insert<KL, false, false, false, KO>(acc, xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm)
+                         : tile_keys_insert<KL, true, false, false, KO>(acc, xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm);
     }
-    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL);
+    half_merge_out<KL, KO>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL, gthr ? gthr + q : nullptr, r0, lowm);
 }
 
 // The same scan for dp = 16 NS (NS <= 4): the lane's query fragments stay in registers for the
 // whole slice, and row fragments and norms are loaded two tiles ahead into a ring of three
 // register sets that rotate by name (three tiles per loop trip; a copy between sets would wait
 // for the loads in flight), so the memory latency hides behind the work of the two tiles before.
-template <int KL, int PREC, int NS>
+template <int KL, int PREC, int NS, bool KO = false>
 __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16 *__restrict__ Xh,
                                            const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
                                            const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
                                            const float *__restrict__ qn, int64_t rows_per_slice,
                                            uint64_t *__restrict__ part, int64_t qsl, int64_t qb, int64_t s,
-                                           int g = 4) {
+                                           int g = 4, uint32_t *__restrict__ gthr = nullptr) {
     constexpr int dp = 16 * NS;
     constexpr int NPL = PREC == 3 ? 2 : 1;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
@@ -267,6 +325,7 @@ __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16
     const int64_t q = qb * 32 * g + 32 * (w % g) + col;
     s = s * (4 / g) + w / g;
     const int64_t r0 = s * rows_per_slice, r1 = min(n, r0 + rows_per_slice);
+    const uint32_t lowm = KO ? slice_row_mask(rows_per_slice) : 31u;
     const float qnv = qn[q];
     bf16x8 b[NS][NPL];
 #pragma unroll
@@ -303,6 +362,9 @@ __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16
     };
     const int64_t nt = r0 < r1 ? (r1 - r0 + 31) / 32 : 0;
     auto step = [&](int64_t i, const Tile &cur, Tile &ahead) __attribute__((always_inline)) {
+        const bool share = gthr && i % kShareTiles == 0;
+        uint32_t old = kKey32None;
+        if (share) old = share_issue(gthr, q, half, thr);
         load_tile(r0 + 32 * min(i + 2, nt - 1), ahead);
         const int64_t t = r0 + 32 * i;
         f32x16 acc = {};
@@ -315,12 +377,13 @@ __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16
             }
         }
         thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64)); // the other half-wave's list of this query
+        if (share) thr = share_take(thr, old);
         if (t == r0) // only a slice's last tile can hold rows past its end
-            t + 32 <= r1 ? tile_keys_insert<KL, false, true>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr)
-                         : tile_keys_insert<KL, true, true>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr);
+            t + 32 <= r1 ? tile_keys_insert<KL, false, true, false, KO>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm)
+                         : tile_keys_insert<KL, true, true, false, KO>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm);
         else
-            t + 32 <= r1 ? tile_keys_insert<KL, false, false>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr)
-                         : tile_keys_insert<KL, true, false>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr);
+            t + 32 <= r1 ? tile_keys_insert<KL, false, false, false, KO>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm)
+                         : tile_keys_insert<KL, true, false, false, KO>(acc, cur.xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm);
     };
     if (nt > 0) {
         Tile f0, f1, f2;
@@ -332,26 +395,27 @@ __device__ __forceinline__ void scan_small(uint64_t (*lds)[32][KL], const __bf16
             step(i + 2, f2, f1);
         }
     }
-    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL);
+    half_merge_out<KL, KO>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL, gthr ? gthr + q : nullptr, r0, lowm);
 }
 
 // The wide scan without LDS staging, for workgroups whose waves scan different slices (blocks of
 // at most 64 queries: g = 1 or 2 query groups, as in scan_small): the row fragments stream
 // through NC register sets, one per 64-column chunk of a tile, set c refilled with the next
 // tile's chunk c right after its products.
-template <int KL, int PREC, int NC>
+template <int KL, int PREC, int NC, bool KO = false>
 __device__ __forceinline__ void scan_wide_direct(uint64_t (*lds)[32][KL], const __bf16 *__restrict__ Xh,
                                                  const __bf16 *__restrict__ Xl, const float *__restrict__ xn,
                                                  int64_t n, const __bf16 *__restrict__ Qh,
                                                  const __bf16 *__restrict__ Ql, const float *__restrict__ qn,
                                                  int64_t rows_per_slice, uint64_t *__restrict__ part, int64_t qsl,
-                                                 int64_t qb, int64_t s, int g) {
+                                                 int64_t qb, int64_t s, int g, uint32_t *__restrict__ gthr = nullptr) {
     constexpr int dp = 64 * NC;
     constexpr int NPL = PREC == 3 ? 2 : 1;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
     const int64_t q = qb * 32 * g + 32 * (w % g) + col;
     s = s * (4 / g) + w / g;
     const int64_t r0 = s * rows_per_slice, r1 = min(n, r0 + rows_per_slice);
+    const uint32_t lowm = KO ? slice_row_mask(rows_per_slice) : 31u;
     const float qnv = qn[q];
     bf16x8 b[4 * NC][NPL];
 #pragma unroll
@@ -382,6 +446,9 @@ __device__ __forceinline__ void scan_wide_direct(uint64_t (*lds)[32][KL], const 
             const int64_t t = r0 + 32 * i;
             const int64_t tn = r0 + 32 * min(i + 1, nt - 1); // past the last tile: that tile again
             const __bf16 *th = Xh + tn * dp, *tl = Xl + tn * dp;
+            const bool share = gthr && i % kShareTiles == 0;
+            uint32_t old = kKey32None;
+            if (share) old = share_issue(gthr, q, half, thr);
             f32x16 acc = {};
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
@@ -400,17 +467,18 @@ __device__ __forceinline__ void scan_wide_direct(uint64_t (*lds)[32][KL], const 
                 }
             }
             thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64)); // the other half-wave's list of this query
+            if (share) thr = share_take(thr, old);
             if (t == r0)
-                t + 32 <= r1 ? tile_keys_insert<KL, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
-                             : tile_keys_insert<KL, true, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+                t + 32 <= r1 ? tile_keys_insert<KL, false, true, false, KO>(acc, xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm)
+                             : tile_keys_insert<KL, true, true, false, KO>(acc, xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm);
             else
-                t + 32 <= r1 ? tile_keys_insert<KL, false, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
-                             : tile_keys_insert<KL, true, false>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+                t + 32 <= r1 ? tile_keys_insert<KL, false, false, false, KO>(acc, xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm)
+                             : tile_keys_insert<KL, true, false, false, KO>(acc, xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm);
 #pragma unroll
             for (int gi = 0; gi < 4; ++gi) xv[gi] = *(const float4 *)(xn + tn + noff + 8 * gi);
         }
     }
-    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL);
+    half_merge_out<KL, KO>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL, gthr ? gthr + q : nullptr, r0, lowm);
 }
 
 // LDS staging area of scan_wide (bf16 elements): two buffers of NPL planes of 32 rows of
@@ -424,12 +492,13 @@ template <int NC, int NPL> constexpr int wide_stage_elems() { return 2 * NPL * 3
 // fetch each tile once per wave, and L1 accesses, not products, would bound the scan. The next
 // tile's pieces are loaded into registers during this tile's products and stored into the other
 // buffer after them (one barrier per tile; rows are padded: the loads need no guard).
-template <int KL, int PREC, int NC>
+template <int KL, int PREC, int NC, bool KO = false>
 __device__ __forceinline__ void scan_wide(uint64_t (*lds)[32][KL], __bf16 *stage, const __bf16 *__restrict__ Xh,
                                           const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
                                           const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
                                           const float *__restrict__ qn, int64_t rows_per_slice,
-                                          uint64_t *__restrict__ part, int64_t qsl, int64_t qb, int64_t s) {
+                                          uint64_t *__restrict__ part, int64_t qsl, int64_t qb, int64_t s,
+                                          uint32_t *__restrict__ gthr = nullptr) {
     constexpr int dp = 64 * NC;
     constexpr int NPL = PREC == 3 ? 2 : 1;
     constexpr int RS = dp + 8;          // LDS row stride (bf16)
@@ -439,6 +508,7 @@ __device__ __forceinline__ void scan_wide(uint64_t (*lds)[32][KL], __bf16 *stage
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5, col = lane & 31;
     const int64_t q = qb * 128 + 32 * w + col;
     const int64_t r0 = s * rows_per_slice, r1 = min(n, r0 + rows_per_slice);
+    const uint32_t lowm = KO ? slice_row_mask(rows_per_slice) : 31u;
     const float qnv = qn[q];
     bf16x8 b[4 * NC][NPL];
 #pragma unroll
@@ -483,6 +553,9 @@ __device__ __forceinline__ void scan_wide(uint64_t (*lds)[32][KL], __bf16 *stage
             const int bf = (int)(i & 1);
             const int64_t t = r0 + 32 * i;
             const int64_t tn = r0 + 32 * min(i + 1, nt - 1);
+            const bool share = gthr && i % kShareTiles == 0;
+            uint32_t old = kKey32None;
+            if (share) old = share_issue(gthr, q, half, thr);
             if (i + 1 < nt) gload(tn); // (in flight during the products)
             const __bf16 *sa = stage + bf * BUF + col * RS + 8 * half;
             f32x16 acc = {};
@@ -497,12 +570,13 @@ __device__ __forceinline__ void scan_wide(uint64_t (*lds)[32][KL], __bf16 *stage
                 }
             }
             thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64)); // the other half-wave's list of this query
+            if (share) thr = share_take(thr, old);
             if (t == r0)
-                t + 32 <= r1 ? tile_keys_insert<KL, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
-                             : tile_keys_insert<KL, true, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+                t + 32 <= r1 ? tile_keys_insert<KL, false, true, false, KO>(acc, xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm)
+                             : tile_keys_insert<KL, true, true, false, KO>(acc, xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm);
             else
-                t + 32 <= r1 ? tile_keys_insert<KL, false, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr)
-                             : tile_keys_insert<KL, true, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr);
+                t + 32 <= r1 ? tile_keys_insert<KL, false, false, true, KO>(acc, xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm)
+                             : tile_keys_insert<KL, true, false, true, KO>(acc, xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm);
 #pragma unroll
             for (int g = 0; g < 4; ++g) xv[g] = *(const float4 *)(xn + tn + 8 * g + 4 * half);
             // the other buffer was last read for tile i - 1, which every wave finished before the
@@ -511,7 +585,7 @@ __device__ __forceinline__ void scan_wide(uint64_t (*lds)[32][KL], __bf16 *stage
             __syncthreads();
         }
     }
-    half_merge_out<KL>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL);
+    half_merge_out<KL, KO>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL, gthr ? gthr + q : nullptr, r0, lowm);
 }
 
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
@@ -576,18 +650,19 @@ __device__ __forceinline__ void rerank_regs(const float *__restrict__ X, int64_t
                                             uint64_t mine, int K2c, const uint64_t *__restrict__ part, int64_t nlists,
                                             int KL, int64_t qsl, int kk, int64_t n, float eps_coef,
                                             int32_t *__restrict__ out_idx, float *__restrict__ out_dist,
-                                            int32_t *__restrict__ flags, int64_t q) {
+                                            int32_t *__restrict__ flags, int64_t q, uint32_t lowm = 31u) {
+    // lowm: the row bits of the keys (31: offsets in a tile; keys-only lists: offsets in a slice)
     const int lane = threadIdx.x & 63;
     // the smallest approximation any full list dropped: its last key with the row bits cleared
     uint32_t bmin = kKey32None;
     for (int64_t l = lane; l < nlists; l += 64) {
         const uint32_t last = (uint32_t)(part[q * qsl + l * KL + KL - 1] >> 32);
-        bmin = min(bmin, last == kKey32None ? kKey32None : (last & ~31u));
+        bmin = min(bmin, last == kKey32None ? kKey32None : (last & ~lowm));
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) bmin = min(bmin, (uint32_t)__shfl_xor((int)bmin, off, 64));
 
-    // Candidates come sorted by approximation (a key's high word holds it with the low 5 bits
+    // Candidates come sorted by approximation (a key's high word holds it with the low bits lowm
     // replaced by row bits, so the true approximation lies between the word with those bits clear
     // and with them set). Every approximation is within eps of the exact value, so a candidate
     // whose approximation exceeds the kk-th candidate's by more than 2 eps has an exact value
@@ -599,8 +674,8 @@ __device__ __forceinline__ void rerank_regs(const float *__restrict__ X, int64_t
     if (K2c > kk) {
         const uint32_t hk = (uint32_t)(__shfl(mine, kk - 1, 64) >> 32);
         if (hk != kKey32None && lane < K2c) {
-            const float upper = key_f32(hk | 31u);
-            const float lower = key_f32((uint32_t)(mine >> 32) & ~31u);
+            const float upper = key_f32(hk | lowm);
+            const float lower = key_f32((uint32_t)(mine >> 32) & ~lowm);
             if (lower - upper > 3.f * eps) myidx = -1;
         }
     }
@@ -662,7 +737,7 @@ __device__ __forceinline__ void rerank_regs(const float *__restrict__ X, int64_t
     if (lane == 0) {
         int flag = 0;
         if (n > K2c) {
-            const uint32_t bnd = min(bmin, al == kKey32None ? kKey32None : (al & ~31u));
+            const uint32_t bnd = min(bmin, al == kKey32None ? kKey32None : (al & ~lowm));
             const float bound = bnd == kKey32None ? INFINITY : key_f32(bnd);
             // every row outside the candidates has core >= bound - eps; the output orders by
             // sqrtf(core), and a root equal to the k-th's would tie and could win on its lower
@@ -684,7 +759,8 @@ __device__ __forceinline__ void rerank_query(const float *__restrict__ X, int64_
                                              const int32_t *__restrict__ cidx, const float *__restrict__ capx, int K2c,
                                              const uint64_t *__restrict__ part, int64_t nlists, int KL, int64_t qsl,
                                              int kk, int64_t n, float eps_coef, int32_t *__restrict__ out_idx,
-                                             float *__restrict__ out_dist, int32_t *__restrict__ flags, int64_t q) {
+                                             float *__restrict__ out_dist, int32_t *__restrict__ flags, int64_t q,
+                                             uint32_t lowm = 31u) {
     const int lane = threadIdx.x & 63;
     uint64_t mine = kKey64None;
     if (lane < K2c) {
@@ -692,7 +768,7 @@ __device__ __forceinline__ void rerank_query(const float *__restrict__ X, int64_
         if (ci >= 0) mine = ((uint64_t)f32_key(capx[q * K2c + lane]) << 32) | (uint32_t)ci;
     }
     rerank_regs<REG>(X, xstride, d, Q, qstride, qn, xn_max_bits, mine, K2c, part, nlists, KL, qsl, kk, n, eps_coef,
-                     out_idx, out_dist, flags, q);
+                     out_idx, out_dist, flags, q, lowm);
 }
 
 // Exact full scan of query q by one workgroup (called for flagged queries only).

@@ -64,6 +64,7 @@ static std::map<std::string, int64_t> &tune_table() {
         t["batch.wg_per_cu"] = 2;      // batched exact search: persistent scan workgroups per CU
         t["batch.order"] = 1;          // batched exact search: 1 = tiles by decreasing cost
         t["batch.concurrent_scans"] = 1; // batched exact search: the three scan classes as parallel graph branches
+        t["batch.share"] = 1;          // batched exact search: the slices of a query share their list bound
         if (const char *env = std::getenv("PN_TUNE")) {
             std::string e(env);
             size_t pos = 0;
