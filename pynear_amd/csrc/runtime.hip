@@ -60,9 +60,9 @@ static std::map<std::string, int64_t> &tune_table() {
         t["ivfl2.pass_margin_pct"] = 0; // > 0: scan the first probe, then the others bounded by its
                                          // k-th approximation x (1 + pct / 100); 0: one pass
         t["ivfbin.list_major"] = 1;    // IVF-binary fine search: 1 list-major scan + merge, 0 query-major
-        t["batch.slice_rows"] = 512;   // batched exact search: target rows per slice
+        t["batch.slice_rows"] = 2048;  // batched exact search: target rows per slice (<= 4096)
         t["batch.wg_per_cu"] = 2;      // batched exact search: persistent scan workgroups per CU
-        t["batch.order"] = 1;          // batched exact search: 1 = tiles by decreasing cost
+        t["batch.order"] = 2;          // batched exact search: 1 = tiles by decreasing cost, 2 = by slice, then cost
         t["batch.concurrent_scans"] = 1; // batched exact search: the three scan classes as parallel graph branches
         t["batch.share"] = 1;          // batched exact search: the slices of a query share their list bound
         if (const char *env = std::getenv("PN_TUNE")) {

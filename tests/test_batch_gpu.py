@@ -84,6 +84,36 @@ def test_batch_merge_variants(lean, fuse):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("slice_rows,order,share", [(32, 2, 1), (256, 0, 1), (2048, 2, 1), (4096, 2, 1),
+                                                    (4096, 1, 0), (8192, 2, 1)])
+def test_batch_schedules_keep_results(slice_rows, order, share):
+    """The slice lists hold keys only: a key's low bits carry its row's offset in the slice (5 bits
+    for 32-row slices up to 12 for 4,096; longer settings are cut to 4,096), and the slices of a
+    query share the bound of their lists (in index order, longest first, or slice by slice so that
+    later slices start from finished ones). Every schedule returns the plain searches' results,
+    bit for bit, on indices of many slices (60,000 and 45,000 rows, k = 32 and 10)."""
+    from pynear_amd import _native as N
+
+    subs = _make(5, SPECS[:6] + [(60000, 64, 200, 32), (45000, 256, 300, 10), (20000, 128, 90, 1)])
+    names = ("batch.slice_rows", "batch.order", "batch.share")
+    old = [N.tune_get(n) for n in names]
+    try:
+        for n, v in zip(names, (slice_rows, order, share)):
+            N.tune_set(n, v)
+        b = BatchSearch([(idx, Q, k) for idx, X, Q, k in subs])
+        for _ in range(2):
+            b.run()
+        res = b.results()
+    finally:
+        for n, v in zip(names, old):
+            N.tune_set(n, v)
+    for (idx, X, Q, k), (bi, bd) in zip(subs, res):
+        pi, pd = _plain(idx, Q, k)
+        assert np.array_equal(bi, pi), (slice_rows, order, share, X.shape, Q.shape, k)
+        assert np.array_equal(bd, pd)
+
+
+@pytest.mark.gpu
 def test_batch_fallback_many_flagged_queries():
     """An index of identical rows ties every distance, so none of its 1,500 queries can be proved
     complete from the candidates: all go to the fallback scan (more queries than fallback
