@@ -121,8 +121,17 @@ __device__ __forceinline__ void batch_tile(uint64_t *lds_raw, const BatchEntry &
 // Tiles of rows padded to 128, 192 or 256 columns: the scan with register-resident queries.
 template <int KL>
 __device__ __forceinline__ void batch_tile_wide(uint64_t *lds_raw, __bf16 *stage, const BatchEntry &e, int64_t qb,
-                                                int64_t s) {
+                                                int64_t s, bool pipe) {
     auto lds = reinterpret_cast<uint64_t (*)[32][KL]>(lds_raw);
+    if (pipe) {
+        if (e.dp == 128)
+            scan_wide_pipe<KL, 3, 2>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.gthr);
+        else if (e.dp == 192)
+            scan_wide_pipe<KL, 3, 3>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.gthr);
+        else
+            scan_wide_pipe<KL, 3, 4>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.gthr);
+        return;
+    }
     if (e.dp == 128)
         scan_wide<KL, 3, 2, true>(lds, stage, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.gthr);
     else if (e.dp == 192)
@@ -152,7 +161,7 @@ static bool wide_dp(int dp) { return dp == 128 || dp == 192 || dp == 256; }
 template <int CLS>
 __global__ __launch_bounds__(256) void batch_scan_kernel(const BatchEntry *__restrict__ E,
                                                          const int2 *__restrict__ tiles, int64_t ntiles,
-                                                         int32_t *__restrict__ counter) {
+                                                         int32_t *__restrict__ counter, bool pipe) {
     __shared__ uint64_t lds_raw[4 * 32 * 32];
     __shared__ __align__(16) __bf16 stage[CLS == 1 ? wide_stage_elems<4, 2>() : 8];
     __shared__ int64_t s_slot;
@@ -165,9 +174,9 @@ __global__ __launch_bounds__(256) void batch_scan_kernel(const BatchEntry *__res
         const BatchEntry e = global_entry(E[t.x]);
         const int64_t qb = t.y / e.S, s = t.y % e.S;
         if (CLS == 1) {
-            if (e.KL == 8) batch_tile_wide<8>(lds_raw, stage, e, qb, s);
-            else if (e.KL == 16) batch_tile_wide<16>(lds_raw, stage, e, qb, s);
-            else batch_tile_wide<32>(lds_raw, stage, e, qb, s);
+            if (e.KL == 8) batch_tile_wide<8>(lds_raw, stage, e, qb, s, pipe);
+            else if (e.KL == 16) batch_tile_wide<16>(lds_raw, stage, e, qb, s, pipe);
+            else batch_tile_wide<32>(lds_raw, stage, e, qb, s, pipe);
         } else if (CLS == 2) {
             if (e.KL == 8) batch_tile_wide_direct<8>(lds_raw, e, qb, s);
             else if (e.KL == 16) batch_tile_wide_direct<16>(lds_raw, e, qb, s);
@@ -454,6 +463,7 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     // the merge): the wide class holds one workgroup -- one wave per SIMD -- per CU for its LDS
     // tiles, and the other two classes' workgroups run beside it instead of after it. The wide
     // scan is launched first so that its workgroups find their LDS free.
+    const bool pipe = tune_get("batch.pipe") != 0;
     const bool fork = tune_get("batch.concurrent_scans") != 0 && nc[1] > 0 && (nc[0] > 0 || nc[2] > 0);
     hipStream_t side[2] = {nullptr, nullptr};
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -481,13 +491,13 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
         PN_HIP(hipStreamWaitEvent(side[1], ev[0], 0));
     }
     if (nc[1] > 0)
-        hipLaunchKernelGGL(batch_scan_kernel<1>, dim3(grid_wide), dim3(256), 0, st, dE, dt + nc[0], nc[1], ctr + 1);
+        hipLaunchKernelGGL(batch_scan_kernel<1>, dim3(grid_wide), dim3(256), 0, st, dE, dt + nc[0], nc[1], ctr + 1, pipe);
     if (nc[0] > 0)
         hipLaunchKernelGGL(batch_scan_kernel<0>, dim3(grid_scan), dim3(256), 0, fork ? side[0] : st, dE, dt, nc[0],
-                           ctr);
+                           ctr, pipe);
     if (nc[2] > 0)
         hipLaunchKernelGGL(batch_scan_kernel<2>, dim3(grid_wd), dim3(256), 0, fork ? side[1] : st, dE,
-                           dt + nc[0] + nc[1], nc[2], ctr + 2);
+                           dt + nc[0] + nc[1], nc[2], ctr + 2, pipe);
     if (fork) {
         PN_HIP(hipEventRecord(ev[1], side[0]));
         PN_HIP(hipEventRecord(ev[2], side[1]));

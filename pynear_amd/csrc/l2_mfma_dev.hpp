@@ -588,6 +588,190 @@ __device__ __forceinline__ void scan_wide(uint64_t (*lds)[32][KL], __bf16 *stage
     half_merge_out<KL, KO>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL, gthr ? gthr + q : nullptr, r0, lowm);
 }
 
+// Keys-only tile keys without branches (the scan overlaps them with the next tile's products):
+// the lane's keys of the tile (rows tb + offset in the slice), its three smallest, and the two
+// smallest inserted where below thr (inserting the empty key changes nothing). A lane with its
+// third smallest still below thr has more keys to take: tile_keys_rest.
+template <int KL>
+__device__ __forceinline__ void tile_keys_fast(const f32x16 &acc, const float4 (&xv)[4], float qnv, uint32_t tb,
+                                               uint32_t lowm, int half, uint32_t (&Lk)[KL], uint32_t &thr,
+                                               uint32_t (&key)[16], uint32_t &m2, uint32_t &m3) {
+    uint32_t m1 = kKey32None;
+    m2 = kKey32None;
+    m3 = kKey32None;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const float xr = (r & 3) == 0 ? xv[r >> 2].x : (r & 3) == 1 ? xv[r >> 2].y : (r & 3) == 2 ? xv[r >> 2].z : xv[r >> 2].w;
+        const float v = fmaxf(0.f, (qnv + xr) - 2.f * acc[r]);
+        const int off = 8 * (r >> 2) + 4 * half + (r & 3);
+        const uint32_t k = (f32_key(v) & ~lowm) | (tb + (uint32_t)off);
+        key[r] = k;
+        m3 = min(m3, max(m2, k));
+        m2 = min(m2, max(m1, k));
+        m1 = min(m1, k);
+    }
+    list_insert_u32<KL>(Lk, m1 < thr ? m1 : kKey32None);
+    thr = min(thr, Lk[KL - 1]);
+    list_insert_u32<KL>(Lk, m2 < thr ? m2 : kKey32None);
+    thr = min(thr, Lk[KL - 1]);
+}
+
+// The keys above m2 still below thr (keys are unique): by a sort and merge when a lane holds
+// several (lists of 16 or 32), else one round per key.
+template <int KL>
+__device__ __forceinline__ void tile_keys_rest(uint32_t (&key)[16], uint32_t m2, uint32_t (&Lk)[KL], uint32_t &thr) {
+    if constexpr (KL == 16 || KL == 32) {
+        int below = 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) below += key[r] > m2 && key[r] < thr ? 1 : 0;
+        if (__any(below >= 3)) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) key[r] = key[r] > m2 && key[r] < thr ? key[r] : kKey32None;
+            sort16(key);
+            uint32_t unused[KL];
+            merge_sorted16<KL, true>(key, 0u, Lk, unused);
+            thr = min(thr, Lk[KL - 1]);
+            return;
+        }
+    }
+    uint32_t last = m2;
+    for (;;) {
+        uint32_t mn = kKey32None;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) mn = min(mn, key[r] > last ? key[r] : kKey32None);
+        if (!__any(mn < thr)) break;
+        if (mn < thr) {
+            list_insert_u32<KL>(Lk, mn);
+            thr = min(thr, Lk[KL - 1]);
+        }
+        last = mn;
+    }
+}
+
+// scan_wide for keys-only lists, software-pipelined: a tile's keys are taken in the same
+// straight-line block as the next tile's products, so the key arithmetic issues while the matrix
+// cores work (a wave per SIMD: there is no other wave to fill them), and only lanes with a third
+// key below their bound leave it. Tiles are staged as in scan_wide, one tile further ahead; full
+// tiles go through the pipeline, a slice's partial last tile (rows past n) after it with masking.
+template <int KL, int PREC, int NC>
+__device__ __forceinline__ void scan_wide_pipe(uint64_t (*lds)[32][KL], __bf16 *stage, const __bf16 *__restrict__ Xh,
+                                               const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
+                                               const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
+                                               const float *__restrict__ qn, int64_t rows_per_slice,
+                                               uint64_t *__restrict__ part, int64_t qsl, int64_t qb, int64_t s,
+                                               uint32_t *__restrict__ gthr = nullptr) {
+    constexpr int dp = 64 * NC;
+    constexpr int NPL = PREC == 3 ? 2 : 1;
+    constexpr int RS = dp + 8;
+    constexpr int PLANE = 32 * RS, BUF = NPL * PLANE;
+    constexpr int PPR = dp / 8;
+    constexpr int NPC = NPL * NC;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5, col = lane & 31;
+    const int64_t q = qb * 128 + 32 * w + col;
+    const int64_t r0 = s * rows_per_slice, r1 = min(n, r0 + rows_per_slice);
+    const uint32_t lowm = slice_row_mask(rows_per_slice);
+    const float qnv = qn[q];
+    bf16x8 b[4 * NC][NPL];
+#pragma unroll
+    for (int k = 0; k < 4 * NC; ++k) {
+        b[k][0] = *(const bf16x8 *)(Qh + q * dp + 16 * k + 8 * half);
+        if (PREC == 3) b[k][NPL - 1] = *(const bf16x8 *)(Ql + q * dp + 16 * k + 8 * half);
+    }
+    uint32_t Lk[KL], Li[KL]; // (Li: unused, keys-only lists)
+#pragma unroll
+    for (int j = 0; j < KL; ++j) Lk[j] = kKey32None;
+    uint32_t thr = kKey32None;
+    const int64_t nt = r0 < r1 ? (r1 - r0 + 31) / 32 : 0; // tiles, the last one possibly partial
+    const int64_t nf = r0 < r1 ? (r1 - r0) / 32 : 0;      // full tiles
+    if (nt > 0) {
+        int goff[NPC], soff[NPC];
+#pragma unroll
+        for (int u = 0; u < NPC; ++u) {
+            const int rem = tid + 256 * (u % NC), row = rem / PPR, pc = rem % PPR;
+            goff[u] = row * dp + 8 * pc;
+            soff[u] = (u / NC) * PLANE + row * RS + 8 * pc;
+        }
+        bf16x8 sreg[NPC];
+        auto gload = [&](int64_t t) __attribute__((always_inline)) {
+#pragma unroll
+            for (int u = 0; u < NPC; ++u)
+                sreg[u] = *(const bf16x8 *)((u / NC == 0 ? Xh : Xl) + t * dp + goff[u]);
+        };
+        auto sstore = [&](int bf) __attribute__((always_inline)) {
+#pragma unroll
+            for (int u = 0; u < NPC; ++u) *(bf16x8 *)(stage + bf * BUF + soff[u]) = sreg[u];
+        };
+        auto products = [&](int bf) __attribute__((always_inline)) {
+            const __bf16 *sa = stage + bf * BUF + col * RS + 8 * half;
+            f32x16 acc = {};
+#pragma unroll
+            for (int k = 0; k < 4 * NC; ++k) {
+                const bf16x8 ah = *(const bf16x8 *)(sa + 16 * k);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b[k][0], acc, 0, 0, 0);
+                if (PREC == 3) {
+                    const bf16x8 al = *(const bf16x8 *)(sa + PLANE + 16 * k);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b[k][NPL - 1], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b[k][0], acc, 0, 0, 0);
+                }
+            }
+            return acc;
+        };
+        auto norms = [&](int64_t t, float4 (&xv)[4]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) xv[g] = *(const float4 *)(xn + t + 8 * g + 4 * half);
+        };
+        // prologue: tile 0 in buffer 0 and its products; tile 1 in buffer 1
+        gload(r0);
+        sstore(0);
+        if (nt > 1) gload(r0 + 32);
+        float4 xv[4];
+        norms(r0, xv);
+        __syncthreads();
+        f32x16 acc = products(0);
+        if (nt > 1) sstore(1);
+        __syncthreads();
+        for (int64_t i = 0; i < nf; ++i) {
+            const int64_t t = r0 + 32 * i;
+            const bool share = gthr && i % kShareTiles == 0;
+            uint32_t old = kKey32None;
+            if (share) old = share_issue(gthr, q, half, thr);
+            if (i + 2 < nt) gload(r0 + 32 * (i + 2)); // (in flight during this step)
+            float4 xn1[4];
+            norms(r0 + 32 * min(i + 1, nt - 1), xn1);
+            // the next tile's products (past the last tile: the stale buffer, unused) and this
+            // tile's keys, one block
+            const f32x16 an = products((int)((i + 1) & 1));
+            thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64)); // the other half-wave's list of this query
+            thr = share_take(thr, old); // (no branch: old is the empty key when not shared)
+            uint32_t key[16], m2, m3;
+            tile_keys_fast<KL>(acc, xv, qnv, (uint32_t)(t - r0), lowm, half, Lk, thr, key, m2, m3);
+            // issue order of the block: each product followed by an LDS read and a few of the key
+            // operations, so that those issue while the matrix core works
+#pragma unroll
+            for (int k = 0; k < 4 * NC * (PREC == 3 ? 3 : 1); ++k) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0); // one MFMA
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0); // one LDS read
+                __builtin_amdgcn_sched_group_barrier(0x002, 5, 0); // five VALU
+            }
+            if (__any(m3 < thr)) tile_keys_rest<KL>(key, m2, Lk, thr);
+            acc = an;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) xv[g] = xn1[g];
+            // tile i + 2 into the buffer tile i was read from (every wave took its products in
+            // the previous step, before the barrier that ended it)
+            if (i + 2 < nt) sstore((int)(i & 1));
+            __syncthreads();
+        }
+        if (nt > nf) { // the partial last tile (its products are in acc): rows past r1 masked
+            const int64_t t = r0 + 32 * nf;
+            thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64));
+            nf == 0 ? tile_keys_insert<KL, true, true, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm)
+                    : tile_keys_insert<KL, true, false, false, true>(acc, xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm);
+        }
+    }
+    half_merge_out<KL, true>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL, gthr ? gthr + q : nullptr, r0, lowm);
+}
+
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
