@@ -95,6 +95,36 @@ __device__ __forceinline__ void wave_sort_u64(uint64_t (&v)[E], int lane) {
         }
 }
 
+// The same bitonic sort for 32-bit keys (one shuffle per cross-lane step).
+template <int E>
+__device__ __forceinline__ void wave_sort_u32(uint32_t (&v)[E], int lane) {
+    constexpr int N = 64 * E;
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+#pragma unroll
+                for (int s = 0; s < E; ++s) {
+                    const int t = s ^ (j / 64);
+                    if (t > s) {
+                        const bool asc = ((lane + 64 * s) & k) == 0;
+                        const uint32_t a = v[s], b = v[t];
+                        v[s] = asc ? min(a, b) : max(a, b);
+                        v[t] = asc ? max(a, b) : min(a, b);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < E; ++s) {
+                    const uint32_t o = (uint32_t)__shfl_xor((int)v[s], j, 64);
+                    const bool asc = ((lane + 64 * s) & k) == 0, lower = (lane & j) == 0;
+                    v[s] = lower == asc ? min(o, v[s]) : max(o, v[s]);
+                }
+            }
+        }
+}
+
 // popcount(x) + acc as one v_bcnt_u32_b32. Written as an (ALU-only) asm statement because the
 // compiler otherwise reassociates chains of __popc(x) + acc into bcnt(x, 0) pairs joined by
 // v_add3_u32, 25% more VALU work per word.

@@ -259,6 +259,22 @@ __global__ __launch_bounds__(256, 3) void l2_matrix_keys_kernel(const __bf16 *__
     }
 }
 
+// The first K2c (<= 64) of the `total` (<= 64 E) keys packed in wb, in order, into query q's
+// cidx / capx: one bitonic sort of E keys per lane (the K2c smallest end in v[0] of lanes
+// 0 .. K2c - 1).
+template <int E>
+__device__ __forceinline__ void select_sorted_out(const uint64_t *wb, int total, int lane, int64_t q, int K2c,
+                                                  int32_t *__restrict__ cidx, float *__restrict__ capx) {
+    uint64_t v[E];
+#pragma unroll
+    for (int u = 0; u < E; ++u) v[u] = lane + 64 * u < total ? wb[lane + 64 * u] : kKey64None;
+    wave_sort_u64<E, 2>(v, lane);
+    if (lane < K2c) {
+        cidx[q * K2c + lane] = v[0] == kKey64None ? -1 : (int32_t)(uint32_t)(v[0] & 0xFFFFFFFFull);
+        capx[q * K2c + lane] = key_f32((uint32_t)(v[0] >> 32));
+    }
+}
+
 // One wave per query: the K2c smallest (key, row) of its key row, ascending, into cidx / capx.
 // A lane sorts its keys 16 at a time (rows lane, lane + 64, ...) and keeps its 16 smallest; the
 // wave then takes the K2c smallest of the 64 lane lists. capx[K2c - 1] is lowered to the smallest
@@ -281,12 +297,14 @@ __global__ __launch_bounds__(256) void l2_matrix_select_kernel(const uint32_t *_
 #pragma unroll
         for (int j = 0; j < 16; ++j) kr[j] = lane + 64 * j < N ? row[lane + 64 * j] : kKey32None;
         auto key = [&](int j) { return ((uint64_t)kr[j] << 32) | (uint32_t)(lane + 64 * j); };
-        uint64_t mn = kKey64None;
+        // (the bound on the keys' high words alone: every key whose word is at most the K2c-th
+        // smallest lane minimum word is taken, at least K2c keys)
+        uint32_t mn = kKey32None;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) mn = lane + 64 * j < N && key(j) < mn ? key(j) : mn;
-        uint64_t sm[1] = {mn};
-        wave_sort_u64<1, 2>(sm, lane);
-        const uint64_t bnd = __shfl(sm[0], K2c - 1, 64);
+        for (int j = 0; j < 16; ++j) mn = lane + 64 * j < N ? min(mn, kr[j]) : mn;
+        uint32_t sm[1] = {mn};
+        wave_sort_u32<1>(sm, lane);
+        const uint64_t bnd = ((uint64_t)__shfl(sm[0], K2c - 1, 64) << 32) | 0xFFFFFFFFull;
         int held = 0;
 #pragma unroll
         for (int j = 0; j < 16; ++j) held += lane + 64 * j < N && key(j) <= bnd ? 1 : 0;
@@ -306,14 +324,13 @@ __global__ __launch_bounds__(256) void l2_matrix_select_kernel(const uint32_t *_
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            uint64_t v[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = lane + 64 * u < total ? wb[lane + 64 * u] : kKey64None;
-            wave_sort_u64<4, 2>(v, lane);
-            if (lane < K2c) {
-                cidx[q * K2c + lane] = v[0] == kKey64None ? -1 : (int32_t)(uint32_t)(v[0] & 0xFFFFFFFFull);
-                capx[q * K2c + lane] = key_f32((uint32_t)(v[0] >> 32));
-            }
+            // the packed keys sorted by a network only as wide as their count (usually under 64)
+            if (total <= 64)
+                select_sorted_out<1>(wb, total, lane, q, K2c, cidx, capx);
+            else if (total <= 128)
+                select_sorted_out<2>(wb, total, lane, q, K2c, cidx, capx);
+            else
+                select_sorted_out<4>(wb, total, lane, q, K2c, cidx, capx);
             return;
         }
     } else {
@@ -344,14 +361,12 @@ __global__ __launch_bounds__(256) void l2_matrix_select_kernel(const uint32_t *_
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            uint64_t v[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = lane + 64 * u < total ? wb[lane + 64 * u] : kKey64None;
-            wave_sort_u64<4, 2>(v, lane);
-            if (lane < K2c) { // (K2c <= 64: the K2c smallest are v[0] of lanes 0 .. K2c - 1)
-                cidx[q * K2c + lane] = v[0] == kKey64None ? -1 : (int32_t)(uint32_t)(v[0] & 0xFFFFFFFFull);
-                capx[q * K2c + lane] = key_f32((uint32_t)(v[0] >> 32));
-            }
+            if (total <= 64)
+                select_sorted_out<1>(wb, total, lane, q, K2c, cidx, capx);
+            else if (total <= 128)
+                select_sorted_out<2>(wb, total, lane, q, K2c, cidx, capx);
+            else
+                select_sorted_out<4>(wb, total, lane, q, K2c, cidx, capx);
             return;
         }
     }
