@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace pn {
 
@@ -93,6 +94,26 @@ __device__ __forceinline__ void wave_sort_u64(uint64_t (&v)[E], int lane) {
                 }
             }
         }
+}
+
+// Sum over the wave by data-parallel-primitive moves (each step one VALU add reading a permuted
+// operand): pairs and quads of lanes, mirrored half rows and rows (sums of 16), then rows 1 and 3
+// add the last lane of the row before and rows 2 and 3 that of row 1, so lane 63 ends with the
+// total, which every lane takes.
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+    auto dpp = [](float x, auto ctrl, auto rows) __attribute__((always_inline)) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x),
+                                                                     decltype(ctrl)::value, decltype(rows)::value,
+                                                                     0xF, false));
+    };
+    using full = std::integral_constant<int, 0xF>;
+    v += dpp(v, std::integral_constant<int, 0xB1>{}, full{});  // quad_perm [1, 0, 3, 2]
+    v += dpp(v, std::integral_constant<int, 0x4E>{}, full{});  // quad_perm [2, 3, 0, 1]
+    v += dpp(v, std::integral_constant<int, 0x141>{}, full{}); // row_half_mirror
+    v += dpp(v, std::integral_constant<int, 0x140>{}, full{}); // row_mirror
+    v += dpp(v, std::integral_constant<int, 0x142>{}, std::integral_constant<int, 0xA>{}); // row_bcast:15
+    v += dpp(v, std::integral_constant<int, 0x143>{}, std::integral_constant<int, 0xC>{}); // row_bcast:31
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
 }
 
 // The same bitonic sort for 32-bit keys (one shuffle per cross-lane step).

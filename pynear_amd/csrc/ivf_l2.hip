@@ -748,7 +748,8 @@ __global__ __launch_bounds__(256) void residual_kernel(const float *__restrict__
 // the fp8 residual (its fp32 sum rounded up by 2^-10) into Rq[pair slot]. One wave per query: its row is
 // read once (dims 4 l + 256 t .. +3 in lane l), then each of its probes p_lo .. p_lo + npw - 1
 // (list probes[q np + p], slot pslot[q np + p]) is converted; slots of absent probes are skipped.
-template <int NV>
+// FULL: d == dp == 256 NV with 16-byte aligned rows, so no dimension is masked.
+template <int NV, bool FULL>
 __global__ __launch_bounds__(256) void query_residual_kernel(const float *__restrict__ Q, int64_t qstride, int d,
                                                              int dp, const float *__restrict__ lcent, float scale,
                                                              const int32_t *__restrict__ probes, int64_t nq, int np,
@@ -759,13 +760,13 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
     const int lane = threadIdx.x & 63;
     if (q >= nq) return;
     const float *qrow = Q + q * qstride;
-    const bool vec = (d & 3) == 0 && (qstride & 3) == 0; // 16-byte loads (rows of whole float4)
+    const bool vec = FULL || ((d & 3) == 0 && (qstride & 3) == 0); // 16-byte loads (rows of whole float4)
     float qv[NV][4]; // NV = dp / 256 rounded up
 #pragma unroll
     for (int t = 0; t < NV; ++t) {
         const int e0 = 4 * lane + 256 * t;
         if (vec) {
-            const float4 v = e0 < d ? *(const float4 *)(qrow + e0) : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 v = FULL || e0 < d ? *(const float4 *)(qrow + e0) : make_float4(0.f, 0.f, 0.f, 0.f);
             qv[t][0] = v.x, qv[t][1] = v.y, qv[t][2] = v.z, qv[t][3] = v.w;
         } else {
 #pragma unroll
@@ -782,7 +783,7 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
         for (int t = 0; t < NV; ++t) {
             const int e0 = 4 * lane + 256 * t;
             if (vec) {
-                const float4 c4 = c >= 0 && e0 < d ? *(const float4 *)(cc + e0) : make_float4(0.f, 0.f, 0.f, 0.f);
+                const float4 c4 = c >= 0 && (FULL || e0 < d) ? *(const float4 *)(cc + e0) : make_float4(0.f, 0.f, 0.f, 0.f);
                 cv[t][0] = c4.x, cv[t][1] = c4.y, cv[t][2] = c4.z, cv[t][3] = c4.w;
             } else {
 #pragma unroll
@@ -797,11 +798,11 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
 #pragma unroll
         for (int t = 0; t < NV; ++t) {
             const int e0 = 4 * lane + 256 * t;
-            if (e0 < dp) {
+            if (FULL || e0 < dp) {
                 float v[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const float r = e0 + u < d ? qv[t][u] - cv[t][u] : 0.f;
+                    const float r = FULL || e0 + u < d ? qv[t][u] - cv[t][u] : 0.f;
                     ss = fmaf(r, r, ss);
                     v[u] = fminf(fmaxf(r * scale, -448.f), 448.f);
                 }
@@ -816,12 +817,10 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
                 }
             }
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+        ss = wave_sum_dpp(ss);
         if (lane == 0) Rn[sl] = ss;
         if (Rq) {
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) s8 += __shfl_xor(s8, off, 64);
+            s8 = wave_sum_dpp(s8);
             if (lane == 0) Rq[sl] = sqrtf(s8) * (1.f + 1.f / 1024.f);
         }
     };
@@ -2369,18 +2368,21 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
             float *Rq = bounded ? (float *)h->qres8n.ensure((size_t)total * 4) : nullptr;
             const dim3 gq((unsigned)ceil_div(nq, (int64_t)4));
             const float *lcp = h->lcent.as<float>();
-            if (dp <= 256)
-                hipLaunchKernelGGL(query_residual_kernel<1>, gq, dim3(256), 0, st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq,
-                                   np, p_lo, npw, psl, R8, Rn, Rq);
-            else if (dp <= 512)
-                hipLaunchKernelGGL(query_residual_kernel<2>, gq, dim3(256), 0, st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq,
-                                   np, p_lo, npw, psl, R8, Rn, Rq);
-            else if (dp <= 768)
-                hipLaunchKernelGGL(query_residual_kernel<3>, gq, dim3(256), 0, st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq,
-                                   np, p_lo, npw, psl, R8, Rn, Rq);
-            else
-                hipLaunchKernelGGL(query_residual_kernel<5>, gq, dim3(256), 0, st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq,
-                                   np, p_lo, npw, psl, R8, Rn, Rq);
+            // rows of whole 256-dim steps (d = dp = 256, 512 or 768, float4-aligned): no masking
+            const int nvr = dp <= 256 ? 1 : (dp <= 512 ? 2 : (dp <= 768 ? 3 : 5));
+            const bool full = d == dp && dp == 256 * nvr && re % 4 == 0 && ((uintptr_t)Q & 15) == 0;
+            auto go = [&](auto nv, auto fl) {
+                hipLaunchKernelGGL((query_residual_kernel<decltype(nv)::value, decltype(fl)::value>), gq, dim3(256), 0,
+                                   st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq, np, p_lo, npw, psl, R8, Rn, Rq);
+            };
+            auto by_nv = [&](auto fl) {
+                if (nvr == 1) go(std::integral_constant<int, 1>{}, fl);
+                else if (nvr == 2) go(std::integral_constant<int, 2>{}, fl);
+                else if (nvr == 3) go(std::integral_constant<int, 3>{}, fl);
+                else go(std::integral_constant<int, 5>{}, std::false_type{});
+            };
+            if (full) by_nv(std::true_type{});
+            else by_nv(std::false_type{});
         }
         PN_HIP(hipGetLastError());
     };
