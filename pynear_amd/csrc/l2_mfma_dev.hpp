@@ -823,8 +823,49 @@ __device__ __forceinline__ float rerank_cores_st(const float *__restrict__ qrow,
     return core;
 }
 
+// rerank_cores_st for rows of a multiple of 256 floats: the chains run through the row in chunks of
+// 32 steps (the query's elements of a chunk re-read from cache, the two candidates' loaded
+// together), so 16 candidates a round fit in registers at any such width.
+__device__ __forceinline__ float rerank_cores_chunked(const float *__restrict__ qrow, const float *__restrict__ X,
+                                                      int64_t xstride, int64_t d, int64_t n, int32_t myidx,
+                                                      int nkeep, int lane) {
+    constexpr int CH = 32;
+    const int grp = lane >> 3, c8 = lane & 7;
+    float core = 0.f;
+    for (int base = 0; base < nkeep; base += 16) {
+        const int ca = base + grp, cb = base + 8 + grp;
+        const int32_t ia = __shfl(myidx, ca & 63, 64), ib = __shfl(myidx, cb & 63, 64);
+        const float *xa = X + (ca < nkeep && ia >= 0 && ia < n ? (int64_t)ia : 0) * xstride;
+        const float *xb = X + (cb < nkeep && ib >= 0 && ib < n ? (int64_t)ib : 0) * xstride;
+        float aa = 0.f, ab = 0.f;
+        for (int64_t e0 = c8; e0 < d; e0 += 8 * CH) {
+            float qa[CH], ta[CH], tb[CH];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                qa[u] = qrow[e0 + 8 * u];
+                ta[u] = xa[e0 + 8 * u];
+                tb[u] = xb[e0 + 8 * u];
+            }
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const float t = qa[u] - ta[u], s = qa[u] - tb[u];
+                aa = fmaf(t, t, aa);
+                ab = fmaf(s, s, ab);
+            }
+        }
+        const float va = l2_group8_finish(aa, qrow, xa, d, c8);
+        const float vb = l2_group8_finish(ab, qrow, xb, d, c8);
+        const int j = lane - base;
+        const float ga = __shfl(va, (j & 7) * 8, 64), gb = __shfl(vb, (j & 7) * 8, 64);
+        if (j >= 0 && j < 8) core = ga;
+        else if (j >= 8 && j < 16) core = gb;
+    }
+    return core;
+}
+
 // One wave per query q: exact rerank of K2c candidates, (distance, index) order, completeness check.
-// REG: rows of 8 u + (0..7) floats for u in {4, 8, 16, 24, 32} go through rerank_cores_st.
+// REG: rows of 8 u + (0..7) floats for u in {4, 8, 16, 24, 32} go through rerank_cores_st; rows of
+// a multiple of 256 floats (other widths) through rerank_cores_chunked.
 // Candidate j (j < K2c) is the key `mine` of lane j: its approximation's bits in the high word, its
 // row in the low word (empty key: no candidate).
 template <bool REG>
@@ -875,6 +916,10 @@ __device__ __forceinline__ void rerank_regs(const float *__restrict__ X, int64_t
     case 24: core = rerank_cores_st<24>(qrow, X, xstride, d, n, myidx, nkeep, lane); break;
     case 32: core = rerank_cores_st<32>(qrow, X, xstride, d, n, myidx, nkeep, lane); break;
     default: {
+        if (d >= 256 && d % 256 == 0) {
+            core = rerank_cores_chunked(qrow, X, xstride, d, n, myidx, nkeep, lane);
+            break;
+        }
         // 8-lane groups (lane c of a group runs the reference's FMA chain c, reading 8
         // consecutive floats with its group), 8 candidates a round; candidate j's sum ends in lane j
         const int grp = lane >> 3, c8 = lane & 7;
