@@ -264,6 +264,74 @@ __global__ __launch_bounds__(256) void select_probes_kernel(const float *__restr
     if (threadIdx.x == 0) counts[row] = min(s_n, np_max);
 }
 
+// The same selection with one wave per query for ncls <= 64 J: the probabilities stay in registers
+// (J per lane, cluster lane + 64 j) for the whole bisection, whose mass and count sums are wave
+// sums by DPP moves, so a step costs three operations per cluster and no barrier. The chosen
+// clusters are written in ascending order (ballot compaction).
+template <int J>
+__global__ __launch_bounds__(256) void select_probes_wave_kernel(const float *__restrict__ logits, int ncls, float mass,
+                                                                 int np_min, int np_max, int64_t B,
+                                                                 int32_t *__restrict__ probes,
+                                                                 int32_t *__restrict__ counts) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= B) return;
+    const float *x = logits + row * ncls;
+    float p[J];
+    float mx = -INFINITY;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int c = lane + 64 * j;
+        p[j] = c < ncls ? x[c] : -INFINITY;
+        mx = fmaxf(mx, p[j]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    float se = 0.f;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        p[j] = expf(p[j] - mx); // (absent clusters: exp(-inf) = 0, never selected: t > 0 below)
+        se += p[j];
+    }
+    se = wave_sum_dpp(se);
+    const float inv = 1.f / se;
+    float pmax = 0.f;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        p[j] *= inv;
+        pmax = fmaxf(pmax, p[j]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) pmax = fmaxf(pmax, __shfl_xor(pmax, off, 64));
+    // bisection on t in [0, pmax] as in select_probes_kernel (t = pmax always selects at least one
+    // cluster, so the answer lies below it or is it)
+    float lo = 0.f, hi = pmax;
+    for (int it = 0; it < 30; ++it) {
+        const float t = 0.5f * (lo + hi);
+        float ms = 0.f, cn = 0.f;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const bool in = p[j] >= t;
+            ms += in ? p[j] : 0.f;
+            cn += in ? 1.f : 0.f;
+        }
+        ms = wave_sum_dpp(ms);
+        cn = wave_sum_dpp(cn);
+        const bool enough = (ms >= mass && cn >= (float)np_min) || cn >= (float)np_max;
+        if (enough) lo = t; else hi = t;
+    }
+    int base = 0;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const bool sel = p[j] >= lo && lane + 64 * j < ncls;
+        const uint64_t bal = __ballot(sel);
+        const int pos = base + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+        if (sel && pos < np_max) probes[row * np_max + pos] = lane + 64 * j;
+        base += __popcll(bal);
+    }
+    if (lane == 0) counts[row] = min(base, np_max);
+}
+
 } // namespace pn
 
 struct pn_probe {
@@ -462,8 +530,19 @@ extern "C" pn_status pn_probe_select_device(pn_probe *p, const float *x, int64_t
         DeviceScope ds(p->device);
         hipStream_t st = (hipStream_t)stream;
         probe_forward(p, x, B, st);
-        hipLaunchKernelGGL(select_probes_kernel, dim3((unsigned)B), dim3(256), 0, st, p->logits.as<float>(), p->ncls,
-                           mass, np_min, np_max, probes, counts);
+        const dim3 gw((unsigned)ceil_div(B, (int64_t)4));
+        if (p->ncls <= 1024)
+            hipLaunchKernelGGL(select_probes_wave_kernel<16>, gw, dim3(256), 0, st, p->logits.as<float>(), p->ncls, mass,
+                               np_min, np_max, B, probes, counts);
+        else if (p->ncls <= 2048)
+            hipLaunchKernelGGL(select_probes_wave_kernel<32>, gw, dim3(256), 0, st, p->logits.as<float>(), p->ncls, mass,
+                               np_min, np_max, B, probes, counts);
+        else if (p->ncls <= 4096)
+            hipLaunchKernelGGL(select_probes_wave_kernel<64>, gw, dim3(256), 0, st, p->logits.as<float>(), p->ncls, mass,
+                               np_min, np_max, B, probes, counts);
+        else
+            hipLaunchKernelGGL(select_probes_kernel, dim3((unsigned)B), dim3(256), 0, st, p->logits.as<float>(),
+                               p->ncls, mass, np_min, np_max, probes, counts);
         PN_HIP(hipGetLastError());
     });
 }
