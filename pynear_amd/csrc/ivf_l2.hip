@@ -880,10 +880,19 @@ __global__ __launch_bounds__(256) void seg_bound_kernel(const uint8_t *__restric
                                                         uint32_t *__restrict__ qthr, uint32_t *__restrict__ act,
                                                         int64_t any_at, int k2) {
     __shared__ float s_ub[4][32][kSegMax]; // MODE 0: per wave, query column and segment: upper bound
+    __shared__ int s_rows[kSegMax], s_ga[kSegMax], s_gb[kSegMax]; // per segment: rows, first and last group
     const int item = blockIdx.x;
     if (item >= bins[3 * kXcds]) return; // (the item total)
     const int4 it = items[item];
     const int c = it.x, start = it.y, cnt = it.z;
+    const int sb0 = sbase[c], sb1 = min(sbase[c + 1], sb0 + kSegMax); // the list's segments (<= kSegMax)
+    if ((int)threadIdx.x < sb1 - sb0) { // the segments' extents, read once for the workgroup
+        const int64_t a = soff[sb0 + threadIdx.x], b = soff[sb0 + threadIdx.x + 1];
+        s_rows[threadIdx.x] = (int)(b - a);
+        s_ga[threadIdx.x] = (int)(a >> 5);
+        s_gb[threadIdx.x] = (int)((b - 1) >> 5);
+    }
+    __syncthreads(); // (before any wave leaves)
     const int lane = threadIdx.x & 63, half = lane >> 5, col = lane & 31;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int q0 = 32 * w;
@@ -917,7 +926,6 @@ __global__ __launch_bounds__(256) void seg_bound_kernel(const uint8_t *__restric
             b[ch].u[0] = *(const uint4 *)(qrow + ch * kIvfKC);
             b[ch].u[1] = *(const uint4 *)(qrow + ch * kIvfKC + 16);
         }
-    const int sb0 = sbase[c], sb1 = sbase[c + 1]; // the list's segments (<= kSegMax)
     uint64_t smask = 0;                            // MODE 1, bit s - sb0: segment s needed
     const uint8_t *lane_c8 = C8 + col * 64 + 32 * half;
     for (int ct = sb0 >> 5; ct <= (sb1 - 1) >> 5; ++ct) {
@@ -970,7 +978,7 @@ __global__ __launch_bounds__(256) void seg_bound_kernel(const uint8_t *__restric
             const float u = s_ub[w][col][t];
             int64_t rows = 0;
             for (int s2 = 0; s2 < ns; ++s2)
-                if (s_ub[w][col][s2] <= u) rows += soff[sb0 + s2 + 1] - soff[sb0 + s2];
+                if (s_ub[w][col][s2] <= u) rows += s_rows[s2];
             if (rows >= k2) best = fminf(best, u);
         }
         best = fminf(best, __shfl_xor(best, 32, 64));
@@ -983,8 +991,8 @@ __global__ __launch_bounds__(256) void seg_bound_kernel(const uint8_t *__restric
         const int wg0 = 32 * (cg0 + wi);
         uint32_t m32 = 0;
         for (uint64_t m = smask; m; m &= m - 1) {
-            const int sg = sb0 + __builtin_ctzll(m);
-            const int ga = (int)(soff[sg] >> 5) - wg0, gb = (int)((soff[sg + 1] - 1) >> 5) - wg0;
+            const int s = __builtin_ctzll(m);
+            const int ga = s_ga[s] - wg0, gb = s_gb[s] - wg0;
             if (gb < 0 || ga > 31) continue;
             const int lo = max(ga, 0), hi = min(gb, 31);
             m32 |= (hi == 31 ? 0xFFFFFFFFu : ((2u << hi) - 1u)) & ~((1u << lo) - 1u);
@@ -2366,11 +2374,11 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
             float *Rn = (float *)h->qresn.ensure((size_t)total * 4);
             // norms of the fp8 residuals for the segment bounds
             float *Rq = bounded ? (float *)h->qres8n.ensure((size_t)total * 4) : nullptr;
-            const dim3 gq((unsigned)ceil_div(nq, (int64_t)4));
             const float *lcp = h->lcent.as<float>();
             // rows of whole 256-dim steps (d = dp = 256, 512 or 768, float4-aligned): no masking
             const int nvr = dp <= 256 ? 1 : (dp <= 512 ? 2 : (dp <= 768 ? 3 : 5));
             const bool full = d == dp && dp == 256 * nvr && re % 4 == 0 && ((uintptr_t)Q & 15) == 0;
+            const dim3 gq((unsigned)ceil_div(nq, (int64_t)4));
             auto go = [&](auto nv, auto fl) {
                 hipLaunchKernelGGL((query_residual_kernel<decltype(nv)::value, decltype(fl)::value>), gq, dim3(256), 0,
                                    st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq, np, p_lo, npw, psl, R8, Rn, Rq);
