@@ -120,6 +120,42 @@ def test_training_lowers_loss_and_selection_bounds():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("ncls", [1500, 3000, 5000])
+def test_selection_at_many_clusters(ncls):
+    """The selection for 1,500 and 3,000 clusters (one wave per query, 32 or 64 probabilities per
+    lane) and 5,000 (a workgroup per query) on an untrained net: the chosen set is the most
+    probable clusters, within [np_min, np_max], in ascending cluster order up to 4,096 clusters,
+    reaches the mass unless capped, and is the fewest that do (dropping its least probable
+    cluster falls short, ties aside)."""
+    from pynear_amd.learned import LearnedProbe
+    from pynear_amd import hipdev
+    rng = np.random.default_rng(11)
+    q = rng.standard_normal((200, D)).astype(np.float32) * 4
+    probe = LearnedProbe(D, ncls, hidden=HID, seed=3)
+    dq = hipdev.DeviceArray.from_numpy(q)
+    logits = probe.logits(q).astype(np.float64)
+    P = np.exp(logits - logits.max(1, keepdims=True))
+    P /= P.sum(1, keepdims=True)
+    for mass, lo, hi in ((0.05, 1, 300), (0.2, 3, 40)):
+        probes, counts = probe.select(dq.ptr, len(q), mass, lo, hi)
+        c, p = counts.numpy(), probes.numpy()
+        assert np.all((c >= lo) & (c <= hi))
+        for i in range(len(q)):
+            sel = p[i, :c[i]]
+            assert len(set(sel.tolist())) == c[i] and np.all((sel >= 0) & (sel < ncls))
+            if ncls <= 4096:
+                assert np.all(np.diff(sel) > 0)
+            rest = np.setdiff1d(np.arange(ncls), sel)
+            assert P[i, sel].min() >= P[i, rest].max() - 1e-6
+            if c[i] < hi:
+                assert P[i, sel].sum() >= mass - 1e-4
+                if c[i] > lo:
+                    smallest = np.sort(P[i, sel])
+                    if smallest[1] - smallest[0] > 1e-6:  # a strict least member (no tie at the cut)
+                        assert P[i, sel].sum() - smallest[0] < mass + 1e-4
+
+
+@pytest.mark.gpu
 def test_ivf_search_with_given_probes_equals_fixed_nprobe():
     from pynear_amd import _native as N, hipdev
     rng = np.random.default_rng(6)
