@@ -247,15 +247,33 @@ __global__ __launch_bounds__(256) void batch_rerank_kernel(const BatchEntry *__r
                  e.S * e.KL, e.kk, e.n, e.eps_coef, e.out_idx, e.out_dist, e.flags, g - e.q_base, e.lowm);
 }
 
-// One workgroup per query: flagged queries run the exact scan in parallel, the others exit at once.
+// One workgroup per chunk of kFallbackChunk queries of the batch: their flags are read at once
+// (each thread finds its query's entry), and the flagged queries run the exact scan one after
+// another (chunks in parallel).
 __global__ __launch_bounds__(256) void batch_fallback_kernel(const BatchEntry *__restrict__ E,
                                                              const int64_t *__restrict__ q_base, int ne, int64_t nq) {
     __shared__ uint64_t wres[4][32];
-    for (int64_t g = blockIdx.x; g < nq; g += gridDim.x) { // (the grid is capped: kFallbackMaxGrid)
-        const BatchEntry e = global_entry(E[find_entry(q_base, ne, g)]);
-        const int64_t q = g - e.q_base;
-        if (e.flags[q])
-            fallback_query<32>(wres, e.X, e.xstride, e.n, e.d, e.Q, e.qstride, e.kk, e.out_idx, e.out_dist, q);
+    __shared__ uint32_t s_flagged;
+    for (int64_t g0 = (int64_t)blockIdx.x * kFallbackChunk; g0 < nq; g0 += (int64_t)gridDim.x * kFallbackChunk) {
+        if (threadIdx.x < 64) { // (wave 0: bit t for query g0 + t)
+            bool f = false;
+            if (threadIdx.x < kFallbackChunk && g0 + threadIdx.x < nq) {
+                const int64_t g = g0 + threadIdx.x;
+                const int ei = find_entry(q_base, ne, g);
+                f = as_global(E[ei].flags)[g - q_base[ei]] != 0;
+            }
+            const uint64_t bal = __ballot(f);
+            if (threadIdx.x == 0) s_flagged = (uint32_t)bal;
+        }
+        __syncthreads();
+        const uint32_t fl = s_flagged;
+        __syncthreads(); // (s_flagged is rewritten for the next chunk)
+        for (uint32_t m = fl; m; m &= m - 1) {
+            const int64_t g = g0 + __builtin_ctz(m);
+            const BatchEntry e = global_entry(E[find_entry(q_base, ne, g)]);
+            fallback_query<32>(wres, e.X, e.xstride, e.n, e.d, e.Q, e.qstride, e.kk, e.out_idx, e.out_dist,
+                               g - e.q_base);
+        }
     }
 }
 
@@ -529,7 +547,10 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     if (!fused)
         hipLaunchKernelGGL(batch_rerank_kernel, dim3((unsigned)ceil_div(queries, (int64_t)4)), dim3(256), 0, st, dE,
                            dQ, ne, queries);
-    hipLaunchKernelGGL(batch_fallback_kernel, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(queries, kFallbackMaxGrid))), dim3(256), 0, st, dE,
+    hipLaunchKernelGGL(batch_fallback_kernel,
+                       dim3((unsigned)std::max<int64_t>(
+                           1, std::min<int64_t>(ceil_div(queries, (int64_t)kFallbackChunk), kFallbackMaxGrid))),
+                       dim3(256), 0, st, dE,
                        dQ, ne, queries);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(st, &g);

@@ -19,9 +19,14 @@ constexpr uint32_t kHamIdxMask = (1u << kHamIdxBits) - 1u;
 constexpr int64_t kMaxRowsPerSlice = int64_t(1) << kHamIdxBits;
 constexpr uint32_t kKey32None = 0xFFFFFFFFu;
 constexpr uint64_t kKey64None = ~uint64_t(0);
-// Largest grid of the per-query fallback launches (workgroups of 256 take queries b, b + grid, ...):
-// a grid of one workgroup per query would pass the 2^32 work-item limit at 16.7 M queries.
+// Largest grid of the per-query fallback launches (workgroups of 256 take query chunks b,
+// b + grid, ...): a grid of one workgroup per query would pass the 2^32 work-item limit at 16.7 M
+// queries.
 constexpr int64_t kFallbackMaxGrid = int64_t(1) << 20;
+// Queries per fallback workgroup: its first kFallbackChunk threads read their queries' flags at
+// once and the workgroup then runs the flagged ones in turn, so a launch over many unflagged
+// queries (the usual case) is a few workgroups per CU, not one per query.
+constexpr int kFallbackChunk = 16;
 
 // Workgroups are dealt to the eight XCDs round robin (workgroup b runs on XCD b % 8). For a grid
 // of a multiple of 8 workgroups this numbering gives each XCD one contiguous eighth of the block

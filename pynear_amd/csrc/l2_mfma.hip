@@ -127,7 +127,8 @@ __global__ __launch_bounds__(256) void l2_merge_rerank_kernel(
                      out_idx, out_dist, flags, q);
 }
 
-// One workgroup per query: flagged queries run the exact scan in parallel, the others exit at once.
+// One workgroup per chunk of kFallbackChunk queries: their flags are read at once, and the
+// flagged queries run the exact scan one after another (chunks in parallel).
 template <int K>
 __global__ __launch_bounds__(256) void l2_fallback_kernel(const float *__restrict__ X, int64_t xstride, int64_t n,
                                                           int64_t d, const float *__restrict__ Q, int64_t qstride,
@@ -135,8 +136,19 @@ __global__ __launch_bounds__(256) void l2_fallback_kernel(const float *__restric
                                                           int32_t *__restrict__ out_idx, float *__restrict__ out_dist,
                                                           int64_t nq) {
     __shared__ uint64_t wres[4][K];
-    for (int64_t q = blockIdx.x; q < nq; q += gridDim.x) // (the grid is capped: kFallbackMaxGrid)
-        if (flags[q]) fallback_query<K>(wres, X, xstride, n, d, Q, qstride, kk, out_idx, out_dist, q);
+    __shared__ uint32_t s_flagged;
+    for (int64_t q0 = (int64_t)blockIdx.x * kFallbackChunk; q0 < nq; q0 += (int64_t)gridDim.x * kFallbackChunk) {
+        if (threadIdx.x < 64) { // (wave 0: bit t for query q0 + t)
+            const bool f = threadIdx.x < kFallbackChunk && q0 + threadIdx.x < nq && flags[q0 + threadIdx.x];
+            const uint64_t bal = __ballot(f);
+            if (threadIdx.x == 0) s_flagged = (uint32_t)bal;
+        }
+        __syncthreads();
+        const uint32_t fl = s_flagged;
+        __syncthreads(); // (s_flagged is rewritten for the next chunk)
+        for (uint32_t m = fl; m; m &= m - 1)
+            fallback_query<K>(wres, X, xstride, n, d, Q, qstride, kk, out_idx, out_dist, q0 + __builtin_ctz(m));
+    }
 }
 
 // ---------------------------------------------------------------- small-database path
@@ -460,7 +472,8 @@ void l2_mfma_search(const L2Set &xs, const float *X, int64_t xstride, const floa
     const int64_t gx = ceil_div(nq, 128);
     // fallback: a workgroup per query, up to kFallbackMaxGrid workgroups (each then takes every
     // kFallbackMaxGrid-th query), so the grid stays within the launch limits at any query count
-    const unsigned fb_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nq, kFallbackMaxGrid));
+    const unsigned fb_grid =
+        (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nq, (int64_t)kFallbackChunk), kFallbackMaxGrid));
     const int64_t nq_pad = gx * 128;
     // queries: split into hi/lo (padded to nq_pad rows of dp; padding rows are zero)
     L2Set &qs = wk.qsplit;
