@@ -91,10 +91,14 @@ def test_batch_schedules_keep_results(slice_rows, order, share):
     for 32-row slices up to 12 for 4,096; longer settings are cut to 4,096), and the slices of a
     query share the bound of their lists (in index order, longest first, or slice by slice so that
     later slices start from finished ones). Every schedule returns the plain searches' results,
-    bit for bit, on indices of many slices (60,000 and 45,000 rows, k = 32 and 10)."""
+    bit for bit, on indices of many slices (60,000 and 45,000 rows, k = 32 and 10) and on the
+    wide-row scan's edge cases."""
     from pynear_amd import _native as N
 
-    subs = _make(5, SPECS[:6] + [(60000, 64, 200, 32), (45000, 256, 300, 10), (20000, 128, 90, 1)])
+    # (the wide-row class with more than 64 queries at 192 and 256 columns and k = 32, a partial
+    # last tile, an index of one partial tile and one of a full and a partial one)
+    subs = _make(5, SPECS[:6] + [(60000, 64, 200, 32), (45000, 256, 300, 10), (20000, 128, 90, 1),
+                                 (7001, 192, 150, 32), (9001, 256, 130, 32), (31, 128, 129, 1), (33, 256, 200, 10)])
     names = ("batch.slice_rows", "batch.order", "batch.share")
     old = [N.tune_get(n) for n in names]
     try:
