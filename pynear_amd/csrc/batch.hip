@@ -106,8 +106,17 @@ __global__ __launch_bounds__(256) void batch_split_kernel(const BatchEntry *__re
 }
 
 template <int KL>
-__device__ __forceinline__ void batch_tile(uint64_t *lds_raw, const BatchEntry &e, int64_t qb, int64_t s) {
+__device__ __forceinline__ void batch_tile(uint64_t *lds_raw, const BatchEntry &e, int64_t qb, int64_t s, bool pipe) {
     auto lds = reinterpret_cast<uint64_t (*)[32][KL]>(lds_raw);
+    if (pipe) {
+        switch (e.dp) {
+        case 16: scan_small_pipe<KL, 3, 1>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G, e.gthr); return;
+        case 32: scan_small_pipe<KL, 3, 2>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G, e.gthr); return;
+        case 48: scan_small_pipe<KL, 3, 3>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G, e.gthr); return;
+        case 64: scan_small_pipe<KL, 3, 4>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G, e.gthr); return;
+        default: break;
+        }
+    }
     switch (e.dp) {
     case 16: scan_small<KL, 3, 1, true>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G, e.gthr); break;
     case 32: scan_small<KL, 3, 2, true>(lds, e.Xh, e.Xl, e.xn, e.n, e.Qh, e.Ql, e.qn, e.rps, e.part, e.S * e.KL, qb, s, e.G, e.gthr); break;
@@ -161,7 +170,7 @@ static bool wide_dp(int dp) { return dp == 128 || dp == 192 || dp == 256; }
 template <int CLS>
 __global__ __launch_bounds__(256) void batch_scan_kernel(const BatchEntry *__restrict__ E,
                                                          const int2 *__restrict__ tiles, int64_t ntiles,
-                                                         int32_t *__restrict__ counter, bool pipe) {
+                                                         int32_t *__restrict__ counter, int pipe) {
     __shared__ uint64_t lds_raw[4 * 32 * 32];
     __shared__ __align__(16) __bf16 stage[CLS == 1 ? wide_stage_elems<4, 2>() : 8];
     __shared__ int64_t s_slot;
@@ -174,17 +183,17 @@ __global__ __launch_bounds__(256) void batch_scan_kernel(const BatchEntry *__res
         const BatchEntry e = global_entry(E[t.x]);
         const int64_t qb = t.y / e.S, s = t.y % e.S;
         if (CLS == 1) {
-            if (e.KL == 8) batch_tile_wide<8>(lds_raw, stage, e, qb, s, pipe);
-            else if (e.KL == 16) batch_tile_wide<16>(lds_raw, stage, e, qb, s, pipe);
-            else batch_tile_wide<32>(lds_raw, stage, e, qb, s, pipe);
+            if (e.KL == 8) batch_tile_wide<8>(lds_raw, stage, e, qb, s, pipe & 1);
+            else if (e.KL == 16) batch_tile_wide<16>(lds_raw, stage, e, qb, s, pipe & 1);
+            else batch_tile_wide<32>(lds_raw, stage, e, qb, s, pipe & 1);
         } else if (CLS == 2) {
             if (e.KL == 8) batch_tile_wide_direct<8>(lds_raw, e, qb, s);
             else if (e.KL == 16) batch_tile_wide_direct<16>(lds_raw, e, qb, s);
             else batch_tile_wide_direct<32>(lds_raw, e, qb, s);
         } else {
-            if (e.KL == 8) batch_tile<8>(lds_raw, e, qb, s);
-            else if (e.KL == 16) batch_tile<16>(lds_raw, e, qb, s);
-            else batch_tile<32>(lds_raw, e, qb, s);
+            if (e.KL == 8) batch_tile<8>(lds_raw, e, qb, s, pipe & 2);
+            else if (e.KL == 16) batch_tile<16>(lds_raw, e, qb, s, pipe & 2);
+            else batch_tile<32>(lds_raw, e, qb, s, pipe & 2);
         }
         __syncthreads(); // LDS and s_slot are reused by the next tile
     }
@@ -481,7 +490,8 @@ static void batch_plan(pn_batch *b, const BatchCfg &cfg) {
     // the merge): the wide class holds one workgroup -- one wave per SIMD -- per CU for its LDS
     // tiles, and the other two classes' workgroups run beside it instead of after it. The wide
     // scan is launched first so that its workgroups find their LDS free.
-    const bool pipe = tune_get("batch.pipe") != 0;
+    // bit 0: the wide rows by the pipelined scan, bit 1: the narrow rows
+    const int pipe = (int)tune_get("batch.pipe");
     const bool fork = tune_get("batch.concurrent_scans") != 0 && nc[1] > 0 && (nc[0] > 0 || nc[2] > 0);
     hipStream_t side[2] = {nullptr, nullptr};
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};

@@ -772,6 +772,120 @@ __device__ __forceinline__ void scan_wide_pipe(uint64_t (*lds)[32][KL], __bf16 *
     half_merge_out<KL, true>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL, gthr ? gthr + q : nullptr, r0, lowm);
 }
 
+// scan_small for keys-only lists, software-pipelined as scan_wide_pipe: a tile's keys are taken
+// in the straight-line block of the next tile's products. The row fragments and norms come from
+// a ring of three register sets loaded two tiles ahead (a set holds tile i for its norms, i + 1 for
+// the products, and i + 2 is loaded into the set of i - 1); g query groups per workgroup as in
+// scan_small.
+template <int KL, int PREC, int NS>
+__device__ __forceinline__ void scan_small_pipe(uint64_t (*lds)[32][KL], const __bf16 *__restrict__ Xh,
+                                                const __bf16 *__restrict__ Xl, const float *__restrict__ xn, int64_t n,
+                                                const __bf16 *__restrict__ Qh, const __bf16 *__restrict__ Ql,
+                                                const float *__restrict__ qn, int64_t rows_per_slice,
+                                                uint64_t *__restrict__ part, int64_t qsl, int64_t qb, int64_t s,
+                                                int g = 4, uint32_t *__restrict__ gthr = nullptr) {
+    constexpr int dp = 16 * NS;
+    constexpr int NPL = PREC == 3 ? 2 : 1;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, col = lane & 31;
+    const int64_t q = qb * 32 * g + 32 * (w % g) + col;
+    s = s * (4 / g) + w / g;
+    const int64_t r0 = s * rows_per_slice, r1 = min(n, r0 + rows_per_slice);
+    const uint32_t lowm = slice_row_mask(rows_per_slice);
+    const float qnv = qn[q];
+    bf16x8 b[NS][NPL];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        b[k][0] = *(const bf16x8 *)(Qh + q * dp + 16 * k + 8 * half);
+        if (PREC == 3) b[k][NPL - 1] = *(const bf16x8 *)(Ql + q * dp + 16 * k + 8 * half);
+    }
+    uint32_t Lk[KL], Li[KL]; // (Li: unused, keys-only lists)
+#pragma unroll
+    for (int j = 0; j < KL; ++j) Lk[j] = kKey32None;
+    uint32_t thr = kKey32None;
+    struct Tile {
+        bf16x8 a[NS][NPL];
+        float4 xv[4];
+    };
+    const int aoff = col * dp + 8 * half, noff = 4 * half;
+    const int64_t nt = r0 < r1 ? (r1 - r0 + 31) / 32 : 0; // tiles, the last one possibly partial
+    const int64_t nf = r0 < r1 ? (r1 - r0) / 32 : 0;      // full tiles
+    auto load_tile = [&](int64_t i, Tile &f) __attribute__((always_inline)) {
+        const int64_t t = r0 + 32 * min(i, nt - 1); // (past the last tile: that tile again, in bounds)
+        const __bf16 *th = Xh + t * dp, *tl = Xl + t * dp;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            f.a[k][0] = *(const bf16x8 *)(th + aoff + 16 * k);
+            if (PREC == 3) f.a[k][NPL - 1] = *(const bf16x8 *)(tl + aoff + 16 * k);
+        }
+#pragma unroll
+        for (int gg = 0; gg < 4; ++gg) f.xv[gg] = *(const float4 *)(xn + t + noff + 8 * gg);
+    };
+    auto products = [&](const Tile &f) __attribute__((always_inline)) {
+        f32x16 acc = {};
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.a[k][0], b[k][0], acc, 0, 0, 0);
+            if (PREC == 3) {
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.a[k][0], b[k][NPL - 1], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.a[k][NPL - 1], b[k][0], acc, 0, 0, 0);
+            }
+        }
+        return acc;
+    };
+    if (nt > 0) {
+        Tile f0, f1, f2;
+        load_tile(0, f0);
+        load_tile(1, f1);
+        f32x16 acc = products(f0);
+        // step i: tile i's keys (acc, its norms in `cur`), tile i + 1's products (`nxt`), tile i + 2
+        // loaded into `ld` (the set of tile i - 1)
+        auto step = [&](int64_t i, const Tile &cur, const Tile &nxt, Tile &ld) __attribute__((always_inline)) {
+            const bool share = gthr && i % kShareTiles == 0;
+            uint32_t old = kKey32None;
+            if (share) old = share_issue(gthr, q, half, thr);
+            load_tile(i + 2, ld);
+            const f32x16 an = products(nxt);
+            thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64)); // the other half-wave's list of this query
+            thr = share_take(thr, old); // (no branch: old is the empty key when not shared)
+            uint32_t key[16], m2, m3;
+            tile_keys_fast<KL>(acc, cur.xv, qnv, (uint32_t)(32 * i), lowm, half, Lk, thr, key, m2, m3);
+            if (__any(m3 < thr)) tile_keys_rest<KL>(key, m2, Lk, thr);
+            acc = an;
+        };
+        int64_t i = 0;
+        for (; i + 3 <= nf; i += 3) {
+            step(i, f0, f1, f2);
+            step(i + 1, f1, f2, f0);
+            step(i + 2, f2, f0, f1);
+        }
+        // the last full tiles (at most two) and the partial one, whose products are then in acc
+        // and its norms in the set that the rotation reached
+        if (i < nf) {
+            step(i, f0, f1, f2);
+            ++i;
+            if (i < nf) {
+                step(i, f1, f2, f0);
+                ++i;
+                if (nt > nf) {
+                    const int64_t t = r0 + 32 * nf;
+                    thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64));
+                    tile_keys_insert<KL, true, false, false, true>(acc, f2.xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm);
+                }
+            } else if (nt > nf) {
+                const int64_t t = r0 + 32 * nf;
+                thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64));
+                tile_keys_insert<KL, true, false, false, true>(acc, f1.xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm);
+            }
+        } else if (nt > nf) {
+            const int64_t t = r0 + 32 * nf;
+            thr = min(thr, (uint32_t)__shfl_xor((int)thr, 32, 64));
+            nf == 0 ? tile_keys_insert<KL, true, true, false, true>(acc, f0.xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm)
+                    : tile_keys_insert<KL, true, false, false, true>(acc, f0.xv, qnv, t, r1, half, Lk, Li, thr, r0, lowm);
+        }
+    }
+    half_merge_out<KL, true>(lds[w] + col, half, Lk, Li, part + q * qsl + s * KL, gthr ? gthr + q : nullptr, r0, lowm);
+}
+
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {

@@ -65,7 +65,7 @@ static std::map<std::string, int64_t> &tune_table() {
         t["batch.order"] = 2;          // batched exact search: 1 = tiles by decreasing cost, 2 = by slice, then cost
         t["batch.concurrent_scans"] = 1; // batched exact search: the three scan classes as parallel graph branches
         t["batch.share"] = 1;          // batched exact search: the slices of a query share their list bound
-        t["batch.pipe"] = 1;           // batched exact search: wide rows by the software-pipelined scan
+        t["batch.pipe"] = 3;           // batched exact search: software-pipelined scans (1: wide rows, 2: narrow rows)
         if (const char *env = std::getenv("PN_TUNE")) {
             std::string e(env);
             size_t pos = 0;

@@ -84,13 +84,16 @@ def test_batch_merge_variants(lean, fuse):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("slice_rows,order,share", [(32, 2, 1), (256, 0, 1), (2048, 2, 1), (4096, 2, 1),
-                                                    (4096, 1, 0), (8192, 2, 1)])
-def test_batch_schedules_keep_results(slice_rows, order, share):
+@pytest.mark.parametrize("slice_rows,order,share,pipe", [(32, 2, 1, 3), (256, 0, 1, 2), (2048, 2, 1, 3),
+                                                         (2048, 2, 1, 0), (4096, 2, 1, 1), (4096, 1, 0, 3),
+                                                         (8192, 2, 1, 3)])
+def test_batch_schedules_keep_results(slice_rows, order, share, pipe):
     """The slice lists hold keys only: a key's low bits carry its row's offset in the slice (5 bits
     for 32-row slices up to 12 for 4,096; longer settings are cut to 4,096), and the slices of a
     query share the bound of their lists (in index order, longest first, or slice by slice so that
-    later slices start from finished ones). Every schedule returns the plain searches' results,
+    later slices start from finished ones), and the wide and narrow rows go through the
+    software-pipelined scans or not (pipe bits 1 and 2). Every schedule returns the plain
+    searches' results,
     bit for bit, on indices of many slices (60,000 and 45,000 rows, k = 32 and 10) and on the
     wide-row scan's edge cases."""
     from pynear_amd import _native as N
@@ -99,10 +102,10 @@ def test_batch_schedules_keep_results(slice_rows, order, share):
     # last tile, an index of one partial tile and one of a full and a partial one)
     subs = _make(5, SPECS[:6] + [(60000, 64, 200, 32), (45000, 256, 300, 10), (20000, 128, 90, 1),
                                  (7001, 192, 150, 32), (9001, 256, 130, 32), (31, 128, 129, 1), (33, 256, 200, 10)])
-    names = ("batch.slice_rows", "batch.order", "batch.share")
+    names = ("batch.slice_rows", "batch.order", "batch.share", "batch.pipe")
     old = [N.tune_get(n) for n in names]
     try:
-        for n, v in zip(names, (slice_rows, order, share)):
+        for n, v in zip(names, (slice_rows, order, share, pipe)):
             N.tune_set(n, v)
         b = BatchSearch([(idx, Q, k) for idx, X, Q, k in subs])
         for _ in range(2):
@@ -113,7 +116,7 @@ def test_batch_schedules_keep_results(slice_rows, order, share):
             N.tune_set(n, v)
     for (idx, X, Q, k), (bi, bd) in zip(subs, res):
         pi, pd = _plain(idx, Q, k)
-        assert np.array_equal(bi, pi), (slice_rows, order, share, X.shape, Q.shape, k)
+        assert np.array_equal(bi, pi), (slice_rows, order, share, pipe, X.shape, Q.shape, k)
         assert np.array_equal(bd, pd)
 
 
