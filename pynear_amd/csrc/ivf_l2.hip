@@ -755,8 +755,12 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
                                                              const int32_t *__restrict__ probes, int64_t nq, int np,
                                                              int p_lo, int npw, const int32_t *__restrict__ pslot,
                                                              uint8_t *__restrict__ R8, float *__restrict__ Rn,
-                                                             float *__restrict__ Rq) {
-    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+                                                             float *__restrict__ Rq, bool xsplit) {
+    // xsplit: eight workgroups per four queries, workgroup b (on XCD b % 8) converting only the
+    // probes of lists c with c % 8 == b % 8, so each XCD's L2 holds an eighth of the centroids
+    // and a query of many probes reads its row once per XCD instead of a centroid row per probe
+    const int64_t q = (int64_t)(xsplit ? blockIdx.x >> 3 : blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int xcd = (int)(blockIdx.x & 7);
     const int lane = threadIdx.x & 63;
     if (q >= nq) return;
     const float *qrow = Q + q * qstride;
@@ -828,15 +832,28 @@ __global__ __launch_bounds__(256) void query_residual_kernel(const float *__rest
         const int nj = min(64, p_lo + npw - pb);
         const int myc = lane < nj ? probes[q * np + pb + lane] : -1;
         const int64_t mysl = lane < nj ? (int64_t)pslot[q * np + pb + lane] : 0;
+        // the probes this wave converts, in order: j walks the set bits of rem (wave-uniform), the
+        // next one's centroid row loaded while this one's residual is made (two register sets
+        // rotated by name)
+        uint64_t rem = __ballot(myc >= 0 && (!xsplit || (myc & 7) == xcd));
+        if (!rem) continue;
+        int j = __builtin_ctzll(rem);
+        rem &= rem - 1;
         Row ca, cb;
-        load_c(__shfl(myc, 0, 64), ca);
-        for (int j = 0; j < nj; j += 2) {
-            load_c(__shfl(myc, min(j + 1, nj - 1), 64), cb);
+        load_c(__shfl(myc, j, 64), ca);
+        for (;;) {
+            const int ja = rem ? __builtin_ctzll(rem) : j;
+            if (rem) load_c(__shfl(myc, ja, 64), cb);
             emit(__shfl(myc, j, 64), __shfl(mysl, j, 64), ca);
-            if (j + 1 < nj) {
-                load_c(__shfl(myc, min(j + 2, nj - 1), 64), ca);
-                emit(__shfl(myc, j + 1, 64), __shfl(mysl, j + 1, 64), cb);
-            }
+            if (!rem) break;
+            rem &= rem - 1;
+            j = ja;
+            const int jb = rem ? __builtin_ctzll(rem) : j;
+            if (rem) load_c(__shfl(myc, jb, 64), ca);
+            emit(__shfl(myc, j, 64), __shfl(mysl, j, 64), cb);
+            if (!rem) break;
+            rem &= rem - 1;
+            j = jb;
         }
     }
 }
@@ -2378,10 +2395,14 @@ static void ivfl2_lists_enqueue(pn_ivfl2 *h, const float *Q, int64_t re, int64_t
             // rows of whole 256-dim steps (d = dp = 256, 512 or 768, float4-aligned): no masking
             const int nvr = dp <= 256 ? 1 : (dp <= 512 ? 2 : (dp <= 768 ? 3 : 5));
             const bool full = d == dp && dp == 256 * nvr && re % 4 == 0 && ((uintptr_t)Q & 15) == 0;
-            const dim3 gq((unsigned)ceil_div(nq, (int64_t)4));
+            // many probes per query (a learned probe's): the XCD split of the centroid rows
+            const int64_t xnp = tune_get("ivfl2.residual_xcd_np");
+            const bool xsplit = xnp > 0 && npw >= xnp;
+            const dim3 gq((unsigned)(ceil_div(nq, (int64_t)4) * (xsplit ? kXcds : 1)));
             auto go = [&](auto nv, auto fl) {
                 hipLaunchKernelGGL((query_residual_kernel<decltype(nv)::value, decltype(fl)::value>), gq, dim3(256), 0,
-                                   st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq, np, p_lo, npw, psl, R8, Rn, Rq);
+                                   st, Q, re, (int)d, dp, lcp, h->sx8, pr, nq, np, p_lo, npw, psl, R8, Rn, Rq,
+                                   xsplit);
             };
             auto by_nv = [&](auto fl) {
                 if (nvr == 1) go(std::integral_constant<int, 1>{}, fl);

@@ -52,6 +52,7 @@ static std::map<std::string, int64_t> &tune_table() {
         t["ivfl2.fp8_wg_per_cu"] = 1;  // fp8 scan: workgroups per CU
         t["ivfl2.bound"] = 1;          // fp8: per-query thresholds from the first probe's segment bounds
         t["ivfl2.prune"] = 1;          // fp8 with bounds: skip row groups that no query can need
+        t["ivfl2.residual_xcd_np"] = 16; // fp8 residuals: per-XCD centroid split from this many probes per query
         t["ivfl2.two_pass"] = 0;       // fp8 without bounds: first probe, then the rest under its K2-th candidate
         t["ivfl2.fp8_iq2"] = 128;      // fp8 second pass of two: queries per work item (32 .. 128)
                                        // (the scan gains less than the second bucketing costs)
