@@ -990,9 +990,15 @@ __global__ __launch_bounds__(256) void seg_bound_kernel(const uint8_t *__restric
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const int ns = sb1 - sb0;
+        // a segment of at least k2 rows is a valid T by itself: the least such bound first (one
+        // pass), then only the smaller bounds need the count over all segments
         float best = INFINITY;
+        for (int t = half; t < ns; t += 2)
+            if (s_rows[t] >= k2) best = fminf(best, s_ub[w][col][t]);
+        best = fminf(best, __shfl_xor(best, 32, 64));
         for (int t = half; t < ns; t += 2) {
             const float u = s_ub[w][col][t];
+            if (u >= best) continue;
             int64_t rows = 0;
             for (int s2 = 0; s2 < ns; ++s2)
                 if (s_ub[w][col][s2] <= u) rows += s_rows[s2];
