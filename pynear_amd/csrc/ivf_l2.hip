@@ -2017,7 +2017,20 @@ __global__ void query_prep_kernel(int32_t *__restrict__ pr, int64_t nq, int np, 
             break;
         }
     qkey[q] = key;
-    atomicAdd(&ohist[key], 1);
+    // one atomic per distinct key of the wave (the queries of a hub list share their first list,
+    // and per-query atomics on its bin would serialize)
+    const int lane = threadIdx.x & 63;
+    bool pending = true;
+    for (;;) {
+        const uint64_t todo = __ballot(pending);
+        if (!todo) break;
+        const int leader = __builtin_ctzll(todo);
+        const int lk = __shfl(key, leader, 64);
+        const bool mine = pending && key == lk;
+        const uint64_t same = __ballot(mine);
+        if (lane == leader) atomicAdd(&ohist[lk], (int)__popcll(same));
+        if (mine) pending = false;
+    }
 }
 
 static void sub_partition_order(const float *dX, int64_t n, int64_t d, const std::vector<int64_t> &off,
