@@ -234,6 +234,7 @@ def run_exact(workload, steps, warmup, cpu, settle=False):
                 "traffic": measured_traffic("hamming_scan_kernel<16, 16, 1, 4, 0>"),
                 "kernel": "hamming_scan_kernel<16, 16, 1, 4, 0>", "kernel_ms": round(scan_s * 1e3, 4),
                 "work": f"{ops:.3e} int32 ops per launch: 16 words x (xor + popcount-accumulate) x Q x N"}
+        _traffic_ratio(roof, X.nbytes)
     else:
         flops = 2.0 * X.shape[1] * nq * n
         ach = flops / scan_s / 1e12
@@ -620,6 +621,7 @@ def run_mih(cpu, m=8, radius=16, k=10, reps=5):
                         "traffic": measured_traffic("hamming_scan_kernel<16, 16, 1, 4, 2>"),
                         "kernel": "hamming_scan_kernel<16, 16, 1, 4, 2>", "kernel_ms": round(scan, 4),
                         "work": f"{ops:.3e} int32 ops per launch: (16 words x 2 + 8 groups x 2) x Q x N"}}
+    _traffic_ratio(out["roofline"], X.nbytes)
     if cpu:
         sys.path.insert(0, ROOT)
         import oracle
@@ -867,6 +869,15 @@ def measured_traffic(kernel):
         return t["kernels"][kernel]["hbm_bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def _traffic_ratio(roof, db_bytes):
+    """Adds the HBM traffic of a Hamming scan as a multiple of its database bytes: the scan re-reads
+    each row block once per query block (scalar loads that mostly hit L2), harmless while the
+    kernel is VALU-bound far below the HBM rate."""
+    t = roof.get("traffic")
+    roof["traffic_over_db_bytes"] = round(t / db_bytes, 1) if t else None
+    roof["hbm_rate_tbs"] = round(t / (roof["kernel_ms"] / 1e3) / 1e12, 3) if t else None
 
 
 # ---------------------------------------------------------------- main
